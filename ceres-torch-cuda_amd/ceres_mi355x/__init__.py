@@ -1,0 +1,63 @@
+"""ceres_mi355x: MI355X-native trust-region solver core with the ceres_torch API.
+
+``import ceres_mi355x as tc`` gives the names code written against the
+reference's ``ceres_torch`` (``src/ceres_torch/__init__.py``) uses on the
+least-squares path: ``Problem``, cost and loss functions, manifolds,
+``SolverOptions``, ``solve``, ``SolverSummary`` and the linear-solver plugin
+registry.  The compute path is the HIP library ``libceres_mi355x.so``.
+"""
+from .costs import (
+    AnalyticCostFunction,
+    AutoDiffCostFunction,
+    CallableCostFunction,
+    CostFunction,
+    DynamicAutoDiffCostFunction,
+    ExpCurveResidual,
+    Pose2DRelativeError,
+    SnavelyReprojectionError,
+    relative_pose,
+    snavely_reprojection,
+)
+from .losses import (
+    ArctanLoss,
+    CauchyLoss,
+    ComposedLoss,
+    HuberLoss,
+    LossFunction,
+    LossFunctionWrapper,
+    ScaledLoss,
+    SoftLOneLoss,
+    TolerantLoss,
+    TrivialLoss,
+    TukeyLoss,
+)
+from .manifolds import EuclideanManifold, Manifold, SubsetManifold
+from .problem import EvaluationResult, ParameterBlock, Problem, ResidualBlock
+from .rotation import angle_axis_rotate_point, angle_axis_to_quaternion, unit_quaternion_rotate_point
+from .solver import set_device_loop, solve
+from .types import (
+    CallbackReturnType,
+    DenseLinearAlgebraLibraryType,
+    DoglegType,
+    IterationSummary,
+    LinearSolverOrderingType,
+    LinearSolverType,
+    LineSearchDirectionType,
+    LineSearchInterpolationType,
+    LineSearchType,
+    LoggingType,
+    MinimizerType,
+    NonlinearConjugateGradientType,
+    PreconditionerType,
+    SolverOptions,
+    SolverSummary,
+    SparseLinearAlgebraLibraryType,
+    TerminationType,
+    TrustRegionStrategyType,
+    VisibilityClusteringType,
+)
+from ._native import NativeLibraryError
+
+Solve = solve
+
+__all__ = [name for name in dir() if not name.startswith("_")]

@@ -1,0 +1,175 @@
+"""ctypes binding of libceres_mi355x.so (the C ABI in include/ceres_mi355x.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+solver entry point raises ``NativeLibraryError``; nothing falls back to a CPU
+or PyTorch implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(os.environ.get("CERES_MI355X_LIB", Path(__file__).resolve().parent / "libceres_mi355x.so"))
+
+CT_OK = 0
+CT_ERR_ARGUMENT = -1
+CT_ERR_HIP = -2
+CT_ERR_NOT_SPD = -3
+CT_ERR_WORKSPACE = -4
+CT_ERR_UNSUPPORTED = -5
+
+CT_REC_FIELDS = 12
+KERNELS = ["eval_points", "camera_blocks", "lm_begin", "rcs_solve", "refine", "candidate", "lm_end"]
+REC = dict(
+    iteration=0,
+    cost=1,
+    cost_change=2,
+    gradient_max_norm=3,
+    gradient_norm=4,
+    step_norm=5,
+    relative_decrease=6,
+    trust_region_radius=7,
+    step_is_valid=8,
+    step_is_successful=9,
+    linear_solver_iterations=10,
+    timestamp=11,
+)
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library is missing, failed to load, or a call failed."""
+
+
+class ct_loss(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("a", ctypes.c_double), ("b", ctypes.c_double), ("scale", ctypes.c_double)]
+
+
+class ct_lm_options(ctypes.Structure):
+    _fields_ = [
+        ("max_num_iterations", ctypes.c_int32),
+        ("max_num_consecutive_invalid_steps", ctypes.c_int32),
+        ("linear_solver", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("loop_mode", ctypes.c_int32),
+        ("min_linear_solver_iterations", ctypes.c_int32),
+        ("max_linear_solver_iterations", ctypes.c_int32),
+        ("max_num_refinement_iterations", ctypes.c_int32),
+        ("jacobi_scaling", ctypes.c_int32),
+        ("iterations_per_graph", ctypes.c_int32),
+        ("initial_trust_region_radius", ctypes.c_double),
+        ("max_trust_region_radius", ctypes.c_double),
+        ("min_trust_region_radius", ctypes.c_double),
+        ("min_relative_decrease", ctypes.c_double),
+        ("min_lm_diagonal", ctypes.c_double),
+        ("max_lm_diagonal", ctypes.c_double),
+        ("function_tolerance", ctypes.c_double),
+        ("gradient_tolerance", ctypes.c_double),
+        ("parameter_tolerance", ctypes.c_double),
+        ("eta", ctypes.c_double),
+    ]
+
+
+class ct_lm_report(ctypes.Structure):
+    _fields_ = [
+        ("termination", ctypes.c_int32),
+        ("reason", ctypes.c_int32),
+        ("num_records", ctypes.c_int32),
+        ("num_successful_steps", ctypes.c_int32),
+        ("num_unsuccessful_steps", ctypes.c_int32),
+        ("num_residual_evaluations", ctypes.c_int32),
+        ("num_jacobian_evaluations", ctypes.c_int32),
+        ("num_linear_solves", ctypes.c_int32),
+        ("initial_cost", ctypes.c_double),
+        ("final_cost", ctypes.c_double),
+        ("device_seconds", ctypes.c_double),
+    ]
+
+
+_vp = ctypes.c_void_p
+
+
+class ct_ba_problem(ctypes.Structure):
+    _fields_ = [
+        ("num_cameras", ctypes.c_int32),
+        ("num_points", ctypes.c_int32),
+        ("num_observations", ctypes.c_int32),
+        ("num_blocks", ctypes.c_int32),
+        ("num_terms", ctypes.c_int64),
+        ("obs_camera", _vp),
+        ("point_offsets", _vp),
+        ("observations", _vp),
+        ("camera_offsets", _vp),
+        ("camera_obs", _vp),
+        ("block_cameras", _vp),
+        ("block_offsets", _vp),
+        ("block_terms", _vp),
+        ("camera_mask", _vp),
+        ("point_mask", _vp),
+        ("loss", ct_loss),
+    ]
+
+
+# (name, restype, argtypes) for every entry point the header declares
+SIGNATURES = {
+    "ct_abi_version": (ctypes.c_int, []),
+    "ct_last_error": (ctypes.c_char_p, []),
+    "ct_ba_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), ctypes.POINTER(ct_lm_options),
+                                             ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_ba_plan_create": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), ctypes.POINTER(ct_lm_options), _vp, _vp, _vp,
+                                         ctypes.c_size_t, _vp, ctypes.POINTER(_vp)]),
+    "ct_ba_plan_run": (ctypes.c_int, [_vp, ctypes.POINTER(ct_lm_report), _vp]),
+    "ct_ba_plan_destroy": (ctypes.c_int, [_vp]),
+    "ct_ba_plan_profile": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "ct_ba_evaluate": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), _vp, _vp, ctypes.POINTER(ctypes.c_double), _vp,
+                                      _vp, ctypes.c_size_t, _vp]),
+}
+
+_LIB = None
+_LOAD_ERROR: str | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the native library, or raise NativeLibraryError."""
+    global _LIB, _LOAD_ERROR
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        _LOAD_ERROR = f"{LIB_PATH} not found; build it with ceres-torch-cuda_amd/build.py"
+        raise NativeLibraryError(_LOAD_ERROR)
+    try:
+        lib = ctypes.CDLL(str(LIB_PATH))
+    except OSError as exc:  # pragma: no cover - depends on the host
+        _LOAD_ERROR = f"failed to load {LIB_PATH}: {exc}"
+        raise NativeLibraryError(_LOAD_ERROR) from exc
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    if lib.ct_abi_version() != 1:
+        raise NativeLibraryError("ABI version mismatch")
+    _LIB = lib
+    return lib
+
+
+def check(status: int, what: str) -> None:
+    if status != CT_OK:
+        msg = load().ct_last_error()
+        raise NativeLibraryError(f"{what} failed ({status}): {msg.decode() if msg else ''}")
+
+
+def exported_symbols() -> list[str]:
+    return list(SIGNATURES)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    return int(t.data_ptr())
+
+
+def current_stream_handle() -> int | None:
+    import torch
+
+    return int(torch.cuda.current_stream().cuda_stream) or None

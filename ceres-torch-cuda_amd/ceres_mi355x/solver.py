@@ -1,0 +1,302 @@
+"""``solve(options, problem)`` (``src/ceres_torch/solver.py:28-32``) on the device.
+
+The reference runs the trust-region loop of ``solver.py:35-357`` in Python
+with one autograd call per residual block.  Here ``solve`` recognises a
+problem made of typed residual blocks, moves it to the device once and runs
+the whole LM loop in the HIP library; the summary it returns has the
+reference's fields and counters.  Problems or options the device path does not
+cover raise ``NotImplementedError`` naming what is unsupported; there is no
+silent fallback.
+"""
+from __future__ import annotations
+
+import time
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .ba import BADevice, lm_options
+from .losses import native_loss
+from .problem import EvaluationResult, Problem
+from .types import (
+    IterationSummary,
+    LinearSolverType,
+    MinimizerType,
+    SolverOptions,
+    SolverSummary,
+    TerminationType,
+    TrustRegionStrategyType,
+)
+
+_TERMINATION = {0: TerminationType.CONVERGENCE, 1: TerminationType.NO_CONVERGENCE, 2: TerminationType.FAILURE}
+_MESSAGES = {
+    1: "Gradient tolerance reached.",
+    2: "Parameter tolerance reached.",
+    3: "Function tolerance reached.",
+    4: "Minimum trust-region radius reached.",
+    5: "Maximum iterations reached.",
+    6: "Too many invalid trust-region steps.",
+    7: "No active parameters.",
+}
+
+# Device-path settings with no counterpart in SolverOptions.
+DEVICE_SETTINGS = dict(loop="device", iterations_per_graph=8)
+
+
+def set_device_loop(mode: str, iterations_per_graph: Optional[int] = None) -> None:
+    """'device' (graph-captured, device-resident LM state) or 'host' (one sync per iteration)."""
+    if mode not in ("device", "host"):
+        raise ValueError("mode must be 'device' or 'host'")
+    DEVICE_SETTINGS["loop"] = mode
+    if iterations_per_graph is not None:
+        DEVICE_SETTINGS["iterations_per_graph"] = int(iterations_per_graph)
+
+
+def _check_supported(options: SolverOptions) -> None:
+    if options.minimizer_type is MinimizerType.LINE_SEARCH:
+        raise NotImplementedError("the MI355X device path implements the TRUST_REGION minimizer only")
+    if options.trust_region_strategy_type is TrustRegionStrategyType.DOGLEG:
+        raise NotImplementedError("DOGLEG is not on the MI355X device path (LEVENBERG_MARQUARDT only)")
+    if options.use_nonmonotonic_steps:
+        raise NotImplementedError("non-monotonic steps are not on the MI355X device path")
+    if options.use_inner_iterations:
+        raise NotImplementedError("inner iterations are not on the MI355X device path")
+    if options.callbacks:
+        raise NotImplementedError("iteration callbacks are not on the MI355X device path")
+    if options.check_gradients:
+        raise NotImplementedError("check_gradients is not on the MI355X device path")
+    if options.linear_solver_type is LinearSolverType.CGNR:
+        raise NotImplementedError("CGNR is not on the MI355X device path")
+
+
+def _linear_kind(options: SolverOptions) -> str:
+    if options.linear_solver_type is LinearSolverType.ITERATIVE_SCHUR:
+        return "iterative_schur"
+    # DENSE_SCHUR / SPARSE_SCHUR, and the exact dense solvers, all produce the
+    # exact solution of the damped normal equations: one Schur + Cholesky path.
+    return "dense_schur"
+
+
+def _effective_solver(t: LinearSolverType) -> LinearSolverType:
+    """solver.py:788-795."""
+    if t is LinearSolverType.SPARSE_NORMAL_CHOLESKY:
+        return LinearSolverType.DENSE_NORMAL_CHOLESKY
+    if t is LinearSolverType.SPARSE_SCHUR:
+        return LinearSolverType.DENSE_SCHUR
+    return t
+
+
+# ---------------------------------------------------------------------------
+# bundle adjustment extraction
+# ---------------------------------------------------------------------------
+
+
+class _BAExtract:
+    def __init__(self, problem: Problem):
+        self.problem = problem
+        if problem.bulk_ba and problem._residual_blocks:
+            raise NotImplementedError("mixing bulk and per-block bundle adjustment residuals is not supported")
+        if problem.bulk_ba:
+            if len(problem.bulk_ba) != 1:
+                raise NotImplementedError("one bulk bundle-adjustment group per problem")
+            bulk = problem.bulk_ba[0]
+            self.cam_blocks = [problem._row_blocks[(id(bulk.cameras), i)] for i in range(bulk.cameras.shape[0])]
+            self.pt_blocks = [problem._row_blocks[(id(bulk.points), i)] for i in range(bulk.points.shape[0])]
+            self.obs_camera = bulk.camera_index
+            self.obs_point = bulk.point_index
+            self.observations = bulk.observations
+            self.loss = native_loss(bulk.loss)
+            self.cam_base, self.pt_base = bulk.cameras, bulk.points
+        else:
+            rbs = problem._residual_blocks
+            if not rbs or any(rb.cost_function.device_kind != "ba" for rb in rbs):
+                raise NotImplementedError("not a bundle-adjustment problem")
+            losses = {native_loss(rb.loss_function) for rb in rbs}
+            if len(losses) != 1:
+                raise NotImplementedError("the device path needs one loss function for all residual blocks")
+            self.loss = losses.pop()
+            cam_index, pt_index = {}, {}
+            self.cam_blocks, self.pt_blocks = [], []
+            oc, op, ob = [], [], []
+            for rb in rbs:
+                cb, pb = rb.parameter_blocks
+                if cb not in cam_index:
+                    cam_index[cb] = len(self.cam_blocks)
+                    self.cam_blocks.append(cb)
+                if pb not in pt_index:
+                    pt_index[pb] = len(self.pt_blocks)
+                    self.pt_blocks.append(pb)
+                oc.append(cam_index[cb])
+                op.append(pt_index[pb])
+                ob.append((rb.cost_function.observed_x, rb.cost_function.observed_y))
+            if set(cam_index) & set(pt_index):
+                raise NotImplementedError("a parameter block is used both as camera and point")
+            self.obs_camera = np.asarray(oc)
+            self.obs_point = np.asarray(op)
+            self.observations = np.asarray(ob, dtype=np.float64)
+            self.cam_base = self.pt_base = None
+        for b in self.cam_blocks + self.pt_blocks:
+            if b.lower_bound is not None or b.upper_bound is not None:
+                raise NotImplementedError("parameter bounds are not on the MI355X device path")
+            if b.tensor.dtype != torch.float64:
+                raise NotImplementedError("bundle-adjustment parameters must be float64 tensors")
+
+    def stacked(self):
+        if self.cam_base is not None:
+            return self.cam_base.detach(), self.pt_base.detach()
+        cams = torch.stack([b.tensor.detach().reshape(-1) for b in self.cam_blocks]) if self.cam_blocks else torch.zeros(0, 9, dtype=torch.float64)
+        pts = torch.stack([b.tensor.detach().reshape(-1) for b in self.pt_blocks]) if self.pt_blocks else torch.zeros(0, 3, dtype=torch.float64)
+        return cams, pts
+
+    def masks(self):
+        cm = np.array([b.free_mask() for b in self.cam_blocks], dtype=np.uint8).reshape(-1, 9)
+        pm = np.array([b.free_mask() for b in self.pt_blocks], dtype=np.uint8).reshape(-1, 3)
+        return (None if cm.all() else cm), (None if pm.all() else pm)
+
+    def write_back(self, cams: torch.Tensor, pts: torch.Tensor) -> None:
+        with torch.no_grad():
+            if self.cam_base is not None:
+                self.cam_base.copy_(cams.to(self.cam_base.device, self.cam_base.dtype))
+                self.pt_base.copy_(pts.to(self.pt_base.device, self.pt_base.dtype))
+                return
+            cams_h = cams.to("cpu")
+            pts_h = pts.to("cpu")
+            for i, b in enumerate(self.cam_blocks):
+                b.tensor.reshape(-1).copy_(cams_h[i].to(b.tensor.device))
+            for i, b in enumerate(self.pt_blocks):
+                b.tensor.reshape(-1).copy_(pts_h[i].to(b.tensor.device))
+
+
+def _device_for(problem: Problem) -> BADevice:
+    cached = getattr(problem, "_device_cache", None)
+    ex = _BAExtract(problem)
+    cams, pts = ex.stacked()
+    if cached is not None and cached[0] == (len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss):
+        dev = cached[1]
+        dev.set_parameters(cams, pts)
+        return dev, ex
+    cm, pm = ex.masks()
+    dev = BADevice(cams, pts, ex.obs_camera, ex.obs_point, ex.observations, loss_native=ex.loss,
+                   camera_mask=cm, point_mask=pm)
+    problem._device_cache = ((len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss), dev)
+    return dev, ex
+
+
+def _records_to_iterations(records: np.ndarray, eta: float) -> list[IterationSummary]:
+    its = []
+    prev_t = 0.0
+    for row in records:
+        t = float(row[nat.REC["timestamp"]])
+        its.append(IterationSummary(
+            iteration=int(row[0]),
+            step_is_valid=bool(row[nat.REC["step_is_valid"]]),
+            step_is_successful=bool(row[nat.REC["step_is_successful"]]),
+            cost=float(row[nat.REC["cost"]]),
+            cost_change=float(row[nat.REC["cost_change"]]),
+            gradient_max_norm=float(row[nat.REC["gradient_max_norm"]]),
+            gradient_norm=float(row[nat.REC["gradient_norm"]]),
+            step_norm=float(row[nat.REC["step_norm"]]),
+            relative_decrease=float(row[nat.REC["relative_decrease"]]),
+            trust_region_radius=float(row[nat.REC["trust_region_radius"]]),
+            eta=eta,
+            linear_solver_iterations=int(row[nat.REC["linear_solver_iterations"]]),
+            iteration_time_in_seconds=max(0.0, t - prev_t),
+            cumulative_time_in_seconds=t,
+        ))
+        prev_t = t
+    return its
+
+
+def solve(options: SolverOptions, problem: Problem) -> SolverSummary:
+    """Run the trust-region solve of ``problem`` on the MI355X."""
+    options.validate()
+    _check_supported(options)
+    start = time.perf_counter()
+    if not torch.cuda.is_available():
+        raise nat.NativeLibraryError("the MI355X solver needs a ROCm GPU (torch.cuda.is_available() is False)")
+    kinds = {rb.cost_function.device_kind for rb in problem._residual_blocks}
+    if problem.bulk_ba:
+        kinds.add("ba")
+    if kinds == {"ba"}:
+        return _solve_ba(options, problem, start)
+    if kinds == {"curve"}:
+        from .curve import solve_curve_problem
+
+        return solve_curve_problem(options, problem, start)
+    if kinds == {"pose2d"}:
+        from .pose2d import solve_pose_problem
+
+        return solve_pose_problem(options, problem, start)
+    raise NotImplementedError(
+        "the MI355X device path solves problems made of SnavelyReprojectionError, ExpCurveResidual or "
+        f"Pose2DRelativeError residual blocks; got {sorted(str(k) for k in kinds)}"
+    )
+
+
+def summary_from_report(options: SolverOptions, problem: Problem, rep, records, start: float,
+                        precision: str) -> SolverSummary:
+    s = SolverSummary(
+        minimizer_type=options.minimizer_type,
+        num_parameter_blocks=problem.num_parameter_blocks(),
+        num_parameters=problem.num_parameters(),
+        num_effective_parameters=problem.num_effective_parameters(),
+        num_residual_blocks=problem.num_residual_blocks(),
+        num_residuals=problem.num_residuals(),
+        linear_solver_type_given=options.linear_solver_type,
+        linear_solver_type_used=_effective_solver(options.linear_solver_type),
+        trust_region_strategy_type=options.trust_region_strategy_type,
+        dogleg_type=options.dogleg_type,
+        line_search_direction_type=options.line_search_direction_type,
+        line_search_type=options.line_search_type,
+        line_search_interpolation_type=options.line_search_interpolation_type,
+        nonlinear_conjugate_gradient_type=options.nonlinear_conjugate_gradient_type,
+        preconditioner_type=options.preconditioner_type,
+        max_lbfgs_rank=options.max_lbfgs_rank,
+        dense_linear_algebra_library_type=options.dense_linear_algebra_library_type,
+        sparse_linear_algebra_library_type=options.sparse_linear_algebra_library_type,
+    )
+    s.termination_type = _TERMINATION[int(rep.termination)]
+    s.message = _MESSAGES.get(int(rep.reason), "Solver failed.")
+    s.initial_cost = float(rep.initial_cost)
+    s.final_cost = float(rep.final_cost)
+    s.num_successful_steps = int(rep.num_successful_steps)
+    s.num_unsuccessful_steps = int(rep.num_unsuccessful_steps)
+    s.num_residual_evaluations = int(rep.num_residual_evaluations)
+    s.num_jacobian_evaluations = int(rep.num_jacobian_evaluations)
+    s.num_linear_solves = int(rep.num_linear_solves)
+    s.iterations = _records_to_iterations(records, options.eta)
+    s.device_time_in_seconds = float(rep.device_seconds)
+    s.minimizer_time_in_seconds = float(rep.device_seconds)
+    s.precision = precision
+    s.total_time_in_seconds = time.perf_counter() - start
+    return s
+
+
+def _solve_ba(options: SolverOptions, problem: Problem, start: float) -> SolverSummary:
+    dev, ex = _device_for(problem)
+    precision = "mixed" if options.use_mixed_precision_solves else "fp64"
+    opts = lm_options(options, precision=precision, loop=DEVICE_SETTINGS["loop"], linear=_linear_kind(options),
+                      iterations_per_graph=DEVICE_SETTINGS["iterations_per_graph"])
+    pre = time.perf_counter()
+    rep, records = dev.run(opts)
+    torch.cuda.synchronize()
+    s = summary_from_report(options, problem, rep, records, start, precision)
+    s.preprocessor_time_in_seconds = pre - start
+    post = time.perf_counter()
+    ex.write_back(dev.cameras, dev.points)
+    s.postprocessor_time_in_seconds = time.perf_counter() - post
+    s.total_time_in_seconds = time.perf_counter() - start
+    return s
+
+
+def evaluate_problem(problem: Problem) -> EvaluationResult:
+    kinds = {rb.cost_function.device_kind for rb in problem._residual_blocks}
+    if problem.bulk_ba:
+        kinds.add("ba")
+    if kinds != {"ba"}:
+        raise NotImplementedError("device evaluation is implemented for bundle-adjustment problems")
+    dev, ex = _device_for(problem)
+    cost, res = dev.evaluate()
+    return EvaluationResult(cost=torch.tensor(cost, dtype=torch.float64), residuals=res.reshape(-1))

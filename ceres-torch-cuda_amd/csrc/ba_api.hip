@@ -1,0 +1,427 @@
+// C ABI of the bundle-adjustment solver: workspace sizing, plans, and the
+// host-driven or graph-captured device-resident LM loop.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ba_kernels.h"
+
+namespace ct {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+namespace {
+
+int validate_problem(const ct_ba_problem* p) {
+  CT_CHECK_ARG(p != nullptr, "problem is null");
+  CT_CHECK_ARG(p->num_cameras >= 0 && p->num_points >= 0 && p->num_observations >= 0, "negative problem size");
+  CT_CHECK_ARG(p->num_blocks >= 0 && p->num_terms >= 0, "negative block/term count");
+  if (p->num_observations > 0) {
+    CT_CHECK_ARG(p->obs_camera && p->point_offsets && p->observations, "observation arrays are null");
+    CT_CHECK_ARG(p->camera_offsets && p->camera_obs, "camera CSR arrays are null");
+  }
+  if (p->num_blocks > 0) {
+    CT_CHECK_ARG(p->block_cameras && p->block_offsets, "block arrays are null");
+    CT_CHECK_ARG(p->num_terms == 0 || p->block_terms, "block_terms is null");
+  }
+  CT_CHECK_ARG(p->loss.type >= CT_LOSS_TRIVIAL && p->loss.type <= CT_LOSS_TUKEY, "unknown loss type");
+  CT_CHECK_ARG((int64_t)p->num_cameras * 9 <= (int64_t)1 << 20, "too many cameras for the dense reduced system");
+  return CT_OK;
+}
+
+int validate_options(const ct_lm_options* o) {
+  CT_CHECK_ARG(o != nullptr, "options is null");
+  CT_CHECK_ARG(o->max_num_iterations >= 0, "max_num_iterations must be >= 0");
+  CT_CHECK_ARG(o->linear_solver == CT_DENSE_SCHUR || o->linear_solver == CT_ITERATIVE_SCHUR, "unknown linear solver");
+  CT_CHECK_ARG(o->precision == CT_PLAIN_FP64 || o->precision == CT_MIXED, "unknown precision");
+  CT_CHECK_ARG(o->loop_mode == CT_LOOP_HOST || o->loop_mode == CT_LOOP_DEVICE, "unknown loop mode");
+  CT_CHECK_ARG(o->initial_trust_region_radius > 0.0, "initial_trust_region_radius must be > 0");
+  return CT_OK;
+}
+
+int eval_blocks(const ct_ba_problem* p) { return (p->num_points + kPointThreads - 1) / kPointThreads; }
+int cand_blocks(const ct_ba_problem* p) {
+  const int work = p->num_points > 9 * p->num_cameras ? p->num_points : 9 * p->num_cameras;
+  return (work + kPointThreads - 1) / kPointThreads;
+}
+
+struct Layout {
+  BAWork w{};
+  BABuffers<float> bf{};
+  BABuffers<double> bd{};
+  double* pcg_work = nullptr;
+  size_t pcg_doubles = 0;
+  size_t bytes = 0;
+};
+
+Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
+  Layout L;
+  Carver cv(base);
+  const size_t C = p->num_cameras, P = p->num_points, N = p->num_observations;
+  const size_t n = 9 * C;
+  L.w.xcam = cv.take<double>(2 * 9 * C);
+  L.w.xpt = cv.take<double>(2 * 3 * P);
+  L.w.S = cv.take<double>(n * n);
+  L.w.rhs = cv.take<double>(n);
+  L.w.dxc = cv.take<double>(n);
+  L.w.dxp = cv.take<double>(3 * P);
+  L.w.part_eval = cv.take<double>(3 * (size_t)eval_blocks(p) + 3);
+  L.w.part_cam = cv.take<double>(2 * C + 2);
+  L.w.part_cand = cv.take<double>(5 * (size_t)cand_blocks(p) + 5);
+  L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
+  L.w.state = cv.take<LMState>(1);
+  L.w.Dp = cv.take<double>(3 * P);
+  L.w.Dc = cv.take<double>(n);
+  L.w.gc = cv.take<double>(n);
+  L.w.res_c = cv.take<double>(n);
+  L.w.res_p = cv.take<double>(3 * P);
+  L.w.u_obs = cv.take<double>(4 * N);
+  L.w.dxc_corr = cv.take<double>(n);
+  L.w.solve_tmp = cv.take<double>(rcs_cholesky_tmp_doubles((int)n) + 1);
+  L.w.status = cv.take<int32_t>(4);
+  if (o->precision == CT_MIXED) {
+    L.bf.Jc = cv.take<float>(18 * N);
+    L.bf.Jp = cv.take<float>(6 * N);
+    L.bf.Pf = cv.take<float>(6 * N);
+    L.bf.e = cv.take<float>(2 * N);
+    L.bf.r = cv.take<double>(2 * N);
+    L.bf.Linv = cv.take<float>(6 * P);
+    L.bf.q = cv.take<float>(3 * P);
+    L.bf.gp = cv.take<double>(3 * P);
+  } else {
+    L.bd.Jc = cv.take<double>(18 * N);
+    L.bd.Jp = cv.take<double>(6 * N);
+    L.bd.Pf = cv.take<double>(6 * N);
+    L.bd.e = cv.take<double>(2 * N);
+    L.bd.r = cv.take<double>(2 * N);
+    L.bd.Linv = cv.take<double>(6 * P);
+    L.bd.q = cv.take<double>(3 * P);
+    L.bd.gp = cv.take<double>(3 * P);
+  }
+  if (o->linear_solver == CT_ITERATIVE_SCHUR) {
+    L.pcg_doubles = rcs_pcg_work_doubles((int)n);
+    L.pcg_work = cv.take<double>(L.pcg_doubles);
+  }
+  L.bytes = cv.offset + 256;
+  return L;
+}
+
+BAView make_view(const ct_ba_problem* p) {
+  BAView v;
+  v.C = p->num_cameras;
+  v.P = p->num_points;
+  v.N = p->num_observations;
+  v.B = p->num_blocks;
+  v.T = p->num_terms;
+  v.obs_camera = p->obs_camera;
+  v.point_offsets = p->point_offsets;
+  v.obs_xy = p->observations;
+  v.camera_offsets = p->camera_offsets;
+  v.camera_obs = p->camera_obs;
+  v.block_cameras = p->block_cameras;
+  v.block_offsets = p->block_offsets;
+  v.block_terms = p->block_terms;
+  v.camera_mask = p->camera_mask;
+  v.point_mask = p->point_mask;
+  v.loss = p->loss;
+  if (v.loss.scale == 0.0) v.loss.scale = 1.0;
+  return v;
+}
+
+}  // namespace
+
+}  // namespace ct
+
+using namespace ct;
+
+struct ct_ba_plan {
+  ct_ba_problem problem;
+  ct_lm_options options;
+  BAView view;
+  Layout layout;
+  LMInit init;
+  double* user_cams = nullptr;
+  double* user_pts = nullptr;
+  hipStream_t stream = nullptr;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_iterations = 0;
+  LMState* host_state = nullptr;  // pinned
+  double wall_rate_hz = 1e8;
+};
+
+namespace {
+
+bool refining(const ct_ba_plan* pl) {
+  return pl->options.precision == CT_MIXED && pl->options.linear_solver == CT_DENSE_SCHUR &&
+         pl->options.max_num_refinement_iterations > 0;
+}
+
+// Kernel group k of one LM iteration (CT_KERNEL_* order).
+int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
+  const BAView& v = pl->view;
+  Layout& L = pl->layout;
+  const int n = 9 * v.C;
+  const bool mixed = pl->options.precision == CT_MIXED;
+  switch (group) {
+    case CT_KERNEL_EVAL_POINTS:
+      return mixed ? launch_eval_points<float>(v, L.bf, L.w, s) : launch_eval_points<double>(v, L.bd, L.w, s);
+    case CT_KERNEL_CAMERA_BLOCKS:
+      return mixed ? launch_camera_blocks<float>(v, L.bf, L.w, s) : launch_camera_blocks<double>(v, L.bd, L.w, s);
+    case CT_KERNEL_LM_BEGIN:
+      return launch_lm_begin(pl->init, L.w, s);
+    case CT_KERNEL_RCS_SOLVE:
+      if (pl->options.linear_solver == CT_ITERATIVE_SCHUR)
+        return launch_rcs_pcg(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.pcg_work, L.pcg_doubles, s);
+      return launch_rcs_cholesky_solve(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.w.solve_tmp, s);
+    case CT_KERNEL_REFINE:
+      if (!refining(pl)) return CT_OK;
+      CT_TRY(launch_backsub<float>(v, L.bf, L.w, s));
+      for (int k = 0; k < pl->options.max_num_refinement_iterations; ++k)
+        CT_TRY(launch_refine_step<float>(v, L.bf, L.w, s));
+      return CT_OK;
+    case CT_KERNEL_CANDIDATE:
+      return mixed ? launch_candidate<float>(v, L.bf, L.w, !refining(pl), s)
+                   : launch_candidate<double>(v, L.bd, L.w, true, s);
+    case CT_KERNEL_LM_END:
+      return launch_lm_end(pl->init, L.w, s);
+    default:
+      return CT_OK;
+  }
+}
+
+// One LM iteration: evaluate+eliminate, camera blocks, begin, RCS solve,
+// [back-substitution + refinement], candidate, end.
+int enqueue_iteration(ct_ba_plan* pl, hipStream_t s) {
+  for (int g = 0; g < CT_KERNEL_COUNT; ++g) CT_TRY(enqueue_group(pl, g, s));
+  return CT_OK;
+}
+
+int read_state(ct_ba_plan* pl, hipStream_t s) {
+  CT_HIP(hipMemcpyAsync(pl->host_state, pl->layout.w.state, sizeof(LMState), hipMemcpyDeviceToHost, s));
+  CT_HIP(hipStreamSynchronize(s));
+  return CT_OK;
+}
+
+int build_graph(ct_ba_plan* pl) {
+  if (pl->graph_exec) return CT_OK;
+  CT_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
+  int st = CT_OK;
+  for (int k = 0; k < pl->graph_iterations && st == CT_OK; ++k) st = enqueue_iteration(pl, pl->capture_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
+  if (st != CT_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e));
+    return CT_ERR_HIP;
+  }
+  pl->graph = g;
+  CT_HIP(hipGraphInstantiate(&pl->graph_exec, g, nullptr, nullptr, 0));
+  return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_abi_version(void) { return CT_ABI_VERSION; }
+
+const char* ct_last_error(void) { return g_last_error.c_str(); }
+
+int ct_ba_workspace_bytes(const ct_ba_problem* problem, const ct_lm_options* options, size_t* bytes) {
+  clear_error();
+  CT_TRY(validate_problem(problem));
+  CT_TRY(validate_options(options));
+  CT_CHECK_ARG(bytes != nullptr, "bytes is null");
+  *bytes = carve(problem, options, nullptr).bytes;
+  return CT_OK;
+}
+
+int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options, double* cameras, double* points,
+                      void* workspace, size_t workspace_bytes, ct_stream_t stream, ct_ba_plan** plan) {
+  clear_error();
+  CT_TRY(validate_problem(problem));
+  CT_TRY(validate_options(options));
+  CT_CHECK_ARG(plan != nullptr, "plan is null");
+  CT_CHECK_ARG(workspace != nullptr, "workspace is null");
+  CT_CHECK_ARG(problem->num_cameras == 0 || cameras != nullptr, "cameras is null");
+  CT_CHECK_ARG(problem->num_points == 0 || points != nullptr, "points is null");
+  const size_t need = carve(problem, options, nullptr).bytes;
+  if (workspace_bytes < need) {
+    set_error("workspace too small: need " + std::to_string(need) + " bytes");
+    return CT_ERR_WORKSPACE;
+  }
+  auto pl = std::make_unique<ct_ba_plan>();
+  pl->problem = *problem;
+  pl->options = *options;
+  pl->view = make_view(problem);
+  pl->layout = carve(problem, options, workspace);
+  pl->user_cams = cameras;
+  pl->user_pts = points;
+  pl->stream = reinterpret_cast<hipStream_t>(stream);
+  pl->init.opt = *options;
+  pl->init.num_eval_blocks = eval_blocks(problem);
+  pl->init.num_cand_blocks = cand_blocks(problem);
+  pl->init.num_cameras = problem->num_cameras;
+  pl->init.n_cam_params = 9 * problem->num_cameras;
+  pl->init.n_pt_params = 3 * problem->num_points;
+  int per = options->iterations_per_graph > 0 ? options->iterations_per_graph : 8;
+  if (per > options->max_num_iterations + 2) per = options->max_num_iterations + 2;
+  pl->graph_iterations = per;
+  CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pl->host_state), sizeof(LMState), hipHostMallocDefault));
+  if (options->loop_mode == CT_LOOP_DEVICE) {
+    CT_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
+  }
+  *plan = pl.release();
+  return CT_OK;
+}
+
+int ct_ba_plan_destroy(ct_ba_plan* plan) {
+  clear_error();
+  if (!plan) return CT_OK;
+  if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
+  if (plan->host_state) (void)hipHostFree(plan->host_state);
+  delete plan;
+  return CT_OK;
+}
+
+int ct_ba_plan_run(ct_ba_plan* pl, ct_lm_report* report, double* records) {
+  clear_error();
+  CT_CHECK_ARG(pl != nullptr, "plan is null");
+  const int max_passes = pl->options.max_num_iterations + 2;  // iterations 0..max + deferred checks
+  hipStream_t s = pl->stream;
+  if (pl->options.loop_mode == CT_LOOP_DEVICE) {
+    // order the capture stream after the caller's stream and back
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    CT_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+    CT_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    CT_HIP(hipEventRecord(ev_in, s));
+    CT_HIP(hipStreamWaitEvent(pl->capture_stream, ev_in, 0));
+    int st = build_graph(pl);
+    if (st != CT_OK) {
+      (void)hipEventDestroy(ev_in);
+      (void)hipEventDestroy(ev_out);
+      return st;
+    }
+    hipStream_t cs = pl->capture_stream;
+    CT_TRY(launch_lm_init(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, cs));
+    const int launches = (max_passes + pl->graph_iterations - 1) / pl->graph_iterations;
+    for (int k = 0; k < launches; ++k) CT_HIP(hipGraphLaunch(pl->graph_exec, cs));
+    CT_TRY(launch_lm_finalize(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, cs));
+    CT_HIP(hipEventRecord(ev_out, cs));
+    CT_HIP(hipStreamWaitEvent(s, ev_out, 0));
+    (void)hipEventDestroy(ev_in);
+    (void)hipEventDestroy(ev_out);
+    CT_TRY(read_state(pl, s));
+  } else {
+    CT_TRY(launch_lm_init(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, s));
+    for (int k = 0; k < max_passes; ++k) {
+      CT_TRY(enqueue_iteration(pl, s));
+      CT_TRY(read_state(pl, s));  // the one host sync of the iteration
+      if (pl->host_state->done) break;
+    }
+    CT_TRY(launch_lm_finalize(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, s));
+    CT_TRY(read_state(pl, s));
+  }
+  const LMState& h = *pl->host_state;
+  if (report) {
+    report->termination = h.termination;
+    report->reason = h.reason;
+    report->num_records = h.num_records;
+    report->num_successful_steps = h.num_successful;
+    report->num_unsuccessful_steps = h.num_unsuccessful;
+    report->num_residual_evaluations = h.num_residual_evals;
+    report->num_jacobian_evaluations = h.num_jacobian_evals;
+    report->num_linear_solves = h.num_linear_solves;
+    report->initial_cost = h.initial_cost;
+    report->final_cost = h.best_cost;
+    report->device_seconds = (double)(h.t_end - h.t0) * 1e-8;
+  }
+  if (records) {
+    const size_t nrec = (size_t)(pl->options.max_num_iterations + 2) * CT_REC_FIELDS;
+    CT_HIP(hipMemcpyAsync(records, pl->layout.w.records, nrec * sizeof(double), hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+  }
+  if (!h.done) {
+    set_error("LM loop did not terminate within the pass budget");
+    return CT_ERR_HIP;
+  }
+  return CT_OK;
+}
+
+int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_t* kernel_launches) {
+  clear_error();
+  CT_CHECK_ARG(pl != nullptr, "plan is null");
+  CT_CHECK_ARG(iterations > 0, "iterations must be > 0");
+  CT_CHECK_ARG(kernel_ms != nullptr && kernel_launches != nullptr, "output arrays are null");
+  hipStream_t s = pl->stream;
+  Layout& L = pl->layout;
+  hipEvent_t ev[CT_KERNEL_COUNT + 1];
+  for (int k = 0; k <= CT_KERNEL_COUNT; ++k) CT_HIP(hipEventCreate(&ev[k]));
+  for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
+    kernel_ms[k] = 0.0;
+    kernel_launches[k] = 0;
+  }
+  int status = launch_lm_init(pl->init, L.w, pl->user_cams, pl->user_pts, s);
+  // iteration 0 (evaluation only) is not timed
+  if (status == CT_OK) status = enqueue_iteration(pl, s);
+  for (int it = 0; it < iterations && status == CT_OK; ++it) {
+    for (int g = 0; g < CT_KERNEL_COUNT && status == CT_OK; ++g) {
+      hipEventRecord(ev[g], s);
+      status = enqueue_group(pl, g, s);
+    }
+    hipEventRecord(ev[CT_KERNEL_COUNT], s);
+    if (status != CT_OK) break;
+    if (hipEventSynchronize(ev[CT_KERNEL_COUNT]) != hipSuccess) {
+      set_error("event synchronize failed");
+      status = CT_ERR_HIP;
+      break;
+    }
+    for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      kernel_ms[k] += ms;
+      kernel_launches[k] += 1;
+    }
+  }
+  for (int k = 0; k <= CT_KERNEL_COUNT; ++k) (void)hipEventDestroy(ev[k]);
+  if (status == CT_OK) status = launch_lm_finalize(pl->init, L.w, pl->user_cams, pl->user_pts, s);
+  if (status == CT_OK) status = read_state(pl, s);
+  return status;
+}
+
+int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const double* points, double* cost_out_host,
+                   double* residuals, void* workspace, size_t workspace_bytes, ct_stream_t stream) {
+  clear_error();
+  CT_TRY(validate_problem(problem));
+  CT_CHECK_ARG(cost_out_host != nullptr, "cost_out_host is null");
+  CT_CHECK_ARG(workspace != nullptr, "workspace is null");
+  const int grid = eval_blocks(problem);
+  CT_CHECK_ARG(workspace_bytes >= sizeof(double) * (size_t)(grid + 1), "workspace too small for partials");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  BAView v = make_view(problem);
+  double* part = static_cast<double*>(workspace);
+  CT_TRY(launch_evaluate_only(v, cameras, points, residuals, part, s));
+  std::vector<double> h(grid > 0 ? grid : 1, 0.0);
+  if (grid > 0) {
+    CT_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * grid, hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+  }
+  double c = 0.0;
+  for (int i = 0; i < grid; ++i) c += h[i];
+  *cost_out_host = c;
+  return CT_OK;
+}
+
+}  // extern "C"

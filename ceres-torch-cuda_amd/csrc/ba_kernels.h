@@ -1,0 +1,51 @@
+// Launchers of the bundle-adjustment, Cholesky and LM-state kernels.
+#pragma once
+
+#include "ba_device.h"
+
+namespace ct {
+
+template <typename real>
+int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+template <typename real>
+int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+template <typename real>
+int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
+template <typename real>
+int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+template <typename real>
+int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,
+                         hipStream_t s);
+
+// LM state machine (lm_kernels.hip)
+struct LMInit {
+  ct_lm_options opt;
+  int num_eval_blocks;   // partial rows written by k_eval_points
+  int num_cand_blocks;   // partial rows written by k_candidate
+  int num_cameras;
+  int n_cam_params;
+  int n_pt_params;
+};
+int launch_lm_init(const LMInit& init, const BAWork& w, const double* user_cams, const double* user_pts,
+                   hipStream_t s);
+int launch_lm_begin(const LMInit& init, const BAWork& w, hipStream_t s);
+int launch_lm_end(const LMInit& init, const BAWork& w, hipStream_t s);
+int launch_lm_finalize(const LMInit& init, const BAWork& w, double* user_cams, double* user_pts, hipStream_t s);
+
+// Dense SPD solves of the reduced camera system (chol_kernels.hip).
+//   factor S (n x n row-major, lower triangle) in place and solve S x = rhs;
+//   skip when state->done or state->skip_step; flag state->step_invalid on a
+//   non-positive or non-finite pivot.
+int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* state, double* tmp,
+                              hipStream_t s);
+// Triangular solves only, with the factor left in S by the Cholesky.
+int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, LMState* state, double* tmp,
+                         hipStream_t s);
+// Explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning.
+int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState* state, double* work,
+                   size_t work_doubles, hipStream_t s);
+size_t rcs_pcg_work_doubles(int n);
+size_t rcs_cholesky_tmp_doubles(int n);
+
+}  // namespace ct

@@ -1,0 +1,810 @@
+// Bundle-adjustment kernels for gfx950: evaluation fused with point
+// elimination, camera-pair Schur assembly, back-substitution fused with the
+// candidate evaluation.  Templated on the assembly precision (float: mixed,
+// double: plain FP64); parameters, residuals, costs and the model decrease are
+// always FP64.
+#include "ba_kernels.h"
+
+namespace ct {
+
+// ---------------------------------------------------------------------------
+// Snavely camera model (simple_bundle_adjuster.py:73-79).  The rotation is the
+// reference's angle-axis -> unit quaternion -> rotate formula
+// (rotation.py:104-110, 211-222); its derivative uses Rodrigues' form, which
+// is the same function.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void aa_rotate(const double* aa, const double* X, double* out) {
+  const double t2 = aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2];
+  const double theta = sqrt(t2);
+  const double half = 0.5 * theta;
+  double s, c;
+  sincos(half, &s, &c);
+  const double k = theta <= 2.220446049250313e-16 ? 0.5 - t2 / 48.0 : s / theta;
+  double w = c, u0 = k * aa[0], u1 = k * aa[1], u2 = k * aa[2];
+  const double nrm = fmax(sqrt(w * w + u0 * u0 + u1 * u1 + u2 * u2), 2.2250738585072014e-308);
+  w /= nrm; u0 /= nrm; u1 /= nrm; u2 /= nrm;
+  const double v0 = 2.0 * (u1 * X[2] - u2 * X[1]);
+  const double v1 = 2.0 * (u2 * X[0] - u0 * X[2]);
+  const double v2 = 2.0 * (u0 * X[1] - u1 * X[0]);
+  out[0] = X[0] + w * v0 + (u1 * v2 - u2 * v1);
+  out[1] = X[1] + w * v1 + (u2 * v0 - u0 * v2);
+  out[2] = X[2] + w * v2 + (u0 * v1 - u1 * v0);
+}
+
+__device__ __forceinline__ void snavely_residual(const double* cam, const double* X, const double* obs,
+                                                 double* r) {
+  double p[3];
+  aa_rotate(cam, X, p);
+  p[0] += cam[3]; p[1] += cam[4]; p[2] += cam[5];
+  const double xp = -p[0] / p[2];
+  const double yp = -p[1] / p[2];
+  const double r2 = xp * xp + yp * yp;
+  const double d = 1.0 + r2 * (cam[7] + cam[8] * r2);
+  const double fd = cam[6] * d;
+  r[0] = fd * xp - obs[0];
+  r[1] = fd * yp - obs[1];
+}
+
+// Residual and full Jacobian of one observation.
+__device__ __forceinline__ void snavely_jacobian(const double* cam, const double* X, const double* obs, double* r,
+                                                 double (*Jc)[9], double (*Jp)[3]) {
+  const double a0 = cam[0], a1 = cam[1], a2 = cam[2];
+  double p[3];
+  aa_rotate(cam, X, p);
+  p[0] += cam[3]; p[1] += cam[4]; p[2] += cam[5];
+  // Rodrigues coefficients (series below theta = 1e-4)
+  const double t2 = a0 * a0 + a1 * a1 + a2 * a2;
+  double A, B, C1, C2;
+  if (t2 < 1e-8) {
+    A = 1.0 - t2 / 6.0 + t2 * t2 / 120.0;
+    B = 0.5 - t2 / 24.0 + t2 * t2 / 720.0;
+    C1 = -1.0 / 3.0 + t2 / 30.0 - t2 * t2 / 840.0;
+    C2 = -1.0 / 12.0 + t2 / 180.0 - t2 * t2 / 6720.0;
+  } else {
+    const double t = sqrt(t2);
+    double st, cth;
+    sincos(t, &st, &cth);
+    A = st / t;
+    B = (1.0 - cth) / t2;
+    C1 = (t * cth - st) / (t2 * t);
+    C2 = (t * st - 2.0 * (1.0 - cth)) / (t2 * t2);
+  }
+  const double x0 = X[0], x1 = X[1], x2 = X[2];
+  const double wx0 = a1 * x2 - a2 * x1, wx1 = a2 * x0 - a0 * x2, wx2 = a0 * x1 - a1 * x0;      // w x X
+  const double ww0 = a1 * wx2 - a2 * wx1, ww1 = a2 * wx0 - a0 * wx2, ww2 = a0 * wx1 - a1 * wx0;  // w x (w x X)
+  const double wd = a0 * x0 + a1 * x1 + a2 * x2;
+  const double aa[3] = {a0, a1, a2};
+  const double xx[3] = {x0, x1, x2};
+  const double wx[3] = {wx0, wx1, wx2};
+  const double ww[3] = {ww0, ww1, ww2};
+  // dR[i][j] = d(Rx)_i / d w_j
+  double dR[3][3];
+  // -A [x]_x
+  const double sk[3][3] = {{0.0, -x2, x1}, {x2, 0.0, -x0}, {-x1, x0, 0.0}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      dR[i][j] = -A * sk[i][j] + C1 * wx[i] * aa[j] +
+                 B * ((i == j ? wd : 0.0) + aa[i] * xx[j] - 2.0 * xx[i] * aa[j]) + C2 * ww[i] * aa[j];
+    }
+  }
+  // R = I + A W + B W^2, W = [w]_x
+  const double W[3][3] = {{0.0, -a2, a1}, {a2, 0.0, -a0}, {-a1, a0, 0.0}};
+  double R[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double w2 = W[i][0] * W[0][j] + W[i][1] * W[1][j] + W[i][2] * W[2][j];
+      R[i][j] = (i == j ? 1.0 : 0.0) + A * W[i][j] + B * w2;
+    }
+  }
+  const double f = cam[6], k1 = cam[7], k2 = cam[8];
+  const double iz = 1.0 / p[2];
+  const double xp = -p[0] * iz;
+  const double yp = -p[1] * iz;
+  const double r2 = xp * xp + yp * yp;
+  const double d = 1.0 + r2 * (k1 + k2 * r2);
+  const double fd = f * d;
+  r[0] = fd * xp - obs[0];
+  r[1] = fd * yp - obs[1];
+  const double dd = k1 + 2.0 * k2 * r2;
+  const double u_x = f * (d + 2.0 * xp * xp * dd);
+  const double u_y = f * 2.0 * xp * yp * dd;
+  const double v_y = f * (d + 2.0 * yp * yp * dd);
+  // d(xp,yp)/dp = [[-iz, 0, -xp iz], [0, -iz, -yp iz]]
+  double G[2][3];
+  G[0][0] = -u_x * iz; G[0][1] = -u_y * iz; G[0][2] = -(u_x * xp + u_y * yp) * iz;
+  G[1][0] = -u_y * iz; G[1][1] = -v_y * iz; G[1][2] = -(u_y * xp + v_y * yp) * iz;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Jc[k][j] = G[k][0] * dR[0][j] + G[k][1] * dR[1][j] + G[k][2] * dR[2][j];
+      Jc[k][3 + j] = G[k][j];
+      Jp[k][j] = G[k][0] * R[0][j] + G[k][1] * R[1][j] + G[k][2] * R[2][j];
+    }
+  }
+  Jc[0][6] = d * xp; Jc[1][6] = d * yp;
+  Jc[0][7] = f * xp * r2; Jc[1][7] = f * yp * r2;
+  Jc[0][8] = f * xp * r2 * r2; Jc[1][8] = f * yp * r2 * r2;
+}
+
+// Ceres corrector on a 2-residual block (losses.py:158-193).  Returns the
+// block cost 1/2 rho0 and rescales r, Jc, Jp in place.
+__device__ __forceinline__ double robustify2(const ct_loss& L, double* r, double (*Jc)[9], double (*Jp)[3]) {
+  const double s = r[0] * r[0] + r[1] * r[1];
+  if (L.type == CT_LOSS_TRIVIAL && L.scale == 1.0) return 0.5 * s;
+  double rho1, rho2;
+  const double rho0 = loss_rho(L, s, &rho1, &rho2);
+  const double sq = sqrt(fmax(rho1, 0.0));
+  double scale = sq, alpha_s = 0.0;
+  if (s > 0.0 && rho2 > 0.0) {
+    const double alpha = 1.0 - sqrt(1.0 + 2.0 * s * rho2 / rho1);
+    scale = sq / (1.0 - alpha);
+    alpha_s = alpha / s;
+  }
+  if (Jc != nullptr) {
+    if (alpha_s == 0.0) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) Jc[k][j] *= sq;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Jp[k][j] *= sq;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double rtj = r[0] * Jc[0][j] + r[1] * Jc[1][j];
+        Jc[0][j] = sq * (Jc[0][j] - alpha_s * r[0] * rtj);
+        Jc[1][j] = sq * (Jc[1][j] - alpha_s * r[1] * rtj);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double rtj = r[0] * Jp[0][j] + r[1] * Jp[1][j];
+        Jp[0][j] = sq * (Jp[0][j] - alpha_s * r[0] * rtj);
+        Jp[1][j] = sq * (Jp[1][j] - alpha_s * r[1] * rtj);
+      }
+    }
+  }
+  r[0] *= scale;
+  r[1] *= scale;
+  return 0.5 * rho0;
+}
+
+__device__ __forceinline__ double robust_cost2(const ct_loss& L, const double* r) {
+  const double s = r[0] * r[0] + r[1] * r[1];
+  if (L.type == CT_LOSS_TRIVIAL && L.scale == 1.0) return 0.5 * s;
+  double rho1, rho2;
+  return 0.5 * loss_rho(L, s, &rho1, &rho2);
+}
+
+__device__ __forceinline__ double damping_of(double colsq, double min_diag, double max_diag, double radius,
+                                             int jacobi_scaling) {
+  const double dg = jacobi_scaling ? colsq : 1.0;
+  return fmin(fmax(dg, min_diag), max_diag) / fmax(radius, 2.220446049250313e-16);
+}
+
+// ---------------------------------------------------------------------------
+// K1: evaluation fused with point elimination.  One thread per point walks
+// its observations: residual, Jacobian, loss correction, then V_p = Jp^T Jp
+// (+ damping), its 3x3 Cholesky L, L^-1, q = L^-1 g_p, P_o = Jp_o L^-T and
+// e_o = P_o q - r_o.  Block partials: cost, |g_p|^2, max|g_p|.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kPointThreads) void k_eval_points(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double red[kPointThreads / kWave];
+  LMState* st = w.state;
+  if (st->done) return;
+  const int cur = st->cur;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
+  const double radius = st->radius;
+  const int p = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0, g2 = 0.0, gmax = 0.0;
+  if (p < v.P) {
+    const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
+    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    double pmask[3] = {1.0, 1.0, 1.0};
+    if (v.point_mask) {
+      for (int j = 0; j < 3; ++j) pmask[j] = v.point_mask[3 * p + j] ? 1.0 : 0.0;
+    }
+    real V00 = 0, V10 = 0, V11 = 0, V20 = 0, V21 = 0, V22 = 0;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int o = o0; o < o1; ++o) {
+      const int ci = v.obs_camera[o];
+      double cam[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
+      const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+      double r[2], Jc[2][9], Jp[2][3];
+      snavely_jacobian(cam, X, obs, r, Jc, Jp);
+      if (v.camera_mask) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+          if (!v.camera_mask[9 * ci + j]) { Jc[0][j] = 0.0; Jc[1][j] = 0.0; }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { Jp[0][j] *= pmask[j]; Jp[1][j] *= pmask[j]; }
+      cost += robustify2(v.loss, r, Jc, Jp);
+      b.r[2 * o] = r[0];
+      b.r[2 * o + 1] = r[1];
+      real jc[18], jp[6];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) jc[9 * k + j] = (real)Jc[k][j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) jp[3 * k + j] = (real)Jp[k][j];
+      }
+#pragma unroll
+      for (int j = 0; j < 18; ++j) b.Jc[18 * (size_t)o + j] = jc[j];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) b.Jp[6 * (size_t)o + j] = jp[j];
+      V00 += jp[0] * jp[0] + jp[3] * jp[3];
+      V10 += jp[1] * jp[0] + jp[4] * jp[3];
+      V11 += jp[1] * jp[1] + jp[4] * jp[4];
+      V20 += jp[2] * jp[0] + jp[5] * jp[3];
+      V21 += jp[2] * jp[1] + jp[5] * jp[4];
+      V22 += jp[2] * jp[2] + jp[5] * jp[5];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g[j] += (double)jp[j] * r[0] + (double)jp[3 + j] * r[1];
+    }
+    // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
+    const double D0 = damping_of((double)V00, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D1 = damping_of((double)V11, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D2 = damping_of((double)V22, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    w.Dp[3 * p] = D0; w.Dp[3 * p + 1] = D1; w.Dp[3 * p + 2] = D2;
+    V00 += (real)D0;
+    V11 += (real)D1;
+    V22 += (real)D2;
+    // Cholesky of the damped 3x3 block and its inverse
+    const real l00 = sqrt(V00);
+    const real l10 = V10 / l00, l20 = V20 / l00;
+    const real l11 = sqrt(V11 - l10 * l10);
+    const real l21 = (V21 - l20 * l10) / l11;
+    const real l22 = sqrt(V22 - l20 * l20 - l21 * l21);
+    const real i00 = real(1) / l00, i11 = real(1) / l11, i22 = real(1) / l22;
+    const real i10 = -l10 * i00 * i11;
+    const real i21 = -l21 * i11 * i22;
+    const real i20 = -(l20 * i00 + l21 * i10) * i22;
+    const real q0 = i00 * (real)g[0];
+    const real q1 = i10 * (real)g[0] + i11 * (real)g[1];
+    const real q2 = i20 * (real)g[0] + i21 * (real)g[1] + i22 * (real)g[2];
+    real* Li = b.Linv + 6 * (size_t)p;
+    Li[0] = i00; Li[1] = i10; Li[2] = i11; Li[3] = i20; Li[4] = i21; Li[5] = i22;
+    b.q[3 * p] = q0; b.q[3 * p + 1] = q1; b.q[3 * p + 2] = q2;
+    b.gp[3 * p] = g[0]; b.gp[3 * p + 1] = g[1]; b.gp[3 * p + 2] = g[2];
+    for (int o = o0; o < o1; ++o) {
+      const real* jp = b.Jp + 6 * (size_t)o;
+      real P[6];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const real j0 = jp[3 * k], j1 = jp[3 * k + 1], j2 = jp[3 * k + 2];
+        // P = Jp L^-T : P[k][m] = sum_{l<=m} Jp[k][l] Linv[m][l]
+        P[3 * k + 0] = j0 * i00;
+        P[3 * k + 1] = j0 * i10 + j1 * i11;
+        P[3 * k + 2] = j0 * i20 + j1 * i21 + j2 * i22;
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) b.Pf[6 * (size_t)o + j] = P[j];
+      b.e[2 * (size_t)o] = (real)((double)(P[0] * q0 + P[1] * q1 + P[2] * q2) - b.r[2 * o]);
+      b.e[2 * (size_t)o + 1] = (real)((double)(P[3] * q0 + P[4] * q1 + P[5] * q2) - b.r[2 * o + 1]);
+    }
+    g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    gmax = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
+  }
+  const double tc = block_sum<kPointThreads>(cost, red);
+  const double tg = block_sum<kPointThreads>(g2, red);
+  const double tm = block_max<kPointThreads>(gmax, red);
+  if (threadIdx.x == 0) {
+    w.part_eval[3 * blockIdx.x] = tc;
+    w.part_eval[3 * blockIdx.x + 1] = tg;
+    w.part_eval[3 * blockIdx.x + 2] = tm;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: one wavefront per camera pair block (i <= j).
+//   S_ij = d_ij (U_i + D_i) - sum_{(a,b)} Jc_a^T (P_a P_b^T) Jc_b
+// 63 lanes form 7 groups of 3x3 sub-block owners; group g takes terms
+// g, g+7, ...; the 7 partial blocks are summed in LDS in fixed order.
+// Diagonal blocks also produce rhs_i = sum Jc^T e, |g_c|^2 and max|g_c|.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ real red[7][81];
+  __shared__ double vred[kWave][27];
+  LMState* st = w.state;
+  if (st->done) return;
+  const int blk = blockIdx.x;
+  const int ci = v.block_cameras[2 * blk];
+  const int cj = v.block_cameras[2 * blk + 1];
+  const int n = 9 * v.C;
+  const int tid = threadIdx.x;
+  const int grp = tid / 9;
+  const int br = (tid % 9) / 3;
+  const int bc = (tid % 9) % 3;
+  real acc[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[r][c] = 0;
+  if (grp < 7) {
+    if (ci == cj) {
+      const int s0 = v.camera_offsets[ci], s1 = v.camera_offsets[ci + 1];
+      for (int t = s0 + grp; t < s1; t += 7) {
+        const int o = v.camera_obs[t];
+        const real* J = b.Jc + 18 * (size_t)o;
+        real ja0[3], ja1[3], jb0[3], jb1[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          ja0[r] = J[3 * br + r]; ja1[r] = J[9 + 3 * br + r];
+          jb0[r] = J[3 * bc + r]; jb1[r] = J[9 + 3 * bc + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[r][c] += ja0[r] * jb0[c] + ja1[r] * jb1[c];
+      }
+    }
+    const int t0 = v.block_offsets[blk], t1 = v.block_offsets[blk + 1];
+    for (int t = t0 + grp; t < t1; t += 7) {
+      const int oa = v.block_terms[2 * (size_t)t];
+      const int ob = v.block_terms[2 * (size_t)t + 1];
+      const real* Pa = b.Pf + 6 * (size_t)oa;
+      const real* Pb = b.Pf + 6 * (size_t)ob;
+      const real pa0 = Pa[0], pa1 = Pa[1], pa2 = Pa[2], pa3 = Pa[3], pa4 = Pa[4], pa5 = Pa[5];
+      const real pb0 = Pb[0], pb1 = Pb[1], pb2 = Pb[2], pb3 = Pb[3], pb4 = Pb[4], pb5 = Pb[5];
+      const real m00 = pa0 * pb0 + pa1 * pb1 + pa2 * pb2;
+      const real m01 = pa0 * pb3 + pa1 * pb4 + pa2 * pb5;
+      const real m10 = pa3 * pb0 + pa4 * pb1 + pa5 * pb2;
+      const real m11 = pa3 * pb3 + pa4 * pb4 + pa5 * pb5;
+      const real* Ja = b.Jc + 18 * (size_t)oa;
+      const real* Jb = b.Jc + 18 * (size_t)ob;
+      real ja0[3], ja1[3], t0v[3], t1v[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const real b0 = Jb[3 * bc + c], b1 = Jb[9 + 3 * bc + c];
+        t0v[c] = m00 * b0 + m01 * b1;
+        t1v[c] = m10 * b0 + m11 * b1;
+        ja0[c] = Ja[3 * br + c];
+        ja1[c] = Ja[9 + 3 * br + c];
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r][c] -= ja0[r] * t0v[c] + ja1[r] * t1v[c];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) red[grp][(3 * br + r) * 9 + 3 * bc + c] = acc[r][c];
+  }
+  // diagonal blocks: rhs, gradient and column norms over the camera's observations
+  double rv[9], gv[9], dv[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) { rv[j] = 0.0; gv[j] = 0.0; dv[j] = 0.0; }
+  if (ci == cj) {
+    const int s0 = v.camera_offsets[ci], s1 = v.camera_offsets[ci + 1];
+    for (int t = s0 + tid; t < s1; t += kWave) {
+      const int o = v.camera_obs[t];
+      const real* J = b.Jc + 18 * (size_t)o;
+      const double e0 = (double)b.e[2 * (size_t)o], e1 = (double)b.e[2 * (size_t)o + 1];
+      const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double j0 = (double)J[j], j1 = (double)J[9 + j];
+        rv[j] += j0 * e0 + j1 * e1;
+        gv[j] += j0 * r0 + j1 * r1;
+        dv[j] += j0 * j0 + j1 * j1;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      vred[tid][j] = rv[j];
+      vred[tid][9 + j] = gv[j];
+      vred[tid][18 + j] = dv[j];
+    }
+  }
+  __syncthreads();
+  const double radius = st->radius;
+  for (int idx = tid; idx < 81; idx += kWave) {
+    real s = 0;
+#pragma unroll
+    for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
+    const int r = idx / 9, c = idx % 9;
+    if (ci == cj) {
+      // column sums of J^T J on the diagonal come from the same observations
+      double colsq = 0.0;
+      if (r == c) {
+        for (int l = 0; l < kWave; ++l) colsq += vred[l][18 + r];
+      }
+      double val;
+      if (r >= c) {
+        val = (double)s;
+      } else {
+        real s2 = 0;
+#pragma unroll
+        for (int gi = 0; gi < 7; ++gi) s2 += red[gi][c * 9 + r];
+        val = (double)s2;
+      }
+      if (r == c) {
+        const double Dv = damping_of(colsq, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+        w.Dc[9 * ci + r] = Dv;
+        val += Dv;
+      }
+      w.S[(size_t)(9 * ci + r) * n + 9 * ci + c] = val;
+    } else {
+      w.S[(size_t)(9 * ci + r) * n + 9 * cj + c] = (double)s;
+      w.S[(size_t)(9 * cj + c) * n + 9 * ci + r] = (double)s;
+    }
+  }
+  if (ci == cj && tid < 9) {
+    double rs = 0.0;
+    for (int l = 0; l < kWave; ++l) rs += vred[l][tid];
+    w.rhs[9 * ci + tid] = rs;
+    double gs = 0.0;
+    for (int l = 0; l < kWave; ++l) gs += vred[l][9 + tid];
+    w.gc[9 * ci + tid] = gs;
+  }
+  if (ci == cj && tid == 0) {
+    double g2 = 0.0, gm = 0.0;
+    for (int j = 0; j < 9; ++j) {
+      double gsum = 0.0;
+      for (int l = 0; l < kWave; ++l) gsum += vred[l][9 + j];
+      g2 += gsum * gsum;
+      gm = fmax(gm, fabs(gsum));
+    }
+    w.part_cam[2 * ci] = g2;
+    w.part_cam[2 * ci + 1] = gm;
+  }
+}
+
+// dx_p = -L^-T (q + sum_o P_o^T Jc_o dx_c)   (schur_solve_dense's xa, linear.py:698)
+template <typename real>
+__device__ __forceinline__ void backsub_point(const BAView& v, const BABuffers<real>& b, const double* __restrict__ dxc,
+                                              int p, double* d0, double* d1, double* d2) {
+  const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
+  const real* Li = b.Linv + 6 * (size_t)p;
+  const double i00 = Li[0], i10 = Li[1], i11 = Li[2], i20 = Li[3], i21 = Li[4], i22 = Li[5];
+  double s0 = (double)b.q[3 * p], s1 = (double)b.q[3 * p + 1], s2 = (double)b.q[3 * p + 2];
+  for (int o = o0; o < o1; ++o) {
+    const int ci = v.obs_camera[o];
+    const real* J = b.Jc + 18 * (size_t)o;
+    double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const double d = dxc[9 * ci + j];
+      y0 += (double)J[j] * d;
+      y1 += (double)J[9 + j] * d;
+    }
+    const real* P = b.Pf + 6 * (size_t)o;
+    s0 += (double)P[0] * y0 + (double)P[3] * y1;
+    s1 += (double)P[1] * y0 + (double)P[4] * y1;
+    s2 += (double)P[2] * y0 + (double)P[5] * y1;
+  }
+  *d0 = -(i00 * s0 + i10 * s1 + i20 * s2);
+  *d1 = -(i11 * s1 + i21 * s2);
+  *d2 = -(i22 * s2);
+}
+
+template <typename real>
+__global__ __launch_bounds__(kPointThreads) void k_backsub(BAView v, BABuffers<real> b, BAWork w) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int p = blockIdx.x * kPointThreads + threadIdx.x;
+  if (p >= v.P) return;
+  double d0, d1, d2;
+  backsub_point(v, b, w.dxc, p, &d0, &d1, &d2);
+  w.dxp[3 * p] = d0; w.dxp[3 * p + 1] = d1; w.dxp[3 * p + 2] = d2;
+}
+
+// ---------------------------------------------------------------------------
+// Iterative refinement of (J^T J + D) dx = -g with the FP32 Jacobian and FP64
+// accumulation (mixed precision).  One step:
+//   rho = -g - (J^T J + D) dx                         (FP64)
+//   delta_c = S^-1 (rho_c - sum_o Jc_o^T P_o L_p^-1 rho_p)
+//   delta_p = L_p^-T (L_p^-1 rho_p - sum_o P_o^T Jc_o delta_c)
+//   dx += delta
+// reusing the point factors and the FP64 Cholesky factor of S.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kPointThreads) void k_refine_points(BAView v, BABuffers<real> b, BAWork w) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int p = blockIdx.x * kPointThreads + threadIdx.x;
+  if (p >= v.P) return;
+  const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
+  const double dp0 = w.dxp[3 * p], dp1 = w.dxp[3 * p + 1], dp2 = w.dxp[3 * p + 2];
+  double rp0 = -b.gp[3 * p] - w.Dp[3 * p] * dp0;
+  double rp1 = -b.gp[3 * p + 1] - w.Dp[3 * p + 1] * dp1;
+  double rp2 = -b.gp[3 * p + 2] - w.Dp[3 * p + 2] * dp2;
+  for (int o = o0; o < o1; ++o) {
+    const int ci = v.obs_camera[o];
+    const real* J = b.Jc + 18 * (size_t)o;
+    const real* JP = b.Jp + 6 * (size_t)o;
+    double u0 = (double)JP[0] * dp0 + (double)JP[1] * dp1 + (double)JP[2] * dp2;
+    double u1 = (double)JP[3] * dp0 + (double)JP[4] * dp1 + (double)JP[5] * dp2;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const double d = w.dxc[9 * ci + j];
+      u0 += (double)J[j] * d;
+      u1 += (double)J[9 + j] * d;
+    }
+    w.u_obs[4 * (size_t)o] = u0;
+    w.u_obs[4 * (size_t)o + 1] = u1;
+    rp0 -= (double)JP[0] * u0 + (double)JP[3] * u1;
+    rp1 -= (double)JP[1] * u0 + (double)JP[4] * u1;
+    rp2 -= (double)JP[2] * u0 + (double)JP[5] * u1;
+  }
+  const real* Li = b.Linv + 6 * (size_t)p;
+  const double t0 = (double)Li[0] * rp0;
+  const double t1 = (double)Li[1] * rp0 + (double)Li[2] * rp1;
+  const double t2 = (double)Li[3] * rp0 + (double)Li[4] * rp1 + (double)Li[5] * rp2;
+  w.res_p[3 * p] = t0; w.res_p[3 * p + 1] = t1; w.res_p[3 * p + 2] = t2;
+  for (int o = o0; o < o1; ++o) {
+    const real* P = b.Pf + 6 * (size_t)o;
+    w.u_obs[4 * (size_t)o + 2] = (double)P[0] * t0 + (double)P[1] * t1 + (double)P[2] * t2;
+    w.u_obs[4 * (size_t)o + 3] = (double)P[3] * t0 + (double)P[4] * t1 + (double)P[5] * t2;
+  }
+}
+
+// One wavefront per camera: rho_c and the reduced right-hand side.
+template <typename real>
+__global__ __launch_bounds__(kWave) void k_refine_cameras(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double part[kWave][9];
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int ci = blockIdx.x;
+  const int tid = threadIdx.x;
+  double acc[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) acc[j] = 0.0;
+  for (int t = v.camera_offsets[ci] + tid; t < v.camera_offsets[ci + 1]; t += kWave) {
+    const int o = v.camera_obs[t];
+    const real* J = b.Jc + 18 * (size_t)o;
+    const double a0 = w.u_obs[4 * (size_t)o] + w.u_obs[4 * (size_t)o + 2];
+    const double a1 = w.u_obs[4 * (size_t)o + 1] + w.u_obs[4 * (size_t)o + 3];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) acc[j] += (double)J[j] * a0 + (double)J[9 + j] * a1;
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) part[tid][j] = acc[j];
+  __syncthreads();
+  if (tid < 9) {
+    double ssum = 0.0;
+    for (int l = 0; l < kWave; ++l) ssum += part[l][tid];
+    const int k = 9 * ci + tid;
+    w.res_c[k] = -w.gc[k] - w.Dc[k] * w.dxc[k] - ssum;
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(kPointThreads) void k_refine_update(BAView v, BABuffers<real> b, BAWork w) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int t = blockIdx.x * kPointThreads + threadIdx.x;
+  if (t < v.P) {
+    const int p = t;
+    double s0 = w.res_p[3 * p], s1 = w.res_p[3 * p + 1], s2 = w.res_p[3 * p + 2];
+    for (int o = v.point_offsets[p]; o < v.point_offsets[p + 1]; ++o) {
+      const int ci = v.obs_camera[o];
+      const real* J = b.Jc + 18 * (size_t)o;
+      double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double d = w.dxc_corr[9 * ci + j];
+        y0 += (double)J[j] * d;
+        y1 += (double)J[9 + j] * d;
+      }
+      const real* P = b.Pf + 6 * (size_t)o;
+      s0 -= (double)P[0] * y0 + (double)P[3] * y1;
+      s1 -= (double)P[1] * y0 + (double)P[4] * y1;
+      s2 -= (double)P[2] * y0 + (double)P[5] * y1;
+    }
+    const real* Li = b.Linv + 6 * (size_t)p;
+    w.dxp[3 * p] += (double)Li[0] * s0 + (double)Li[1] * s1 + (double)Li[3] * s2;
+    w.dxp[3 * p + 1] += (double)Li[2] * s1 + (double)Li[4] * s2;
+    w.dxp[3 * p + 2] += (double)Li[5] * s2;
+  }
+  if (t < 9 * v.C) w.dxc[t] += w.dxc_corr[t];
+}
+
+// ---------------------------------------------------------------------------
+// K5: back-substitution fused with the candidate evaluation.
+//   dx_p = -L^-T (q + sum_o P_o^T Jc_o dx_c),  x' = x + dx
+//   cost(x'), model decrease sum_o (-r.u - u.u/2) with u = J dx, |dx|^2, |x'|^2
+// ---------------------------------------------------------------------------
+template <typename real, bool kFusedBacksub>
+__global__ __launch_bounds__(kPointThreads) void k_candidate(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double red[kPointThreads / kWave];
+  LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int cur = st->cur;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
+  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
+  const double* __restrict__ dxc = w.dxc;
+  const int t = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0, md = 0.0, dx2 = 0.0, x2 = 0.0, finite = 1.0;
+  if (t < 9 * v.C) {
+    const double d = dxc[t];
+    const double nx = cams[t] + d;
+    ncams[t] = nx;
+    dx2 += d * d;
+    x2 += nx * nx;
+    if (!isfinite(d)) finite = 0.0;
+  }
+  if (t < v.P) {
+    const int p = t;
+    const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
+    double d0, d1, d2;
+    if (kFusedBacksub) {
+      backsub_point(v, b, dxc, p, &d0, &d1, &d2);
+      w.dxp[3 * p] = d0; w.dxp[3 * p + 1] = d1; w.dxp[3 * p + 2] = d2;
+    } else {
+      d0 = w.dxp[3 * p]; d1 = w.dxp[3 * p + 1]; d2 = w.dxp[3 * p + 2];
+    }
+    const double X[3] = {pts[3 * p] + d0, pts[3 * p + 1] + d1, pts[3 * p + 2] + d2};
+    npts[3 * p] = X[0]; npts[3 * p + 1] = X[1]; npts[3 * p + 2] = X[2];
+    dx2 += d0 * d0 + d1 * d1 + d2 * d2;
+    x2 += X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+    if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) finite = 0.0;
+    for (int o = o0; o < o1; ++o) {
+      const int ci = v.obs_camera[o];
+      const real* J = b.Jc + 18 * (size_t)o;
+      const real* JP = b.Jp + 6 * (size_t)o;
+      double cam[9];
+      double u0 = 0.0, u1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double d = dxc[9 * ci + j];
+        cam[j] = cams[9 * ci + j] + d;
+        u0 += (double)J[j] * d;
+        u1 += (double)J[9 + j] * d;
+      }
+      u0 += (double)JP[0] * d0 + (double)JP[1] * d1 + (double)JP[2] * d2;
+      u1 += (double)JP[3] * d0 + (double)JP[4] * d1 + (double)JP[5] * d2;
+      const double r0 = b.r[2 * o], r1 = b.r[2 * o + 1];
+      md += -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
+      const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+      double rr[2];
+      snavely_residual(cam, X, obs, rr);
+      cost += robust_cost2(v.loss, rr);
+    }
+  }
+  const double a0 = block_sum<kPointThreads>(cost, red);
+  const double a1 = block_sum<kPointThreads>(md, red);
+  const double a2 = block_sum<kPointThreads>(dx2, red);
+  const double a3 = block_sum<kPointThreads>(x2, red);
+  const double a4 = block_sum<kPointThreads>(finite == 1.0 ? 0.0 : 1.0, red);
+  if (threadIdx.x == 0) {
+    double* out = w.part_cand + 5 * blockIdx.x;
+    out[0] = a0; out[1] = a1; out[2] = a2; out[3] = a3; out[4] = a4;
+  }
+}
+
+// Plain evaluation (cost + corrected residuals) at given parameters.
+__global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const double* cams, const double* pts,
+                                                                  double* residuals, double* part) {
+  __shared__ double red[kPointThreads / kWave];
+  const int p = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0;
+  if (p < v.P) {
+    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    for (int o = v.point_offsets[p]; o < v.point_offsets[p + 1]; ++o) {
+      const int ci = v.obs_camera[o];
+      const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+      double r[2];
+      snavely_residual(cams + 9 * ci, X, obs, r);
+      const double s = r[0] * r[0] + r[1] * r[1];
+      double rho1, rho2;
+      const double rho0 = loss_rho(v.loss, s, &rho1, &rho2);
+      cost += 0.5 * rho0;
+      if (residuals) {
+        double sq = sqrt(fmax(rho1, 0.0)), scale = sq;
+        if (s > 0.0 && rho2 > 0.0) scale = sq / sqrt(1.0 + 2.0 * s * rho2 / rho1);
+        residuals[2 * o] = scale * r[0];
+        residuals[2 * o + 1] = scale * r[1];
+      }
+    }
+  }
+  const double tc = block_sum<kPointThreads>(cost, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = tc;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename real>
+int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
+  if (grid > 0) {
+    hipLaunchKernelGGL(k_eval_points<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+  if (v.B > 0) {
+    hipLaunchKernelGGL(k_camera_blocks<real>, dim3(v.B), dim3(kWave), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s) {
+  const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
+  const int grid = (work + kPointThreads - 1) / kPointThreads;
+  if (grid > 0) {
+    if (fused_backsub) {
+      hipLaunchKernelGGL((k_candidate<real, true>), dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+    } else {
+      hipLaunchKernelGGL((k_candidate<real, false>), dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+    }
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
+  if (grid > 0) {
+    hipLaunchKernelGGL(k_backsub<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+  const int gp = (v.P + kPointThreads - 1) / kPointThreads;
+  const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
+  const int gu = (work + kPointThreads - 1) / kPointThreads;
+  if (gp > 0) {
+    hipLaunchKernelGGL(k_refine_points<real>, dim3(gp), dim3(kPointThreads), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  if (v.C > 0) {
+    hipLaunchKernelGGL(k_refine_cameras<real>, dim3(v.C), dim3(kWave), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+    CT_TRY(launch_rcs_tri_solve(w.S, w.res_c, w.dxc_corr, 9 * v.C, w.state, w.solve_tmp, s));
+  }
+  if (gu > 0) {
+    hipLaunchKernelGGL(k_refine_update<real>, dim3(gu), dim3(kPointThreads), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,
+                         hipStream_t s) {
+  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
+  if (grid > 0) {
+    hipLaunchKernelGGL(k_evaluate_only, dim3(grid), dim3(kPointThreads), 0, s, v, cams, pts, residuals, part);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template int launch_eval_points<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
+template int launch_eval_points<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_camera_blocks<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
+template int launch_camera_blocks<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_candidate<float>(const BAView&, const BABuffers<float>&, const BAWork&, bool, hipStream_t);
+template int launch_candidate<double>(const BAView&, const BABuffers<double>&, const BAWork&, bool, hipStream_t);
+template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
+template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_refine_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
+template int launch_refine_step<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+
+}  // namespace ct

@@ -1,0 +1,244 @@
+/*
+ * ceres_mi355x C ABI: the MI355X (gfx950) solver core behind the ceres_torch
+ * Problem / SolverOptions / solve API.
+ *
+ * Conventions for every entry point:
+ *   - returns int status: CT_OK (0) or a negative ct_status code; a message for
+ *     the last failure on the calling thread is available from ct_last_error();
+ *   - no C++ exception crosses the boundary;
+ *   - all array arguments are caller-owned device pointers (unless marked
+ *     "host"), with explicit sizes; the library never allocates device memory
+ *     except where a function says so (plans keep host-side graph objects);
+ *   - work is enqueued on the caller's hipStream_t (0 = the null stream) and
+ *     functions are re-entrant per stream.
+ *
+ * The reference exposes its native tier as a pybind11 module with two
+ * functions (native/cuda/ceres_torch_cuda.cpp:3-16, bodies in
+ * native/cuda/ceres_torch_cuda_kernel.cu:48-132) plus a Python linear-solver
+ * plugin registry (src/ceres_torch/linear.py:82-103).  This header replaces
+ * that module; the per-function comments cite what each entry point replaces.
+ */
+#ifndef CERES_MI355X_H_
+#define CERES_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
+
+#define CT_ABI_VERSION 1
+
+typedef enum ct_status {
+  CT_OK = 0,
+  CT_ERR_ARGUMENT = -1,    /* bad argument: message names it */
+  CT_ERR_HIP = -2,         /* a HIP runtime call failed */
+  CT_ERR_NOT_SPD = -3,     /* factorisation met a non-positive pivot */
+  CT_ERR_WORKSPACE = -4,   /* workspace smaller than *_workspace_bytes() */
+  CT_ERR_UNSUPPORTED = -5  /* valid request the device path does not cover */
+} ct_status;
+
+/* Version of this ABI (CT_ABI_VERSION).  No reference counterpart. */
+int ct_abi_version(void);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char* ct_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Robust losses: losses.py:21-122 (Trivial, Huber, SoftLOne, Cauchy, Arctan,
+ * Tolerant, Tukey, each optionally scaled as ScaledLoss).                    */
+/* ------------------------------------------------------------------------ */
+typedef enum ct_loss_type {
+  CT_LOSS_TRIVIAL = 0,
+  CT_LOSS_HUBER = 1,
+  CT_LOSS_SOFTLONE = 2,
+  CT_LOSS_CAUCHY = 3,
+  CT_LOSS_ARCTAN = 4,
+  CT_LOSS_TOLERANT = 5,
+  CT_LOSS_TUKEY = 6
+} ct_loss_type;
+
+typedef struct ct_loss {
+  int32_t type;  /* ct_loss_type */
+  double a;      /* scale parameter */
+  double b;      /* second parameter (Tolerant) */
+  double scale;  /* ScaledLoss factor; 1.0 for none */
+} ct_loss;
+
+/* ------------------------------------------------------------------------ */
+/* Trust-region (Levenberg-Marquardt) options: the fields of SolverOptions
+ * (src/ceres_torch/types.py:164-229) that solver.py:35-357 reads on the LM
+ * path.                                                                     */
+/* ------------------------------------------------------------------------ */
+typedef enum ct_linear_solver {
+  CT_DENSE_SCHUR = 0,      /* Schur complement + FP64 Cholesky of S */
+  CT_ITERATIVE_SCHUR = 1   /* explicit S + SCHUR_JACOBI PCG */
+} ct_linear_solver;
+
+typedef enum ct_precision {
+  CT_PLAIN_FP64 = 0,  /* FP64 everywhere */
+  CT_MIXED = 1        /* FP32 J, J^T J and Schur assembly; FP64 Cholesky,
+                         cost, gain ratio and iterative refinement */
+} ct_precision;
+
+typedef enum ct_loop_mode {
+  CT_LOOP_HOST = 0,   /* host-driven LM loop, one host sync per iteration */
+  CT_LOOP_DEVICE = 1  /* device-resident LM state, graph-captured iterations */
+} ct_loop_mode;
+
+typedef struct ct_lm_options {
+  int32_t max_num_iterations;
+  int32_t max_num_consecutive_invalid_steps;
+  int32_t linear_solver;                /* ct_linear_solver */
+  int32_t precision;                    /* ct_precision */
+  int32_t loop_mode;                    /* ct_loop_mode */
+  int32_t min_linear_solver_iterations;
+  int32_t max_linear_solver_iterations;
+  int32_t max_num_refinement_iterations;
+  int32_t jacobi_scaling;               /* bool */
+  int32_t iterations_per_graph;         /* device loop: iterations per graph */
+  double initial_trust_region_radius;
+  double max_trust_region_radius;
+  double min_trust_region_radius;
+  double min_relative_decrease;
+  double min_lm_diagonal;
+  double max_lm_diagonal;
+  double function_tolerance;
+  double gradient_tolerance;
+  double parameter_tolerance;
+  double eta;
+} ct_lm_options;
+
+/* Termination codes match TerminationType (types.py:98-103). */
+typedef enum ct_termination {
+  CT_TERM_CONVERGENCE = 0,
+  CT_TERM_NO_CONVERGENCE = 1,
+  CT_TERM_FAILURE = 2
+} ct_termination;
+
+/* Why the loop stopped; messages of solver.py:35-357. */
+typedef enum ct_reason {
+  CT_REASON_NONE = 0,
+  CT_REASON_GRADIENT = 1,         /* "Gradient tolerance reached." */
+  CT_REASON_PARAMETER = 2,        /* "Parameter tolerance reached." */
+  CT_REASON_FUNCTION = 3,         /* "Function tolerance reached." */
+  CT_REASON_MIN_RADIUS = 4,       /* "Minimum trust-region radius reached." */
+  CT_REASON_MAX_ITERATIONS = 5,   /* "Maximum iterations reached." */
+  CT_REASON_INVALID_STEPS = 6,    /* "Too many invalid trust-region steps." */
+  CT_REASON_NO_PARAMETERS = 7     /* "No active parameters." */
+} ct_reason;
+
+/* Per-iteration record layout (doubles), mirroring IterationSummary
+ * (types.py:134-161).  ct_lm_report.num_records rows are valid.            */
+enum {
+  CT_REC_ITERATION = 0,
+  CT_REC_COST = 1,
+  CT_REC_COST_CHANGE = 2,
+  CT_REC_GRADIENT_MAX_NORM = 3,
+  CT_REC_GRADIENT_NORM = 4,
+  CT_REC_STEP_NORM = 5,
+  CT_REC_RELATIVE_DECREASE = 6,
+  CT_REC_TRUST_REGION_RADIUS = 7,
+  CT_REC_STEP_IS_VALID = 8,
+  CT_REC_STEP_IS_SUCCESSFUL = 9,
+  CT_REC_LINEAR_SOLVER_ITERATIONS = 10,
+  CT_REC_TIMESTAMP = 11,          /* device wall clock, seconds since start */
+  CT_REC_FIELDS = 12
+};
+
+/* Host-side result of a solve (SolverSummary counters, types.py:360-387). */
+typedef struct ct_lm_report {
+  int32_t termination;       /* ct_termination */
+  int32_t reason;            /* ct_reason */
+  int32_t num_records;       /* rows of the iteration record table */
+  int32_t num_successful_steps;
+  int32_t num_unsuccessful_steps;
+  int32_t num_residual_evaluations;
+  int32_t num_jacobian_evaluations;
+  int32_t num_linear_solves;
+  double initial_cost;
+  double final_cost;
+  double device_seconds;     /* device wall clock from start to finish */
+} ct_lm_report;
+
+/* ------------------------------------------------------------------------ */
+/* Bundle adjustment (Snavely/BAL cameras: angle-axis, translation, f, k1, k2;
+ * residual of examples/simple_bundle_adjuster.py:73-79).  Observations are
+ * grouped by point.  The camera-pair tables come from ct_ba_* helpers in the
+ * Python layer (built once per problem structure).                          */
+/* ------------------------------------------------------------------------ */
+typedef struct ct_ba_problem {
+  int32_t num_cameras;
+  int32_t num_points;
+  int32_t num_observations;
+  int32_t num_blocks;              /* camera pair blocks (i <= j) */
+  int64_t num_terms;               /* observation pairs over all blocks */
+  const int32_t* obs_camera;       /* [N] camera of each observation */
+  const int32_t* point_offsets;    /* [P+1] observations of point p */
+  const double* observations;      /* [N*2] observed pixels */
+  const int32_t* camera_offsets;   /* [C+1] CSR of observations by camera */
+  const int32_t* camera_obs;       /* [N]   observation ids by camera */
+  const int32_t* block_cameras;    /* [B*2] (i, j), i <= j */
+  const int32_t* block_offsets;    /* [B+1] term ranges */
+  const int32_t* block_terms;      /* [T*2] (obs a of camera i, obs b of camera j) */
+  const uint8_t* camera_mask;      /* [C*9] 1 = free, 0 = held constant */
+  const uint8_t* point_mask;       /* [P*3] */
+  ct_loss loss;
+} ct_ba_problem;
+
+/* Device workspace needed by ct_ba_plan_create for this problem/options. */
+int ct_ba_workspace_bytes(const ct_ba_problem* problem, const ct_lm_options* options, size_t* bytes);
+
+typedef struct ct_ba_plan ct_ba_plan; /* opaque, host memory only */
+
+/* Bind a problem, options, parameter buffers and workspace to a plan.
+ * cameras [C*9] and points [P*3] (device, float64) hold the initial estimate
+ * and receive the final (best) state, as Problem parameter tensors do in the
+ * reference (problem.py:457-460, solver.py:672).  Replaces the per-iteration
+ * Python loop of solver.py:77-345 + Problem.evaluate (problem.py:377-413) +
+ * solve_linear_system(DENSE_SCHUR / ITERATIVE_SCHUR) (linear.py:141-246).   */
+int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options, double* cameras,
+                      double* points, void* workspace, size_t workspace_bytes, ct_stream_t stream,
+                      ct_ba_plan** plan);
+
+/* Run the LM solve; report and records (host memory, may be NULL) receive the
+ * summary and (max_num_iterations + 2) * CT_REC_FIELDS doubles.            */
+int ct_ba_plan_run(ct_ba_plan* plan, ct_lm_report* report, double* records);
+
+int ct_ba_plan_destroy(ct_ba_plan* plan);
+
+/* Kernel groups of one LM iteration, in launch order. */
+enum {
+  CT_KERNEL_EVAL_POINTS = 0,    /* evaluation + point elimination */
+  CT_KERNEL_CAMERA_BLOCKS = 1,  /* reduced camera system assembly */
+  CT_KERNEL_LM_BEGIN = 2,       /* cost/gradient reduction + checks */
+  CT_KERNEL_RCS_SOLVE = 3,      /* Cholesky (or PCG) of S */
+  CT_KERNEL_REFINE = 4,         /* mixed: back-substitution + iterative refinement */
+  CT_KERNEL_CANDIDATE = 5,      /* (back-substitution +) candidate cost */
+  CT_KERNEL_LM_END = 6,         /* gain ratio, radius, accept/reject */
+  CT_KERNEL_COUNT = 7
+};
+
+/* Run `iterations` LM iterations (after an untimed iteration 0) with HIP
+ * events around each kernel group on the plan's stream; kernel_ms[k] and
+ * kernel_launches[k] (host, CT_KERNEL_COUNT entries) receive the summed
+ * milliseconds and the number of timed launches.  The parameters end at the
+ * state reached, as after ct_ba_plan_run.  No reference counterpart: the
+ * reference times whole calls on the host (benchmarking.py:32-69).         */
+int ct_ba_plan_profile(ct_ba_plan* plan, int iterations, double* kernel_ms, int32_t* kernel_launches);
+
+/* Single evaluation at the current parameters (Problem.evaluate with
+ * compute_jacobian=False, problem.py:377-413): cost and residuals [N*2]
+ * (loss-corrected).  residuals may be NULL.                                 */
+int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const double* points,
+                   double* cost_out_host, double* residuals, void* workspace, size_t workspace_bytes,
+                   ct_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CERES_MI355X_H_ */
