@@ -7,6 +7,7 @@ with ctypes.  Run ``python ceres-torch-cuda_amd/build.py`` or call ``build()``.
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -32,14 +33,39 @@ COMMON = [
 ]
 
 
+# Kernels may not spill: no VGPR spills, no AGPRs used as spill space (no
+# kernel here issues MFMA), no scratch.  A register-allocator spill copy placed
+# inside divergent control flow only saves the lanes active there, so a value
+# that is live in other lanes comes back as garbage when restored under a wider
+# exec mask; kernels are kept small enough that the allocator never spills.
+RESOURCE_RE = re.compile(r"remark: .*?(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|VGPRs Spill|SGPRs Spill): (\S+)")
+
+
+def check_resources(stderr: str, src: str) -> list[str]:
+    problems, name = [], None
+    for key, val in RESOURCE_RE.findall(stderr):
+        if key == "Function Name":
+            name = val
+        elif key == "AGPRs" and int(val) > 0:
+            problems.append(f"{src}:{name} uses {val} AGPRs")
+        elif key == "VGPRs Spill" and int(val) > 0:
+            problems.append(f"{src}:{name} spills {val} VGPRs")
+        elif key.startswith("ScratchSize") and int(val) > 0:
+            problems.append(f"{src}:{name} uses {val} bytes of scratch")
+    return problems
+
+
 def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ_DIR / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, "-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
+    problems = check_resources(res.stderr, src.name)
+    if problems:
+        raise RuntimeError("kernel resource check failed:\n  " + "\n  ".join(problems))
     return obj
 
 

@@ -97,6 +97,7 @@ class ct_ba_problem(ctypes.Structure):
         ("num_blocks", ctypes.c_int32),
         ("num_terms", ctypes.c_int64),
         ("obs_camera", _vp),
+        ("obs_point", _vp),
         ("point_offsets", _vp),
         ("observations", _vp),
         ("camera_offsets", _vp),

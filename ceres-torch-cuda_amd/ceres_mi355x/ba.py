@@ -28,6 +28,7 @@ class BATables:
     num_points: int
     order: np.ndarray            # device observation k = caller observation order[k]
     obs_camera: np.ndarray       # [N] (point-grouped)
+    obs_point: np.ndarray        # [N] (non-decreasing)
     point_offsets: np.ndarray    # [P+1]
     camera_offsets: np.ndarray   # [C+1]
     camera_obs: np.ndarray       # [N]
@@ -81,6 +82,7 @@ def build_tables(obs_camera, obs_point, num_cameras: int, num_points: int) -> BA
         num_points=num_points,
         order=order,
         obs_camera=cam.astype(np.int32),
+        obs_point=pt.astype(np.int32),
         point_offsets=point_offsets.astype(np.int32),
         camera_offsets=camera_offsets.astype(np.int32),
         camera_obs=corder.astype(np.int32),
@@ -112,6 +114,7 @@ class BADevice:
             return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
 
         self.obs_camera = i32(t.obs_camera)
+        self.obs_point = i32(t.obs_point)
         self.point_offsets = i32(t.point_offsets)
         self.observations = torch.from_numpy(np.ascontiguousarray(obs)).to(dev)
         self.camera_offsets = i32(t.camera_offsets)
@@ -138,6 +141,7 @@ class BADevice:
         p.num_blocks = int(self.block_cameras.shape[0])
         p.num_terms = int(self.block_terms.shape[0])
         p.obs_camera = nat.ptr(self.obs_camera)
+        p.obs_point = nat.ptr(self.obs_point)
         p.point_offsets = nat.ptr(self.point_offsets)
         p.observations = nat.ptr(self.observations)
         p.camera_offsets = nat.ptr(self.camera_offsets)

@@ -22,7 +22,7 @@ int validate_problem(const ct_ba_problem* p) {
   CT_CHECK_ARG(p->num_cameras >= 0 && p->num_points >= 0 && p->num_observations >= 0, "negative problem size");
   CT_CHECK_ARG(p->num_blocks >= 0 && p->num_terms >= 0, "negative block/term count");
   if (p->num_observations > 0) {
-    CT_CHECK_ARG(p->obs_camera && p->point_offsets && p->observations, "observation arrays are null");
+    CT_CHECK_ARG(p->obs_camera && p->obs_point && p->point_offsets && p->observations, "observation arrays are null");
     CT_CHECK_ARG(p->camera_offsets && p->camera_obs, "camera CSR arrays are null");
   }
   if (p->num_blocks > 0) {
@@ -44,8 +44,9 @@ int validate_options(const ct_lm_options* o) {
   return CT_OK;
 }
 
+int obs_blocks(const ct_ba_problem* p) { return (p->num_observations + kPointThreads - 1) / kPointThreads; }
 int eval_blocks(const ct_ba_problem* p) { return (p->num_points + kPointThreads - 1) / kPointThreads; }
-int cand_blocks(const ct_ba_problem* p) {
+int step_blocks(const ct_ba_problem* p) {
   const int work = p->num_points > 9 * p->num_cameras ? p->num_points : 9 * p->num_cameras;
   return (work + kPointThreads - 1) / kPointThreads;
 }
@@ -70,9 +71,11 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.rhs = cv.take<double>(n);
   L.w.dxc = cv.take<double>(n);
   L.w.dxp = cv.take<double>(3 * P);
-  L.w.part_eval = cv.take<double>(3 * (size_t)eval_blocks(p) + 3);
+  L.w.part_cost = cv.take<double>((size_t)obs_blocks(p) + 1);
+  L.w.part_eval = cv.take<double>(2 * (size_t)eval_blocks(p) + 2);
   L.w.part_cam = cv.take<double>(2 * C + 2);
-  L.w.part_cand = cv.take<double>(5 * (size_t)cand_blocks(p) + 5);
+  L.w.part_step = cv.take<double>(3 * (size_t)step_blocks(p) + 3);
+  L.w.part_cand = cv.take<double>(2 * (size_t)obs_blocks(p) + 2);
   L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
   L.w.state = cv.take<LMState>(1);
   L.w.Dp = cv.take<double>(3 * P);
@@ -119,6 +122,7 @@ BAView make_view(const ct_ba_problem* p) {
   v.B = p->num_blocks;
   v.T = p->num_terms;
   v.obs_camera = p->obs_camera;
+  v.obs_point = p->obs_point;
   v.point_offsets = p->point_offsets;
   v.obs_xy = p->observations;
   v.camera_offsets = p->camera_offsets;
@@ -269,8 +273,9 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->user_pts = points;
   pl->stream = reinterpret_cast<hipStream_t>(stream);
   pl->init.opt = *options;
+  pl->init.num_obs_blocks = obs_blocks(problem);
   pl->init.num_eval_blocks = eval_blocks(problem);
-  pl->init.num_cand_blocks = cand_blocks(problem);
+  pl->init.num_step_blocks = step_blocks(problem);
   pl->init.num_cameras = problem->num_cameras;
   pl->init.n_cam_params = 9 * problem->num_cameras;
   pl->init.n_pt_params = 3 * problem->num_points;

@@ -44,6 +44,7 @@ struct BAView {
   int32_t C, P, N, B;
   int64_t T;
   const int32_t* __restrict__ obs_camera;
+  const int32_t* __restrict__ obs_point;
   const int32_t* __restrict__ point_offsets;
   const double* __restrict__ obs_xy;
   const int32_t* __restrict__ camera_offsets;
@@ -76,9 +77,11 @@ struct BAWork {
   double* rhs;        // [n]
   double* dxc;        // [n]
   double* dxp;        // [P*3]
-  double* part_eval;  // [G][3]: cost, |g_p|^2, max|g_p|
-  double* part_cam;   // [C][2]: |g_c|^2, max|g_c|
-  double* part_cand;  // [G][5]: cand cost, model decrease, |dx|^2, |x'|^2, finite
+  double* part_cost;  // [Gobs]   cost at x
+  double* part_eval;  // [Gpt][2] |g_p|^2, max|g_p|
+  double* part_cam;   // [C][2]   |g_c|^2, max|g_c|
+  double* part_step;  // [Gstep][3] |dx|^2, |x'|^2, non-finite count
+  double* part_cand;  // [Gobs][2] candidate cost, model decrease
   double* records;    // [(max_iter+2)][CT_REC_FIELDS]
   LMState* state;
   // damping and gradient kept for the refinement residual

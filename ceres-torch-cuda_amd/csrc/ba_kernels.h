@@ -21,8 +21,9 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 // LM state machine (lm_kernels.hip)
 struct LMInit {
   ct_lm_options opt;
+  int num_obs_blocks;    // partial rows written by observation-parallel kernels
   int num_eval_blocks;   // partial rows written by k_eval_points
-  int num_cand_blocks;   // partial rows written by k_candidate
+  int num_step_blocks;   // partial rows written by k_step_points
   int num_cameras;
   int n_cam_params;
   int n_pt_params;

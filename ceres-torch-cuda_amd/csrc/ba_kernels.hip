@@ -1,6 +1,6 @@
-// Bundle-adjustment kernels for gfx950: evaluation fused with point
-// elimination, camera-pair Schur assembly, back-substitution fused with the
-// candidate evaluation.  Templated on the assembly precision (float: mixed,
+// Bundle-adjustment kernels for gfx950: per-observation evaluation, per-point
+// elimination, camera-pair Schur assembly, step application and the
+// per-observation candidate evaluation.  Templated on the assembly precision (float: mixed,
 // double: plain FP64); parameters, residuals, costs and the model decrease are
 // always FP64.
 #include "ba_kernels.h"
@@ -46,8 +46,8 @@ __device__ __forceinline__ void snavely_residual(const double* cam, const double
 }
 
 // Residual and full Jacobian of one observation.
-__device__ __forceinline__ void snavely_jacobian(const double* cam, const double* X, const double* obs, double* r,
-                                                 double (*Jc)[9], double (*Jp)[3]) {
+__device__ __forceinline__ void snavely_jacobian(const double* cam, const double* X, const double* obs, double (&r)[2],
+                                                 double (&Jc)[2][9], double (&Jp)[2][3]) {
   const double a0 = cam[0], a1 = cam[1], a2 = cam[2];
   double p[3];
   aa_rotate(cam, X, p);
@@ -133,7 +133,8 @@ __device__ __forceinline__ void snavely_jacobian(const double* cam, const double
 
 // Ceres corrector on a 2-residual block (losses.py:158-193).  Returns the
 // block cost 1/2 rho0 and rescales r, Jc, Jp in place.
-__device__ __forceinline__ double robustify2(const ct_loss& L, double* r, double (*Jc)[9], double (*Jp)[3]) {
+__device__ __forceinline__ double robustify2(const ct_loss& L, double (&r)[2], double (&Jc)[2][9],
+                                             double (&Jp)[2][3]) {
   const double s = r[0] * r[0] + r[1] * r[1];
   if (L.type == CT_LOSS_TRIVIAL && L.scale == 1.0) return 0.5 * s;
   double rho1, rho2;
@@ -145,29 +146,18 @@ __device__ __forceinline__ double robustify2(const ct_loss& L, double* r, double
     scale = sq / (1.0 - alpha);
     alpha_s = alpha / s;
   }
-  if (Jc != nullptr) {
-    if (alpha_s == 0.0) {
+  // J <- sqrt(rho1) (J - alpha/s r r^T J); alpha = 0 gives the plain scaling
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
+  for (int j = 0; j < 9; ++j) {
+    const double rtj = alpha_s * (r[0] * Jc[0][j] + r[1] * Jc[1][j]);
+    Jc[0][j] = sq * (Jc[0][j] - r[0] * rtj);
+    Jc[1][j] = sq * (Jc[1][j] - r[1] * rtj);
+  }
 #pragma unroll
-        for (int j = 0; j < 9; ++j) Jc[k][j] *= sq;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Jp[k][j] *= sq;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        const double rtj = r[0] * Jc[0][j] + r[1] * Jc[1][j];
-        Jc[0][j] = sq * (Jc[0][j] - alpha_s * r[0] * rtj);
-        Jc[1][j] = sq * (Jc[1][j] - alpha_s * r[1] * rtj);
-      }
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double rtj = r[0] * Jp[0][j] + r[1] * Jp[1][j];
-        Jp[0][j] = sq * (Jp[0][j] - alpha_s * r[0] * rtj);
-        Jp[1][j] = sq * (Jp[1][j] - alpha_s * r[1] * rtj);
-      }
-    }
+  for (int j = 0; j < 3; ++j) {
+    const double rtj = alpha_s * (r[0] * Jp[0][j] + r[1] * Jp[1][j]);
+    Jp[0][j] = sq * (Jp[0][j] - r[0] * rtj);
+    Jp[1][j] = sq * (Jp[1][j] - r[1] * rtj);
   }
   r[0] *= scale;
   r[1] *= scale;
@@ -188,80 +178,105 @@ __device__ __forceinline__ double damping_of(double colsq, double min_diag, doub
 }
 
 // ---------------------------------------------------------------------------
-// K1: evaluation fused with point elimination.  One thread per point walks
-// its observations: residual, Jacobian, loss correction, then V_p = Jp^T Jp
-// (+ damping), its 3x3 Cholesky L, L^-1, q = L^-1 g_p, P_o = Jp_o L^-T and
-// e_o = P_o q - r_o.  Block partials: cost, |g_p|^2, max|g_p|.
+// K1a: one thread per observation - residual, Jacobian, loss correction.
+// Writes Jc [2x9], Jp [2x3] (assembly precision) and r (FP64); block partial
+// of the cost.
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ __launch_bounds__(kPointThreads) void k_eval_points(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kPointThreads, 2) void k_eval_obs(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ double red[kPointThreads / kWave];
-  LMState* st = w.state;
+  const LMState* st = w.state;
   if (st->done) return;
   const int cur = st->cur;
   const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
+  const int o = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0;
+  if (o < v.N) {
+    const int ci = v.obs_camera[o];
+    const int p = v.obs_point[o];
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
+    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+    double r[2], Jc[2][9], Jp[2][3];
+    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    if (v.camera_mask) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double m = v.camera_mask[9 * ci + j] ? 1.0 : 0.0;
+        Jc[0][j] *= m;
+        Jc[1][j] *= m;
+      }
+    }
+    if (v.point_mask) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double m = v.point_mask[3 * p + j] ? 1.0 : 0.0;
+        Jp[0][j] *= m;
+        Jp[1][j] *= m;
+      }
+    }
+    cost = robustify2(v.loss, r, Jc, Jp);
+    b.r[2 * (size_t)o] = r[0];
+    b.r[2 * (size_t)o + 1] = r[1];
+    real* jc = b.Jc + 18 * (size_t)o;
+    real* jp = b.Jp + 6 * (size_t)o;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) jc[9 * k + j] = (real)Jc[k][j];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) jp[3 * k + j] = (real)Jp[k][j];
+    }
+  }
+  const double tc = block_sum<kPointThreads>(cost, red);
+  if (threadIdx.x == 0) w.part_cost[blockIdx.x] = tc;
+}
+
+// ---------------------------------------------------------------------------
+// K1b: one thread per point - point elimination.
+//   V_p = sum Jp^T Jp + D_p, g_p = sum Jp^T r, L = chol(V_p),
+//   q = L^-1 g_p, P_o = Jp_o L^-T, e_o = P_o q - r_o.
+// Block partials: |g_p|^2, max|g_p|.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double red[kPointThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done) return;
   const double radius = st->radius;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
-  double cost = 0.0, g2 = 0.0, gmax = 0.0;
+  double g2 = 0.0, gmax = 0.0;
   if (p < v.P) {
     const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
-    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
-    double pmask[3] = {1.0, 1.0, 1.0};
-    if (v.point_mask) {
-      for (int j = 0; j < 3; ++j) pmask[j] = v.point_mask[3 * p + j] ? 1.0 : 0.0;
-    }
     real V00 = 0, V10 = 0, V11 = 0, V20 = 0, V21 = 0, V22 = 0;
-    double g[3] = {0.0, 0.0, 0.0};
+    double g0 = 0.0, g1 = 0.0, g2v = 0.0;
     for (int o = o0; o < o1; ++o) {
-      const int ci = v.obs_camera[o];
-      double cam[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
-      const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
-      double r[2], Jc[2][9], Jp[2][3];
-      snavely_jacobian(cam, X, obs, r, Jc, Jp);
-      if (v.camera_mask) {
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-          if (!v.camera_mask[9 * ci + j]) { Jc[0][j] = 0.0; Jc[1][j] = 0.0; }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 3; ++j) { Jp[0][j] *= pmask[j]; Jp[1][j] *= pmask[j]; }
-      cost += robustify2(v.loss, r, Jc, Jp);
-      b.r[2 * o] = r[0];
-      b.r[2 * o + 1] = r[1];
-      real jc[18], jp[6];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-#pragma unroll
-        for (int j = 0; j < 9; ++j) jc[9 * k + j] = (real)Jc[k][j];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) jp[3 * k + j] = (real)Jp[k][j];
-      }
-#pragma unroll
-      for (int j = 0; j < 18; ++j) b.Jc[18 * (size_t)o + j] = jc[j];
-#pragma unroll
-      for (int j = 0; j < 6; ++j) b.Jp[6 * (size_t)o + j] = jp[j];
-      V00 += jp[0] * jp[0] + jp[3] * jp[3];
-      V10 += jp[1] * jp[0] + jp[4] * jp[3];
-      V11 += jp[1] * jp[1] + jp[4] * jp[4];
-      V20 += jp[2] * jp[0] + jp[5] * jp[3];
-      V21 += jp[2] * jp[1] + jp[5] * jp[4];
-      V22 += jp[2] * jp[2] + jp[5] * jp[5];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) g[j] += (double)jp[j] * r[0] + (double)jp[3 + j] * r[1];
+      const real* jp = b.Jp + 6 * (size_t)o;
+      const real a0 = jp[0], a1 = jp[1], a2 = jp[2], c0 = jp[3], c1 = jp[4], c2 = jp[5];
+      const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
+      V00 += a0 * a0 + c0 * c0;
+      V10 += a1 * a0 + c1 * c0;
+      V11 += a1 * a1 + c1 * c1;
+      V20 += a2 * a0 + c2 * c0;
+      V21 += a2 * a1 + c2 * c1;
+      V22 += a2 * a2 + c2 * c2;
+      g0 += (double)a0 * r0 + (double)c0 * r1;
+      g1 += (double)a1 * r0 + (double)c1 * r1;
+      g2v += (double)a2 * r0 + (double)c2 * r1;
     }
     // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
     const double D0 = damping_of((double)V00, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
     const double D1 = damping_of((double)V11, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
     const double D2 = damping_of((double)V22, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    w.Dp[3 * p] = D0; w.Dp[3 * p + 1] = D1; w.Dp[3 * p + 2] = D2;
+    w.Dp[3 * p] = D0;
+    w.Dp[3 * p + 1] = D1;
+    w.Dp[3 * p + 2] = D2;
     V00 += (real)D0;
     V11 += (real)D1;
     V22 += (real)D2;
-    // Cholesky of the damped 3x3 block and its inverse
     const real l00 = sqrt(V00);
     const real l10 = V10 / l00, l20 = V20 / l00;
     const real l11 = sqrt(V11 - l10 * l10);
@@ -271,39 +286,44 @@ __global__ __launch_bounds__(kPointThreads) void k_eval_points(BAView v, BABuffe
     const real i10 = -l10 * i00 * i11;
     const real i21 = -l21 * i11 * i22;
     const real i20 = -(l20 * i00 + l21 * i10) * i22;
-    const real q0 = i00 * (real)g[0];
-    const real q1 = i10 * (real)g[0] + i11 * (real)g[1];
-    const real q2 = i20 * (real)g[0] + i21 * (real)g[1] + i22 * (real)g[2];
+    const real q0 = i00 * (real)g0;
+    const real q1 = i10 * (real)g0 + i11 * (real)g1;
+    const real q2 = i20 * (real)g0 + i21 * (real)g1 + i22 * (real)g2v;
     real* Li = b.Linv + 6 * (size_t)p;
     Li[0] = i00; Li[1] = i10; Li[2] = i11; Li[3] = i20; Li[4] = i21; Li[5] = i22;
-    b.q[3 * p] = q0; b.q[3 * p + 1] = q1; b.q[3 * p + 2] = q2;
-    b.gp[3 * p] = g[0]; b.gp[3 * p + 1] = g[1]; b.gp[3 * p + 2] = g[2];
+    b.q[3 * p] = q0;
+    b.q[3 * p + 1] = q1;
+    b.q[3 * p + 2] = q2;
+    b.gp[3 * p] = g0;
+    b.gp[3 * p + 1] = g1;
+    b.gp[3 * p + 2] = g2v;
     for (int o = o0; o < o1; ++o) {
       const real* jp = b.Jp + 6 * (size_t)o;
-      real P[6];
+      real* P = b.Pf + 6 * (size_t)o;
+      double e[2];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const real j0 = jp[3 * k], j1 = jp[3 * k + 1], j2 = jp[3 * k + 2];
         // P = Jp L^-T : P[k][m] = sum_{l<=m} Jp[k][l] Linv[m][l]
-        P[3 * k + 0] = j0 * i00;
-        P[3 * k + 1] = j0 * i10 + j1 * i11;
-        P[3 * k + 2] = j0 * i20 + j1 * i21 + j2 * i22;
+        const real p0 = j0 * i00;
+        const real p1 = j0 * i10 + j1 * i11;
+        const real p2 = j0 * i20 + j1 * i21 + j2 * i22;
+        P[3 * k] = p0;
+        P[3 * k + 1] = p1;
+        P[3 * k + 2] = p2;
+        e[k] = (double)(p0 * q0 + p1 * q1 + p2 * q2) - b.r[2 * (size_t)o + k];
       }
-#pragma unroll
-      for (int j = 0; j < 6; ++j) b.Pf[6 * (size_t)o + j] = P[j];
-      b.e[2 * (size_t)o] = (real)((double)(P[0] * q0 + P[1] * q1 + P[2] * q2) - b.r[2 * o]);
-      b.e[2 * (size_t)o + 1] = (real)((double)(P[3] * q0 + P[4] * q1 + P[5] * q2) - b.r[2 * o + 1]);
+      b.e[2 * (size_t)o] = (real)e[0];
+      b.e[2 * (size_t)o + 1] = (real)e[1];
     }
-    g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-    gmax = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
+    g2 = g0 * g0 + g1 * g1 + g2v * g2v;
+    gmax = fmax(fabs(g0), fmax(fabs(g1), fabs(g2v)));
   }
-  const double tc = block_sum<kPointThreads>(cost, red);
   const double tg = block_sum<kPointThreads>(g2, red);
   const double tm = block_max<kPointThreads>(gmax, red);
   if (threadIdx.x == 0) {
-    w.part_eval[3 * blockIdx.x] = tc;
-    w.part_eval[3 * blockIdx.x + 1] = tg;
-    w.part_eval[3 * blockIdx.x + 2] = tm;
+    w.part_eval[2 * blockIdx.x] = tg;
+    w.part_eval[2 * blockIdx.x + 1] = tm;
   }
 }
 
@@ -311,14 +331,16 @@ __global__ __launch_bounds__(kPointThreads) void k_eval_points(BAView v, BABuffe
 // K2: one wavefront per camera pair block (i <= j).
 //   S_ij = d_ij (U_i + D_i) - sum_{(a,b)} Jc_a^T (P_a P_b^T) Jc_b
 // 63 lanes form 7 groups of 3x3 sub-block owners; group g takes terms
-// g, g+7, ...; the 7 partial blocks are summed in LDS in fixed order.
-// Diagonal blocks also produce rhs_i = sum Jc^T e, |g_c|^2 and max|g_c|.
+// g, g+7, ...; the 7 partial blocks are summed in LDS in a fixed order.
+// Diagonal blocks also produce rhs_i = sum Jc^T e, g_c = sum Jc^T r and the
+// column norms that set D_i, from the same pass over the camera's
+// observations.
 // ---------------------------------------------------------------------------
 template <typename real>
 __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ real red[7][81];
-  __shared__ double vred[kWave][27];
-  LMState* st = w.state;
+  __shared__ double vec[7][27];  // per group: rhs (9), g_c (9), colsq (9)
+  const LMState* st = w.state;
   if (st->done) return;
   const int blk = blockIdx.x;
   const int ci = v.block_cameras[2 * blk];
@@ -328,13 +350,15 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
   const int grp = tid / 9;
   const int br = (tid % 9) / 3;
   const int bc = (tid % 9) % 3;
+  const bool diag = ci == cj;
   real acc[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[r][c] = 0;
+  double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
   if (grp < 7) {
-    if (ci == cj) {
+    if (diag) {
       const int s0 = v.camera_offsets[ci], s1 = v.camera_offsets[ci + 1];
       for (int t = s0 + grp; t < s1; t += 7) {
         const int o = v.camera_obs[t];
@@ -342,13 +366,36 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
         real ja0[3], ja1[3], jb0[3], jb1[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          ja0[r] = J[3 * br + r]; ja1[r] = J[9 + 3 * br + r];
-          jb0[r] = J[3 * bc + r]; jb1[r] = J[9 + 3 * bc + r];
+          ja0[r] = J[3 * br + r];
+          ja1[r] = J[9 + 3 * br + r];
+          jb0[r] = J[3 * bc + r];
+          jb1[r] = J[9 + 3 * bc + r];
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
           for (int c = 0; c < 3; ++c) acc[r][c] += ja0[r] * jb0[c] + ja1[r] * jb1[c];
+        if (bc == 0) {
+          const double e0 = (double)b.e[2 * (size_t)o], e1 = (double)b.e[2 * (size_t)o + 1];
+          const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            rv[r] += (double)ja0[r] * e0 + (double)ja1[r] * e1;
+            gv[r] += (double)ja0[r] * r0 + (double)ja1[r] * r1;
+          }
+        }
+      }
+      // diagonal of U_i (column norms of the camera's Jacobian columns)
+      if (br == bc) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) vec[grp][18 + 3 * br + r] = (double)acc[r][r];
+      }
+      if (bc == 0) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          vec[grp][3 * br + r] = rv[r];
+          vec[grp][9 + 3 * br + r] = gv[r];
+        }
       }
     }
     const int t0 = v.block_offsets[blk], t1 = v.block_offsets[blk + 1];
@@ -357,12 +404,10 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
       const int ob = v.block_terms[2 * (size_t)t + 1];
       const real* Pa = b.Pf + 6 * (size_t)oa;
       const real* Pb = b.Pf + 6 * (size_t)ob;
-      const real pa0 = Pa[0], pa1 = Pa[1], pa2 = Pa[2], pa3 = Pa[3], pa4 = Pa[4], pa5 = Pa[5];
-      const real pb0 = Pb[0], pb1 = Pb[1], pb2 = Pb[2], pb3 = Pb[3], pb4 = Pb[4], pb5 = Pb[5];
-      const real m00 = pa0 * pb0 + pa1 * pb1 + pa2 * pb2;
-      const real m01 = pa0 * pb3 + pa1 * pb4 + pa2 * pb5;
-      const real m10 = pa3 * pb0 + pa4 * pb1 + pa5 * pb2;
-      const real m11 = pa3 * pb3 + pa4 * pb4 + pa5 * pb5;
+      const real m00 = Pa[0] * Pb[0] + Pa[1] * Pb[1] + Pa[2] * Pb[2];
+      const real m01 = Pa[0] * Pb[3] + Pa[1] * Pb[4] + Pa[2] * Pb[5];
+      const real m10 = Pa[3] * Pb[0] + Pa[4] * Pb[1] + Pa[5] * Pb[2];
+      const real m11 = Pa[3] * Pb[3] + Pa[4] * Pb[4] + Pa[5] * Pb[5];
       const real* Ja = b.Jc + 18 * (size_t)oa;
       const real* Jb = b.Jc + 18 * (size_t)ob;
       real ja0[3], ja1[3], t0v[3], t1v[3];
@@ -384,80 +429,42 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
 #pragma unroll
       for (int c = 0; c < 3; ++c) red[grp][(3 * br + r) * 9 + 3 * bc + c] = acc[r][c];
   }
-  // diagonal blocks: rhs, gradient and column norms over the camera's observations
-  double rv[9], gv[9], dv[9];
-#pragma unroll
-  for (int j = 0; j < 9; ++j) { rv[j] = 0.0; gv[j] = 0.0; dv[j] = 0.0; }
-  if (ci == cj) {
-    const int s0 = v.camera_offsets[ci], s1 = v.camera_offsets[ci + 1];
-    for (int t = s0 + tid; t < s1; t += kWave) {
-      const int o = v.camera_obs[t];
-      const real* J = b.Jc + 18 * (size_t)o;
-      const double e0 = (double)b.e[2 * (size_t)o], e1 = (double)b.e[2 * (size_t)o + 1];
-      const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        const double j0 = (double)J[j], j1 = (double)J[9 + j];
-        rv[j] += j0 * e0 + j1 * e1;
-        gv[j] += j0 * r0 + j1 * r1;
-        dv[j] += j0 * j0 + j1 * j1;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      vred[tid][j] = rv[j];
-      vred[tid][9 + j] = gv[j];
-      vred[tid][18 + j] = dv[j];
-    }
-  }
   __syncthreads();
   const double radius = st->radius;
+  if (diag && tid < 9) {
+    double rs = 0.0, gs = 0.0, cs = 0.0;
+    for (int gi = 0; gi < 7; ++gi) {
+      rs += vec[gi][tid];
+      gs += vec[gi][9 + tid];
+      cs += vec[gi][18 + tid];
+    }
+    const int k = 9 * ci + tid;
+    w.rhs[k] = rs;
+    w.gc[k] = gs;
+    w.Dc[k] = damping_of(cs, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    vec[0][tid] = gs;  // gradient entries for the norms below (group 0 slot is free now)
+  }
+  __syncthreads();
   for (int idx = tid; idx < 81; idx += kWave) {
-    real s = 0;
-#pragma unroll
-    for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
     const int r = idx / 9, c = idx % 9;
-    if (ci == cj) {
-      // column sums of J^T J on the diagonal come from the same observations
-      double colsq = 0.0;
-      if (r == c) {
-        for (int l = 0; l < kWave; ++l) colsq += vred[l][18 + r];
-      }
-      double val;
-      if (r >= c) {
-        val = (double)s;
-      } else {
-        real s2 = 0;
-#pragma unroll
-        for (int gi = 0; gi < 7; ++gi) s2 += red[gi][c * 9 + r];
-        val = (double)s2;
-      }
-      if (r == c) {
-        const double Dv = damping_of(colsq, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-        w.Dc[9 * ci + r] = Dv;
-        val += Dv;
-      }
+    real s = 0;
+    if (diag) {
+      const int src = r >= c ? idx : c * 9 + r;  // symmetric: lower triangle value
+      for (int gi = 0; gi < 7; ++gi) s += red[gi][src];
+      double val = (double)s;
+      if (r == c) val += w.Dc[9 * ci + r];
       w.S[(size_t)(9 * ci + r) * n + 9 * ci + c] = val;
     } else {
+      for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
       w.S[(size_t)(9 * ci + r) * n + 9 * cj + c] = (double)s;
       w.S[(size_t)(9 * cj + c) * n + 9 * ci + r] = (double)s;
     }
   }
-  if (ci == cj && tid < 9) {
-    double rs = 0.0;
-    for (int l = 0; l < kWave; ++l) rs += vred[l][tid];
-    w.rhs[9 * ci + tid] = rs;
-    double gs = 0.0;
-    for (int l = 0; l < kWave; ++l) gs += vred[l][9 + tid];
-    w.gc[9 * ci + tid] = gs;
-  }
-  if (ci == cj && tid == 0) {
+  if (diag && tid == 0) {
     double g2 = 0.0, gm = 0.0;
     for (int j = 0; j < 9; ++j) {
-      double gsum = 0.0;
-      for (int l = 0; l < kWave; ++l) gsum += vred[l][9 + j];
-      g2 += gsum * gsum;
-      gm = fmax(gm, fabs(gsum));
+      g2 += vec[0][j] * vec[0][j];
+      gm = fmax(gm, fabs(vec[0][j]));
     }
     w.part_cam[2 * ci] = g2;
     w.part_cam[2 * ci + 1] = gm;
@@ -470,7 +477,6 @@ __device__ __forceinline__ void backsub_point(const BAView& v, const BABuffers<r
                                               int p, double* d0, double* d1, double* d2) {
   const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
   const real* Li = b.Linv + 6 * (size_t)p;
-  const double i00 = Li[0], i10 = Li[1], i11 = Li[2], i20 = Li[3], i21 = Li[4], i22 = Li[5];
   double s0 = (double)b.q[3 * p], s1 = (double)b.q[3 * p + 1], s2 = (double)b.q[3 * p + 2];
   for (int o = o0; o < o1; ++o) {
     const int ci = v.obs_camera[o];
@@ -487,20 +493,22 @@ __device__ __forceinline__ void backsub_point(const BAView& v, const BABuffers<r
     s1 += (double)P[1] * y0 + (double)P[4] * y1;
     s2 += (double)P[2] * y0 + (double)P[5] * y1;
   }
-  *d0 = -(i00 * s0 + i10 * s1 + i20 * s2);
-  *d1 = -(i11 * s1 + i21 * s2);
-  *d2 = -(i22 * s2);
+  *d0 = -((double)Li[0] * s0 + (double)Li[1] * s1 + (double)Li[3] * s2);
+  *d1 = -((double)Li[2] * s1 + (double)Li[4] * s2);
+  *d2 = -((double)Li[5] * s2);
 }
 
 template <typename real>
-__global__ __launch_bounds__(kPointThreads) void k_backsub(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kPointThreads, 2) void k_backsub(BAView v, BABuffers<real> b, BAWork w) {
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
   if (p >= v.P) return;
   double d0, d1, d2;
   backsub_point(v, b, w.dxc, p, &d0, &d1, &d2);
-  w.dxp[3 * p] = d0; w.dxp[3 * p + 1] = d1; w.dxp[3 * p + 2] = d2;
+  w.dxp[3 * p] = d0;
+  w.dxp[3 * p + 1] = d1;
+  w.dxp[3 * p + 2] = d2;
 }
 
 // ---------------------------------------------------------------------------
@@ -513,7 +521,7 @@ __global__ __launch_bounds__(kPointThreads) void k_backsub(BAView v, BABuffers<r
 // reusing the point factors and the FP64 Cholesky factor of S.
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ __launch_bounds__(kPointThreads) void k_refine_points(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BABuffers<real> b, BAWork w) {
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
@@ -545,7 +553,9 @@ __global__ __launch_bounds__(kPointThreads) void k_refine_points(BAView v, BABuf
   const double t0 = (double)Li[0] * rp0;
   const double t1 = (double)Li[1] * rp0 + (double)Li[2] * rp1;
   const double t2 = (double)Li[3] * rp0 + (double)Li[4] * rp1 + (double)Li[5] * rp2;
-  w.res_p[3 * p] = t0; w.res_p[3 * p + 1] = t1; w.res_p[3 * p + 2] = t2;
+  w.res_p[3 * p] = t0;
+  w.res_p[3 * p + 1] = t1;
+  w.res_p[3 * p + 2] = t2;
   for (int o = o0; o < o1; ++o) {
     const real* P = b.Pf + 6 * (size_t)o;
     w.u_obs[4 * (size_t)o + 2] = (double)P[0] * t0 + (double)P[1] * t1 + (double)P[2] * t2;
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(kWave) void k_refine_cameras(BAView v, BABuffers<re
 }
 
 template <typename real>
-__global__ __launch_bounds__(kPointThreads) void k_refine_update(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kPointThreads, 2) void k_refine_update(BAView v, BABuffers<real> b, BAWork w) {
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int t = blockIdx.x * kPointThreads + threadIdx.x;
@@ -615,77 +625,105 @@ __global__ __launch_bounds__(kPointThreads) void k_refine_update(BAView v, BABuf
 }
 
 // ---------------------------------------------------------------------------
-// K5: back-substitution fused with the candidate evaluation.
-//   dx_p = -L^-T (q + sum_o P_o^T Jc_o dx_c),  x' = x + dx
-//   cost(x'), model decrease sum_o (-r.u - u.u/2) with u = J dx, |dx|^2, |x'|^2
+// K5a: apply the step, x' = x + dx (Euclidean Plus, manifolds.py:69-91), with
+// the point back-substitution fused when the solve did not already do it.
+// Block partials: |dx|^2, |x'|^2 over all parameters, non-finite count.
 // ---------------------------------------------------------------------------
-template <typename real, bool kFusedBacksub>
-__global__ __launch_bounds__(kPointThreads) void k_candidate(BAView v, BABuffers<real> b, BAWork w) {
+template <typename real, bool kBacksub>
+__global__ __launch_bounds__(kPointThreads, 2) void k_step_points(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ double red[kPointThreads / kWave];
-  LMState* st = w.state;
+  const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int cur = st->cur;
   const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
   double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
   double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
-  const double* __restrict__ dxc = w.dxc;
   const int t = blockIdx.x * kPointThreads + threadIdx.x;
-  double cost = 0.0, md = 0.0, dx2 = 0.0, x2 = 0.0, finite = 1.0;
+  double dx2 = 0.0, x2 = 0.0, bad = 0.0;
   if (t < 9 * v.C) {
-    const double d = dxc[t];
+    const double d = w.dxc[t];
     const double nx = cams[t] + d;
     ncams[t] = nx;
     dx2 += d * d;
     x2 += nx * nx;
-    if (!isfinite(d)) finite = 0.0;
+    if (!isfinite(d)) bad = 1.0;
   }
   if (t < v.P) {
-    const int p = t;
-    const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
     double d0, d1, d2;
-    if (kFusedBacksub) {
-      backsub_point(v, b, dxc, p, &d0, &d1, &d2);
-      w.dxp[3 * p] = d0; w.dxp[3 * p + 1] = d1; w.dxp[3 * p + 2] = d2;
+    if (kBacksub) {
+      backsub_point(v, b, w.dxc, t, &d0, &d1, &d2);
+      w.dxp[3 * t] = d0;
+      w.dxp[3 * t + 1] = d1;
+      w.dxp[3 * t + 2] = d2;
     } else {
-      d0 = w.dxp[3 * p]; d1 = w.dxp[3 * p + 1]; d2 = w.dxp[3 * p + 2];
+      d0 = w.dxp[3 * t];
+      d1 = w.dxp[3 * t + 1];
+      d2 = w.dxp[3 * t + 2];
     }
-    const double X[3] = {pts[3 * p] + d0, pts[3 * p + 1] + d1, pts[3 * p + 2] + d2};
-    npts[3 * p] = X[0]; npts[3 * p + 1] = X[1]; npts[3 * p + 2] = X[2];
+    const double n0 = pts[3 * t] + d0, n1 = pts[3 * t + 1] + d1, n2 = pts[3 * t + 2] + d2;
+    npts[3 * t] = n0;
+    npts[3 * t + 1] = n1;
+    npts[3 * t + 2] = n2;
     dx2 += d0 * d0 + d1 * d1 + d2 * d2;
-    x2 += X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
-    if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) finite = 0.0;
-    for (int o = o0; o < o1; ++o) {
-      const int ci = v.obs_camera[o];
-      const real* J = b.Jc + 18 * (size_t)o;
-      const real* JP = b.Jp + 6 * (size_t)o;
-      double cam[9];
-      double u0 = 0.0, u1 = 0.0;
+    x2 += n0 * n0 + n1 * n1 + n2 * n2;
+    if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) bad = 1.0;
+  }
+  const double a0 = block_sum<kPointThreads>(dx2, red);
+  const double a1 = block_sum<kPointThreads>(x2, red);
+  const double a2 = block_sum<kPointThreads>(bad, red);
+  if (threadIdx.x == 0) {
+    w.part_step[3 * blockIdx.x] = a0;
+    w.part_step[3 * blockIdx.x + 1] = a1;
+    w.part_step[3 * blockIdx.x + 2] = a2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5b: one thread per observation - candidate cost at x' and the model
+// decrease term -r.u - u.u/2 with u = Jc dx_c + Jp dx_p (solver.py:798-802,
+// summed per observation in FP64).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kPointThreads, 2) void k_candidate_obs(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double red[kPointThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int cur = st->cur;
+  const double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  const double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
+  const int o = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0, md = 0.0;
+  if (o < v.N) {
+    const int ci = v.obs_camera[o];
+    const int p = v.obs_point[o];
+    const real* J = b.Jc + 18 * (size_t)o;
+    const real* JP = b.Jp + 6 * (size_t)o;
+    const double d0 = w.dxp[3 * p], d1 = w.dxp[3 * p + 1], d2 = w.dxp[3 * p + 2];
+    double u0 = (double)JP[0] * d0 + (double)JP[1] * d1 + (double)JP[2] * d2;
+    double u1 = (double)JP[3] * d0 + (double)JP[4] * d1 + (double)JP[5] * d2;
 #pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        const double d = dxc[9 * ci + j];
-        cam[j] = cams[9 * ci + j] + d;
-        u0 += (double)J[j] * d;
-        u1 += (double)J[9 + j] * d;
-      }
-      u0 += (double)JP[0] * d0 + (double)JP[1] * d1 + (double)JP[2] * d2;
-      u1 += (double)JP[3] * d0 + (double)JP[4] * d1 + (double)JP[5] * d2;
-      const double r0 = b.r[2 * o], r1 = b.r[2 * o + 1];
-      md += -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
-      const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
-      double rr[2];
-      snavely_residual(cam, X, obs, rr);
-      cost += robust_cost2(v.loss, rr);
+    for (int j = 0; j < 9; ++j) {
+      const double d = w.dxc[9 * ci + j];
+      u0 += (double)J[j] * d;
+      u1 += (double)J[9 + j] * d;
     }
+    const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
+    md = -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = ncams[9 * ci + j];
+    const double X[3] = {npts[3 * p], npts[3 * p + 1], npts[3 * p + 2]};
+    const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+    double rr[2];
+    snavely_residual(cam, X, obs, rr);
+    cost = robust_cost2(v.loss, rr);
   }
   const double a0 = block_sum<kPointThreads>(cost, red);
   const double a1 = block_sum<kPointThreads>(md, red);
-  const double a2 = block_sum<kPointThreads>(dx2, red);
-  const double a3 = block_sum<kPointThreads>(x2, red);
-  const double a4 = block_sum<kPointThreads>(finite == 1.0 ? 0.0 : 1.0, red);
   if (threadIdx.x == 0) {
-    double* out = w.part_cand + 5 * blockIdx.x;
-    out[0] = a0; out[1] = a1; out[2] = a2; out[3] = a3; out[4] = a4;
+    w.part_cand[2 * blockIdx.x] = a0;
+    w.part_cand[2 * blockIdx.x + 1] = a1;
   }
 }
 
@@ -721,11 +759,18 @@ __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+namespace {
+inline int blocks_for(int n) { return (n + kPointThreads - 1) / kPointThreads; }
+}  // namespace
+
 template <typename real>
 int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
-  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
-  if (grid > 0) {
-    hipLaunchKernelGGL(k_eval_points<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+  if (v.N > 0) {
+    hipLaunchKernelGGL(k_eval_obs<real>, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
+  if (v.P > 0) {
+    hipLaunchKernelGGL(k_eval_points<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -743,13 +788,16 @@ int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork
 template <typename real>
 int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s) {
   const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
-  const int grid = (work + kPointThreads - 1) / kPointThreads;
-  if (grid > 0) {
+  if (work > 0) {
     if (fused_backsub) {
-      hipLaunchKernelGGL((k_candidate<real, true>), dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+      hipLaunchKernelGGL((k_step_points<real, true>), dim3(blocks_for(work)), dim3(kPointThreads), 0, s, v, b, w);
     } else {
-      hipLaunchKernelGGL((k_candidate<real, false>), dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+      hipLaunchKernelGGL((k_step_points<real, false>), dim3(blocks_for(work)), dim3(kPointThreads), 0, s, v, b, w);
     }
+    CT_LAUNCH_CHECK();
+  }
+  if (v.N > 0) {
+    hipLaunchKernelGGL(k_candidate_obs<real>, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -757,9 +805,8 @@ int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w,
 
 template <typename real>
 int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
-  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
-  if (grid > 0) {
-    hipLaunchKernelGGL(k_backsub<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+  if (v.P > 0) {
+    hipLaunchKernelGGL(k_backsub<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -767,11 +814,9 @@ int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, h
 
 template <typename real>
 int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
-  const int gp = (v.P + kPointThreads - 1) / kPointThreads;
   const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
-  const int gu = (work + kPointThreads - 1) / kPointThreads;
-  if (gp > 0) {
-    hipLaunchKernelGGL(k_refine_points<real>, dim3(gp), dim3(kPointThreads), 0, s, v, b, w);
+  if (v.P > 0) {
+    hipLaunchKernelGGL(k_refine_points<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   if (v.C > 0) {
@@ -779,8 +824,8 @@ int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& 
     CT_LAUNCH_CHECK();
     CT_TRY(launch_rcs_tri_solve(w.S, w.res_c, w.dxc_corr, 9 * v.C, w.state, w.solve_tmp, s));
   }
-  if (gu > 0) {
-    hipLaunchKernelGGL(k_refine_update<real>, dim3(gu), dim3(kPointThreads), 0, s, v, b, w);
+  if (work > 0) {
+    hipLaunchKernelGGL(k_refine_update<real>, dim3(blocks_for(work)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -788,9 +833,9 @@ int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& 
 
 int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,
                          hipStream_t s) {
-  const int grid = (v.P + kPointThreads - 1) / kPointThreads;
-  if (grid > 0) {
-    hipLaunchKernelGGL(k_evaluate_only, dim3(grid), dim3(kPointThreads), 0, s, v, cams, pts, residuals, part);
+  if (v.P > 0) {
+    hipLaunchKernelGGL(k_evaluate_only, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, cams, pts, residuals,
+                       part);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;

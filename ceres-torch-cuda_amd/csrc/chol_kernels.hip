@@ -1,13 +1,14 @@
 // FP64 solves of the reduced camera system S x = rhs (S symmetric positive
 // definite, n x n row-major, lower triangle referenced).
 //
-//  * n <= 256: one workgroup factors S blockwise (16-column panels: wave 0
-//    factors the diagonal block in registers, all waves solve the panel and
-//    apply the trailing rank-16 update) and runs both triangular solves.
+//  * n <= 256: one workgroup factors S blockwise (16-column panels: the
+//    diagonal block is factored in LDS by all threads, the rows below are
+//    solved one row per thread and the trailing rank-16 update runs on 4x4
+//    register tiles) and runs both triangular solves.
 //  * n > 256: right-looking blocked Cholesky, 32-column panels; per panel one
-//    kernel factors the diagonal block (redundantly per workgroup) and solves
-//    the rows below it, and one kernel applies the trailing update on 64x64
-//    tiles; a single workgroup then runs the blocked triangular solves.
+//    workgroup factors the diagonal block, one kernel solves the rows below it
+//    and one applies the trailing update on 64x64 tiles; a single workgroup
+//    then runs the blocked triangular solves.
 //  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning as
 //    one persistent kernel (linear.py:1216-1256, preconditioner :1156-1182).
 #include "ba_kernels.h"
@@ -16,47 +17,88 @@ namespace ct {
 
 namespace {
 
-__device__ __forceinline__ void lds_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 __device__ __forceinline__ bool skip_solve(const LMState* st) { return st != nullptr && (st->done || st->skip_step); }
 
-// Factor the kb x kb lower block held in Ld (row stride NBS) in place using
-// wave 0.  Lanes own rows; column j is broadcast through colbuf.  Returns
-// (in lane 0's *bad) whether a pivot was not positive and finite.
-template <int NB, int NBS>
-__device__ void factor_diag_block_wave0(double (*Ld)[NBS], int kb, double* colbuf, int* bad) {
-  const int l = threadIdx.x;
-  double a[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) a[c] = (l < kb && c <= l) ? Ld[l][c] : 0.0;
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < kb) {
-      colbuf[l] = a[j];
-      lds_wave_sync();
-      const double piv = colbuf[j];
-      if (l == 0 && !(piv > 0.0 && isfinite(piv))) *bad = 1;
-      const double d = sqrt(piv);
-      const double inv = 1.0 / d;
-      const double lj = (l == j) ? d : a[j] * inv;
-#pragma unroll
-      for (int m = j + 1; m < NB; ++m) {
-        const double lmj = colbuf[m] * inv;
-        if (m <= l) a[m] -= lj * lmj;
-      }
-      if (l >= j) a[j] = lj;
-      lds_wave_sync();
+// Factor the kb x kb lower block held in LDS (row stride NBS) in place with
+// every thread of the workgroup: per column, all threads read the pivot, the
+// column below it is scaled by 1/sqrt(pivot) and each trailing lower entry
+// gets its rank-1 update from one thread.  dinv[j] = 1/L[j][j].  A pivot that
+// is not positive and finite sets *bad.
+template <int NB, int NBS, int THREADS>
+__device__ void factor_diag_block(double (*Ld)[NBS], double* dinv, int kb, int* bad) {
+  const int tid = threadIdx.x;
+  for (int j = 0; j < kb; ++j) {
+    const double piv = Ld[j][j];
+    const double d = sqrt(piv);
+    const double inv = 1.0 / d;
+    __syncthreads();  // the pivot is read by all threads before it is replaced
+    if (tid == 0) {
+      if (!(piv > 0.0 && isfinite(piv))) *bad = 1;
+      Ld[j][j] = d;
+      dinv[j] = inv;
     }
+    for (int i = j + 1 + tid; i < kb; i += THREADS) Ld[i][j] *= inv;
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += THREADS) {
+      const int i = idx / NB, k = idx % NB;
+      if (k > j && k <= i && i < kb) Ld[i][k] -= Ld[i][j] * Ld[k][j];
+    }
+    __syncthreads();
   }
-  if (l < kb) {
+}
+
+// One row of the panel solve v = a L11^-T, i.e.
+//   v[c] = (a[c] - sum_{l<c} v[l] L[c][l]) / L[c][c],
+// in chunks of 8 columns held in registers.  prow (the thread's own LDS row)
+// holds a on entry and v on exit.  Ld rows >= kb must be zero.
+template <int NB, int NBS>
+__device__ __forceinline__ void trsm_row(const double (*Ld)[NBS], const double* dinv, int kb, double* prow) {
+  static_assert(NB % 8 == 0, "panel width must be a multiple of 8");
+  for (int c0 = 0; c0 < kb; c0 += 8) {
+    double v[8];
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c <= l) Ld[l][c] = a[c];
+    for (int k = 0; k < 8; ++k) v[k] = c0 + k < kb ? prow[c0 + k] : 0.0;
+    for (int l = 0; l < c0; ++l) {
+      const double pl = prow[l];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] -= pl * Ld[c0 + k][l];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double s = v[k];
+#pragma unroll
+      for (int l = 0; l < k; ++l) s -= v[l] * Ld[c0 + k][c0 + l];
+      v[k] = c0 + k < kb ? s * dinv[c0 + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c0 + k < kb) prow[c0 + k] = v[k];
   }
+}
+
+// Forward substitution of one kb-row block, wave 0 only (lane l owns row l):
+// solves L[k0:k0+kb, k0:k0+kb] yb = y[k0:k0+kb] with the pivots' reciprocals.
+__device__ __forceinline__ double tri_forward_block(const double* __restrict__ A, const double* __restrict__ dinv,
+                                                    int n, int k0, int kb, double yl) {
+  const int l = threadIdx.x;
+  for (int j = 0; j < kb; ++j) {
+    const double yj = __shfl(yl, j) * dinv[k0 + j];
+    if (l == j) yl = yj;
+    else if (l > j && l < kb) yl -= A[(size_t)(k0 + l) * n + k0 + j] * yj;
+  }
+  return yl;
+}
+
+// Backward substitution of one block with L^T, wave 0 only.
+__device__ __forceinline__ double tri_backward_block(const double* __restrict__ A, const double* __restrict__ dinv,
+                                                     int n, int k0, int kb, double yl) {
+  const int l = threadIdx.x;
+  for (int j = kb - 1; j >= 0; --j) {
+    const double yj = __shfl(yl, j) * dinv[k0 + j];
+    if (l == j) yl = yj;
+    else if (l < j) yl -= A[(size_t)(k0 + j) * n + k0 + l] * yj;
+  }
+  return yl;
 }
 
 // ---------------------------------------------------------------------------
@@ -68,12 +110,14 @@ constexpr int kSmallThreads = 256;
 
 __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __restrict__ A,
                                                                     const double* __restrict__ rhs,
-                                                                    double* __restrict__ x, int n, LMState* st) {
+                                                                    double* __restrict__ x, double* __restrict__ tmp,
+                                                                    int n, LMState* st) {
   if (skip_solve(st)) return;
   __shared__ double Ld[kSmallNB][kSmallNB + 1];
   __shared__ double Pn[kSmallMax][kSmallNB + 1];
-  __shared__ double colbuf[kWave];
+  __shared__ double dinv[kSmallMax];
   __shared__ double y[kSmallMax];
+  __shared__ double yb[kSmallNB];
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (tid == 0) bad = 0;
@@ -85,37 +129,21 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __re
       Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
     }
     __syncthreads();
-    if (tid < kWave) {
-      factor_diag_block_wave0<kSmallNB, kSmallNB + 1>(Ld, kb, colbuf, &bad);
-      if (tid < kb) {
-        for (int c = 0; c <= tid; ++c) A[(size_t)(k0 + tid) * n + k0 + c] = Ld[tid][c];
-      }
+    factor_diag_block<kSmallNB, kSmallNB + 1, kSmallThreads>(Ld, dinv + k0, kb, &bad);
+    for (int idx = tid; idx < kSmallNB * kSmallNB; idx += kSmallThreads) {
+      const int r = idx / kSmallNB, c = idx % kSmallNB;
+      if (r < kb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = Ld[r][c];
     }
-    __syncthreads();
+    // rows below the block: L21 = A21 L11^-T
     const int rows = n - k0 - kb;
     for (int i = tid; i < rows; i += kSmallThreads) {
       const int row = k0 + kb + i;
-      double v[kSmallNB];
-#pragma unroll
-      for (int c = 0; c < kSmallNB; ++c) v[c] = c < kb ? A[(size_t)row * n + k0 + c] : 0.0;
-#pragma unroll
-      for (int c = 0; c < kSmallNB; ++c) {
-        if (c < kb) {
-          double s = v[c];
-#pragma unroll
-          for (int l = 0; l < c; ++l) s -= v[l] * Ld[c][l];
-          v[c] = s / Ld[c][c];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < kSmallNB; ++c) {
-        if (c < kb) {
-          A[(size_t)row * n + k0 + c] = v[c];
-          Pn[i][c] = v[c];
-        }
-      }
+      for (int c = 0; c < kb; ++c) Pn[i][c] = A[(size_t)row * n + k0 + c];
+      trsm_row<kSmallNB, kSmallNB + 1>(Ld, dinv + k0, kb, Pn[i]);
+      for (int c = 0; c < kb; ++c) A[(size_t)row * n + k0 + c] = Pn[i][c];
     }
     __syncthreads();
+    // trailing update A22 -= L21 L21^T on 4x4 register tiles of the lower triangle
     const int T = (rows + 3) / 4;
     const int ntiles = T * (T + 1) / 2;
     for (int tt = tid; tt < ntiles; tt += kSmallThreads) {
@@ -152,28 +180,26 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __re
     }
     __syncthreads();
   }
-  // forward substitution L y = rhs
-  for (int i = tid; i < n; i += kSmallThreads) y[i] = rhs[i];
+  // keep the pivots' reciprocals for later triangular solves with this factor
+  for (int i = tid; i < n; i += kSmallThreads) {
+    tmp[n + i] = dinv[i];
+    y[i] = rhs[i];
+  }
   __syncthreads();
+  // forward substitution L y = rhs
   for (int k0 = 0; k0 < n; k0 += kSmallNB) {
     const int kb = min(kSmallNB, n - k0);
     if (tid < kWave) {
-      const int l = tid;
-      double yl = l < kb ? y[k0 + l] : 0.0;
-      for (int j = 0; j < kb; ++j) {
-        if (l == j) {
-          yl /= A[(size_t)(k0 + j) * n + k0 + j];
-          colbuf[j] = yl;
-        }
-        lds_wave_sync();
-        if (l > j && l < kb) yl -= A[(size_t)(k0 + l) * n + k0 + j] * colbuf[j];
+      const double yl = tri_forward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
+      if (tid < kb) {
+        y[k0 + tid] = yl;
+        yb[tid] = yl;
       }
-      if (l < kb) y[k0 + l] = yl;
     }
     __syncthreads();
     for (int i = k0 + kb + tid; i < n; i += kSmallThreads) {
       double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)i * n + k0 + c] * y[k0 + c];
+      for (int c = 0; c < kb; ++c) s += A[(size_t)i * n + k0 + c] * yb[c];
       y[i] -= s;
     }
     __syncthreads();
@@ -183,30 +209,26 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __re
   for (int k0 = last; k0 >= 0; k0 -= kSmallNB) {
     const int kb = min(kSmallNB, n - k0);
     if (tid < kWave) {
-      const int l = tid;
-      double yl = l < kb ? y[k0 + l] : 0.0;
-      for (int j = kb - 1; j >= 0; --j) {
-        if (l == j) {
-          yl /= A[(size_t)(k0 + j) * n + k0 + j];
-          colbuf[j] = yl;
-        }
-        lds_wave_sync();
-        if (l < j) yl -= A[(size_t)(k0 + j) * n + k0 + l] * colbuf[j];
+      const double yl = tri_backward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
+      if (tid < kb) {
+        y[k0 + tid] = yl;
+        yb[tid] = yl;
       }
-      if (l < kb) y[k0 + l] = yl;
     }
     __syncthreads();
     for (int i = tid; i < k0; i += kSmallThreads) {
       double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)(k0 + c) * n + i] * y[k0 + c];
+      for (int c = 0; c < kb; ++c) s += A[(size_t)(k0 + c) * n + i] * yb[c];
       y[i] -= s;
     }
     __syncthreads();
   }
+  int my_bad = 0;
   for (int i = tid; i < n; i += kSmallThreads) {
     x[i] = y[i];
-    if (!isfinite(y[i])) bad = 1;
+    if (!isfinite(y[i])) my_bad = 1;
   }
+  if (my_bad) bad = 1;
   __syncthreads();
   if (tid == 0 && st) {
     if (bad) st->step_invalid = 1;
@@ -221,10 +243,13 @@ constexpr int kNB = 32;
 constexpr int kPanelThreads = 256;
 constexpr int kTile = 64;
 
-__global__ __launch_bounds__(kPanelThreads) void k_chol_panel(double* __restrict__ A, int n, int k0, LMState* st) {
+// Factor the diagonal block of panel k0 in place (one workgroup) and store
+// the pivots' reciprocals.
+__global__ __launch_bounds__(kPanelThreads) void k_chol_diag(double* __restrict__ A, double* __restrict__ dinv_out,
+                                                             int n, int k0, LMState* st) {
   if (skip_solve(st)) return;
   __shared__ double Ld[kNB][kNB + 1];
-  __shared__ double colbuf[kWave];
+  __shared__ double dinv[kNB];
   __shared__ int bad;
   const int tid = threadIdx.x;
   const int kb = min(kNB, n - k0);
@@ -234,32 +259,38 @@ __global__ __launch_bounds__(kPanelThreads) void k_chol_panel(double* __restrict
     Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
   }
   __syncthreads();
-  if (tid < kWave) factor_diag_block_wave0<kNB, kNB + 1>(Ld, kb, colbuf, &bad);
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    for (int idx = tid; idx < kb * kb; idx += kPanelThreads) {
-      const int r = idx / kb, c = idx % kb;
-      if (c <= r) A[(size_t)(k0 + r) * n + k0 + c] = Ld[r][c];
-    }
-    if (tid == 0 && bad && st) st->step_invalid = 1;
+  factor_diag_block<kNB, kNB + 1, kPanelThreads>(Ld, dinv, kb, &bad);
+  for (int idx = tid; idx < kb * kb; idx += kPanelThreads) {
+    const int r = idx / kb, c = idx % kb;
+    if (c <= r) A[(size_t)(k0 + r) * n + k0 + c] = Ld[r][c];
   }
+  if (tid < kb) dinv_out[k0 + tid] = dinv[tid];
+  if (tid == 0 && bad && st) st->step_invalid = 1;
+}
+
+// Rows below the factored diagonal block of panel k0: L21 = A21 L11^-T, one
+// row per thread.
+__global__ __launch_bounds__(kPanelThreads) void k_chol_panel(double* __restrict__ A,
+                                                              const double* __restrict__ dinv_in, int n, int k0,
+                                                              LMState* st) {
+  if (skip_solve(st)) return;
+  __shared__ double Ld[kNB][kNB + 1];
+  __shared__ double P[kPanelThreads][kNB + 1];
+  __shared__ double dinv[kNB];
+  const int tid = threadIdx.x;
+  const int kb = min(kNB, n - k0);
+  for (int idx = tid; idx < kNB * kNB; idx += kPanelThreads) {
+    const int r = idx / kNB, c = idx % kNB;
+    Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+  }
+  if (tid < kNB) dinv[tid] = tid < kb ? dinv_in[k0 + tid] : 0.0;
+  __syncthreads();
   const int row = k0 + kb + blockIdx.x * kPanelThreads + tid;
   if (row < n) {
-    double v[kNB];
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) v[c] = c < kb ? A[(size_t)row * n + k0 + c] : 0.0;
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) {
-      if (c < kb) {
-        double s = v[c];
-#pragma unroll
-        for (int l = 0; l < c; ++l) s -= v[l] * Ld[c][l];
-        v[c] = s / Ld[c][c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kNB; ++c)
-      if (c < kb) A[(size_t)row * n + k0 + c] = v[c];
+    double* prow = P[tid];
+    for (int c = 0; c < kb; ++c) prow[c] = A[(size_t)row * n + k0 + c];
+    trsm_row<kNB, kNB + 1>(Ld, dinv, kb, prow);
+    for (int c = 0; c < kb; ++c) A[(size_t)row * n + k0 + c] = prow[c];
   }
 }
 
@@ -312,15 +343,17 @@ __global__ __launch_bounds__(256) void k_chol_syrk(double* __restrict__ A, int n
   }
 }
 
-// Blocked triangular solves with one workgroup; y lives in global scratch.
+// Blocked triangular solves with one workgroup against the factor in A and
+// the pivots' reciprocals tmp[n:2n]; y lives in tmp[0:n].
 constexpr int kTriThreads = 1024;
 __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restrict__ A,
                                                             const double* __restrict__ rhs, double* __restrict__ x,
-                                                            double* __restrict__ y, int n, LMState* st) {
+                                                            double* __restrict__ tmp, int n, LMState* st) {
   if (skip_solve(st)) return;
-  __shared__ double colbuf[kWave];
   __shared__ double yb[kSmallNB];
   __shared__ int bad;
+  double* __restrict__ y = tmp;
+  const double* __restrict__ dinv = tmp + n;
   const int tid = threadIdx.x;
   if (tid == 0) bad = 0;
   for (int i = tid; i < n; i += kTriThreads) y[i] = rhs[i];
@@ -328,19 +361,10 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
   for (int k0 = 0; k0 < n; k0 += kSmallNB) {
     const int kb = min(kSmallNB, n - k0);
     if (tid < kWave) {
-      const int l = tid;
-      double yl = l < kb ? y[k0 + l] : 0.0;
-      for (int j = 0; j < kb; ++j) {
-        if (l == j) {
-          yl /= A[(size_t)(k0 + j) * n + k0 + j];
-          colbuf[j] = yl;
-        }
-        lds_wave_sync();
-        if (l > j && l < kb) yl -= A[(size_t)(k0 + l) * n + k0 + j] * colbuf[j];
-      }
-      if (l < kb) {
-        y[k0 + l] = yl;
-        yb[l] = yl;
+      const double yl = tri_forward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
+      if (tid < kb) {
+        y[k0 + tid] = yl;
+        yb[tid] = yl;
       }
     }
     __syncthreads();
@@ -355,19 +379,10 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
   for (int k0 = last; k0 >= 0; k0 -= kSmallNB) {
     const int kb = min(kSmallNB, n - k0);
     if (tid < kWave) {
-      const int l = tid;
-      double yl = l < kb ? y[k0 + l] : 0.0;
-      for (int j = kb - 1; j >= 0; --j) {
-        if (l == j) {
-          yl /= A[(size_t)(k0 + j) * n + k0 + j];
-          colbuf[j] = yl;
-        }
-        lds_wave_sync();
-        if (l < j) yl -= A[(size_t)(k0 + j) * n + k0 + l] * colbuf[j];
-      }
-      if (l < kb) {
-        y[k0 + l] = yl;
-        yb[l] = yl;
+      const double yl = tri_backward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
+      if (tid < kb) {
+        y[k0 + tid] = yl;
+        yb[tid] = yl;
       }
     }
     __syncthreads();
@@ -378,10 +393,12 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
     }
     __syncthreads();
   }
+  int my_bad = 0;
   for (int i = tid; i < n; i += kTriThreads) {
     x[i] = y[i];
-    if (!isfinite(y[i])) bad = 1;
+    if (!isfinite(y[i])) my_bad = 1;
   }
+  if (my_bad) bad = 1;
   __syncthreads();
   if (tid == 0 && st) {
     if (bad) st->step_invalid = 1;
@@ -467,6 +484,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   __shared__ double red[kPcgThreads / kWave];
   __shared__ int timed_out;
   __shared__ double sums[4];
+  __shared__ double L[81];
   const int tid = threadIdx.x;
   const int g = blockIdx.x;
   const int nb = n / kBlk;
@@ -476,7 +494,6 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   double rr = 0.0, rz = 0.0, bb = 0.0;
   for (int blk = g; blk < nb; blk += G) {
     if (tid == 0) {
-      double L[81];
       bool ok = true;
       for (int i = 0; i < kBlk; ++i)
         for (int j = 0; j < kBlk; ++j) L[i * kBlk + j] = j <= i ? S[(size_t)(kBlk * blk + i) * n + kBlk * blk + j] : 0.0;
@@ -641,24 +658,25 @@ __global__ void k_reset_barrier(unsigned int* bar) {
 
 }  // namespace
 
-size_t rcs_cholesky_tmp_doubles(int n) { return (size_t)n; }
+size_t rcs_cholesky_tmp_doubles(int n) { return 2 * (size_t)n; }
 
 int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* st, double* tmp,
                               hipStream_t s) {
   if (n <= 0) return CT_OK;
   if (n <= kSmallMax) {
-    hipLaunchKernelGGL(k_chol_solve_small, dim3(1), dim3(kSmallThreads), 0, s, S, rhs, x, n, st);
+    hipLaunchKernelGGL(k_chol_solve_small, dim3(1), dim3(kSmallThreads), 0, s, S, rhs, x, tmp, n, st);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
   for (int k0 = 0; k0 < n; k0 += kNB) {
     const int kb = n - k0 < kNB ? n - k0 : kNB;
     const int rows = n - k0 - kb;
-    int grid = (rows + kPanelThreads - 1) / kPanelThreads;
-    if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_chol_panel, dim3(grid), dim3(kPanelThreads), 0, s, S, n, k0, st);
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(kPanelThreads), 0, s, S, tmp + n, n, k0, st);
     CT_LAUNCH_CHECK();
     if (rows > 0) {
+      const int grid = (rows + kPanelThreads - 1) / kPanelThreads;
+      hipLaunchKernelGGL(k_chol_panel, dim3(grid), dim3(kPanelThreads), 0, s, S, tmp + n, n, k0, st);
+      CT_LAUNCH_CHECK();
       const int T = (rows + kTile - 1) / kTile;
       hipLaunchKernelGGL(k_chol_syrk, dim3(T * (T + 1) / 2), dim3(256), 0, s, S, n, k0, kb, st);
       CT_LAUNCH_CHECK();

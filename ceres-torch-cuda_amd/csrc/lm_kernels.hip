@@ -1,7 +1,8 @@
 // Device-resident Levenberg-Marquardt bookkeeping (solver.py:35-357).
 //
 // An LM iteration t is the kernel sequence
-//   eval_points -> camera_blocks -> lm_begin -> RCS solve -> candidate -> lm_end
+//   eval (observations, points) -> camera_blocks -> lm_begin -> RCS solve
+//   -> step (points) -> candidate (observations) -> lm_end
 // lm_begin reduces the cost/gradient partials of the evaluation at x and runs
 // the checks that solver.py performs at the top of an iteration and, deferred
 // by one evaluation, the gradient and radius checks that follow an accepted
@@ -99,10 +100,10 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   LMState* st = w.state;
   if (st->done) return;
   double c = 0.0, g2 = 0.0, gm = 0.0;
+  for (int i = threadIdx.x; i < init.num_obs_blocks; i += kLMThreads) c += w.part_cost[i];
   for (int i = threadIdx.x; i < init.num_eval_blocks; i += kLMThreads) {
-    c += w.part_eval[3 * i];
-    g2 += w.part_eval[3 * i + 1];
-    gm = fmax(gm, w.part_eval[3 * i + 2]);
+    g2 += w.part_eval[2 * i];
+    gm = fmax(gm, w.part_eval[2 * i + 1]);
   }
   for (int i = threadIdx.x; i < init.num_cameras; i += kLMThreads) {
     g2 += w.part_cam[2 * i];
@@ -170,8 +171,12 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
 __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
   __shared__ double red[kLMThreads / kWave];
   LMState* st = w.state;
-  if (st->done) return;
-  if (st->skip_step) {
+  // every thread reads the flags before thread 0 may change them
+  const int done = st->done;
+  const int skip = st->skip_step;
+  __syncthreads();
+  if (done) return;
+  if (skip) {
     if (threadIdx.x == 0) {
       st->skip_step = 0;
       st->iteration += 1;
@@ -179,9 +184,14 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
     return;
   }
   double a[5] = {0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < init.num_cand_blocks; i += kLMThreads) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) a[k] += w.part_cand[5 * i + k];
+  for (int i = threadIdx.x; i < init.num_obs_blocks; i += kLMThreads) {
+    a[0] += w.part_cand[2 * i];
+    a[1] += w.part_cand[2 * i + 1];
+  }
+  for (int i = threadIdx.x; i < init.num_step_blocks; i += kLMThreads) {
+    a[2] += w.part_step[3 * i];
+    a[3] += w.part_step[3 * i + 1];
+    a[4] += w.part_step[3 * i + 2];
   }
   double s[5];
 #pragma unroll

@@ -177,6 +177,7 @@ typedef struct ct_ba_problem {
   int32_t num_blocks;              /* camera pair blocks (i <= j) */
   int64_t num_terms;               /* observation pairs over all blocks */
   const int32_t* obs_camera;       /* [N] camera of each observation */
+  const int32_t* obs_point;        /* [N] point of each observation (non-decreasing) */
   const int32_t* point_offsets;    /* [P+1] observations of point p */
   const double* observations;      /* [N*2] observed pixels */
   const int32_t* camera_offsets;   /* [C+1] CSR of observations by camera */
@@ -212,12 +213,12 @@ int ct_ba_plan_destroy(ct_ba_plan* plan);
 
 /* Kernel groups of one LM iteration, in launch order. */
 enum {
-  CT_KERNEL_EVAL_POINTS = 0,    /* evaluation + point elimination */
+  CT_KERNEL_EVAL_POINTS = 0,    /* evaluation (per observation) + point elimination */
   CT_KERNEL_CAMERA_BLOCKS = 1,  /* reduced camera system assembly */
   CT_KERNEL_LM_BEGIN = 2,       /* cost/gradient reduction + checks */
   CT_KERNEL_RCS_SOLVE = 3,      /* Cholesky (or PCG) of S */
   CT_KERNEL_REFINE = 4,         /* mixed: back-substitution + iterative refinement */
-  CT_KERNEL_CANDIDATE = 5,      /* (back-substitution +) candidate cost */
+  CT_KERNEL_CANDIDATE = 5,      /* step application + candidate cost */
   CT_KERNEL_LM_END = 6,         /* gain ratio, radius, accept/reject */
   CT_KERNEL_COUNT = 7
 };

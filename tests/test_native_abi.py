@@ -37,7 +37,7 @@ def _problem_struct(C=3, P=4, N=8):
     p.num_blocks = C * (C + 1) // 2
     p.num_terms = 10
     fake = ctypes.c_void_p(0x1000)
-    for f in ("obs_camera", "point_offsets", "observations", "camera_offsets", "camera_obs", "block_cameras",
+    for f in ("obs_camera", "obs_point", "point_offsets", "observations", "camera_offsets", "camera_obs", "block_cameras",
               "block_offsets", "block_terms"):
         setattr(p, f, fake)
     p.loss = nat.ct_loss(0, 0.0, 0.0, 1.0)
