@@ -51,8 +51,18 @@ int step_blocks(const ct_ba_problem* p) {
   return (work + kPointThreads - 1) / kPointThreads;
 }
 
+// Upper bound of the camera-block work items: a block (or a camera's
+// observation list) of t entries is split into ceil(t / kItemMaxWork) items.
+size_t max_items(const ct_ba_problem* p) {
+  const size_t W = kItemMaxWork;
+  return (size_t)p->num_blocks + (size_t)p->num_cameras + ((size_t)p->num_terms + W - 1) / W +
+         ((size_t)p->num_observations + W - 1) / W + 1;
+}
+
 struct Layout {
   BAWork w{};
+  BlockItem* items = nullptr;
+  int32_t* block_item_offsets = nullptr;
   BABuffers<float> bf{};
   BABuffers<double> bd{};
   double* pcg_work = nullptr;
@@ -87,6 +97,12 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.dxc_corr = cv.take<double>(n);
   L.w.solve_tmp = cv.take<double>(rcs_cholesky_tmp_doubles((int)n) + 1);
   L.w.status = cv.take<int32_t>(4);
+  const size_t I = max_items(p);
+  L.items = cv.take<BlockItem>(I);
+  L.block_item_offsets = cv.take<int32_t>((size_t)p->num_blocks + 1);
+  L.w.items = L.items;
+  L.w.block_item_offsets = L.block_item_offsets;
+  L.w.item_vec = cv.take<double>(27 * I);
   if (o->precision == CT_MIXED) {
     L.bf.Jc = cv.take<float>(18 * N);
     L.bf.Jp = cv.take<float>(6 * N);
@@ -96,6 +112,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
     L.bf.Linv = cv.take<float>(6 * P);
     L.bf.q = cv.take<float>(3 * P);
     L.bf.gp = cv.take<double>(3 * P);
+    L.bf.item_S = cv.take<float>(81 * I);
   } else {
     L.bd.Jc = cv.take<double>(18 * N);
     L.bd.Jp = cv.take<double>(6 * N);
@@ -105,6 +122,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
     L.bd.Linv = cv.take<double>(6 * P);
     L.bd.q = cv.take<double>(3 * P);
     L.bd.gp = cv.take<double>(3 * P);
+    L.bd.item_S = cv.take<double>(81 * I);
   }
   if (o->linear_solver == CT_ITERATIVE_SCHUR) {
     L.pcg_doubles = rcs_pcg_work_doubles((int)n);
@@ -158,6 +176,7 @@ struct ct_ba_plan {
   int graph_iterations = 0;
   LMState* host_state = nullptr;  // pinned
   double wall_rate_hz = 1e8;
+  int num_items = 0;
 };
 
 namespace {
@@ -177,7 +196,8 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
     case CT_KERNEL_EVAL_POINTS:
       return mixed ? launch_eval_points<float>(v, L.bf, L.w, s) : launch_eval_points<double>(v, L.bd, L.w, s);
     case CT_KERNEL_CAMERA_BLOCKS:
-      return mixed ? launch_camera_blocks<float>(v, L.bf, L.w, s) : launch_camera_blocks<double>(v, L.bd, L.w, s);
+      return mixed ? launch_camera_blocks<float>(v, L.bf, L.w, pl->num_items, s)
+                   : launch_camera_blocks<double>(v, L.bd, L.w, pl->num_items, s);
     case CT_KERNEL_LM_BEGIN:
       return launch_lm_begin(pl->init, L.w, s);
     case CT_KERNEL_RCS_SOLVE:
@@ -210,6 +230,102 @@ int enqueue_iteration(ct_ba_plan* pl, hipStream_t s) {
 int read_state(ct_ba_plan* pl, hipStream_t s) {
   CT_HIP(hipMemcpyAsync(pl->host_state, pl->layout.w.state, sizeof(LMState), hipMemcpyDeviceToHost, s));
   CT_HIP(hipStreamSynchronize(s));
+  return CT_OK;
+}
+
+template <typename T>
+int copy_back(std::vector<T>& h, const T* d, size_t count) {
+  h.resize(count);
+  if (count) CT_HIP(hipMemcpy(h.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost));
+  return CT_OK;
+}
+
+// Every index the kernels follow is checked once on the host, so a malformed
+// table is an argument error instead of an out-of-bounds access.
+int validate_point_indices(const ct_ba_problem& p) {
+  const int64_t C = p.num_cameras, P = p.num_points, N = p.num_observations;
+  if (N == 0) return CT_OK;
+  std::vector<int32_t> oc, op, po;
+  CT_TRY(copy_back(oc, p.obs_camera, N));
+  CT_TRY(copy_back(op, p.obs_point, N));
+  CT_TRY(copy_back(po, p.point_offsets, P + 1));
+  CT_CHECK_ARG(po[0] == 0 && po[P] == N, "point_offsets must start at 0 and end at num_observations");
+  for (int64_t i = 0; i < P; ++i) CT_CHECK_ARG(po[i] <= po[i + 1], "point_offsets must be non-decreasing");
+  for (int64_t o = 0; o < N; ++o) {
+    CT_CHECK_ARG(oc[o] >= 0 && oc[o] < C, "obs_camera out of range");
+    CT_CHECK_ARG(op[o] >= 0 && op[o] < P && o >= po[op[o]] && o < po[op[o] + 1],
+                 "obs_point must agree with point_offsets (observations grouped by point)");
+  }
+  return CT_OK;
+}
+
+int validate_indices(const ct_ba_problem& p, const std::vector<int32_t>& camera_offsets) {
+  CT_TRY(validate_point_indices(p));
+  const int64_t C = p.num_cameras, N = p.num_observations, T = p.num_terms;
+  if (N == 0) return CT_OK;
+  std::vector<int32_t> oc, cobs, terms;
+  CT_TRY(copy_back(oc, p.obs_camera, N));
+  CT_TRY(copy_back(cobs, p.camera_obs, N));
+  CT_TRY(copy_back(terms, p.block_terms, 2 * T));
+  CT_CHECK_ARG(camera_offsets[0] == 0 && camera_offsets[C] == N,
+               "camera_offsets must start at 0 and end at num_observations");
+  for (int64_t c = 0; c < C; ++c)
+    CT_CHECK_ARG(camera_offsets[c] <= camera_offsets[c + 1], "camera_offsets must be non-decreasing");
+  for (int64_t o = 0; o < N; ++o) CT_CHECK_ARG(cobs[o] >= 0 && cobs[o] < N, "camera_obs out of range");
+  for (int64_t c = 0; c < C; ++c)
+    for (int32_t t = camera_offsets[c]; t < camera_offsets[c + 1]; ++t)
+      CT_CHECK_ARG(oc[cobs[t]] == c, "camera_obs must list each camera's observations");
+  for (int64_t t = 0; t < 2 * T; ++t) CT_CHECK_ARG(terms[t] >= 0 && terms[t] < N, "block_terms out of range");
+  return CT_OK;
+}
+
+void split_range(std::vector<BlockItem>& out, int block, int kind, int begin, int end) {
+  const int count = end - begin;
+  if (count <= 0) return;
+  const int pieces = (count + kItemMaxWork - 1) / kItemMaxWork;
+  const int chunk = (count + pieces - 1) / pieces;
+  for (int b = begin; b < end; b += chunk) out.push_back(BlockItem{block, kind, b, end < b + chunk ? end : b + chunk});
+}
+
+// Split every camera-pair block into work items of at most kItemMaxWork terms
+// or observations (balanced within a block) and upload the list.  Reads the
+// block and camera offsets back from the device once.
+int build_items(ct_ba_plan* pl) {
+  const ct_ba_problem& p = pl->problem;
+  const int B = p.num_blocks, C = p.num_cameras;
+  std::vector<int32_t> bo(B + 1, 0), co(C + 1, 0), bc(2 * (size_t)B + 1, 0);
+  if (B > 0) {
+    CT_HIP(hipMemcpy(bo.data(), p.block_offsets, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+    CT_HIP(hipMemcpy(bc.data(), p.block_cameras, sizeof(int32_t) * 2 * (size_t)B, hipMemcpyDeviceToHost));
+  }
+  if (C > 0 && p.num_observations > 0)
+    CT_HIP(hipMemcpy(co.data(), p.camera_offsets, sizeof(int32_t) * (C + 1), hipMemcpyDeviceToHost));
+  CT_TRY(validate_indices(p, co));
+  std::vector<BlockItem> items;
+  std::vector<int32_t> offsets(B + 1, 0);
+  for (int blk = 0; blk < B; ++blk) {
+    offsets[blk] = (int32_t)items.size();
+    const int ci = bc[2 * (size_t)blk], cj = bc[2 * (size_t)blk + 1];
+    if (ci < 0 || ci >= C || cj < ci || cj >= C) {
+      set_error("block_cameras must hold pairs i <= j of valid cameras");
+      return CT_ERR_ARGUMENT;
+    }
+    if (bo[blk + 1] < bo[blk] || bo[blk] < 0 || bo[blk + 1] > p.num_terms) {
+      set_error("block_offsets must be non-decreasing within [0, num_terms]");
+      return CT_ERR_ARGUMENT;
+    }
+    if (ci == cj) split_range(items, blk, 1, co[ci], co[ci + 1]);
+    split_range(items, blk, 0, bo[blk], bo[blk + 1]);
+  }
+  offsets[B] = (int32_t)items.size();
+  if (items.size() > max_items(&p)) {
+    set_error("camera-block item bound exceeded");
+    return CT_ERR_ARGUMENT;
+  }
+  pl->num_items = (int)items.size();
+  if (!items.empty())
+    CT_HIP(hipMemcpy(pl->layout.items, items.data(), sizeof(BlockItem) * items.size(), hipMemcpyHostToDevice));
+  CT_HIP(hipMemcpy(pl->layout.block_item_offsets, offsets.data(), sizeof(int32_t) * (B + 1), hipMemcpyHostToDevice));
   return CT_OK;
 }
 
@@ -282,6 +398,7 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   int per = options->iterations_per_graph > 0 ? options->iterations_per_graph : 8;
   if (per > options->max_num_iterations + 2) per = options->max_num_iterations + 2;
   pl->graph_iterations = per;
+  CT_TRY(build_items(pl.get()));
   CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pl->host_state), sizeof(LMState), hipHostMallocDefault));
   if (options->loop_mode == CT_LOOP_DEVICE) {
     CT_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
@@ -415,6 +532,7 @@ int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const do
   const int grid = eval_blocks(problem);
   CT_CHECK_ARG(workspace_bytes >= sizeof(double) * (size_t)(grid + 1), "workspace too small for partials");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  CT_TRY(validate_point_indices(*problem));
   BAView v = make_view(problem);
   double* part = static_cast<double*>(workspace);
   CT_TRY(launch_evaluate_only(v, cameras, points, residuals, part, s));

@@ -57,6 +57,15 @@ struct BAView {
   ct_loss loss;
 };
 
+// One unit of camera-block assembly work: a bounded slice of one camera-pair
+// block.  kind 0: Schur terms [begin, end) of block_terms; kind 1: camera
+// observations [begin, end) of camera_obs (diagonal blocks: U, rhs, g_c and
+// the column norms).
+struct BlockItem {
+  int32_t block, kind, begin, end;
+};
+constexpr int kItemMaxWork = 224;  // terms or observations per item (7 lane groups x 32)
+
 // Per-observation and per-point buffers written by the evaluation kernel.
 template <typename real>
 struct BABuffers {
@@ -68,6 +77,7 @@ struct BABuffers {
   real* Linv;      // [P][6]    L^-1, lower triangle row-major
   real* q;         // [P][3]    L^-1 g_p
   double* gp;      // [P][3]    gradient point block (FP64 accumulation)
+  real* item_S;    // [items][81] partial 9x9 camera blocks
 };
 
 struct BAWork {
@@ -93,8 +103,12 @@ struct BAWork {
   double* res_p;      // [P*3] L^-1 rho_p
   double* u_obs;      // [N*4] J dx and P L^-1 rho_p per observation
   double* dxc_corr;   // [n]   camera correction
+  // camera-block work items (built at plan creation)
+  const BlockItem* items;            // [num_items], grouped by block
+  const int32_t* block_item_offsets; // [B+1]
+  double* item_vec;   // [items][27] rhs, g_c, column norms (kind 1 items)
   // dense solve scratch
-  double* solve_tmp;  // [n]
+  double* solve_tmp;  // [2n]
   int32_t* status;    // [4] device status words
 };
 

@@ -8,7 +8,7 @@ namespace ct {
 template <typename real>
 int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
-int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s);
 template <typename real>
 int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
 template <typename real>

@@ -328,39 +328,37 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
 }
 
 // ---------------------------------------------------------------------------
-// K2: one wavefront per camera pair block (i <= j).
-//   S_ij = d_ij (U_i + D_i) - sum_{(a,b)} Jc_a^T (P_a P_b^T) Jc_b
-// 63 lanes form 7 groups of 3x3 sub-block owners; group g takes terms
-// g, g+7, ...; the 7 partial blocks are summed in LDS in a fixed order.
-// Diagonal blocks also produce rhs_i = sum Jc^T e, g_c = sum Jc^T r and the
-// column norms that set D_i, from the same pass over the camera's
-// observations.
+// K2a: one wavefront per work item (a bounded slice of one camera-pair block
+// (i, j), i <= j).  63 lanes form 7 groups of 3x3 sub-block owners; group g
+// takes terms g, g+7, ... of the slice; the 7 partial blocks are summed in
+// LDS in a fixed order and written as the item's partial:
+//   kind 0 (Schur terms):  - sum Jc_a^T (P_a P_b^T) Jc_b
+//   kind 1 (observations of camera i, diagonal blocks only):
+//                          sum Jc^T Jc, and rhs = sum Jc^T e, g_c = sum Jc^T r,
+//                          column norms diag(sum Jc^T Jc) in FP64.
+// Splitting blocks into items bounds the serial work per wavefront: a
+// diagonal block covers every observation of its camera.
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ real red[7][81];
-  __shared__ double vec[7][27];  // per group: rhs (9), g_c (9), colsq (9)
+  __shared__ double vec[7][27];  // per group: rhs (9), g_c (9), column norms (9)
   const LMState* st = w.state;
   if (st->done) return;
-  const int blk = blockIdx.x;
-  const int ci = v.block_cameras[2 * blk];
-  const int cj = v.block_cameras[2 * blk + 1];
-  const int n = 9 * v.C;
+  const BlockItem it = w.items[blockIdx.x];
   const int tid = threadIdx.x;
   const int grp = tid / 9;
   const int br = (tid % 9) / 3;
   const int bc = (tid % 9) % 3;
-  const bool diag = ci == cj;
   real acc[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[r][c] = 0;
-  double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
   if (grp < 7) {
-    if (diag) {
-      const int s0 = v.camera_offsets[ci], s1 = v.camera_offsets[ci + 1];
-      for (int t = s0 + grp; t < s1; t += 7) {
+    if (it.kind == 1) {
+      double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+      for (int t = it.begin + grp; t < it.end; t += 7) {
         const int o = v.camera_obs[t];
         const real* J = b.Jc + 18 * (size_t)o;
         real ja0[3], ja1[3], jb0[3], jb1[3];
@@ -385,7 +383,6 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
           }
         }
       }
-      // diagonal of U_i (column norms of the camera's Jacobian columns)
       if (br == bc) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) vec[grp][18 + 3 * br + r] = (double)acc[r][r];
@@ -397,32 +394,33 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
           vec[grp][9 + 3 * br + r] = gv[r];
         }
       }
-    }
-    const int t0 = v.block_offsets[blk], t1 = v.block_offsets[blk + 1];
-    for (int t = t0 + grp; t < t1; t += 7) {
-      const int oa = v.block_terms[2 * (size_t)t];
-      const int ob = v.block_terms[2 * (size_t)t + 1];
-      const real* Pa = b.Pf + 6 * (size_t)oa;
-      const real* Pb = b.Pf + 6 * (size_t)ob;
-      const real m00 = Pa[0] * Pb[0] + Pa[1] * Pb[1] + Pa[2] * Pb[2];
-      const real m01 = Pa[0] * Pb[3] + Pa[1] * Pb[4] + Pa[2] * Pb[5];
-      const real m10 = Pa[3] * Pb[0] + Pa[4] * Pb[1] + Pa[5] * Pb[2];
-      const real m11 = Pa[3] * Pb[3] + Pa[4] * Pb[4] + Pa[5] * Pb[5];
-      const real* Ja = b.Jc + 18 * (size_t)oa;
-      const real* Jb = b.Jc + 18 * (size_t)ob;
-      real ja0[3], ja1[3], t0v[3], t1v[3];
+    } else {
+#pragma unroll 2
+      for (int t = it.begin + grp; t < it.end; t += 7) {
+        const int oa = v.block_terms[2 * (size_t)t];
+        const int ob = v.block_terms[2 * (size_t)t + 1];
+        const real* Pa = b.Pf + 6 * (size_t)oa;
+        const real* Pb = b.Pf + 6 * (size_t)ob;
+        const real m00 = Pa[0] * Pb[0] + Pa[1] * Pb[1] + Pa[2] * Pb[2];
+        const real m01 = Pa[0] * Pb[3] + Pa[1] * Pb[4] + Pa[2] * Pb[5];
+        const real m10 = Pa[3] * Pb[0] + Pa[4] * Pb[1] + Pa[5] * Pb[2];
+        const real m11 = Pa[3] * Pb[3] + Pa[4] * Pb[4] + Pa[5] * Pb[5];
+        const real* Ja = b.Jc + 18 * (size_t)oa;
+        const real* Jb = b.Jc + 18 * (size_t)ob;
+        real ja0[3], ja1[3], t0v[3], t1v[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const real b0 = Jb[3 * bc + c], b1 = Jb[9 + 3 * bc + c];
-        t0v[c] = m00 * b0 + m01 * b1;
-        t1v[c] = m10 * b0 + m11 * b1;
-        ja0[c] = Ja[3 * br + c];
-        ja1[c] = Ja[9 + 3 * br + c];
+        for (int c = 0; c < 3; ++c) {
+          const real b0 = Jb[3 * bc + c], b1 = Jb[9 + 3 * bc + c];
+          t0v[c] = m00 * b0 + m01 * b1;
+          t1v[c] = m10 * b0 + m11 * b1;
+          ja0[c] = Ja[3 * br + c];
+          ja1[c] = Ja[9 + 3 * br + c];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[r][c] -= ja0[r] * t0v[c] + ja1[r] * t1v[c];
       }
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[r][c] -= ja0[r] * t0v[c] + ja1[r] * t1v[c];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -430,41 +428,82 @@ __global__ __launch_bounds__(kWave) void k_camera_blocks(BAView v, BABuffers<rea
       for (int c = 0; c < 3; ++c) red[grp][(3 * br + r) * 9 + 3 * bc + c] = acc[r][c];
   }
   __syncthreads();
-  const double radius = st->radius;
-  if (diag && tid < 9) {
-    double rs = 0.0, gs = 0.0, cs = 0.0;
-    for (int gi = 0; gi < 7; ++gi) {
-      rs += vec[gi][tid];
-      gs += vec[gi][9 + tid];
-      cs += vec[gi][18 + tid];
+  real* out = b.item_S + 81 * (size_t)blockIdx.x;
+  for (int idx = tid; idx < 81; idx += kWave) {
+    real s = 0;
+    for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
+    out[idx] = s;
+  }
+  if (it.kind == 1 && tid < 27) {
+    double s = 0.0;
+    for (int gi = 0; gi < 7; ++gi) s += vec[gi][tid];
+    w.item_vec[27 * (size_t)blockIdx.x + tid] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2b: one workgroup per camera-pair block: sum the block's item partials in
+// item order and write
+//   S_ij = d_ij (U_i + D_i) - sum Jc_a^T (P_a P_b^T) Jc_b   (both triangles)
+// and, for diagonal blocks, rhs_i, g_c, D_i (linear.py:539-551) and the
+// camera gradient norm partials.
+// ---------------------------------------------------------------------------
+constexpr int kReduceThreads = 128;
+
+template <typename real>
+__global__ __launch_bounds__(kReduceThreads) void k_camera_reduce(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ real sblk[81];
+  __shared__ double svec[27];
+  __shared__ double sD[9];
+  const LMState* st = w.state;
+  if (st->done) return;
+  const int blk = blockIdx.x;
+  const int ci = v.block_cameras[2 * blk];
+  const int cj = v.block_cameras[2 * blk + 1];
+  const bool diag = ci == cj;
+  const int n = 9 * v.C;
+  const int i0 = w.block_item_offsets[blk], i1 = w.block_item_offsets[blk + 1];
+  const int tid = threadIdx.x;
+  if (tid < 81) {
+    real s = 0;
+    for (int k = i0; k < i1; ++k) s += b.item_S[81 * (size_t)k + tid];
+    sblk[tid] = s;
+  } else if (tid < 81 + 27) {
+    double s = 0.0;
+    if (diag) {
+      for (int k = i0; k < i1; ++k)
+        if (w.items[k].kind == 1) s += w.item_vec[27 * (size_t)k + tid - 81];
     }
-    const int k = 9 * ci + tid;
-    w.rhs[k] = rs;
-    w.gc[k] = gs;
-    w.Dc[k] = damping_of(cs, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    vec[0][tid] = gs;  // gradient entries for the norms below (group 0 slot is free now)
+    svec[tid - 81] = s;
   }
   __syncthreads();
-  for (int idx = tid; idx < 81; idx += kWave) {
+  if (diag && tid < 9) {
+    const int k = 9 * ci + tid;
+    const double d = damping_of(svec[18 + tid], st->min_diag, st->max_diag, st->radius, st->jacobi_scaling);
+    w.rhs[k] = svec[tid];
+    w.gc[k] = svec[9 + tid];
+    w.Dc[k] = d;
+    sD[tid] = d;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 81; idx += kReduceThreads) {
     const int r = idx / 9, c = idx % 9;
-    real s = 0;
     if (diag) {
       const int src = r >= c ? idx : c * 9 + r;  // symmetric: lower triangle value
-      for (int gi = 0; gi < 7; ++gi) s += red[gi][src];
-      double val = (double)s;
-      if (r == c) val += w.Dc[9 * ci + r];
+      double val = (double)sblk[src];
+      if (r == c) val += sD[r];
       w.S[(size_t)(9 * ci + r) * n + 9 * ci + c] = val;
     } else {
-      for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
-      w.S[(size_t)(9 * ci + r) * n + 9 * cj + c] = (double)s;
-      w.S[(size_t)(9 * cj + c) * n + 9 * ci + r] = (double)s;
+      const double val = (double)sblk[idx];
+      w.S[(size_t)(9 * ci + r) * n + 9 * cj + c] = val;
+      w.S[(size_t)(9 * cj + c) * n + 9 * ci + r] = val;
     }
   }
   if (diag && tid == 0) {
     double g2 = 0.0, gm = 0.0;
     for (int j = 0; j < 9; ++j) {
-      g2 += vec[0][j] * vec[0][j];
-      gm = fmax(gm, fabs(vec[0][j]));
+      g2 += svec[9 + j] * svec[9 + j];
+      gm = fmax(gm, fabs(svec[9 + j]));
     }
     w.part_cam[2 * ci] = g2;
     w.part_cam[2 * ci + 1] = gm;
@@ -777,9 +816,13 @@ int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& 
 }
 
 template <typename real>
-int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s) {
+  if (num_items > 0) {
+    hipLaunchKernelGGL(k_camera_items<real>, dim3(num_items), dim3(kWave), 0, s, v, b, w);
+    CT_LAUNCH_CHECK();
+  }
   if (v.B > 0) {
-    hipLaunchKernelGGL(k_camera_blocks<real>, dim3(v.B), dim3(kWave), 0, s, v, b, w);
+    hipLaunchKernelGGL(k_camera_reduce<real>, dim3(v.B), dim3(kReduceThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -843,8 +886,8 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 
 template int launch_eval_points<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_eval_points<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
-template int launch_camera_blocks<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
-template int launch_camera_blocks<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_camera_blocks<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
+template int launch_camera_blocks<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
 template int launch_candidate<float>(const BAView&, const BABuffers<float>&, const BAWork&, bool, hipStream_t);
 template int launch_candidate<double>(const BAView&, const BABuffers<double>&, const BAWork&, bool, hipStream_t);
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);

@@ -165,8 +165,9 @@ class BAStructure:
     point_offsets: np.ndarray      # [P+1]
     camera_active: np.ndarray      # [C] bool
     point_active: np.ndarray       # [P] bool
-    pair_a: np.ndarray             # ordered observation pairs of a common point
-    pair_b: np.ndarray
+    pair_a: np.ndarray             # ordered observation pairs of a common point,
+    pair_b: np.ndarray             # sorted (stably) by camera block
+    pair_key: np.ndarray           # cam(a) * C + cam(b), non-decreasing
 
     @classmethod
     def build(cls, obs_camera, obs_point, num_cameras, num_points, camera_active=None, point_active=None):
@@ -189,6 +190,10 @@ class BAStructure:
             pb.append((base[:, None] + jj.reshape(1, -1)).reshape(-1))
         pair_a = np.concatenate(pa) if pa else np.zeros(0, np.int64)
         pair_b = np.concatenate(pb) if pb else np.zeros(0, np.int64)
+        cam = np.asarray(obs_camera, dtype=np.int64)
+        key = cam[pair_a] * num_cameras + cam[pair_b]
+        order = np.argsort(key, kind="stable")
+        pair_a, pair_b, key = pair_a[order], pair_b[order], key[order]
         return cls(
             num_cameras=num_cameras,
             num_points=num_points,
@@ -199,6 +204,7 @@ class BAStructure:
             point_active=np.ones(num_points, bool) if point_active is None else np.asarray(point_active, bool),
             pair_a=pair_a,
             pair_b=pair_b,
+            pair_key=key,
         )
 
 
@@ -276,28 +282,26 @@ def schur_system(struct: BAStructure, ev: BAEvaluation, Dp: np.ndarray, Dc: np.n
     # constant points have zero Jacobian columns; a unit diagonal keeps their
     # (unused) 3x3 block invertible without touching any active quantity
     Dp = np.where(struct.point_active[:, None], Dp, 1.0)
-    V = _segment_sum(np.einsum("nki,nkj->nij", ev.Jp, ev.Jp), struct.point_offsets)
+    V = _segment_sum(np.matmul(ev.Jp.transpose(0, 2, 1), ev.Jp), struct.point_offsets)
     V = V + Dp[:, :, None] * np.eye(3)[None]
     ga = -_segment_sum(np.einsum("nij,ni->nj", ev.Jp, ev.r), struct.point_offsets)
     gb = np.zeros((C, 9))
     np.add.at(gb, struct.obs_camera, -np.einsum("nij,ni->nj", ev.Jc, ev.r))
     U = np.zeros((C, 9, 9))
-    np.add.at(U, struct.obs_camera, np.einsum("nki,nkj->nij", ev.Jc, ev.Jc))
+    np.add.at(U, struct.obs_camera, np.matmul(ev.Jc.transpose(0, 2, 1), ev.Jc))
     U = U + Dc[:, :, None] * np.eye(9)[None]
     Vinv = np.linalg.inv(V)
-    W = np.einsum("nki,nkj->nij", ev.Jc, ev.Jp)               # [N, 9, 3]
-    Y = np.einsum("nij,njk->nik", W, Vinv[struct.obs_point])   # W V^-1
+    W = np.matmul(ev.Jc.transpose(0, 2, 1), ev.Jp)              # [N, 9, 3]
+    Y = np.matmul(W, Vinv[struct.obs_point])                    # W V^-1
+    Wt = np.ascontiguousarray(W.transpose(0, 2, 1))             # [N, 3, 9]
     S = np.zeros((C * C, 9, 9))
     for s in range(0, struct.pair_a.shape[0], chunk):
         a = struct.pair_a[s : s + chunk]
         b = struct.pair_b[s : s + chunk]
-        blk = struct.obs_camera[a] * C + struct.obs_camera[b]
-        contrib = np.einsum("tik,tjk->tij", Y[a], W[b])
-        order = np.argsort(blk, kind="stable")
-        blk_s = blk[order]
-        starts = np.nonzero(np.r_[True, blk_s[1:] != blk_s[:-1]])[0]
-        sums = np.add.reduceat(contrib[order], starts, axis=0)
-        S[blk_s[starts]] -= sums
+        blk = struct.pair_key[s : s + chunk]
+        contrib = np.matmul(Y[a], Wt[b])
+        starts = np.nonzero(np.r_[True, blk[1:] != blk[:-1]])[0]
+        S[blk[starts]] -= np.add.reduceat(contrib, starts, axis=0)
     S = S.reshape(C, C, 9, 9).transpose(0, 2, 1, 3).reshape(9 * C, 9 * C)
     for i in range(C):
         S[9 * i : 9 * i + 9, 9 * i : 9 * i + 9] += U[i]

@@ -147,13 +147,22 @@ def test_evaluate_matches_oracle_cost_and_residuals():
 
 
 def test_mixed_precision_reaches_fp64_final_cost():
+    """Mixed form (FP32 J / J^T J / Schur, FP64 Cholesky, cost, gain ratio, 2 refinement steps) vs plain FP64.
+
+    Both are run to convergence (function tolerance 0: the solve ends at the
+    minimum trust-region radius); the Huber problem with outliers descends
+    slowly, so a looser tolerance stops the two runs at different points of
+    the same descent.  Bar: final cost within 1e-6 relative.
+    """
     data = gen.ba_config("ba16_huber")
-    o = tc.SolverOptions(max_num_iterations=100, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
-                         function_tolerance=1e-12, gradient_tolerance=0.0, parameter_tolerance=1e-14)
-    problem, _, pts64 = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=2000, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
+                         function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+    problem, _, _ = ba_problem(data)
     s64 = tc.solve(o, problem)
     o2 = tc.SolverOptions(**{**o.__dict__, "use_mixed_precision_solves": True, "max_num_refinement_iterations": 2})
-    problem2, _, pts32 = ba_problem(data)
+    problem2, _, _ = ba_problem(data)
     s32 = tc.solve(o2, problem2)
+    assert s64.termination_type == tc.TerminationType.CONVERGENCE
+    assert s32.termination_type == tc.TerminationType.CONVERGENCE
     assert s32.final_cost == pytest.approx(s64.final_cost, rel=1e-6)
     assert s32.precision == "mixed"
