@@ -141,29 +141,54 @@ def time_to_tol(dev, data, *, precision, loop, linear, max_iters, target):
 
 
 def profile_kernels(dev, data, *, precision, linear, iters):
+    """Per-kernel-group device time (HIP events on the library's stream), host-driven loop.
+
+    Returns ({group: (ms per launch, launches)}, mean linear-solver iterations
+    per timed LM iteration).  The host-driven loop launches no kernel that
+    returns early (no evaluation-only step groups, nothing after convergence),
+    so a rocprofv3 kernel trace of the same run averages the same launches.
+    """
     import ctypes
 
-    import numpy as np
     import torch
 
     import ceres_mi355x as tc
     from ceres_mi355x import _native as nat
     from ceres_mi355x.ba import lm_options
 
-    o = tc.SolverOptions(max_num_iterations=iters + 2, function_tolerance=0.0, gradient_tolerance=0.0,
+    o = tc.SolverOptions(max_num_iterations=iters, function_tolerance=0.0, gradient_tolerance=0.0,
                          parameter_tolerance=0.0, max_num_refinement_iterations=2 if precision == "mixed" else 0)
     opts = lm_options(o, precision=precision, loop="host", linear=linear)
-    run_solve(dev, data, opts)
+    run_solve(dev, data, opts)  # builds the plan
     handle = dev._plans[tuple(getattr(opts, f) for f, _ in nat.ct_lm_options._fields_)][0]
     dev.set_parameters(torch.tensor(data.cameras), torch.tensor(data.points))
     ms = (ctypes.c_double * len(nat.KERNELS))()
     cnt = (ctypes.c_int32 * len(nat.KERNELS))()
     nat.check(nat.load().ct_ba_plan_profile(handle, iters, ms, cnt), "ct_ba_plan_profile")
-    return {k: (ms[i] / max(cnt[i], 1), cnt[i]) for i, k in enumerate(nat.KERNELS)}
+    # linear-solver iterations of the same LM iterations, from a recorded run of the same plan
+    _, rec, _ = run_solve(dev, data, opts)
+    lin = rec[1:, 10]
+    mean_lin = float(lin.mean()) if lin.size else 0.0
+    return {k: (ms[i] / max(cnt[i], 1), cnt[i]) for i, k in enumerate(nat.KERNELS)}, mean_lin
 
 
-def algorithmic_work(kernel: str, data, precision: str):
-    """Compulsory bytes (or flops) of one launch of a kernel group.
+def num_items(data, max_work=224):
+    """Camera-block work items (the library splits each block's terms / observations into <= max_work pieces)."""
+    import numpy as np
+
+    from ceres_mi355x.ba import build_tables
+
+    t = build_tables(data.obs_camera, data.obs_point, data.num_cameras, data.num_points)
+    terms = np.diff(t.block_offsets)
+    cam_obs = np.diff(t.camera_offsets)
+    diag = t.block_cameras[:, 0] == t.block_cameras[:, 1]
+    items = int(np.sum(-(-terms // max_work)))
+    items1 = int(np.sum(-(-cam_obs[t.block_cameras[diag, 0]] // max_work)))
+    return items + items1, items1, int(t.block_terms.shape[0]), int(t.block_cameras.shape[0])
+
+
+def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0, items=None):
+    """Compulsory HBM bytes (or FP64 flops) of one launch of a kernel group.
 
     Every input is read once and every output written once; gathered re-reads
     are not counted.  s = bytes per assembly scalar (4 mixed, 8 FP64).
@@ -171,25 +196,54 @@ def algorithmic_work(kernel: str, data, precision: str):
     C, P, N = data.num_cameras, data.num_points, data.num_observations
     s = 4 if precision == "mixed" else 8
     n = 9 * C
+    if kernel == "eval_obs":
+        reads = N * (4 + 4 + 16) + C * 72 + P * 24
+        writes = N * (16 + 24 * s)
+        return "hbm", reads + writes
     if kernel == "eval_points":
-        reads = N * (16 + 4) + P * (24 + 4) + C * 72
-        writes = N * (18 * s + 6 * s + 6 * s + 2 * s + 16) + P * (6 * s + 3 * s + 24 + 24)
+        reads = 4 * (P + 1) + N * (6 * s + 16)
+        writes = P * (24 + 9 * s + 24) + N * 8 * s
         return "hbm", reads + writes
-    if kernel == "camera_blocks":
-        import numpy as np
-
-        counts = np.bincount(data.obs_point, minlength=P)
-        terms = int(np.sum(counts * (counts + 1) // 2))
-        reads = N * (18 * s + 6 * s + 2 * s + 16 + 4) + terms * 8 + (C * (C + 1) // 2 + 1) * 4
-        writes = n * n * 8 + n * 8 * 3
+    if kernel == "camera_items":
+        n_items, n_items1, T, B = items
+        reads = 16 * n_items + N * (4 + 18 * s + 2 * s + 16 + 6 * s) + 8 * T
+        writes = n_items * 81 * s + n_items1 * 27 * 8
         return "hbm", reads + writes
-    if kernel == "candidate":
-        reads = N * (18 * s + 6 * s + 16 + 16 + 4) + P * (6 * s + 3 * s + 24 + 4) + C * 72 + n * 8
-        writes = P * (24 + 24) + n * 8
+    if kernel == "camera_reduce":
+        n_items, n_items1, T, B = items
+        reads = n_items * 81 * s + n_items1 * 27 * 8 + 8 * (B + 1) + 16 * n_items
+        writes = n * n * 8 + 3 * n * 8 + 16 * C
         return "hbm", reads + writes
     if kernel == "rcs_solve":
+        if linear_iterations > 0:  # PCG: S read once per iteration, plus its diagonal blocks once
+            return "hbm", linear_iterations * (n * n * 8 + 8 * n * 8) + n * 9 * 8 + n * 8
         return "mfma", n ** 3 / 3.0 + 2.0 * n * n  # FP64 flops of potrf + 2 triangular solves
+    if kernel == "step":
+        reads = 8 * n + 72 * C + 24 * P + 4 * (P + 1) + N * (4 + 18 * s + 6 * s) + P * 9 * s
+        writes = 72 * C + 24 * P + 24 * P
+        return "hbm", reads + writes
+    if kernel == "candidate":
+        reads = N * (8 + 18 * s + 6 * s + 16 + 16) + 24 * P + 8 * n + 72 * C + 24 * P
+        writes = 0
+        return "hbm", reads + writes
     return None, None
+
+
+def roofline_entry(kernels, data, precision, mean_lin, items):
+    """Roofline of the kernel group with the largest time per launch (lm_begin/lm_end/refine excluded)."""
+    cands = {k: v for k, v in kernels.items() if k not in ("lm_begin", "lm_end", "refine") and v[1] > 0}
+    dominant = max(cands, key=lambda k: cands[k][0])
+    bound, work = algorithmic_work(dominant, data, precision, linear_iterations=mean_lin, items=items)
+    sec = kernels[dominant][0] * 1e-3
+    if bound == "hbm":
+        ach = work / sec / 1e9
+        return dict(kernel=dominant, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=ach / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=work, ms_per_launch=kernels[dominant][0],
+                    launches=kernels[dominant][1], linear_iterations_per_launch=mean_lin if dominant == "rcs_solve" else None)
+    ach = work / sec / 1e12
+    return dict(kernel=dominant, bound="mfma", achieved=ach, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=ach / FP64_PEAK_TFLOPS, traffic=None, algorithmic_flops=work, ms_per_launch=kernels[dominant][0],
+                launches=kernels[dominant][1])
 
 
 def cpu_baseline(data, iterations=2, sample_points=20_000):
@@ -266,7 +320,9 @@ def main() -> int:
     ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--only-value", action="store_true", help="time the value path only (for profiling)")
+    ap.add_argument("--only-value", action="store_true", help="time the value path only")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the per-kernel profile of the value path (the run a rocprofv3 trace is taken of)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (sharding the points across GPUs "
@@ -296,9 +352,11 @@ def main() -> int:
             f"{r['wall_s'] * 1e3:.1f} ms)")
         return r
 
-    value_run = timed("fp64_iterative_schur_device_loop", precision="fp64", loop="device", linear="iterative_schur")
     extra = {}
-    if not args.only_value:
+    if not args.profile_only:
+        value_run = timed("fp64_iterative_schur_device_loop", precision="fp64", loop="device",
+                          linear="iterative_schur")
+    if not (args.only_value or args.profile_only):
         timed("fp64_iterative_schur_host_loop", precision="fp64", loop="host", linear="iterative_schur")
         timed("fp64_dense_schur_device_loop", precision="fp64", loop="device", linear="dense_schur")
         timed("fp64_dense_schur_host_loop", precision="fp64", loop="host", linear="dense_schur")
@@ -315,20 +373,24 @@ def main() -> int:
 
     roofline = None
     kernels = None
+    per_kernel = None
     if not args.no_profile:
-        kernels = profile_kernels(dev, data, precision="fp64", linear="iterative_schur", iters=min(args.steps, 10))
-        log("[bench] fp64 PCG kernel groups (ms/launch): " + ", ".join(f"{k}={v[0]:.3f}" for k, v in kernels.items()))
-        dominant = max((k for k in kernels if algorithmic_work(k, data, "fp64")[0]), key=lambda k: kernels[k][0])
-        bound, work = algorithmic_work(dominant, data, "fp64")
-        sec = kernels[dominant][0] * 1e-3
-        if bound == "hbm":
-            ach = work / sec / 1e9
-            roofline = dict(kernel=dominant, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                            frac=ach / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=work)
-        else:
-            ach = work / sec / 1e12
-            roofline = dict(kernel=dominant, bound="mfma", achieved=ach, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                            frac=ach / FP64_PEAK_TFLOPS, traffic=None, algorithmic_flops=work)
+        items = num_items(data)
+        kernels, mean_lin = profile_kernels(dev, data, precision="fp64", linear="iterative_schur",
+                                            iters=min(args.steps, 10))
+        log("[bench] fp64 PCG kernel groups (ms/launch): " + ", ".join(f"{k}={v[0]:.4f}" for k, v in kernels.items())
+            + f"; mean PCG iterations {mean_lin:.2f}")
+        roofline = roofline_entry(kernels, data, "fp64", mean_lin, items)
+        per_kernel = {}
+        for k, (ms, cnt) in kernels.items():
+            bound, work = algorithmic_work(k, data, "fp64", linear_iterations=mean_lin, items=items)
+            if bound == "hbm" and cnt:
+                per_kernel[k] = {"ms": ms, "GB/s": work / (ms * 1e-3) / 1e9, "algorithmic_bytes": work}
+        if args.profile_only:
+            print(json.dumps({"roofline": roofline, "kernel_ms": {k: v[0] for k, v in kernels.items()},
+                              "per_kernel_hbm": per_kernel}), flush=True)
+            dev.close()
+            return 0
 
     cpu = None if args.no_cpu else cpu_baseline(data)
     if cpu:
@@ -358,6 +420,7 @@ def main() -> int:
         "runs": runs,
         **extra,
         "kernel_ms": None if kernels is None else {k: v[0] for k, v in kernels.items()},
+        "per_kernel_hbm": per_kernel,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

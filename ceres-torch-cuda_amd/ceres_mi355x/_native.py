@@ -20,7 +20,9 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-KERNELS = ["eval_points", "camera_blocks", "lm_begin", "rcs_solve", "refine", "candidate", "lm_end"]
+ABI_VERSION = 2  # CT_ABI_VERSION of include/ceres_mi355x.h
+KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "step",
+           "candidate", "lm_end"]
 REC = dict(
     iteration=0,
     cost=1,
@@ -147,7 +149,7 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
-    if lib.ct_abi_version() != 1:
+    if lib.ct_abi_version() != ABI_VERSION:
         raise NativeLibraryError("ABI version mismatch")
     _LIB = lib
     return lib

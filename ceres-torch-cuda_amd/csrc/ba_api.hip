@@ -127,6 +127,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   if (o->linear_solver == CT_ITERATIVE_SCHUR) {
     L.pcg_doubles = rcs_pcg_work_doubles((int)n);
     L.pcg_work = cv.take<double>(L.pcg_doubles);
+    L.w.pcg_bar = base ? rcs_pcg_barrier(L.pcg_work, (int)n) : nullptr;
   }
   L.bytes = cv.offset + 256;
   return L;
@@ -193,11 +194,15 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
   const int n = 9 * v.C;
   const bool mixed = pl->options.precision == CT_MIXED;
   switch (group) {
+    case CT_KERNEL_EVAL_OBS:
+      return mixed ? launch_eval_obs<float>(v, L.bf, L.w, s) : launch_eval_obs<double>(v, L.bd, L.w, s);
     case CT_KERNEL_EVAL_POINTS:
       return mixed ? launch_eval_points<float>(v, L.bf, L.w, s) : launch_eval_points<double>(v, L.bd, L.w, s);
-    case CT_KERNEL_CAMERA_BLOCKS:
-      return mixed ? launch_camera_blocks<float>(v, L.bf, L.w, pl->num_items, s)
-                   : launch_camera_blocks<double>(v, L.bd, L.w, pl->num_items, s);
+    case CT_KERNEL_CAMERA_ITEMS:
+      return mixed ? launch_camera_items<float>(v, L.bf, L.w, pl->num_items, s)
+                   : launch_camera_items<double>(v, L.bd, L.w, pl->num_items, s);
+    case CT_KERNEL_CAMERA_REDUCE:
+      return mixed ? launch_camera_reduce<float>(v, L.bf, L.w, s) : launch_camera_reduce<double>(v, L.bd, L.w, s);
     case CT_KERNEL_LM_BEGIN:
       return launch_lm_begin(pl->init, L.w, s);
     case CT_KERNEL_RCS_SOLVE:
@@ -210,9 +215,10 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
       for (int k = 0; k < pl->options.max_num_refinement_iterations; ++k)
         CT_TRY(launch_refine_step<float>(v, L.bf, L.w, s));
       return CT_OK;
+    case CT_KERNEL_STEP:
+      return mixed ? launch_step<float>(v, L.bf, L.w, !refining(pl), s) : launch_step<double>(v, L.bd, L.w, true, s);
     case CT_KERNEL_CANDIDATE:
-      return mixed ? launch_candidate<float>(v, L.bf, L.w, !refining(pl), s)
-                   : launch_candidate<double>(v, L.bd, L.w, true, s);
+      return mixed ? launch_candidate<float>(v, L.bf, L.w, s) : launch_candidate<double>(v, L.bd, L.w, s);
     case CT_KERNEL_LM_END:
       return launch_lm_end(pl->init, L.w, s);
     default:
@@ -220,10 +226,19 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
   }
 }
 
-// One LM iteration: evaluate+eliminate, camera blocks, begin, RCS solve,
-// [back-substitution + refinement], candidate, end.
-int enqueue_iteration(ct_ba_plan* pl, hipStream_t s) {
-  for (int g = 0; g < CT_KERNEL_COUNT; ++g) CT_TRY(enqueue_group(pl, g, s));
+// Groups that compute and test a step; iteration 0 only evaluates.
+bool step_group(int group) {
+  return group == CT_KERNEL_RCS_SOLVE || group == CT_KERNEL_REFINE || group == CT_KERNEL_STEP ||
+         group == CT_KERNEL_CANDIDATE;
+}
+
+// One LM iteration.  evaluation_only (host-driven iteration 0) leaves out the
+// step groups, which would return at once; a captured graph keeps them.
+int enqueue_iteration(ct_ba_plan* pl, hipStream_t s, bool evaluation_only = false) {
+  for (int g = 0; g < CT_KERNEL_COUNT; ++g) {
+    if (evaluation_only && step_group(g)) continue;
+    CT_TRY(enqueue_group(pl, g, s));
+  }
   return CT_OK;
 }
 
@@ -449,7 +464,9 @@ int ct_ba_plan_run(ct_ba_plan* pl, ct_lm_report* report, double* records) {
   } else {
     CT_TRY(launch_lm_init(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, s));
     for (int k = 0; k < max_passes; ++k) {
-      CT_TRY(enqueue_iteration(pl, s));
+      // pass 0 only evaluates; the last pass only runs the deferred checks of
+      // the last accepted step and ends at the iteration limit
+      CT_TRY(enqueue_iteration(pl, s, k == 0 || k == max_passes - 1));
       CT_TRY(read_state(pl, s));  // the one host sync of the iteration
       if (pl->host_state->done) break;
     }
@@ -496,9 +513,10 @@ int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_
     kernel_launches[k] = 0;
   }
   int status = launch_lm_init(pl->init, L.w, pl->user_cams, pl->user_pts, s);
-  // iteration 0 (evaluation only) is not timed
-  if (status == CT_OK) status = enqueue_iteration(pl, s);
-  for (int it = 0; it < iterations && status == CT_OK; ++it) {
+  // iteration 0 (evaluation only) is not timed; timing stops when LM ends
+  if (status == CT_OK) status = enqueue_iteration(pl, s, true);
+  if (status == CT_OK) status = read_state(pl, s);
+  for (int it = 0; it < iterations && status == CT_OK && !pl->host_state->done; ++it) {
     for (int g = 0; g < CT_KERNEL_COUNT && status == CT_OK; ++g) {
       hipEventRecord(ev[g], s);
       status = enqueue_group(pl, g, s);
@@ -516,6 +534,7 @@ int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_
       kernel_ms[k] += ms;
       kernel_launches[k] += 1;
     }
+    status = read_state(pl, s);
   }
   for (int k = 0; k <= CT_KERNEL_COUNT; ++k) (void)hipEventDestroy(ev[k]);
   if (status == CT_OK) status = launch_lm_finalize(pl->init, L.w, pl->user_cams, pl->user_pts, s);

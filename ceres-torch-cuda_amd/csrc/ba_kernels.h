@@ -6,11 +6,17 @@
 namespace ct {
 
 template <typename real>
+int launch_eval_obs(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+template <typename real>
 int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
-int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s);
+int launch_camera_items(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s);
 template <typename real>
-int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
+int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+template <typename real>
+int launch_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
+template <typename real>
+int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
 int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
@@ -47,6 +53,7 @@ int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, L
 int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState* state, double* work,
                    size_t work_doubles, hipStream_t s);
 size_t rcs_pcg_work_doubles(int n);
+unsigned int* rcs_pcg_barrier(double* work, int n);  // arrival counter inside the PCG workspace
 size_t rcs_cholesky_tmp_doubles(int n);
 
 }  // namespace ct

@@ -803,11 +803,16 @@ inline int blocks_for(int n) { return (n + kPointThreads - 1) / kPointThreads; }
 }  // namespace
 
 template <typename real>
-int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+int launch_eval_obs(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.N > 0) {
     hipLaunchKernelGGL(k_eval_obs<real>, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.P > 0) {
     hipLaunchKernelGGL(k_eval_points<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
@@ -816,11 +821,16 @@ int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& 
 }
 
 template <typename real>
-int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s) {
+int launch_camera_items(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s) {
   if (num_items > 0) {
     hipLaunchKernelGGL(k_camera_items<real>, dim3(num_items), dim3(kWave), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.B > 0) {
     hipLaunchKernelGGL(k_camera_reduce<real>, dim3(v.B), dim3(kReduceThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
@@ -829,7 +839,7 @@ int launch_camera_blocks(const BAView& v, const BABuffers<real>& b, const BAWork
 }
 
 template <typename real>
-int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s) {
+int launch_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s) {
   const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
   if (work > 0) {
     if (fused_backsub) {
@@ -839,6 +849,11 @@ int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w,
     }
     CT_LAUNCH_CHECK();
   }
+  return CT_OK;
+}
+
+template <typename real>
+int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.N > 0) {
     hipLaunchKernelGGL(k_candidate_obs<real>, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
@@ -884,12 +899,16 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
   return CT_OK;
 }
 
-template int launch_eval_points<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
-template int launch_eval_points<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
-template int launch_camera_blocks<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
-template int launch_camera_blocks<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
-template int launch_candidate<float>(const BAView&, const BABuffers<float>&, const BAWork&, bool, hipStream_t);
-template int launch_candidate<double>(const BAView&, const BABuffers<double>&, const BAWork&, bool, hipStream_t);
+#define CT_INSTANTIATE(real)                                                                           \
+  template int launch_eval_obs<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);        \
+  template int launch_eval_points<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);     \
+  template int launch_camera_items<real>(const BAView&, const BABuffers<real>&, const BAWork&, int, hipStream_t); \
+  template int launch_camera_reduce<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);   \
+  template int launch_step<real>(const BAView&, const BABuffers<real>&, const BAWork&, bool, hipStream_t);      \
+  template int launch_candidate<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);
+CT_INSTANTIATE(float)
+CT_INSTANTIATE(double)
+#undef CT_INSTANTIATE
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 template int launch_refine_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);

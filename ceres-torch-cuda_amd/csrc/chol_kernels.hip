@@ -407,45 +407,43 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 }
 
 // ---------------------------------------------------------------------------
-// explicit-S PCG, one persistent kernel.  Workgroup g owns 9-row blocks
-// g, g+G, ...  Reductions go through per-workgroup partials that every
-// workgroup sums in the same order, so all of them take identical decisions.
-// The grid barrier spins a bounded number of times; on timeout the solve is
-// flagged invalid instead of waiting forever.
+// explicit-S PCG with the SCHUR_JACOBI preconditioner, one persistent kernel
+// (linear.py:1216-1256; preconditioner linear.py:1156-1182, applied as in
+// linear.py:69-77).  Workgroup g owns the 9-row camera blocks g, g+G, ...;
+// its 9 wavefronts compute one row each of S p.  Every workgroup keeps the
+// whole search direction p in LDS and updates it identically from the
+// shared z, so an iteration needs two grid barriers (after p.Ap and after
+// r.z).  Cross-workgroup sums read the per-workgroup partials in one fixed
+// order in every workgroup, so all of them take identical decisions.  The
+// grid barrier is a monotonic arrival counter (cleared by the LM begin
+// kernel) with a bounded spin; on timeout the step is flagged invalid.
 // ---------------------------------------------------------------------------
-constexpr int kPcgThreads = 256;
 constexpr int kBlk = 9;
+constexpr int kPcgWaves = kBlk;  // one wavefront per row of a camera block
+constexpr int kPcgThreads = kPcgWaves * kWave;
+constexpr int kPcgMaxOwn = 8;    // camera blocks per workgroup
+constexpr int kPcgMaxGroups = 1024;
+constexpr int kPcgMaxN = 12288;  // p is held in LDS
+constexpr double kEps = 2.220446049250313e-16;
 
 struct PcgWork {
-  double* x;        // [n]
-  double* r;        // [n]
-  double* z;        // [n]
-  double* p;        // [n]
-  double* Ap;       // [n]
-  double* Lblk;     // [nb][81] Cholesky factors of diagonal blocks
-  double* part;     // [G][4]
-  unsigned int* bar;  // [2] count, generation
-  int* flags;       // [nb] block is Cholesky-factored (1) or LU fallback (0)
+  double* z;          // [n]  preconditioned residual, read by every workgroup
+  double* part;       // [3][kPcgMaxGroups] partials: r.r, r.z, p.Ap
+  unsigned int* bar;  // arrival counter
 };
 
-__device__ bool grid_barrier(unsigned int* bar, unsigned int nblocks, int* timed_out) {
+__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int target, int* timed_out) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned int arrived = atomicAdd(&bar[0], 1u) + 1u;
-    if (arrived == nblocks) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > (1L << 26)) {
-          *timed_out = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
+    atomicAdd(bar, 1u);
+    long spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > (1L << 26)) {
+        *timed_out = 1;
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
     }
     __threadfence();
   }
@@ -453,207 +451,264 @@ __device__ bool grid_barrier(unsigned int* bar, unsigned int nblocks, int* timed
   return *timed_out == 0;
 }
 
-__device__ void block_apply(const double* L, const double* rhs, double* out) {
-  // solve (L L^T) out = rhs for a 9x9 Cholesky factor L (row-major lower)
-  double t[kBlk];
-#pragma unroll
-  for (int i = 0; i < kBlk; ++i) {
-    double s = rhs[i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) s -= L[i * kBlk + k] * t[k];
-    t[i] = s / L[i * kBlk + i];
+// Sum of part[0:G] in one fixed order (identical in every workgroup); valid
+// in every thread after the call.
+__device__ __forceinline__ double group_total(const double* part, unsigned int G, double* slot) {
+  if (threadIdx.x < kWave) {
+    double v = 0.0;
+    for (unsigned int i = threadIdx.x; i < G; i += kWave) v += part[i];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) *slot = v;
   }
+  __syncthreads();
+  const double t = *slot;
+  __syncthreads();
+  return t;
+}
+
+// Cholesky of a 9x9 block in place (lower triangle), reciprocal pivots in dinv.
+__device__ __forceinline__ bool chol9(double* A, double* dinv) {
+  for (int j = 0; j < kBlk; ++j) {
+    double d = A[j * kBlk + j];
+    for (int k = 0; k < j; ++k) d -= A[j * kBlk + k] * A[j * kBlk + k];
+    if (!(d > 0.0) || !isfinite(d)) return false;
+    d = sqrt(d);
+    const double inv = 1.0 / d;
+    A[j * kBlk + j] = d;
+    dinv[j] = inv;
+    for (int i = j + 1; i < kBlk; ++i) {
+      double t = A[i * kBlk + j];
+      for (int k = 0; k < j; ++k) t -= A[i * kBlk + k] * A[j * kBlk + k];
+      A[i * kBlk + j] = t * inv;
+    }
+  }
+  return true;
+}
+
+// LU with partial pivoting of a 9x9 block in place (torch.linalg.solve's
+// factorisation); reciprocal pivots in dinv.  False if a pivot is zero.
+__device__ __forceinline__ bool lu9(double* A, int* piv, double* dinv) {
+  for (int j = 0; j < kBlk; ++j) {
+    int p = j;
+    double best = fabs(A[j * kBlk + j]);
+    for (int i = j + 1; i < kBlk; ++i) {
+      const double v = fabs(A[i * kBlk + j]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    piv[j] = p;
+    if (!(best > 0.0) || !isfinite(best)) return false;
+    if (p != j)
+      for (int k = 0; k < kBlk; ++k) {
+        const double t = A[j * kBlk + k];
+        A[j * kBlk + k] = A[p * kBlk + k];
+        A[p * kBlk + k] = t;
+      }
+    const double inv = 1.0 / A[j * kBlk + j];
+    dinv[j] = inv;
+    for (int i = j + 1; i < kBlk; ++i) {
+      const double l = A[i * kBlk + j] * inv;
+      A[i * kBlk + j] = l;
+      for (int k = j + 1; k < kBlk; ++k) A[i * kBlk + k] -= l * A[j * kBlk + k];
+    }
+  }
+  return true;
+}
+
+// z = B^-1 r for one block: Cholesky solve (kind 1, in registers) or LU
+// solve (kind 0, rare: worked in place in z, which must not alias r).
+__device__ __forceinline__ void block_solve(const double* A, const double* dinv, const int* piv, int kind, const double* r,
+                            double* z) {
+  if (kind == 1) {
+    double t[kBlk];
 #pragma unroll
+    for (int i = 0; i < kBlk; ++i) {
+      double s = r[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= A[i * kBlk + k] * t[k];
+      t[i] = s * dinv[i];
+    }
+#pragma unroll
+    for (int i = kBlk - 1; i >= 0; --i) {
+      double s = t[i];
+#pragma unroll
+      for (int k = i + 1; k < kBlk; ++k) s -= A[k * kBlk + i] * t[k];
+      t[i] = s * dinv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kBlk; ++i) z[i] = t[i];
+    return;
+  }
+  for (int i = 0; i < kBlk; ++i) z[i] = r[i];
+  for (int j = 0; j < kBlk; ++j) {
+    const int p = piv[j];
+    const double t = z[j];
+    z[j] = z[p];
+    z[p] = t;
+  }
+  for (int i = 0; i < kBlk; ++i) {
+    double s = z[i];
+    for (int k = 0; k < i; ++k) s -= A[i * kBlk + k] * z[k];
+    z[i] = s;
+  }
   for (int i = kBlk - 1; i >= 0; --i) {
-    double s = t[i];
-#pragma unroll
-    for (int k = i + 1; k < kBlk; ++k) s -= L[k * kBlk + i] * out[k];
-    out[i] = s / L[i * kBlk + i];
+    double s = z[i];
+    for (int k = i + 1; k < kBlk; ++k) s -= A[i * kBlk + k] * z[k];
+    z[i] = s * dinv[i];
   }
 }
 
 __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ S, const double* __restrict__ rhs,
                                                       double* __restrict__ xout, int n, PcgWork wk, LMState* st,
-                                                      int min_iter, int max_iter, double tol, unsigned int G) {
+                                                      unsigned int G) {
+  __shared__ double p[kPcgMaxN];  // search direction (first n entries), identical in every workgroup
+  __shared__ double Lf[kPcgMaxOwn][kBlk * kBlk];
+  __shared__ double Sd[kPcgMaxOwn][kBlk * kBlk];
+  __shared__ double dv[kPcgMaxOwn][kBlk];
+  __shared__ int piv[kPcgMaxOwn][kBlk];
+  __shared__ int kind[kPcgMaxOwn];
+  __shared__ double xo[kPcgMaxOwn][kBlk], ro[kPcgMaxOwn][kBlk], zo[kPcgMaxOwn][kBlk], apo[kPcgMaxOwn][kBlk];
+  __shared__ double slot;
+  __shared__ int timed_out, singular;
   if (skip_solve(st)) return;
-  if (st != nullptr) {
-    min_iter = st->min_lin_iter;
-    max_iter = st->max_lin_iter;
-    tol = st->eta;
-  }
-  __shared__ double red[kPcgThreads / kWave];
-  __shared__ int timed_out;
-  __shared__ double sums[4];
-  __shared__ double L[81];
-  const int tid = threadIdx.x;
-  const int g = blockIdx.x;
+  const int min_iter = st->min_lin_iter, max_iter = st->max_lin_iter;
+  const double tol = st->eta;
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const unsigned int g = blockIdx.x;
   const int nb = n / kBlk;
-  if (tid == 0) timed_out = 0;
-  __syncthreads();
-  // factor own diagonal blocks; x = 0, r = rhs, z = M^-1 r, p = z
-  double rr = 0.0, rz = 0.0, bb = 0.0;
-  for (int blk = g; blk < nb; blk += G) {
-    if (tid == 0) {
-      bool ok = true;
-      for (int i = 0; i < kBlk; ++i)
-        for (int j = 0; j < kBlk; ++j) L[i * kBlk + j] = j <= i ? S[(size_t)(kBlk * blk + i) * n + kBlk * blk + j] : 0.0;
-      for (int j = 0; j < kBlk && ok; ++j) {
-        double d = L[j * kBlk + j];
-        for (int k = 0; k < j; ++k) d -= L[j * kBlk + k] * L[j * kBlk + k];
-        if (!(d > 0.0)) { ok = false; break; }
-        d = sqrt(d);
-        L[j * kBlk + j] = d;
-        for (int i = j + 1; i < kBlk; ++i) {
-          double s = L[i * kBlk + j];
-          for (int k = 0; k < j; ++k) s -= L[i * kBlk + k] * L[j * kBlk + k];
-          L[i * kBlk + j] = s / d;
-        }
-      }
-      for (int k = 0; k < 81; ++k) wk.Lblk[81 * blk + k] = ok ? L[k] : 0.0;
-      wk.flags[blk] = ok ? 1 : 0;
-      double r[kBlk], zz[kBlk];
-      for (int i = 0; i < kBlk; ++i) r[i] = rhs[kBlk * blk + i];
-      if (ok) {
-        block_apply(L, r, zz);
-      } else {
-        // non-SPD block: Jacobi on its diagonal (rare; the reference falls
-        // back to an LU solve of the block)
-        for (int i = 0; i < kBlk; ++i) {
-          const double dgl = S[(size_t)(kBlk * blk + i) * n + kBlk * blk + i];
-          zz[i] = r[i] / (fabs(dgl) > 0.0 ? dgl : 1.0);
-        }
-      }
-      for (int i = 0; i < kBlk; ++i) {
-        wk.x[kBlk * blk + i] = 0.0;
-        wk.r[kBlk * blk + i] = r[i];
-        wk.z[kBlk * blk + i] = zz[i];
-        wk.p[kBlk * blk + i] = zz[i];
-        rr += r[i] * r[i];
-        rz += r[i] * zz[i];
-        bb += r[i] * r[i];
-      }
-    }
-  }
+  const int nown = ((int)g < nb) ? (nb - (int)g + (int)G - 1) / (int)G : 0;
+  double* part_rr = wk.part;
+  double* part_rz = wk.part + kPcgMaxGroups;
+  double* part_pap = wk.part + 2 * kPcgMaxGroups;
+  unsigned int barriers = 0;
+  int iterations = 0;
   if (tid == 0) {
-    wk.part[4 * g] = rr;
-    wk.part[4 * g + 1] = rz;
-    wk.part[4 * g + 2] = bb;
+    timed_out = 0;
+    singular = 0;
   }
-  if (!grid_barrier(wk.bar, G, &timed_out)) goto fail;
-  {
-    if (tid == 0) {
-      double a = 0, b2 = 0, c = 0;
-      for (unsigned int k = 0; k < G; ++k) {
-        a += wk.part[4 * k];
-        b2 += wk.part[4 * k + 1];
-        c += wk.part[4 * k + 2];
-      }
-      sums[0] = a;
-      sums[1] = b2;
-      sums[2] = c;
+  // diagonal blocks of the owned rows, r = rhs, x = 0
+  for (int i = tid; i < nown * kBlk * kBlk; i += kPcgThreads) {
+    const int b = i / (kBlk * kBlk), e = i % (kBlk * kBlk);
+    const int row = kBlk * (g + b * G) + e / kBlk, col = kBlk * (g + b * G) + e % kBlk;
+    Lf[b][e] = S[(size_t)row * n + col];
+    Sd[b][e] = Lf[b][e];
+  }
+  for (int i = tid; i < nown * kBlk; i += kPcgThreads) {
+    const int b = i / kBlk, k = i % kBlk;
+    ro[b][k] = rhs[kBlk * (g + b * G) + k];
+    xo[b][k] = 0.0;
+  }
+  __syncthreads();
+  if (tid < nown) {
+    // Cholesky when the block is SPD, else an LU solve of the block
+    if (chol9(Lf[tid], dv[tid])) {
+      kind[tid] = 1;
+    } else {
+      for (int e = 0; e < kBlk * kBlk; ++e) Lf[tid][e] = Sd[tid][e];
+      kind[tid] = lu9(Lf[tid], piv[tid], dv[tid]) ? 0 : -1;
+      if (kind[tid] < 0) singular = 1;
     }
+  }
+  __syncthreads();
+  // z = M^-1 r (own blocks), partials r.r and r.z
+  if (tid < nown && kind[tid] >= 0) block_solve(Lf[tid], dv[tid], piv[tid], kind[tid], ro[tid], zo[tid]);
+  __syncthreads();
+  if (wave == 0) {
+    double rr = 0.0, rz = 0.0;
+    for (int i = lane; i < nown * kBlk; i += kWave) {
+      const int b = i / kBlk, k = i % kBlk;
+      const double r = ro[b][k], z = singular ? 0.0 : zo[b][k];
+      rr += r * r;
+      rz += r * z;
+      wk.z[kBlk * (g + b * G) + k] = z;
+    }
+    rr = wave_sum(rr);
+    rz = wave_sum(rz);
+    if (lane == 0) {
+      part_rr[g] = rr;
+      part_rz[g] = rz;
+    }
+  }
+  if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+  {
+    const double rr0 = group_total(part_rr, G, &slot);
+    double rz_old = group_total(part_rz, G, &slot);
+    const double rhs_norm = fmax(sqrt(rr0), kEps);
+    for (int c = tid; c < n; c += kPcgThreads) p[c] = wk.z[c];
     __syncthreads();
-    double rz_old = sums[1];
-    const double rhs_norm = fmax(sqrt(sums[2]), 2.220446049250313e-16);
-    double rnorm = sqrt(sums[0]);
-    int iterations = 0;
-    if (!(min_iter <= 0 && rnorm <= tol * rhs_norm)) {
+    if (!(min_iter <= 0 && sqrt(rr0) <= tol * rhs_norm)) {
       for (int k = 0; k < max_iter; ++k) {
-        // Ap = S p on own rows, p.Ap partial
-        double pap = 0.0;
-        for (int blk = g; blk < nb; blk += G) {
-          for (int rrow = 0; rrow < kBlk; ++rrow) {
-            const int row = kBlk * blk + rrow;
-            double s = 0.0;
-            for (int c = tid; c < n; c += kPcgThreads) s += S[(size_t)row * n + c] * wk.p[c];
-            const double tot = block_sum<kPcgThreads>(s, red);
-            if (tid == 0) {
-              wk.Ap[row] = tot;
-              pap += wk.p[row] * tot;
-            }
-          }
-        }
-        if (tid == 0) wk.part[4 * g] = pap;
-        if (!grid_barrier(wk.bar, G, &timed_out)) goto fail;
-        if (tid == 0) {
-          double a = 0;
-          for (unsigned int q = 0; q < G; ++q) a += wk.part[4 * q];
-          sums[3] = a;
+        // Ap on the owned rows; wavefront w computes row w of each owned block
+        for (int b = 0; b < nown; ++b) {
+          const size_t row = (size_t)kBlk * (g + b * G) + wave;
+          const double* Srow = S + row * n;
+          double acc = 0.0;
+          for (int c = lane; c < n; c += kWave) acc += Srow[c] * p[c];
+          acc = wave_sum(acc);
+          if (lane == 0) apo[b][wave] = acc;
         }
         __syncthreads();
-        const double denom = sums[3];
-        if (fabs(denom) <= 2.220446049250313e-16) break;
-        const double alpha = rz_old / denom;
-        // x, r update; z = M^-1 r; partial |r|^2, r.z
-        double rr2 = 0.0, rz2 = 0.0;
-        if (tid == 0) {
-          for (int blk = g; blk < nb; blk += G) {
-            double r[kBlk], zz[kBlk];
-            for (int i = 0; i < kBlk; ++i) {
-              const int row = kBlk * blk + i;
-              wk.x[row] += alpha * wk.p[row];
-              r[i] = wk.r[row] - alpha * wk.Ap[row];
-              wk.r[row] = r[i];
-              rr2 += r[i] * r[i];
-            }
-            if (wk.flags[blk]) {
-              block_apply(wk.Lblk + 81 * blk, r, zz);
-            } else {
-              for (int i = 0; i < kBlk; ++i) {
-                const double dgl = S[(size_t)(kBlk * blk + i) * n + kBlk * blk + i];
-                zz[i] = r[i] / (fabs(dgl) > 0.0 ? dgl : 1.0);
-              }
-            }
-            for (int i = 0; i < kBlk; ++i) {
-              wk.z[kBlk * blk + i] = zz[i];
-              rz2 += r[i] * zz[i];
-            }
+        if (wave == 0) {
+          double pap = 0.0;
+          for (int i = lane; i < nown * kBlk; i += kWave) {
+            const int b = i / kBlk, kk = i % kBlk;
+            pap += p[kBlk * (g + b * G) + kk] * apo[b][kk];
           }
-          wk.part[4 * g + 1] = rr2;
-          wk.part[4 * g + 2] = rz2;
+          pap = wave_sum(pap);
+          if (lane == 0) part_pap[g] = pap;
+        }
+        if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+        const double denom = group_total(part_pap, G, &slot);
+        if (fabs(denom) <= kEps) break;
+        const double alpha = rz_old / denom;
+        if (tid < nown * kBlk) {
+          const int b = tid / kBlk, kk = tid % kBlk;
+          xo[b][kk] += alpha * p[kBlk * (g + b * G) + kk];
+          ro[b][kk] -= alpha * apo[b][kk];
         }
         iterations = k + 1;
-        if (!grid_barrier(wk.bar, G, &timed_out)) goto fail;
-        if (tid == 0) {
-          double a = 0, b2 = 0;
-          for (unsigned int q = 0; q < G; ++q) {
-            a += wk.part[4 * q + 1];
-            b2 += wk.part[4 * q + 2];
-          }
-          sums[0] = a;
-          sums[1] = b2;
-        }
         __syncthreads();
-        rnorm = sqrt(sums[0]);
-        if (iterations >= min_iter && rnorm <= tol * rhs_norm) break;
-        const double rz_new = sums[1];
-        if (fabs(rz_old) <= 2.220446049250313e-16) break;
-        const double beta = rz_new / rz_old;
-        if (tid == 0) {
-          for (int blk = g; blk < nb; blk += G)
-            for (int i = 0; i < kBlk; ++i) {
-              const int row = kBlk * blk + i;
-              wk.p[row] = wk.z[row] + beta * wk.p[row];
-            }
+        if (tid < nown && kind[tid] >= 0) block_solve(Lf[tid], dv[tid], piv[tid], kind[tid], ro[tid], zo[tid]);
+        __syncthreads();
+        if (wave == 0) {
+          double rr = 0.0, rz = 0.0;
+          for (int i = lane; i < nown * kBlk; i += kWave) {
+            const int b = i / kBlk, kk = i % kBlk;
+            const double r = ro[b][kk], z = singular ? 0.0 : zo[b][kk];
+            rr += r * r;
+            rz += r * z;
+            wk.z[kBlk * (g + b * G) + kk] = z;
+          }
+          rr = wave_sum(rr);
+          rz = wave_sum(rz);
+          if (lane == 0) {
+            part_rr[g] = rr;
+            part_rz[g] = rz;
+          }
         }
+        if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+        const double rr = group_total(part_rr, G, &slot);
+        const double rz_new = group_total(part_rz, G, &slot);
+        if (iterations >= min_iter && sqrt(rr) <= tol * rhs_norm) break;
+        if (fabs(rz_old) <= kEps) break;
+        const double beta = rz_new / rz_old;
+        for (int c = tid; c < n; c += kPcgThreads) p[c] = wk.z[c] + beta * p[c];
+        __syncthreads();
         rz_old = rz_new;
-        if (!grid_barrier(wk.bar, G, &timed_out)) goto fail;
       }
     }
-    for (int blk = g; blk < nb; blk += G)
-      for (int i = tid; i < kBlk; i += kPcgThreads) {
-        const double v = wk.x[kBlk * blk + i];
-        xout[kBlk * blk + i] = v;
-      }
-    if (g == 0 && tid == 0 && st) st->linear_iterations = iterations;
+    for (int i = tid; i < nown * kBlk; i += kPcgThreads) {
+      const int b = i / kBlk, kk = i % kBlk;
+      xout[kBlk * (g + b * G) + kk] = xo[b][kk];
+    }
+    if (g == 0 && tid == 0) st->linear_iterations = iterations;
+    if (tid == 0 && singular) st->step_invalid = 1;
     return;
   }
 fail:
-  if (tid == 0 && st) st->step_invalid = 1;
-}
-
-__global__ void k_reset_barrier(unsigned int* bar) {
-  if (threadIdx.x < 2) bar[threadIdx.x] = 0u;
+  if (tid == 0) st->step_invalid = 1;
 }
 
 }  // namespace
@@ -695,9 +750,26 @@ int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, L
   return CT_OK;
 }
 
-size_t rcs_pcg_work_doubles(int n) {
+size_t rcs_pcg_work_doubles(int n) { return (size_t)n + 3 * (size_t)kPcgMaxGroups + 8; }
+
+unsigned int* rcs_pcg_barrier(double* work, int n) {
+  return reinterpret_cast<unsigned int*>(work + n + 3 * (size_t)kPcgMaxGroups);
+}
+
+int rcs_pcg_groups(int n, int* groups) {
   const int nb = n / kBlk;
-  return (size_t)5 * n + (size_t)81 * nb + 4 * 1024 + 8 + nb;
+  int dev = 0, cus = 0;
+  CT_HIP(hipGetDevice(&dev));
+  CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int G = nb < cus ? nb : cus;
+  if (G > kPcgMaxGroups) G = kPcgMaxGroups;
+  if (G < 1) G = 1;
+  if ((nb + G - 1) / G > kPcgMaxOwn) {
+    set_error("ITERATIVE_SCHUR supports at most " + std::to_string(kPcgMaxOwn * G) + " cameras on this device");
+    return CT_ERR_UNSUPPORTED;
+  }
+  *groups = G;
+  return CT_OK;
 }
 
 int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState* st, double* work,
@@ -707,29 +779,25 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
     set_error("PCG expects 9-parameter camera blocks");
     return CT_ERR_ARGUMENT;
   }
+  if (n > kPcgMaxN) {
+    set_error("ITERATIVE_SCHUR supports at most " + std::to_string(kPcgMaxN / kBlk) + " cameras");
+    return CT_ERR_UNSUPPORTED;
+  }
+  if (st == nullptr) {
+    set_error("PCG needs the LM state (options and the cleared barrier)");
+    return CT_ERR_ARGUMENT;
+  }
   if (work_doubles < rcs_pcg_work_doubles(n)) {
     set_error("PCG workspace too small");
     return CT_ERR_WORKSPACE;
   }
-  const int nb = n / kBlk;
-  int dev = 0, cus = 0;
-  CT_HIP(hipGetDevice(&dev));
-  CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  unsigned int G = (unsigned int)(nb < cus ? nb : cus);
-  if (G > 1024) G = 1024;
+  int G = 0;
+  CT_TRY(rcs_pcg_groups(n, &G));
   PcgWork wk;
-  wk.x = work;
-  wk.r = work + n;
-  wk.z = work + 2 * (size_t)n;
-  wk.p = work + 3 * (size_t)n;
-  wk.Ap = work + 4 * (size_t)n;
-  wk.Lblk = work + 5 * (size_t)n;
-  wk.part = wk.Lblk + 81 * (size_t)nb;
-  wk.bar = reinterpret_cast<unsigned int*>(wk.part + 4 * 1024);
-  wk.flags = reinterpret_cast<int*>(wk.part + 4 * 1024 + 8);
-  hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, wk.bar);
-  CT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_pcg, dim3(G), dim3(kPcgThreads), 0, s, S, rhs, x, n, wk, st, 0, 500, 1e-10, G);
+  wk.z = work;
+  wk.part = work + n;
+  wk.bar = rcs_pcg_barrier(work, n);
+  hipLaunchKernelGGL(k_pcg, dim3(G), dim3(kPcgThreads), 0, s, S, rhs, x, n, wk, st, (unsigned int)G);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

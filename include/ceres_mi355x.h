@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 1
+#define CT_ABI_VERSION 2
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -213,20 +213,24 @@ int ct_ba_plan_destroy(ct_ba_plan* plan);
 
 /* Kernel groups of one LM iteration, in launch order. */
 enum {
-  CT_KERNEL_EVAL_POINTS = 0,    /* evaluation (per observation) + point elimination */
-  CT_KERNEL_CAMERA_BLOCKS = 1,  /* reduced camera system assembly */
-  CT_KERNEL_LM_BEGIN = 2,       /* cost/gradient reduction + checks */
-  CT_KERNEL_RCS_SOLVE = 3,      /* Cholesky (or PCG) of S */
-  CT_KERNEL_REFINE = 4,         /* mixed: back-substitution + iterative refinement */
-  CT_KERNEL_CANDIDATE = 5,      /* step application + candidate cost */
-  CT_KERNEL_LM_END = 6,         /* gain ratio, radius, accept/reject */
-  CT_KERNEL_COUNT = 7
+  CT_KERNEL_EVAL_OBS = 0,        /* residual, Jacobian, loss correction per observation */
+  CT_KERNEL_EVAL_POINTS = 1,     /* point elimination: V, its Cholesky, q, P, e */
+  CT_KERNEL_CAMERA_ITEMS = 2,    /* reduced camera system: partial camera-pair blocks */
+  CT_KERNEL_CAMERA_REDUCE = 3,   /* reduced camera system: block sums, damping, rhs */
+  CT_KERNEL_LM_BEGIN = 4,        /* cost/gradient reduction + checks */
+  CT_KERNEL_RCS_SOLVE = 5,       /* Cholesky (or PCG) of S */
+  CT_KERNEL_REFINE = 6,          /* mixed: back-substitution + iterative refinement */
+  CT_KERNEL_STEP = 7,            /* step application (+ point back-substitution) */
+  CT_KERNEL_CANDIDATE = 8,       /* candidate cost + model decrease per observation */
+  CT_KERNEL_LM_END = 9,          /* gain ratio, radius, accept/reject */
+  CT_KERNEL_COUNT = 10
 };
 
-/* Run `iterations` LM iterations (after an untimed iteration 0) with HIP
- * events around each kernel group on the plan's stream; kernel_ms[k] and
- * kernel_launches[k] (host, CT_KERNEL_COUNT entries) receive the summed
- * milliseconds and the number of timed launches.  The parameters end at the
+/* Run up to `iterations` LM iterations (after an untimed iteration 0; fewer
+ * if LM terminates) host-driven, with HIP events around each kernel group
+ * on the plan's stream; kernel_ms[k] and kernel_launches[k] (host,
+ * CT_KERNEL_COUNT entries) receive the summed milliseconds and the number
+ * of timed launches.  The parameters end at the
  * state reached, as after ct_ba_plan_run.  No reference counterpart: the
  * reference times whole calls on the host (benchmarking.py:32-69).         */
 int ct_ba_plan_profile(ct_ba_plan* plan, int iterations, double* kernel_ms, int32_t* kernel_launches);

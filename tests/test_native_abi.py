@@ -28,7 +28,9 @@ def test_library_loads_and_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(nat.exported_symbols())
-    assert lib.ct_abi_version() == 1
+    assert lib.ct_abi_version() == nat.ABI_VERSION
+    header_version = re.search(r"#define CT_ABI_VERSION (\d+)", HEADER.read_text()).group(1)
+    assert int(header_version) == nat.ABI_VERSION
 
 
 def _problem_struct(C=3, P=4, N=8):
@@ -146,3 +148,11 @@ def test_losses_native_descriptors_match_torch_values():
         ref = loss_rho(tc.losses.oracle_loss(loss), s.numpy())
         for a, b in zip(mine, ref):
             np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def test_kernel_group_names_follow_header_enum():
+    text = HEADER.read_text()
+    groups = re.findall(r"CT_KERNEL_([A-Z_]+) = (\d+)", text)
+    names = [name.lower() for name, idx in sorted(groups, key=lambda g: int(g[1])) if name != "COUNT"]
+    assert names == nat.KERNELS
+    assert int(dict(groups)["COUNT"]) == len(nat.KERNELS)

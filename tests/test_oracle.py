@@ -304,6 +304,9 @@ def assert_trajectory_matches(summary, golden_summary, *, final_rtol=1e-10, cost
         assert bool(a["step_is_successful"]) == b["step_is_successful"], k
         assert a["cost"] == pytest.approx(b["cost"], rel=cost_rtol), k
         assert a["trust_region_radius"] == pytest.approx(b["trust_region_radius"], rel=1e-3), k
+        if "linear_solver_iterations" in b and "linear_solver_iterations" in a:
+            # PCG iteration counts (1 for direct solves) are part of the trajectory
+            assert int(a["linear_solver_iterations"]) == int(b["linear_solver_iterations"]), k
     assert summary.final_cost == pytest.approx(final, rel=final_rtol)
     assert summary.initial_cost == pytest.approx(golden_summary["initial_cost"], rel=1e-12)
 
