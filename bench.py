@@ -323,6 +323,9 @@ def main() -> int:
     ap.add_argument("--only-value", action="store_true", help="time the value path only")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the per-kernel profile of the value path (the run a rocprofv3 trace is taken of)")
+    ap.add_argument("--profile-linear", default="iterative_schur", choices=["iterative_schur", "dense_schur"],
+                    help="linear solver of the profiled path")
+    ap.add_argument("--profile-precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (sharding the points across GPUs "
@@ -376,14 +379,16 @@ def main() -> int:
     per_kernel = None
     if not args.no_profile:
         items = num_items(data)
-        kernels, mean_lin = profile_kernels(dev, data, precision="fp64", linear="iterative_schur",
-                                            iters=min(args.steps, 10))
-        log("[bench] fp64 PCG kernel groups (ms/launch): " + ", ".join(f"{k}={v[0]:.4f}" for k, v in kernels.items())
-            + f"; mean PCG iterations {mean_lin:.2f}")
-        roofline = roofline_entry(kernels, data, "fp64", mean_lin, items)
+        prec, lin_solver = args.profile_precision, args.profile_linear
+        kernels, mean_lin = profile_kernels(dev, data, precision=prec, linear=lin_solver, iters=min(args.steps, 10))
+        if lin_solver != "iterative_schur":
+            mean_lin = 0.0
+        log(f"[bench] {prec} {lin_solver} kernel groups (ms/launch): "
+            + ", ".join(f"{k}={v[0]:.4f}" for k, v in kernels.items()) + f"; mean linear iterations {mean_lin:.2f}")
+        roofline = roofline_entry(kernels, data, prec, mean_lin, items)
         per_kernel = {}
         for k, (ms, cnt) in kernels.items():
-            bound, work = algorithmic_work(k, data, "fp64", linear_iterations=mean_lin, items=items)
+            bound, work = algorithmic_work(k, data, prec, linear_iterations=mean_lin, items=items)
             if bound == "hbm" and cnt:
                 per_kernel[k] = {"ms": ms, "GB/s": work / (ms * 1e-3) / 1e9, "algorithmic_bytes": work}
         if args.profile_only:

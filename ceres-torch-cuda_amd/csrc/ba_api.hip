@@ -414,6 +414,10 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   if (per > options->max_num_iterations + 2) per = options->max_num_iterations + 2;
   pl->graph_iterations = per;
   CT_TRY(build_items(pl.get()));
+  if (options->linear_solver == CT_DENSE_SCHUR && problem->num_cameras > 0) {
+    CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
+    CT_HIP(hipStreamSynchronize(pl->stream));
+  }
   CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pl->host_state), sizeof(LMState), hipHostMallocDefault));
   if (options->loop_mode == CT_LOOP_DEVICE) {
     CT_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));

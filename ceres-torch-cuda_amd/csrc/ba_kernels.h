@@ -55,5 +55,7 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
 size_t rcs_pcg_work_doubles(int n);
 unsigned int* rcs_pcg_barrier(double* work, int n);  // arrival counter inside the PCG workspace
 size_t rcs_cholesky_tmp_doubles(int n);
+// Clears the synchronisation words inside the solve scratch; once per plan, before the first solve.
+int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s);
 
 }  // namespace ct

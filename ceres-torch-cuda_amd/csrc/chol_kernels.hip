@@ -6,9 +6,10 @@
 //    solved one row per thread and the trailing rank-16 update runs on 4x4
 //    register tiles) and runs both triangular solves.
 //  * n > 256: right-looking blocked Cholesky, 32-column panels; per panel one
-//    workgroup factors the diagonal block, one kernel solves the rows below it
-//    and one applies the trailing update on 64x64 tiles; a single workgroup
-//    then runs the blocked triangular solves.
+//    wavefront factors the diagonal block, one kernel solves the rows below it
+//    and one applies the trailing update on 64x64 tiles; the triangular solves
+//    run in one persistent many-workgroup launch after the 64x64 diagonal
+//    blocks have been inverted.
 //  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning as
 //    one persistent kernel (linear.py:1216-1256, preconditioner :1156-1182).
 #include "ba_kernels.h"
@@ -243,29 +244,50 @@ constexpr int kNB = 32;
 constexpr int kPanelThreads = 256;
 constexpr int kTile = 64;
 
-// Factor the diagonal block of panel k0 in place (one workgroup) and store
-// the pivots' reciprocals.
-__global__ __launch_bounds__(kPanelThreads) void k_chol_diag(double* __restrict__ A, double* __restrict__ dinv_out,
-                                                             int n, int k0, LMState* st) {
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Factor the diagonal block of panel k0 in place with one wavefront: lane l
+// holds row l in registers; column j is broadcast through LDS.  Stores the
+// pivots' reciprocals.
+__global__ __launch_bounds__(kWave) void k_chol_diag(double* __restrict__ A, double* __restrict__ dinv_out, int n,
+                                                     int k0, LMState* st) {
   if (skip_solve(st)) return;
-  __shared__ double Ld[kNB][kNB + 1];
-  __shared__ double dinv[kNB];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
+  __shared__ double col[kNB];
+  const int l = threadIdx.x;
   const int kb = min(kNB, n - k0);
-  if (tid == 0) bad = 0;
-  for (int idx = tid; idx < kNB * kNB; idx += kPanelThreads) {
-    const int r = idx / kNB, c = idx % kNB;
-    Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+  const bool own = l < kb;
+  double a[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) a[c] = (own && c <= l) ? A[(size_t)(k0 + l) * n + k0 + c] : 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < kb) {
+      const double piv = __shfl(a[j], j);
+      if (!(piv > 0.0) || !isfinite(piv)) bad = 1;
+      const double d = sqrt(piv);
+      const double inv = 1.0 / d;
+      const double lj = (l == j) ? d : a[j] * inv;
+      if (l >= j) a[j] = lj;
+      if (l < kNB) col[l] = (l >= j && own) ? lj : 0.0;
+      lds_wave_sync();
+#pragma unroll
+      for (int m = j + 1; m < kNB; ++m)
+        if (m <= l) a[m] -= lj * col[m];
+      if (l == j && own) dinv_out[k0 + j] = inv;
+      lds_wave_sync();
+    }
   }
-  __syncthreads();
-  factor_diag_block<kNB, kNB + 1, kPanelThreads>(Ld, dinv, kb, &bad);
-  for (int idx = tid; idx < kb * kb; idx += kPanelThreads) {
-    const int r = idx / kb, c = idx % kb;
-    if (c <= r) A[(size_t)(k0 + r) * n + k0 + c] = Ld[r][c];
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c <= l) A[(size_t)(k0 + l) * n + k0 + c] = a[c];
   }
-  if (tid < kb) dinv_out[k0 + tid] = dinv[tid];
-  if (tid == 0 && bad && st) st->step_invalid = 1;
+  if (l == 0 && bad && st) st->step_invalid = 1;
 }
 
 // Rows below the factored diagonal block of panel k0: L21 = A21 L11^-T, one
@@ -403,6 +425,202 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
   if (tid == 0 && st) {
     if (bad) st->step_invalid = 1;
     st->linear_iterations = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// large n: triangular solves with many workgroups
+//
+// The factor is cut into kTB-row blocks.  k_diag_inverse inverts every
+// diagonal block (one workgroup each, all at once).  k_tri_solve_mt then
+// solves L y = b and L^T x = y in one persistent launch: workgroup g owns
+// forward blocks g, g+G, ... (increasing) and backward blocks nt-1-g,
+// nt-1-g-G, ... (decreasing).  Block r of the forward solve subtracts
+// L_rj y_j as each y_j is published and finishes with y_r = L_rr^-1 (...);
+// the backward solve mirrors it with L^T.  Publication is a per-block flag
+// stamped with this launch's epoch (read once at start; the last workgroup
+// to finish advances it), so flags never need clearing.  Each workgroup
+// waits only on blocks that precede its own in the dependency order and all
+// G <= #CU workgroups are resident, so every wait ends; waits are bounded
+// anyway and a timeout flags the step invalid.
+// ---------------------------------------------------------------------------
+constexpr int kTB = 64;
+constexpr int kTriMtThreads = 256;
+constexpr int kTriSlices = kTriMtThreads / kTB;  // column slices per row in a block product
+
+__host__ __device__ inline size_t tri_blocks(int n) { return ((size_t)n + kTB - 1) / kTB; }
+
+// inverse of the lower-triangular diagonal block b of the factor in A, row-major
+// kTB x kTB (zero outside the block's rows), one lane per column
+__global__ __launch_bounds__(kWave) void k_diag_inverse(const double* __restrict__ A, double* __restrict__ linv,
+                                                        int n, LMState* st) {
+  if (skip_solve(st)) return;
+  __shared__ double Ls[kTB][kTB + 1];
+  __shared__ double dinv[kTB];
+  const int b = blockIdx.x;
+  const int r0 = b * kTB;
+  const int rb = min(kTB, n - r0);
+  const int c = threadIdx.x;
+  for (int i = 0; i < kTB; ++i) Ls[i][c] = (i < rb && c <= i) ? A[(size_t)(r0 + i) * n + r0 + c] : 0.0;
+  lds_wave_sync();
+  dinv[c] = c < rb ? 1.0 / Ls[c][c] : 0.0;
+  lds_wave_sync();
+  double x[kTB];
+#pragma unroll
+  for (int i = 0; i < kTB; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s += Ls[i][k] * x[k];
+    x[i] = (i < c) ? 0.0 : (i == c ? dinv[i] : -s * dinv[i]);
+  }
+  double* out = linv + (size_t)b * kTB * kTB;
+#pragma unroll
+  for (int i = 0; i < kTB; ++i) out[i * kTB + c] = (i < rb && c < rb) ? x[i] : 0.0;
+}
+
+struct TriWork {
+  double* y;             // [n] forward solution
+  const double* linv;    // [nt][kTB][kTB] inverses of the diagonal blocks
+  int* flags;            // [2 nt] epoch stamps: y block ready, x block ready
+  unsigned int* epoch;   // [2] completed launches, finished workgroups of this launch
+};
+
+__device__ __forceinline__ bool wait_stamp(const int* flag, int epoch, int* timed_out) {
+  if (threadIdx.x == 0 && !*timed_out) {
+    long spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      if (++spins > (1L << 24)) {
+        *timed_out = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  return *timed_out == 0;
+}
+
+__device__ __forceinline__ void publish_stamp(int* flag, int epoch) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __restrict__ A,
+                                                                const double* __restrict__ rhs,
+                                                                double* __restrict__ x, int n, TriWork tw,
+                                                                LMState* st, int G) {
+  if (skip_solve(st)) return;
+  __shared__ double acc[kTB];
+  __shared__ double part[kTriSlices][kTB];
+  __shared__ int timed_out;
+  const int tid = threadIdx.x;
+  const int row = tid % kTB, sl = tid / kTB;
+  const int g = blockIdx.x;
+  const int nt = (int)tri_blocks(n);
+  const int epoch = (int)__hip_atomic_load(&tw.epoch[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+  if (tid == 0) timed_out = 0;
+  __syncthreads();
+  // forward: y_r = L_rr^-1 (b_r - sum_{j<r} L_rj y_j)
+  for (int r = g; r < nt; r += G) {
+    const int r0 = r * kTB, rb = min(kTB, n - r0);
+    if (tid < kTB) acc[tid] = tid < rb ? rhs[r0 + tid] : 0.0;
+    __syncthreads();
+    for (int j = 0; j < r; ++j) {
+      if (!wait_stamp(&tw.flags[j], epoch, &timed_out)) break;
+      double p = 0.0;
+      if (row < rb) {
+        const double* Lrow = A + (size_t)(r0 + row) * n + j * kTB + sl * (kTB / kTriSlices);
+        const double* yj = tw.y + j * kTB + sl * (kTB / kTriSlices);
+#pragma unroll
+        for (int c = 0; c < kTB / kTriSlices; ++c) p += Lrow[c] * yj[c];
+      }
+      part[sl][row] = p;
+      __syncthreads();
+      if (tid < kTB) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < kTriSlices; ++q) t += part[q][tid];
+        acc[tid] -= t;
+      }
+      __syncthreads();
+    }
+    {
+      const double* Li = tw.linv + (size_t)r * kTB * kTB + row * kTB + sl * (kTB / kTriSlices);
+      double p = 0.0;
+#pragma unroll
+      for (int c = 0; c < kTB / kTriSlices; ++c) p += Li[c] * acc[sl * (kTB / kTriSlices) + c];
+      part[sl][row] = p;
+      __syncthreads();
+      if (tid < rb) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < kTriSlices; ++q) t += part[q][tid];
+        tw.y[r0 + tid] = t;
+      }
+    }
+    publish_stamp(&tw.flags[r], epoch);
+  }
+  // backward: x_r = L_rr^-T (y_r - sum_{j>r} L_jr^T x_j)
+  for (int q0 = g; q0 < nt; q0 += G) {
+    const int r = nt - 1 - q0;
+    const int r0 = r * kTB, rb = min(kTB, n - r0);
+    if (!wait_stamp(&tw.flags[r], epoch, &timed_out)) break;
+    if (tid < kTB) acc[tid] = tid < rb ? tw.y[r0 + tid] : 0.0;
+    __syncthreads();
+    for (int j = nt - 1; j > r; --j) {
+      if (!wait_stamp(&tw.flags[nt + j], epoch, &timed_out)) break;
+      const int j0 = j * kTB, jb = min(kTB, n - j0);
+      double p = 0.0;
+      if (row < rb) {
+        for (int k = sl * (kTB / kTriSlices); k < (sl + 1) * (kTB / kTriSlices); ++k)
+          if (k < jb) p += A[(size_t)(j0 + k) * n + r0 + row] * x[j0 + k];
+      }
+      part[sl][row] = p;
+      __syncthreads();
+      if (tid < kTB) {
+        double t = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
+        acc[tid] -= t;
+      }
+      __syncthreads();
+    }
+    {
+      const double* Lb = tw.linv + (size_t)r * kTB * kTB;
+      double p = 0.0;
+      for (int c = sl * (kTB / kTriSlices); c < (sl + 1) * (kTB / kTriSlices); ++c) p += Lb[c * kTB + row] * acc[c];
+      part[sl][row] = p;
+      __syncthreads();
+      if (tid < rb) {
+        double t = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
+        x[r0 + tid] = t;
+      }
+    }
+    publish_stamp(&tw.flags[nt + r], epoch);
+  }
+  // non-finite solution or a timed-out wait invalidates the step
+  int my_bad = 0;
+  for (int q0 = g; q0 < nt; q0 += G) {
+    const int r0 = (nt - 1 - q0) * kTB;
+    if (tid < kTB && r0 + tid < n && !isfinite(x[r0 + tid])) my_bad = 1;
+  }
+  if (my_bad || timed_out) st->step_invalid = 1;
+  if (g == 0 && tid == 0) st->linear_iterations = 1;
+  // the last workgroup to finish advances the epoch for the next launch
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    const unsigned int done = atomicAdd(&tw.epoch[1], 1u);
+    if (done == (unsigned int)G - 1) {
+      tw.epoch[1] = 0u;
+      __threadfence();
+      __hip_atomic_store(&tw.epoch[0], (unsigned int)epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -713,7 +931,49 @@ fail:
 
 }  // namespace
 
-size_t rcs_cholesky_tmp_doubles(int n) { return 2 * (size_t)n; }
+// solve scratch layout (doubles): [0,n) y, [n,2n) pivot reciprocals,
+// [2n, 2n + nt kTB^2) diagonal-block inverses, then 2 nt + 2 ints of flags
+size_t rcs_cholesky_tmp_doubles(int n) {
+  const size_t nt = tri_blocks(n);
+  return 2 * (size_t)n + nt * kTB * kTB + nt + 2;
+}
+
+namespace {
+TriWork tri_work(double* tmp, int n) {
+  const size_t nt = tri_blocks(n);
+  TriWork tw;
+  tw.y = tmp;
+  tw.linv = tmp + 2 * (size_t)n;
+  tw.flags = reinterpret_cast<int*>(tmp + 2 * (size_t)n + nt * kTB * kTB);
+  tw.epoch = reinterpret_cast<unsigned int*>(tw.flags + 2 * nt);
+  return tw;
+}
+
+int tri_groups(int n, int* groups) {
+  int dev = 0, cus = 0;
+  CT_HIP(hipGetDevice(&dev));
+  CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nt = (int)tri_blocks(n);
+  *groups = nt < cus ? nt : cus;
+  if (*groups < 1) *groups = 1;
+  return CT_OK;
+}
+
+int launch_tri_mt(const double* L, const double* rhs, double* x, int n, LMState* st, double* tmp, hipStream_t s) {
+  int G = 0;
+  CT_TRY(tri_groups(n, &G));
+  hipLaunchKernelGGL(k_tri_solve_mt, dim3(G), dim3(kTriMtThreads), 0, s, L, rhs, x, n, tri_work(tmp, n), st, G);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+}  // namespace
+
+int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s) {
+  // zero the flags and epoch of the many-workgroup triangular solves (once, before first use)
+  const size_t nt = tri_blocks(n);
+  CT_HIP(hipMemsetAsync(tmp + 2 * (size_t)n + nt * kTB * kTB, 0, (nt + 2) * sizeof(double), s));
+  return CT_OK;
+}
 
 int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* st, double* tmp,
                               hipStream_t s) {
@@ -726,7 +986,7 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
   for (int k0 = 0; k0 < n; k0 += kNB) {
     const int kb = n - k0 < kNB ? n - k0 : kNB;
     const int rows = n - k0 - kb;
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(kPanelThreads), 0, s, S, tmp + n, n, k0, st);
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(kWave), 0, s, S, tmp + n, n, k0, st);
     CT_LAUNCH_CHECK();
     if (rows > 0) {
       const int grid = (rows + kPanelThreads - 1) / kPanelThreads;
@@ -737,14 +997,16 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
       CT_LAUNCH_CHECK();
     }
   }
-  hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(kTriThreads), 0, s, S, rhs, x, tmp, n, st);
+  hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned int)tri_blocks(n)), dim3(kWave), 0, s, S,
+                     tmp + 2 * (size_t)n, n, st);
   CT_LAUNCH_CHECK();
-  return CT_OK;
+  return launch_tri_mt(S, rhs, x, n, st, tmp, s);
 }
 
 int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, LMState* st, double* tmp,
                          hipStream_t s) {
   if (n <= 0) return CT_OK;
+  if (n > kSmallMax) return launch_tri_mt(L, rhs, x, n, st, tmp, s);
   hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(kTriThreads), 0, s, L, rhs, x, tmp, n, st);
   CT_LAUNCH_CHECK();
   return CT_OK;
