@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 2  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 3  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "step",
            "candidate", "lm_end"]
 REC = dict(
@@ -113,6 +113,30 @@ class ct_ba_problem(ctypes.Structure):
     ]
 
 
+class ct_curve_batch(ctypes.Structure):
+    _fields_ = [
+        ("num_problems", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("num_samples", ctypes.c_int64),
+        ("offsets", _vp),
+        ("x", _vp),
+        ("y", _vp),
+        ("order", _vp),
+        ("loss", ct_loss),
+    ]
+
+
+class ct_batch_schedule(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int32), ("groups", ctypes.c_int32)]
+
+
+# numpy view of a device array of ct_lm_report
+import numpy as _np  # noqa: E402
+
+REPORT_DTYPE = _np.dtype([(name, _np.int32 if ctype is ctypes.c_int32 else _np.float64)
+                          for name, ctype in ct_lm_report._fields_], align=True)
+assert REPORT_DTYPE.itemsize == ctypes.sizeof(ct_lm_report)
+
 # (name, restype, argtypes) for every entry point the header declares
 SIGNATURES = {
     "ct_abi_version": (ctypes.c_int, []),
@@ -126,6 +150,10 @@ SIGNATURES = {
     "ct_ba_plan_profile": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "ct_ba_evaluate": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), _vp, _vp, ctypes.POINTER(ctypes.c_double), _vp,
                                       _vp, ctypes.c_size_t, _vp]),
+    "ct_curve_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
+                                      ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "ct_curve_validate": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _LIB = None

@@ -39,6 +39,7 @@ _MESSAGES = {
     5: "Maximum iterations reached.",
     6: "Too many invalid trust-region steps.",
     7: "No active parameters.",
+    8: "Invalid problem input.",
 }
 
 # Device-path settings with no counterpart in SolverOptions.

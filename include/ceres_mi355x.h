@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 2
+#define CT_ABI_VERSION 3
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -128,7 +128,8 @@ typedef enum ct_reason {
   CT_REASON_MIN_RADIUS = 4,       /* "Minimum trust-region radius reached." */
   CT_REASON_MAX_ITERATIONS = 5,   /* "Maximum iterations reached." */
   CT_REASON_INVALID_STEPS = 6,    /* "Too many invalid trust-region steps." */
-  CT_REASON_NO_PARAMETERS = 7     /* "No active parameters." */
+  CT_REASON_NO_PARAMETERS = 7,    /* "No active parameters." */
+  CT_REASON_INVALID_INPUT = 8     /* batched problems: malformed sample range */
 } ct_reason;
 
 /* Per-iteration record layout (doubles), mirroring IterationSummary
@@ -241,6 +242,48 @@ int ct_ba_plan_profile(ct_ba_plan* plan, int iterations, double* kernel_ms, int3
 int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const double* points,
                    double* cost_out_host, double* residuals, void* workspace, size_t workspace_bytes,
                    ct_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched curve fits: y = exp(m x + c), one residual block per sample with a
+ * robust loss (examples/robust_curve_fitting.py:80-97; parameter blocks m and
+ * c), many independent problems of uneven size solved by the LM loop of
+ * solver.py:35-357 in one persistent launch.  For two parameters the damped
+ * normal equations give the reference's DENSE_QR step (linear.py:359-371).  */
+/* ------------------------------------------------------------------------ */
+typedef struct ct_curve_batch {
+  int32_t num_problems;
+  int32_t reserved;
+  int64_t num_samples;
+  const int64_t* offsets;   /* [num_problems+1] device: problem p owns samples offsets[p]..offsets[p+1]-1 */
+  const double* x;          /* [num_samples] device */
+  const double* y;          /* [num_samples] device */
+  const int32_t* order;     /* [num_problems] device processing order (a permutation), or NULL */
+  ct_loss loss;
+} ct_curve_batch;
+
+/* Launch shape of a persistent batched solve (chosen by the autotuner of
+ * ceres_mi355x.autotune).  No reference counterpart.                        */
+typedef struct ct_batch_schedule {
+  int32_t threads;          /* workgroup size: 64, 128, 256 or 512 */
+  int32_t groups;           /* persistent workgroups; 0 = 8 per compute unit */
+} ct_batch_schedule;
+
+/* Device workspace of ct_curve_solve (a work-queue counter). */
+int ct_curve_workspace_bytes(const ct_curve_batch* batch, size_t* bytes);
+
+/* Solve every problem of the batch: params [num_problems][2] (m, c) FP64
+ * device, start values in, best values out; results [num_problems] device
+ * (may be NULL); records [num_problems][max_iter+2][CT_REC_FIELDS] device (may
+ * be NULL).  options->precision CT_MIXED accumulates J^T J in FP32.  Enqueues
+ * two kernels on stream and returns: no host synchronisation, so a caller may
+ * capture it in a graph.  Malformed sample ranges end that problem with
+ * CT_REASON_INVALID_INPUT; ct_curve_validate checks them on the host.        */
+int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, const ct_batch_schedule* schedule,
+                   double* params, ct_lm_report* results, double* records, void* workspace, size_t workspace_bytes,
+                   ct_stream_t stream);
+
+/* Host check of offsets (and order) of a batch; synchronous.                */
+int ct_curve_validate(const ct_curve_batch* batch, int32_t* first_bad_problem);
 
 #ifdef __cplusplus
 }
