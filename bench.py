@@ -312,6 +312,80 @@ def mixed_vs_fp64(dev, data, *, max_iters):
     }
 
 
+def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=256):
+    """Batched uneven curve fits (y = exp(m x + c), Cauchy(0.5), 30 LM iterations) in one persistent launch.
+
+    The launch shape is autotuned; the solve (start values reset + batched LM)
+    is captured in a HIP graph and replayed; time per batch is the mean over
+    the replays, bracketed by device synchronisations.  Also reports the FP64
+    form and a one-thread oracle sample.
+    """
+    import numpy as np
+    import torch
+
+    import ceres_mi355x as tc
+    from ceres_mi355x.autotune import curve_schedule
+    from ceres_mi355x.ba import lm_options
+    from ceres_mi355x.curve import CurveBatch
+    from oracle import generators as gen
+
+    data = gen.make_curve_batch(num_problems=num_problems, seed=3)
+    batch = CurveBatch(data.offsets, data.x, data.y, loss=tc.CauchyLoss(0.5))
+    out = {"workload": f"{num_problems} curve fits y = exp(m x + c), N ~ U[50, 2000] samples "
+                       f"({int(data.offsets[-1])} in total), Cauchy(0.5), {iterations} LM iterations max, "
+                       "reference default tolerances, start m = c = 0"}
+    for precision in ("mixed", "fp64"):
+        opts = lm_options(tc.SolverOptions(max_num_iterations=iterations), precision=precision)
+        res = batch.solve(opts, records=False)  # tunes the launch shape on first use
+        (threads, groups), order = curve_schedule(batch, opts)  # the cached choice
+        start = torch.zeros(num_problems, 2, dtype=torch.float64, device="cuda")
+        params = start.clone()
+        graph = torch.cuda.CUDAGraph()
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            batch.launch(opts, params, schedule=(threads, groups), order=order)
+            torch.cuda.synchronize()
+            with torch.cuda.graph(graph, stream=stream):
+                params.zero_()
+                params.add_(start)
+                batch.launch(opts, params, schedule=(threads, groups), order=order)
+        graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(replays):
+            graph.replay()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / replays
+        iters = int(np.sum(res.reports["num_records"]) - num_problems)
+        conv = float(np.mean(res.reports["termination"] == 0))
+        out[precision] = {"seconds_per_batch": sec, "solves_per_s": num_problems / sec,
+                          "lm_iterations_per_s": iters / sec, "lm_iterations": iters, "converged_fraction": conv,
+                          "schedule": {"threads": threads, "groups": groups, "order": order},
+                          "median_abs_err_m_vs_truth": float(np.median(np.abs(params[:, 0].cpu().numpy()
+                                                                              - data.true_m)))}
+        log(f"[bench] curve batch {precision}: {sec * 1e3:.3f} ms/batch, {num_problems / sec:.0f} solves/s, "
+            f"schedule {threads}x{groups} {order}")
+    # one-thread oracle on the first problems of the batch
+    from threadpoolctl import threadpool_limits
+
+    from oracle.lm import OracleOptions, trust_region_solve
+    from oracle.small_models import CurveModel
+
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for i in range(cpu_problems):
+            lo, hi = data.offsets[i], data.offsets[i + 1]
+            trust_region_solve(CurveModel(data.x[lo:hi], data.y[lo:hi]), OracleOptions(max_num_iterations=iterations))
+        el = time.perf_counter() - t0
+    frac = float(data.offsets[cpu_problems]) / float(data.offsets[-1])
+    out["cpu_baseline"] = {"value": num_problems * frac / el, "unit": "solves/s",
+                           "cores": 1, "kind": "port",
+                           "sample": f"float64 numpy oracle (DENSE_QR step) on the first {cpu_problems} problems "
+                                     f"({int(data.offsets[cpu_problems])} samples) in {el:.2f} s; scaled to the "
+                                     "batch's sample mix"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +447,7 @@ def main() -> int:
         extra["time_to_tol_ba16_huber"] = mixed_vs_fp64(sdev, small, max_iters=2000)
         log(f"[bench] ba16_huber time-to-tol {extra['time_to_tol_ba16_huber']}")
         sdev.close()
+        extra["curve_batch"] = bench_curve_batch()
 
     roofline = None
     kernels = None
