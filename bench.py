@@ -467,8 +467,14 @@ def main() -> int:
             if bound == "hbm" and cnt:
                 per_kernel[k] = {"ms": ms, "GB/s": work / (ms * 1e-3) / 1e9, "algorithmic_bytes": work}
         if args.profile_only:
+            # calibration for hardware byte counters: an elementwise copy of exactly 256 MiB read + 256 MiB written
+            src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            torch.mul(src, 1.0, out=dst)
+            torch.cuda.synchronize()
             print(json.dumps({"roofline": roofline, "kernel_ms": {k: v[0] for k, v in kernels.items()},
-                              "per_kernel_hbm": per_kernel}), flush=True)
+                              "per_kernel_hbm": per_kernel, "calibration_copy_bytes": 256 * 1024 * 1024}),
+                  flush=True)
             dev.close()
             return 0
 

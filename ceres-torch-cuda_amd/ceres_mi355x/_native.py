@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 3  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 4  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "step",
            "candidate", "lm_end"]
 REC = dict(
@@ -154,6 +154,14 @@ SIGNATURES = {
     "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
                                       ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_curve_validate": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_int32)]),
+    "ct_dense_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_normal_equations_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
+                                                     _vp, _vp, ctypes.c_size_t, _vp]),
+    "ct_block_schur_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp,
+                                                ctypes.c_int64, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "ct_rcs_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_rcs_potrf_f64": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_size_t, _vp]),
+    "ct_rcs_potrs_f64": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
 }
 
 _LIB = None

@@ -63,6 +63,7 @@ struct Layout {
   BAWork w{};
   BlockItem* items = nullptr;
   int32_t* block_item_offsets = nullptr;
+  int32_t* item_perm = nullptr;
   BABuffers<float> bf{};
   BABuffers<double> bd{};
   double* pcg_work = nullptr;
@@ -100,13 +101,14 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   const size_t I = max_items(p);
   L.items = cv.take<BlockItem>(I);
   L.block_item_offsets = cv.take<int32_t>((size_t)p->num_blocks + 1);
+  L.item_perm = cv.take<int32_t>(I);
   L.w.items = L.items;
   L.w.block_item_offsets = L.block_item_offsets;
+  L.w.item_perm = L.item_perm;
   L.w.item_vec = cv.take<double>(27 * I);
   if (o->precision == CT_MIXED) {
-    L.bf.Jc = cv.take<float>(18 * N);
+    L.bf.JP = cv.take<float>(kJPStride * N);
     L.bf.Jp = cv.take<float>(6 * N);
-    L.bf.Pf = cv.take<float>(6 * N);
     L.bf.e = cv.take<float>(2 * N);
     L.bf.r = cv.take<double>(2 * N);
     L.bf.Linv = cv.take<float>(6 * P);
@@ -114,9 +116,8 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
     L.bf.gp = cv.take<double>(3 * P);
     L.bf.item_S = cv.take<float>(81 * I);
   } else {
-    L.bd.Jc = cv.take<double>(18 * N);
+    L.bd.JP = cv.take<double>(kJPStride * N);
     L.bd.Jp = cv.take<double>(6 * N);
-    L.bd.Pf = cv.take<double>(6 * N);
     L.bd.e = cv.take<double>(2 * N);
     L.bd.r = cv.take<double>(2 * N);
     L.bd.Linv = cv.take<double>(6 * P);
@@ -294,6 +295,67 @@ int validate_indices(const ct_ba_problem& p, const std::vector<int32_t>& camera_
   return CT_OK;
 }
 
+// Launch order of the camera-block items.  Workgroups are dispatched to the
+// chip's L2 domains (XCDs) round-robin by id, so slot k runs on domain
+// k % kDomains.  Block rows are cut into kDomains bands of about equal work;
+// a band's items go to one domain, row panels of kPanelRows rows at a time
+// with the column camera outermost, so the row cameras' Jacobians stay in that
+// domain's L2 and each column camera's data serves the whole panel.  Results
+// do not depend on the order (every item writes its own partial slot).
+constexpr int kDomains = 8;
+constexpr int kPanelRows = 4;
+
+std::vector<int32_t> item_order(const std::vector<BlockItem>& items, const std::vector<int32_t>& offsets,
+                                const std::vector<int32_t>& bc, int C) {
+  const int B = (int)offsets.size() - 1;
+  std::vector<int32_t> block_of(C * (size_t)C, -1);
+  std::vector<double> row_work(C, 0.0);
+  for (int blk = 0; blk < B; ++blk) {
+    const int i = bc[2 * (size_t)blk], j = bc[2 * (size_t)blk + 1];
+    block_of[(size_t)i * C + j] = blk;
+    for (int k = offsets[blk]; k < offsets[blk + 1]; ++k) row_work[i] += items[k].end - items[k].begin + 1;
+  }
+  double total = 0.0;
+  for (double v : row_work) total += v;
+  // bands of consecutive rows with about total / kDomains work each
+  std::vector<std::vector<int32_t>> seq(kDomains);
+  int row = 0;
+  double done = 0.0;
+  for (int d = 0; d < kDomains && row < C; ++d) {
+    const double target = total * (d + 1) / kDomains;
+    int r1 = row;
+    while (r1 < C && (done + row_work[r1] <= target || r1 == row || d == kDomains - 1)) done += row_work[r1++];
+    for (int p0 = row; p0 < r1; p0 += kPanelRows) {
+      const int p1 = p0 + kPanelRows < r1 ? p0 + kPanelRows : r1;
+      for (int j = p0; j < C; ++j)
+        for (int i = p0; i < p1 && i <= j; ++i) {
+          const int blk = block_of[(size_t)i * C + j];
+          if (blk < 0) continue;
+          for (int k = offsets[blk]; k < offsets[blk + 1]; ++k) seq[d].push_back(k);
+        }
+    }
+    row = r1;
+  }
+  // interleave: slot k takes the next item of domain k % kDomains (or of the
+  // domain with the most items left once that one is exhausted)
+  std::vector<int32_t> perm;
+  perm.reserve(items.size());
+  std::vector<size_t> next(kDomains, 0);
+  while (perm.size() < items.size()) {
+    int d = (int)(perm.size() % kDomains);
+    if (next[d] >= seq[d].size()) {
+      size_t best = 0;
+      for (int e = 0; e < kDomains; ++e)
+        if (seq[e].size() - next[e] > best) {
+          best = seq[e].size() - next[e];
+          d = e;
+        }
+    }
+    perm.push_back(seq[d][next[d]++]);
+  }
+  return perm;
+}
+
 void split_range(std::vector<BlockItem>& out, int block, int kind, int begin, int end) {
   const int count = end - begin;
   if (count <= 0) return;
@@ -338,8 +400,19 @@ int build_items(ct_ba_plan* pl) {
     return CT_ERR_ARGUMENT;
   }
   pl->num_items = (int)items.size();
-  if (!items.empty())
+  if (!items.empty()) {
     CT_HIP(hipMemcpy(pl->layout.items, items.data(), sizeof(BlockItem) * items.size(), hipMemcpyHostToDevice));
+    const std::vector<int32_t> perm = item_order(items, offsets, bc, C);
+    std::vector<char> seen(items.size(), 0);
+    for (int32_t k : perm) {
+      if (k < 0 || k >= (int32_t)items.size() || seen[k]) {
+        set_error("camera-block item order is not a permutation");
+        return CT_ERR_ARGUMENT;
+      }
+      seen[k] = 1;
+    }
+    CT_HIP(hipMemcpy(pl->layout.item_perm, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+  }
   CT_HIP(hipMemcpy(pl->layout.block_item_offsets, offsets.data(), sizeof(int32_t) * (B + 1), hipMemcpyHostToDevice));
   return CT_OK;
 }
@@ -506,41 +579,52 @@ int ct_ba_plan_run(ct_ba_plan* pl, ct_lm_report* report, double* records) {
 int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_t* kernel_launches) {
   clear_error();
   CT_CHECK_ARG(pl != nullptr, "plan is null");
-  CT_CHECK_ARG(iterations > 0, "iterations must be > 0");
+  CT_CHECK_ARG(iterations > 0 && iterations <= 4096, "iterations must be in [1, 4096]");
   CT_CHECK_ARG(kernel_ms != nullptr && kernel_launches != nullptr, "output arrays are null");
   hipStream_t s = pl->stream;
   Layout& L = pl->layout;
-  hipEvent_t ev[CT_KERNEL_COUNT + 1];
-  for (int k = 0; k <= CT_KERNEL_COUNT; ++k) CT_HIP(hipEventCreate(&ev[k]));
   for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
     kernel_ms[k] = 0.0;
     kernel_launches[k] = 0;
   }
-  int status = launch_lm_init(pl->init, L.w, pl->user_cams, pl->user_pts, s);
-  // iteration 0 (evaluation only) is not timed; timing stops when LM ends
-  if (status == CT_OK) status = enqueue_iteration(pl, s, true);
-  if (status == CT_OK) status = read_state(pl, s);
-  for (int it = 0; it < iterations && status == CT_OK && !pl->host_state->done; ++it) {
+  // every iteration's events and kernels are enqueued before one synchronisation,
+  // so no event waits on the host; iterations after LM finished are not counted
+  const int per = CT_KERNEL_COUNT + 1;
+  std::vector<hipEvent_t> ev((size_t)iterations * per, nullptr);
+  int status = CT_OK;
+  for (auto& e : ev)
+    if (status == CT_OK && hipEventCreate(&e) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      status = CT_ERR_HIP;
+    }
+  if (status == CT_OK) status = launch_lm_init(pl->init, L.w, pl->user_cams, pl->user_pts, s);
+  if (status == CT_OK) status = enqueue_iteration(pl, s, true);  // iteration 0, not timed
+  for (int it = 0; it < iterations && status == CT_OK; ++it) {
+    hipEvent_t* e = &ev[(size_t)it * per];
     for (int g = 0; g < CT_KERNEL_COUNT && status == CT_OK; ++g) {
-      hipEventRecord(ev[g], s);
+      hipEventRecord(e[g], s);
       status = enqueue_group(pl, g, s);
     }
-    hipEventRecord(ev[CT_KERNEL_COUNT], s);
-    if (status != CT_OK) break;
-    if (hipEventSynchronize(ev[CT_KERNEL_COUNT]) != hipSuccess) {
-      set_error("event synchronize failed");
-      status = CT_ERR_HIP;
-      break;
-    }
-    for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-      kernel_ms[k] += ms;
-      kernel_launches[k] += 1;
-    }
-    status = read_state(pl, s);
+    hipEventRecord(e[CT_KERNEL_COUNT], s);
   }
-  for (int k = 0; k <= CT_KERNEL_COUNT; ++k) (void)hipEventDestroy(ev[k]);
+  if (status == CT_OK) status = read_state(pl, s);
+  if (status == CT_OK) {
+    // LM iterations 1..completed ran every group with work (a termination in
+    // the begin kernel leaves that iteration's later groups without work)
+    int completed = pl->host_state->iteration - 1;
+    if (completed > iterations) completed = iterations;
+    for (int it = 0; it < completed; ++it) {
+      hipEvent_t* e = &ev[(size_t)it * per];
+      for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e[k], e[k + 1]);
+        kernel_ms[k] += ms;
+        kernel_launches[k] += 1;
+      }
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (status == CT_OK) status = launch_lm_finalize(pl->init, L.w, pl->user_cams, pl->user_pts, s);
   if (status == CT_OK) status = read_state(pl, s);
   return status;

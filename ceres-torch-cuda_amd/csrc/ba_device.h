@@ -66,12 +66,17 @@ struct BlockItem {
 };
 constexpr int kItemMaxWork = 224;  // terms or observations per item (7 lane groups x 32)
 
+// One record per observation holds what the camera-pair assembly gathers for
+// both observations of a term (camera Jacobian and P), so a term touches two
+// records instead of four separate rows.
+constexpr int kJPStride = 24;
+
 // Per-observation and per-point buffers written by the evaluation kernel.
 template <typename real>
 struct BABuffers {
-  real* Jc;        // [N][2][9] corrected camera Jacobian (masked)
+  real* JP;        // [N][kJPStride] per observation: corrected camera Jacobian [2][9] (masked),
+                   //                then P = Jp L^-T [2][3] with L the Cholesky factor of damped V_p
   real* Jp;        // [N][2][3] corrected point Jacobian (masked)
-  real* Pf;        // [N][2][3] Jp L^-T with L the Cholesky factor of damped V_p
   real* e;         // [N][2]    P q - r
   double* r;       // [N][2]    corrected residual (FP64)
   real* Linv;      // [P][6]    L^-1, lower triangle row-major
@@ -106,6 +111,7 @@ struct BAWork {
   // camera-block work items (built at plan creation)
   const BlockItem* items;            // [num_items], grouped by block
   const int32_t* block_item_offsets; // [B+1]
+  const int32_t* item_perm;          // [num_items] launch slot -> item (locality order)
   double* item_vec;   // [items][27] rhs, g_c, column norms (kind 1 items)
   // dense solve scratch
   double* solve_tmp;  // [2n]

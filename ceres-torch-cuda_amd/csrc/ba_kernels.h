@@ -41,7 +41,8 @@ int launch_lm_end(const LMInit& init, const BAWork& w, hipStream_t s);
 int launch_lm_finalize(const LMInit& init, const BAWork& w, double* user_cams, double* user_pts, hipStream_t s);
 
 // Dense SPD solves of the reduced camera system (chol_kernels.hip).
-//   factor S (n x n row-major, lower triangle) in place and solve S x = rhs;
+//   factor S (n x n row-major, lower triangle) in place and solve S x = rhs
+//   (rhs == nullptr: factor only, keeping what later triangular solves need);
 //   skip when state->done or state->skip_step; flag state->step_invalid on a
 //   non-positive or non-finite pivot.
 int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* state, double* tmp,

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_obs(BAView v, BABuffe
     cost = robustify2(v.loss, r, Jc, Jp);
     b.r[2 * (size_t)o] = r[0];
     b.r[2 * (size_t)o + 1] = r[1];
-    real* jc = b.Jc + 18 * (size_t)o;
+    real* jc = b.JP + kJPStride * (size_t)o;
     real* jp = b.Jp + 6 * (size_t)o;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
     b.gp[3 * p + 2] = g2v;
     for (int o = o0; o < o1; ++o) {
       const real* jp = b.Jp + 6 * (size_t)o;
-      real* P = b.Pf + 6 * (size_t)o;
+      real* P = b.JP + kJPStride * (size_t)o + 18;
       double e[2];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
@@ -329,60 +329,136 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
 
 // ---------------------------------------------------------------------------
 // K2a: one wavefront per work item (a bounded slice of one camera-pair block
-// (i, j), i <= j).  63 lanes form 7 groups of 3x3 sub-block owners; group g
-// takes terms g, g+7, ... of the slice; the 7 partial blocks are summed in
-// LDS in a fixed order and written as the item's partial:
+// (i, j), i <= j):
 //   kind 0 (Schur terms):  - sum Jc_a^T (P_a P_b^T) Jc_b
 //   kind 1 (observations of camera i, diagonal blocks only):
 //                          sum Jc^T Jc, and rhs = sum Jc^T e, g_c = sum Jc^T r,
 //                          column norms diag(sum Jc^T Jc) in FP64.
-// Splitting blocks into items bounds the serial work per wavefront: a
-// diagonal block covers every observation of its camera.
+// The slice is processed in rounds of kStageTerms terms.  Staging: lane t gathers term
+// t (its own independent loads, all in flight together), forms
+// T = (P_a P_b^T) Jc_b once, and stores Jc_a and T in LDS.  Accumulation: 63
+// lanes form 7 groups of 3x3 sub-block owners; group g takes staged terms
+// g, g+7, ...  The 7 partial blocks are summed in LDS in a fixed order and
+// written as the item's partial.  Splitting blocks into items bounds the
+// serial work per wavefront: a diagonal block covers every observation of
+// its camera.
 // ---------------------------------------------------------------------------
+constexpr int kStageTerms = 32;  // terms per staging round (LDS per wavefront sets the occupancy)
+
 template <typename real>
 __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real> b, BAWork w) {
-  __shared__ real red[7][81];
-  __shared__ double vec[7][27];  // per group: rhs (9), g_c (9), column norms (9)
+  // staging buffers during the rounds; the group partials reuse the space
+  union Lds {
+    struct {
+      real L[kStageTerms][18];   // Jc_a of each staged term (kind 1: the observation's Jc)
+      real R[kStageTerms][18];   // (P_a P_b^T) Jc_b (kind 1: Jc again)
+      double V[kStageTerms][4];  // kind 1: e0, e1, r0, r1
+    } stage;
+    struct {
+      real red[7][81];
+      double vec[7][27];  // per group: rhs (9), g_c (9), column norms (9)
+    } out;
+  };
+  __shared__ Lds lds;
+  auto& sL = lds.stage.L;
+  auto& sR = lds.stage.R;
+  auto& sV = lds.stage.V;
+  auto& red = lds.out.red;
+  auto& vec = lds.out.vec;
   const LMState* st = w.state;
   if (st->done) return;
-  const BlockItem it = w.items[blockIdx.x];
+  const int slot = w.item_perm[blockIdx.x];
+  const BlockItem it = w.items[slot];
   const int tid = threadIdx.x;
   const int grp = tid / 9;
   const int br = (tid % 9) / 3;
   const int bc = (tid % 9) % 3;
+  const bool diag_obs = it.kind == 1;
   real acc[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[r][c] = 0;
-  if (grp < 7) {
-    if (it.kind == 1) {
-      double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
-      for (int t = it.begin + grp; t < it.end; t += 7) {
-        const int o = v.camera_obs[t];
-        const real* J = b.Jc + 18 * (size_t)o;
-        real ja0[3], ja1[3], jb0[3], jb1[3];
+  double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+  for (int base = it.begin; base < it.end; base += kStageTerms) {
+    const int cnt = min(kStageTerms, it.end - base);
+    if (tid < cnt) {
+      if (diag_obs) {
+        const int o = v.camera_obs[base + tid];
+        const real* J = b.JP + kJPStride * (size_t)o;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          ja0[r] = J[3 * br + r];
-          ja1[r] = J[9 + 3 * br + r];
-          jb0[r] = J[3 * bc + r];
-          jb1[r] = J[9 + 3 * bc + r];
+        for (int k = 0; k < 18; ++k) {
+          const real jv = J[k];
+          sL[tid][k] = jv;
+          sR[tid][k] = jv;
         }
+        sV[tid][0] = (double)b.e[2 * (size_t)o];
+        sV[tid][1] = (double)b.e[2 * (size_t)o + 1];
+        sV[tid][2] = b.r[2 * (size_t)o];
+        sV[tid][3] = b.r[2 * (size_t)o + 1];
+      } else {
+        const int oa = v.block_terms[2 * (size_t)(base + tid)];
+        const int ob = v.block_terms[2 * (size_t)(base + tid) + 1];
+        const real* Pa = b.JP + kJPStride * (size_t)oa + 18;
+        const real* Pb = b.JP + kJPStride * (size_t)ob + 18;
+        real pa[6], pb[6];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 6; ++k) {
+          pa[k] = Pa[k];
+          pb[k] = Pb[k];
+        }
+        const real m00 = pa[0] * pb[0] + pa[1] * pb[1] + pa[2] * pb[2];
+        const real m01 = pa[0] * pb[3] + pa[1] * pb[4] + pa[2] * pb[5];
+        const real m10 = pa[3] * pb[0] + pa[4] * pb[1] + pa[5] * pb[2];
+        const real m11 = pa[3] * pb[3] + pa[4] * pb[4] + pa[5] * pb[5];
+        const real* Ja = b.JP + kJPStride * (size_t)oa;
+        const real* Jb = b.JP + kJPStride * (size_t)ob;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) acc[r][c] += ja0[r] * jb0[c] + ja1[r] * jb1[c];
-        if (bc == 0) {
-          const double e0 = (double)b.e[2 * (size_t)o], e1 = (double)b.e[2 * (size_t)o + 1];
-          const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            rv[r] += (double)ja0[r] * e0 + (double)ja1[r] * e1;
-            gv[r] += (double)ja0[r] * r0 + (double)ja1[r] * r1;
-          }
+        for (int k = 0; k < 9; ++k) {
+          const real b0 = Jb[k], b1 = Jb[9 + k];
+          sR[tid][k] = m00 * b0 + m01 * b1;
+          sR[tid][9 + k] = m10 * b0 + m11 * b1;
+          sL[tid][k] = Ja[k];
+          sL[tid][9 + k] = Ja[9 + k];
         }
       }
+    }
+    __syncthreads();
+    if (grp < 7) {
+      for (int t = grp; t < cnt; t += 7) {
+        real l0[3], l1[3], r0[3], r1[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          l0[k] = sL[t][3 * br + k];
+          l1[k] = sL[t][9 + 3 * br + k];
+          r0[k] = sR[t][3 * bc + k];
+          r1[k] = sR[t][9 + 3 * bc + k];
+        }
+        if (diag_obs) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[r][c] += l0[r] * r0[c] + l1[r] * r1[c];
+          if (bc == 0) {
+            const double e0 = sV[t][0], e1 = sV[t][1], q0 = sV[t][2], q1 = sV[t][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              rv[r] += (double)l0[r] * e0 + (double)l1[r] * e1;
+              gv[r] += (double)l0[r] * q0 + (double)l1[r] * q1;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[r][c] -= l0[r] * r0[c] + l1[r] * r1[c];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (grp < 7) {
+    if (diag_obs) {
       if (br == bc) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) vec[grp][18 + 3 * br + r] = (double)acc[r][r];
@@ -394,33 +470,6 @@ __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real
           vec[grp][9 + 3 * br + r] = gv[r];
         }
       }
-    } else {
-#pragma unroll 2
-      for (int t = it.begin + grp; t < it.end; t += 7) {
-        const int oa = v.block_terms[2 * (size_t)t];
-        const int ob = v.block_terms[2 * (size_t)t + 1];
-        const real* Pa = b.Pf + 6 * (size_t)oa;
-        const real* Pb = b.Pf + 6 * (size_t)ob;
-        const real m00 = Pa[0] * Pb[0] + Pa[1] * Pb[1] + Pa[2] * Pb[2];
-        const real m01 = Pa[0] * Pb[3] + Pa[1] * Pb[4] + Pa[2] * Pb[5];
-        const real m10 = Pa[3] * Pb[0] + Pa[4] * Pb[1] + Pa[5] * Pb[2];
-        const real m11 = Pa[3] * Pb[3] + Pa[4] * Pb[4] + Pa[5] * Pb[5];
-        const real* Ja = b.Jc + 18 * (size_t)oa;
-        const real* Jb = b.Jc + 18 * (size_t)ob;
-        real ja0[3], ja1[3], t0v[3], t1v[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const real b0 = Jb[3 * bc + c], b1 = Jb[9 + 3 * bc + c];
-          t0v[c] = m00 * b0 + m01 * b1;
-          t1v[c] = m10 * b0 + m11 * b1;
-          ja0[c] = Ja[3 * br + c];
-          ja1[c] = Ja[9 + 3 * br + c];
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) acc[r][c] -= ja0[r] * t0v[c] + ja1[r] * t1v[c];
-      }
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -428,16 +477,16 @@ __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real
       for (int c = 0; c < 3; ++c) red[grp][(3 * br + r) * 9 + 3 * bc + c] = acc[r][c];
   }
   __syncthreads();
-  real* out = b.item_S + 81 * (size_t)blockIdx.x;
+  real* out = b.item_S + 81 * (size_t)slot;
   for (int idx = tid; idx < 81; idx += kWave) {
     real s = 0;
     for (int gi = 0; gi < 7; ++gi) s += red[gi][idx];
     out[idx] = s;
   }
-  if (it.kind == 1 && tid < 27) {
+  if (diag_obs && tid < 27) {
     double s = 0.0;
     for (int gi = 0; gi < 7; ++gi) s += vec[gi][tid];
-    w.item_vec[27 * (size_t)blockIdx.x + tid] = s;
+    w.item_vec[27 * (size_t)slot + tid] = s;
   }
 }
 
@@ -519,7 +568,7 @@ __device__ __forceinline__ void backsub_point(const BAView& v, const BABuffers<r
   double s0 = (double)b.q[3 * p], s1 = (double)b.q[3 * p + 1], s2 = (double)b.q[3 * p + 2];
   for (int o = o0; o < o1; ++o) {
     const int ci = v.obs_camera[o];
-    const real* J = b.Jc + 18 * (size_t)o;
+    const real* J = b.JP + kJPStride * (size_t)o;
     double y0 = 0.0, y1 = 0.0;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
@@ -527,7 +576,7 @@ __device__ __forceinline__ void backsub_point(const BAView& v, const BABuffers<r
       y0 += (double)J[j] * d;
       y1 += (double)J[9 + j] * d;
     }
-    const real* P = b.Pf + 6 * (size_t)o;
+    const real* P = b.JP + kJPStride * (size_t)o + 18;
     s0 += (double)P[0] * y0 + (double)P[3] * y1;
     s1 += (double)P[1] * y0 + (double)P[4] * y1;
     s2 += (double)P[2] * y0 + (double)P[5] * y1;
@@ -572,10 +621,10 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
   double rp2 = -b.gp[3 * p + 2] - w.Dp[3 * p + 2] * dp2;
   for (int o = o0; o < o1; ++o) {
     const int ci = v.obs_camera[o];
-    const real* J = b.Jc + 18 * (size_t)o;
-    const real* JP = b.Jp + 6 * (size_t)o;
-    double u0 = (double)JP[0] * dp0 + (double)JP[1] * dp1 + (double)JP[2] * dp2;
-    double u1 = (double)JP[3] * dp0 + (double)JP[4] * dp1 + (double)JP[5] * dp2;
+    const real* J = b.JP + kJPStride * (size_t)o;
+    const real* Jq = b.Jp + 6 * (size_t)o;
+    double u0 = (double)Jq[0] * dp0 + (double)Jq[1] * dp1 + (double)Jq[2] * dp2;
+    double u1 = (double)Jq[3] * dp0 + (double)Jq[4] * dp1 + (double)Jq[5] * dp2;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       const double d = w.dxc[9 * ci + j];
@@ -584,9 +633,9 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
     }
     w.u_obs[4 * (size_t)o] = u0;
     w.u_obs[4 * (size_t)o + 1] = u1;
-    rp0 -= (double)JP[0] * u0 + (double)JP[3] * u1;
-    rp1 -= (double)JP[1] * u0 + (double)JP[4] * u1;
-    rp2 -= (double)JP[2] * u0 + (double)JP[5] * u1;
+    rp0 -= (double)Jq[0] * u0 + (double)Jq[3] * u1;
+    rp1 -= (double)Jq[1] * u0 + (double)Jq[4] * u1;
+    rp2 -= (double)Jq[2] * u0 + (double)Jq[5] * u1;
   }
   const real* Li = b.Linv + 6 * (size_t)p;
   const double t0 = (double)Li[0] * rp0;
@@ -596,7 +645,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
   w.res_p[3 * p + 1] = t1;
   w.res_p[3 * p + 2] = t2;
   for (int o = o0; o < o1; ++o) {
-    const real* P = b.Pf + 6 * (size_t)o;
+    const real* P = b.JP + kJPStride * (size_t)o + 18;
     w.u_obs[4 * (size_t)o + 2] = (double)P[0] * t0 + (double)P[1] * t1 + (double)P[2] * t2;
     w.u_obs[4 * (size_t)o + 3] = (double)P[3] * t0 + (double)P[4] * t1 + (double)P[5] * t2;
   }
@@ -615,7 +664,7 @@ __global__ __launch_bounds__(kWave) void k_refine_cameras(BAView v, BABuffers<re
   for (int j = 0; j < 9; ++j) acc[j] = 0.0;
   for (int t = v.camera_offsets[ci] + tid; t < v.camera_offsets[ci + 1]; t += kWave) {
     const int o = v.camera_obs[t];
-    const real* J = b.Jc + 18 * (size_t)o;
+    const real* J = b.JP + kJPStride * (size_t)o;
     const double a0 = w.u_obs[4 * (size_t)o] + w.u_obs[4 * (size_t)o + 2];
     const double a1 = w.u_obs[4 * (size_t)o + 1] + w.u_obs[4 * (size_t)o + 3];
 #pragma unroll
@@ -642,7 +691,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_update(BAView v, BA
     double s0 = w.res_p[3 * p], s1 = w.res_p[3 * p + 1], s2 = w.res_p[3 * p + 2];
     for (int o = v.point_offsets[p]; o < v.point_offsets[p + 1]; ++o) {
       const int ci = v.obs_camera[o];
-      const real* J = b.Jc + 18 * (size_t)o;
+      const real* J = b.JP + kJPStride * (size_t)o;
       double y0 = 0.0, y1 = 0.0;
 #pragma unroll
       for (int j = 0; j < 9; ++j) {
@@ -650,7 +699,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_update(BAView v, BA
         y0 += (double)J[j] * d;
         y1 += (double)J[9 + j] * d;
       }
-      const real* P = b.Pf + 6 * (size_t)o;
+      const real* P = b.JP + kJPStride * (size_t)o + 18;
       s0 -= (double)P[0] * y0 + (double)P[3] * y1;
       s1 -= (double)P[1] * y0 + (double)P[4] * y1;
       s2 -= (double)P[2] * y0 + (double)P[5] * y1;
@@ -736,11 +785,11 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_candidate_obs(BAView v, BA
   if (o < v.N) {
     const int ci = v.obs_camera[o];
     const int p = v.obs_point[o];
-    const real* J = b.Jc + 18 * (size_t)o;
-    const real* JP = b.Jp + 6 * (size_t)o;
+    const real* J = b.JP + kJPStride * (size_t)o;
+    const real* Jq = b.Jp + 6 * (size_t)o;
     const double d0 = w.dxp[3 * p], d1 = w.dxp[3 * p + 1], d2 = w.dxp[3 * p + 2];
-    double u0 = (double)JP[0] * d0 + (double)JP[1] * d1 + (double)JP[2] * d2;
-    double u1 = (double)JP[3] * d0 + (double)JP[4] * d1 + (double)JP[5] * d2;
+    double u0 = (double)Jq[0] * d0 + (double)Jq[1] * d1 + (double)Jq[2] * d2;
+    double u1 = (double)Jq[3] * d0 + (double)Jq[4] * d1 + (double)Jq[5] * d2;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       const double d = w.dxc[9 * ci + j];

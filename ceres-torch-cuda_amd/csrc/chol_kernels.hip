@@ -184,9 +184,13 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __re
   // keep the pivots' reciprocals for later triangular solves with this factor
   for (int i = tid; i < n; i += kSmallThreads) {
     tmp[n + i] = dinv[i];
-    y[i] = rhs[i];
+    y[i] = rhs ? rhs[i] : 0.0;
   }
   __syncthreads();
+  if (rhs == nullptr) {  // factorisation only
+    if (tid == 0 && st && bad) st->step_invalid = 1;
+    return;
+  }
   // forward substitution L y = rhs
   for (int k0 = 0; k0 < n; k0 += kSmallNB) {
     const int kb = min(kSmallNB, n - k0);
@@ -609,8 +613,8 @@ __global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __
     const int r0 = (nt - 1 - q0) * kTB;
     if (tid < kTB && r0 + tid < n && !isfinite(x[r0 + tid])) my_bad = 1;
   }
-  if (my_bad || timed_out) st->step_invalid = 1;
-  if (g == 0 && tid == 0) st->linear_iterations = 1;
+  if (st && (my_bad || timed_out)) st->step_invalid = 1;
+  if (st && g == 0 && tid == 0) st->linear_iterations = 1;
   // the last workgroup to finish advances the epoch for the next launch
   __syncthreads();
   if (tid == 0) {
@@ -968,10 +972,19 @@ int launch_tri_mt(const double* L, const double* rhs, double* x, int n, LMState*
 }
 }  // namespace
 
+namespace {
+__global__ void k_zero_words(double* p, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+}  // namespace
+
 int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s) {
-  // zero the flags and epoch of the many-workgroup triangular solves (once, before first use)
+  // zero the flags and epoch of the many-workgroup triangular solves (a fill
+  // kernel, so the call may sit inside a captured graph)
   const size_t nt = tri_blocks(n);
-  CT_HIP(hipMemsetAsync(tmp + 2 * (size_t)n + nt * kTB * kTB, 0, (nt + 2) * sizeof(double), s));
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, s, tmp + 2 * (size_t)n + nt * kTB * kTB, nt + 2);
+  CT_LAUNCH_CHECK();
   return CT_OK;
 }
 
@@ -1000,6 +1013,7 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
   hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned int)tri_blocks(n)), dim3(kWave), 0, s, S,
                      tmp + 2 * (size_t)n, n, st);
   CT_LAUNCH_CHECK();
+  if (rhs == nullptr) return CT_OK;  // factorisation only
   return launch_tri_mt(S, rhs, x, n, st, tmp, s);
 }
 

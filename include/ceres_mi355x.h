@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 3
+#define CT_ABI_VERSION 4
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -284,6 +284,34 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
 
 /* Host check of offsets (and order) of a batch; synchronous.                */
 int ct_curve_validate(const ct_curve_batch* batch, int32_t* first_bad_problem);
+
+/* ------------------------------------------------------------------------ */
+/* Dense damped normal equations for a row-major A [m x n] (leading dimension
+ * lda): x = (A^T A + diag(damping))^-1 A^T b, damping may be NULL.  Replaces
+ * the reference's native normal_equations_solve / block_schur_solve
+ * (ceres_torch_cuda.cpp:3-16, cuda_backends.py:267-307) behind the plugin
+ * names sparse_normal_cholesky, sparse_cholesky, sparse_schur, block_schur.
+ * FP64.  Synchronises the stream to report CT_ERR_NOT_SPD.                  */
+/* ------------------------------------------------------------------------ */
+int ct_dense_workspace_bytes(int64_t n, size_t* bytes);
+int ct_normal_equations_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
+                                  const double* damping, double* x, void* workspace, size_t workspace_bytes,
+                                  ct_stream_t stream);
+/* Schur complement solve with the first num_eliminate unknowns eliminated
+ * (cuda_backends.py:232-264); the solution is that of the whole system.     */
+int ct_block_schur_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
+                             int64_t num_eliminate, const double* damping, double* x, void* workspace,
+                             size_t workspace_bytes, ct_stream_t stream);
+
+/* Standalone FP64 Cholesky of an SPD S [n x n] row-major (lower triangle
+ * used): potrf factors in place; potrs solves L L^T x = rhs with that factor
+ * and the same workspace (torch.linalg.cholesky / cholesky_solve at
+ * linear.py:1151, 1170).  Both synchronise the stream to report
+ * CT_ERR_NOT_SPD.                                                            */
+int ct_rcs_workspace_bytes(int64_t n, size_t* bytes);
+int ct_rcs_potrf_f64(double* S, int64_t n, void* workspace, size_t workspace_bytes, ct_stream_t stream);
+int ct_rcs_potrs_f64(const double* L, int64_t n, const double* rhs, double* x, void* workspace,
+                     size_t workspace_bytes, ct_stream_t stream);
 
 #ifdef __cplusplus
 }

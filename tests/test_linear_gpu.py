@@ -1,0 +1,108 @@
+"""Plugin-registry backends (dense damped normal equations) and the standalone Cholesky on the MI355X.
+
+Reference semantics (cuda_backends.py:215-307): x solves (A^T A + diag(d)) x = A^T b.
+The oracle here is the textbook identity with the damped least-squares problem
+min |A x - b|^2 + sum d x^2, solved by numpy lstsq on [A; sqrt(d)] in float64.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from ceres_mi355x import _native as nat
+from ceres_mi355x import linear as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _damped_lstsq(A, b, d):
+    aug = np.vstack([A, np.diag(np.sqrt(d))]) if d is not None else A
+    rhs = np.concatenate([b, np.zeros(A.shape[1])]) if d is not None else b
+    return np.linalg.lstsq(aug, rhs, rcond=None)[0]
+
+
+@pytest.mark.parametrize("m,n", [(40, 7), (300, 144), (900, 257), (2000, 520)])
+@pytest.mark.parametrize("damped", [False, True])
+def test_normal_equations_backend_matches_damped_least_squares(m, n, damped):
+    rng = np.random.default_rng(m + n)
+    A = rng.normal(size=(m, n))
+    b = rng.normal(size=m)
+    d = rng.uniform(0.1, 2.0, size=n) if damped else None
+    ref = _damped_lstsq(A, b, d)
+    At = torch.tensor(A, device="cuda")
+    res = L.rocm_normal_equations(At, torch.tensor(b, device="cuda"),
+                                  damping=None if d is None else torch.tensor(d, device="cuda"))
+    np.testing.assert_allclose(res.x.cpu().numpy(), ref, rtol=1e-9, atol=1e-11)
+    assert res.summary.success and res.summary.num_iterations == 1
+    # the reported residual is that of the damped least-squares problem
+    expect = np.sqrt(np.sum((A @ ref - b) ** 2) + (0.0 if d is None else np.sum(d * ref * ref)))
+    assert res.summary.residual_norm == pytest.approx(expect, rel=1e-9)
+
+
+def test_block_schur_backend_same_solution_and_float32():
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(120, 30))
+    b = rng.normal(size=120)
+    d = rng.uniform(0.5, 1.0, size=30)
+    ref = _damped_lstsq(A, b, d)
+    r64 = L.rocm_block_schur(torch.tensor(A, device="cuda"), torch.tensor(b, device="cuda"),
+                             damping=torch.tensor(d, device="cuda"), num_eliminate=12)
+    np.testing.assert_allclose(r64.x.cpu().numpy(), ref, rtol=1e-9, atol=1e-11)
+    r32 = L.rocm_block_schur(torch.tensor(A, device="cuda", dtype=torch.float32),
+                             torch.tensor(b, device="cuda", dtype=torch.float32),
+                             damping=torch.tensor(d, device="cuda", dtype=torch.float32), num_eliminate=12)
+    assert r32.x.dtype == torch.float32
+    np.testing.assert_allclose(r32.x.cpu().numpy(), ref, rtol=1e-4, atol=1e-5)
+
+
+def test_registry_contract_declines_host_tensors_and_dispatches():
+    L.clear_optional_backends()
+    names = L.register_rocm_backends()
+    assert set(names) == set(L.ROCM_BACKEND_NAMES)
+    A = torch.randn(50, 8, dtype=torch.float64)
+    b = torch.randn(50, dtype=torch.float64)
+    assert L.call_optional_backend("block_schur", A, b, num_eliminate=3) is None  # CPU tensors: declined
+    res = L.call_optional_backend("sparse_normal_cholesky", A.cuda(), b.cuda(), damping=None, num_eliminate=0,
+                                  min_iterations=0, max_iterations=500)
+    ref = np.linalg.lstsq(A.numpy(), b.numpy(), rcond=None)[0]
+    np.testing.assert_allclose(res.x.cpu().numpy(), ref, rtol=1e-9, atol=1e-11)
+    L.unregister_rocm_backends()
+    assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
+
+
+@pytest.mark.parametrize("n", [9, 144, 300, 1800])
+def test_standalone_potrf_potrs(n):
+    rng = np.random.default_rng(n)
+    M = rng.normal(size=(n, n))
+    S = M @ M.T + n * np.eye(n)
+    rhs = rng.normal(size=n)
+    lib = nat.load()
+    nbytes = ctypes.c_size_t(0)
+    assert lib.ct_rcs_workspace_bytes(n, ctypes.byref(nbytes)) == 0
+    ws = torch.zeros(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    St = torch.tensor(S, device="cuda")
+    assert lib.ct_rcs_potrf_f64(nat.ptr(St), n, nat.ptr(ws), ws.numel(), nat.current_stream_handle()) == 0
+    Lf = np.tril(St.cpu().numpy())
+    np.testing.assert_allclose(Lf, np.linalg.cholesky(S), rtol=1e-10, atol=1e-10)
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    for k in range(2):  # two solves with one factor
+        r = torch.tensor(rhs * (k + 1), device="cuda")
+        assert lib.ct_rcs_potrs_f64(nat.ptr(St), n, nat.ptr(r), nat.ptr(x), nat.ptr(ws), ws.numel(),
+                                    nat.current_stream_handle()) == 0
+        np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(S, rhs * (k + 1)), rtol=1e-9, atol=1e-12)
+
+
+def test_potrf_reports_not_positive_definite():
+    n = 40
+    S = np.eye(n)
+    S[17, 17] = -1.0
+    lib = nat.load()
+    nbytes = ctypes.c_size_t(0)
+    lib.ct_rcs_workspace_bytes(n, ctypes.byref(nbytes))
+    ws = torch.zeros(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    St = torch.tensor(S, device="cuda")
+    assert lib.ct_rcs_potrf_f64(nat.ptr(St), n, nat.ptr(ws), ws.numel(), nat.current_stream_handle()) == \
+        nat.CT_ERR_NOT_SPD

@@ -156,3 +156,24 @@ def test_kernel_group_names_follow_header_enum():
     names = [name.lower() for name, idx in sorted(groups, key=lambda g: int(g[1])) if name != "COUNT"]
     assert names == nat.KERNELS
     assert int(dict(groups)["COUNT"]) == len(nat.KERNELS)
+
+
+def test_registry_declines_host_tensors_like_the_reference():
+    import torch
+
+    from ceres_mi355x import linear as L
+
+    L.clear_optional_backends()
+    assert set(L.register_rocm_backends()) == set(L.ROCM_BACKEND_NAMES)
+    A = torch.randn(20, 4, dtype=torch.float64)
+    b = torch.randn(20, dtype=torch.float64)
+    for name in L.ROCM_BACKEND_NAMES:
+        assert L.call_optional_backend(name, A, b, damping=None, num_eliminate=1) is None
+    with pytest.raises(L.OptionalBackendUnavailable):
+        L.rocm_normal_equations(A.to(torch.int64), b)
+    # overwrite=False keeps an existing registration
+    L.register_optional_backend("block_schur", lambda A, b, **kw: "mine")
+    L.register_rocm_backends(overwrite=False)
+    assert L.get_optional_backend("block_schur")(A, b) == "mine"
+    L.unregister_rocm_backends()
+    assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
