@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 4  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 5  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "step",
            "candidate", "lm_end"]
 REC = dict(
@@ -126,6 +126,23 @@ class ct_curve_batch(ctypes.Structure):
     ]
 
 
+class ct_pose2d_problem(ctypes.Structure):
+    _fields_ = [
+        ("num_nodes", ctypes.c_int32),
+        ("num_edges", ctypes.c_int32),
+        ("num_pairs", ctypes.c_int32),
+        ("fixed_node", ctypes.c_int32),
+        ("edges", _vp),
+        ("measurements", _vp),
+        ("node_offsets", _vp),
+        ("node_edges", _vp),
+        ("pair_nodes", _vp),
+        ("pair_offsets", _vp),
+        ("pair_edges", _vp),
+        ("loss", ct_loss),
+    ]
+
+
 class ct_batch_schedule(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("groups", ctypes.c_int32)]
 
@@ -154,6 +171,12 @@ SIGNATURES = {
     "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
                                       ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_curve_validate": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_int32)]),
+    "ct_pose2d_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_pose2d_problem), ctypes.POINTER(ct_lm_options),
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_pose2d_plan_create": (ctypes.c_int, [ctypes.POINTER(ct_pose2d_problem), ctypes.POINTER(ct_lm_options), _vp,
+                                             _vp, ctypes.c_size_t, _vp, ctypes.POINTER(_vp)]),
+    "ct_pose2d_plan_run": (ctypes.c_int, [_vp, ctypes.POINTER(ct_lm_report), _vp]),
+    "ct_pose2d_plan_destroy": (ctypes.c_int, [_vp]),
     "ct_dense_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "ct_normal_equations_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                                      _vp, _vp, ctypes.c_size_t, _vp]),

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "ba_kernels.h"
+#include "lm_plan.h"
 
 namespace ct {
 
@@ -172,12 +173,7 @@ struct ct_ba_plan {
   double* user_cams = nullptr;
   double* user_pts = nullptr;
   hipStream_t stream = nullptr;
-  hipStream_t capture_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_iterations = 0;
-  LMState* host_state = nullptr;  // pinned
-  double wall_rate_hz = 1e8;
+  LMDriver driver;
   int num_items = 0;
 };
 
@@ -243,11 +239,7 @@ int enqueue_iteration(ct_ba_plan* pl, hipStream_t s, bool evaluation_only = fals
   return CT_OK;
 }
 
-int read_state(ct_ba_plan* pl, hipStream_t s) {
-  CT_HIP(hipMemcpyAsync(pl->host_state, pl->layout.w.state, sizeof(LMState), hipMemcpyDeviceToHost, s));
-  CT_HIP(hipStreamSynchronize(s));
-  return CT_OK;
-}
+int read_state(ct_ba_plan* pl, hipStream_t s) { return lm_driver_read_state(pl->driver, s); }
 
 template <typename T>
 int copy_back(std::vector<T>& h, const T* d, size_t count) {
@@ -417,25 +409,6 @@ int build_items(ct_ba_plan* pl) {
   return CT_OK;
 }
 
-int build_graph(ct_ba_plan* pl) {
-  if (pl->graph_exec) return CT_OK;
-  CT_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
-  int st = CT_OK;
-  for (int k = 0; k < pl->graph_iterations && st == CT_OK; ++k) st = enqueue_iteration(pl, pl->capture_stream);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
-  if (st != CT_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return st;
-  }
-  if (e != hipSuccess) {
-    set_error(std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e));
-    return CT_ERR_HIP;
-  }
-  pl->graph = g;
-  CT_HIP(hipGraphInstantiate(&pl->graph_exec, g, nullptr, nullptr, 0));
-  return CT_OK;
-}
 
 }  // namespace
 
@@ -483,17 +456,26 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->init.num_cameras = problem->num_cameras;
   pl->init.n_cam_params = 9 * problem->num_cameras;
   pl->init.n_pt_params = 3 * problem->num_points;
-  int per = options->iterations_per_graph > 0 ? options->iterations_per_graph : 8;
-  if (per > options->max_num_iterations + 2) per = options->max_num_iterations + 2;
-  pl->graph_iterations = per;
   CT_TRY(build_items(pl.get()));
   if (options->linear_solver == CT_DENSE_SCHUR && problem->num_cameras > 0) {
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));
   }
-  CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pl->host_state), sizeof(LMState), hipHostMallocDefault));
-  if (options->loop_mode == CT_LOOP_DEVICE) {
-    CT_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
+  ct_ba_plan* raw = pl.get();
+  LMDriver& d = pl->driver;
+  d.enqueue_iteration = [raw](hipStream_t s, bool evaluation_only) {
+    return enqueue_iteration(raw, s, evaluation_only);
+  };
+  d.launch_init = [raw](hipStream_t s) {
+    return launch_lm_init(raw->init, raw->layout.w, raw->user_cams, raw->user_pts, s);
+  };
+  d.launch_finalize = [raw](hipStream_t s) {
+    return launch_lm_finalize(raw->init, raw->layout.w, raw->user_cams, raw->user_pts, s);
+  };
+  const int st = lm_driver_setup(d, *options, pl->stream, pl->layout.w.state, pl->layout.w.records);
+  if (st != CT_OK) {
+    lm_driver_release(d);
+    return st;
   }
   *plan = pl.release();
   return CT_OK;
@@ -502,10 +484,7 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
 int ct_ba_plan_destroy(ct_ba_plan* plan) {
   clear_error();
   if (!plan) return CT_OK;
-  if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
-  if (plan->graph) (void)hipGraphDestroy(plan->graph);
-  if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
-  if (plan->host_state) (void)hipHostFree(plan->host_state);
+  lm_driver_release(plan->driver);
   delete plan;
   return CT_OK;
 }
@@ -513,67 +492,7 @@ int ct_ba_plan_destroy(ct_ba_plan* plan) {
 int ct_ba_plan_run(ct_ba_plan* pl, ct_lm_report* report, double* records) {
   clear_error();
   CT_CHECK_ARG(pl != nullptr, "plan is null");
-  const int max_passes = pl->options.max_num_iterations + 2;  // iterations 0..max + deferred checks
-  hipStream_t s = pl->stream;
-  if (pl->options.loop_mode == CT_LOOP_DEVICE) {
-    // order the capture stream after the caller's stream and back
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    CT_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
-    CT_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
-    CT_HIP(hipEventRecord(ev_in, s));
-    CT_HIP(hipStreamWaitEvent(pl->capture_stream, ev_in, 0));
-    int st = build_graph(pl);
-    if (st != CT_OK) {
-      (void)hipEventDestroy(ev_in);
-      (void)hipEventDestroy(ev_out);
-      return st;
-    }
-    hipStream_t cs = pl->capture_stream;
-    CT_TRY(launch_lm_init(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, cs));
-    const int launches = (max_passes + pl->graph_iterations - 1) / pl->graph_iterations;
-    for (int k = 0; k < launches; ++k) CT_HIP(hipGraphLaunch(pl->graph_exec, cs));
-    CT_TRY(launch_lm_finalize(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, cs));
-    CT_HIP(hipEventRecord(ev_out, cs));
-    CT_HIP(hipStreamWaitEvent(s, ev_out, 0));
-    (void)hipEventDestroy(ev_in);
-    (void)hipEventDestroy(ev_out);
-    CT_TRY(read_state(pl, s));
-  } else {
-    CT_TRY(launch_lm_init(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, s));
-    for (int k = 0; k < max_passes; ++k) {
-      // pass 0 only evaluates; the last pass only runs the deferred checks of
-      // the last accepted step and ends at the iteration limit
-      CT_TRY(enqueue_iteration(pl, s, k == 0 || k == max_passes - 1));
-      CT_TRY(read_state(pl, s));  // the one host sync of the iteration
-      if (pl->host_state->done) break;
-    }
-    CT_TRY(launch_lm_finalize(pl->init, pl->layout.w, pl->user_cams, pl->user_pts, s));
-    CT_TRY(read_state(pl, s));
-  }
-  const LMState& h = *pl->host_state;
-  if (report) {
-    report->termination = h.termination;
-    report->reason = h.reason;
-    report->num_records = h.num_records;
-    report->num_successful_steps = h.num_successful;
-    report->num_unsuccessful_steps = h.num_unsuccessful;
-    report->num_residual_evaluations = h.num_residual_evals;
-    report->num_jacobian_evaluations = h.num_jacobian_evals;
-    report->num_linear_solves = h.num_linear_solves;
-    report->initial_cost = h.initial_cost;
-    report->final_cost = h.best_cost;
-    report->device_seconds = (double)(h.t_end - h.t0) * 1e-8;
-  }
-  if (records) {
-    const size_t nrec = (size_t)(pl->options.max_num_iterations + 2) * CT_REC_FIELDS;
-    CT_HIP(hipMemcpyAsync(records, pl->layout.w.records, nrec * sizeof(double), hipMemcpyDeviceToHost, s));
-    CT_HIP(hipStreamSynchronize(s));
-  }
-  if (!h.done) {
-    set_error("LM loop did not terminate within the pass budget");
-    return CT_ERR_HIP;
-  }
-  return CT_OK;
+  return lm_driver_run(pl->driver, report, records);
 }
 
 int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_t* kernel_launches) {
@@ -611,7 +530,7 @@ int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_
   if (status == CT_OK) {
     // LM iterations 1..completed ran every group with work (a termination in
     // the begin kernel leaves that iteration's later groups without work)
-    int completed = pl->host_state->iteration - 1;
+    int completed = pl->driver.host_state->iteration - 1;
     if (completed > iterations) completed = iterations;
     for (int it = 0; it < completed; ++it) {
       hipEvent_t* e = &ev[(size_t)it * per];

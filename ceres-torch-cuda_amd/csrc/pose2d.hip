@@ -1,0 +1,578 @@
+// SE2 pose-graph LM solve on the device (examples/pose_graph_2d.py:6-46; the
+// trust-region loop of solver.py:35-357).
+//
+// Residual of an edge (a, b) with measurement m: relative_pose(a, b) - m,
+// relative_pose = (R(theta_a)^T (p_b - p_a), wrap(theta_b - theta_a))
+// (pose_graph_2d.py:10-15), with a robust loss per edge.  One node is held
+// constant (pose_graph_2d.py:34).  The reference's DENSE_QR step is the
+// solution of the damped normal equations, assembled here densely and
+// deterministically: per-node diagonal blocks (incident edges in table order)
+// and per-node-pair off-diagonal blocks (the pair's edges in table order),
+// then the FP64 Cholesky of the reduced-camera-system solver.
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ba_kernels.h"
+#include "lm_logic.h"
+#include "lm_plan.h"
+
+namespace ct {
+
+namespace {
+
+constexpr int kPgThreads = 256;
+
+struct PGView {
+  int32_t N, E, P, fixed, n;
+  const int32_t* __restrict__ edges;
+  const double* __restrict__ meas;
+  const int32_t* __restrict__ node_offsets;
+  const int32_t* __restrict__ node_edges;
+  const int32_t* __restrict__ pair_nodes;
+  const int32_t* __restrict__ pair_offsets;
+  const int32_t* __restrict__ pair_edges;
+  ct_loss loss;
+};
+
+struct PGWork {
+  double* x;          // [2][N][3] ping-pong poses
+  double* r;          // [E][3]    corrected residuals
+  double* Ja;         // [E][9]    corrected d r / d pose a (row-major 3x3)
+  double* Jb;         // [E][9]
+  double* H;          // [n][n]    damped normal matrix
+  double* rhs;        // [n]       -g
+  double* dx;         // [n]
+  double* part_cost;  // [Ge]
+  double* part_grad;  // [Gn][2]   |g|^2, max|g|
+  double* part_step;  // [Gn][3]   |dx|^2, |x'|^2, non-finite
+  double* part_cand;  // [Ge][2]   candidate cost, model decrease
+  double* records;
+  LMState* state;
+  double* solve_tmp;
+};
+
+__device__ __forceinline__ int col_of(const PGView& v, int k) { return 3 * (v.fixed >= 0 && k > v.fixed ? k - 1 : k); }
+
+__device__ __forceinline__ void pose_residual(const double* a, const double* b, const double* m, double (&r)[3],
+                                              double* ca_out, double* sa_out, double* dx_out, double* dy_out) {
+  double sa, ca;
+  sincos(a[2], &sa, &ca);
+  const double dx = b[0] - a[0], dy = b[1] - a[1], dth = b[2] - a[2];
+  double ss, cc;
+  sincos(dth, &ss, &cc);
+  r[0] = ca * dx + sa * dy - m[0];
+  r[1] = -sa * dx + ca * dy - m[1];
+  r[2] = atan2(ss, cc) - m[2];
+  if (ca_out) {
+    *ca_out = ca;
+    *sa_out = sa;
+    *dx_out = dx;
+    *dy_out = dy;
+  }
+}
+
+// Corrector of a 3-residual block (losses.py:158-193); returns 1/2 rho0.
+__device__ __forceinline__ double robustify3(const ct_loss& L, double (&r)[3], double (&Ja)[9], double (&Jb)[9]) {
+  const double s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  if (L.type == CT_LOSS_TRIVIAL && L.scale == 1.0) return 0.5 * s;
+  double rho1, rho2;
+  const double rho0 = loss_rho(L, s, &rho1, &rho2);
+  const double sq = sqrt(fmax(rho1, 0.0));
+  double scale = sq, alpha_s = 0.0;
+  if (s > 0.0 && rho2 > 0.0) {
+    const double alpha = 1.0 - sqrt(1.0 + 2.0 * s * rho2 / rho1);
+    scale = sq / (1.0 - alpha);
+    alpha_s = alpha / s;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double ta = alpha_s * (r[0] * Ja[j] + r[1] * Ja[3 + j] + r[2] * Ja[6 + j]);
+    const double tb = alpha_s * (r[0] * Jb[j] + r[1] * Jb[3 + j] + r[2] * Jb[6 + j]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Ja[3 * k + j] = sq * (Ja[3 * k + j] - r[k] * ta);
+      Jb[3 * k + j] = sq * (Jb[3 * k + j] - r[k] * tb);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r[k] *= scale;
+  return 0.5 * rho0;
+}
+
+__device__ __forceinline__ double robust_cost3(const ct_loss& L, const double (&r)[3]) {
+  const double s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  if (L.type == CT_LOSS_TRIVIAL && L.scale == 1.0) return 0.5 * s;
+  double rho1, rho2;
+  return 0.5 * loss_rho(L, s, &rho1, &rho2);
+}
+
+__global__ __launch_bounds__(kPgThreads) void k_pg_init(PGView v, PGWork w, const double* __restrict__ poses,
+                                                         ct_lm_options opt) {
+  for (int i = blockIdx.x * kPgThreads + threadIdx.x; i < 3 * v.N; i += gridDim.x * kPgThreads) w.x[i] = poses[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) lm_state_init(w.state, opt);
+}
+
+__global__ __launch_bounds__(kPgThreads) void k_pg_finalize(PGView v, PGWork w, double* __restrict__ poses) {
+  const int cur = w.state->cur;
+  for (int i = blockIdx.x * kPgThreads + threadIdx.x; i < 3 * v.N; i += gridDim.x * kPgThreads)
+    poses[i] = w.x[(size_t)cur * 3 * v.N + i];
+}
+
+// residuals and Jacobians of every edge at x; cost partials
+__global__ __launch_bounds__(kPgThreads) void k_pg_eval(PGView v, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done) return;
+  const double* x = w.x + (size_t)st->cur * 3 * v.N;
+  const int e = blockIdx.x * kPgThreads + threadIdx.x;
+  double cost = 0.0;
+  if (e < v.E) {
+    const int a = v.edges[2 * e], b = v.edges[2 * e + 1];
+    double r[3], ca, sa, dx, dy;
+    pose_residual(x + 3 * a, x + 3 * b, v.meas + 3 * e, r, &ca, &sa, &dx, &dy);
+    double Ja[9] = {-ca, -sa, -sa * dx + ca * dy, sa, -ca, -ca * dx - sa * dy, 0.0, 0.0, -1.0};
+    double Jb[9] = {ca, sa, 0.0, -sa, ca, 0.0, 0.0, 0.0, 1.0};
+    cost = robustify3(v.loss, r, Ja, Jb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w.r[3 * e + k] = r[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      w.Ja[9 * (size_t)e + k] = Ja[k];
+      w.Jb[9 * (size_t)e + k] = Jb[k];
+    }
+  }
+  const double t = block_sum<kPgThreads>(cost, red);
+  if (threadIdx.x == 0) w.part_cost[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kPgThreads) void k_pg_zero(PGWork w, size_t count) {
+  if (w.state->done) return;
+  for (size_t i = blockIdx.x * (size_t)kPgThreads + threadIdx.x; i < count; i += (size_t)gridDim.x * kPgThreads)
+    w.H[i] = 0.0;
+}
+
+// diagonal block, damping and gradient of each free node (incident edges in table order)
+__global__ __launch_bounds__(kPgThreads) void k_pg_node(PGView v, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done) return;
+  const int k = blockIdx.x * kPgThreads + threadIdx.x;
+  double g2 = 0.0, gm = 0.0;
+  if (k < v.N && k != v.fixed) {
+    double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+    for (int t = v.node_offsets[k]; t < v.node_offsets[k + 1]; ++t) {
+      const int tag = v.node_edges[t];
+      const int e = tag >> 1;
+      const double* J = ((tag & 1) ? w.Jb : w.Ja) + 9 * (size_t)e;
+      const double* r = w.r + 3 * e;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) h[3 * i + j] += J[i] * J[j] + J[3 + i] * J[3 + j] + J[6 + i] * J[6 + j];
+        g[i] += J[i] * r[0] + J[3 + i] * r[1] + J[6 + i] * r[2];
+      }
+    }
+    const int c = col_of(v, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
+      const double dg = st->jacobi_scaling ? h[4 * i] : 1.0;
+      const double D = fmin(fmax(dg, st->min_diag), st->max_diag) / fmax(st->radius, 2.220446049250313e-16);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) w.H[(size_t)(c + i) * v.n + c + j] = h[3 * i + j] + (i == j ? D : 0.0);
+      w.rhs[c + i] = -g[i];
+    }
+    g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    gm = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
+  }
+  const double t = block_sum<kPgThreads>(g2, red);
+  const double m = block_max<kPgThreads>(gm, red);
+  if (threadIdx.x == 0) {
+    w.part_grad[2 * blockIdx.x] = t;
+    w.part_grad[2 * blockIdx.x + 1] = m;
+  }
+}
+
+// off-diagonal block of each joined node pair u < v (its edges in table order)
+__global__ __launch_bounds__(kPgThreads) void k_pg_pair(PGView v, PGWork w) {
+  if (w.state->done) return;
+  const int p = blockIdx.x * kPgThreads + threadIdx.x;
+  if (p >= v.P) return;
+  const int u = v.pair_nodes[2 * p], q = v.pair_nodes[2 * p + 1];
+  if (u == v.fixed || q == v.fixed) return;
+  double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = v.pair_offsets[p]; t < v.pair_offsets[p + 1]; ++t) {
+    const int tag = v.pair_edges[t];
+    const int e = tag >> 1;
+    const bool u_is_b = tag & 1;
+    const double* Ju = (u_is_b ? w.Jb : w.Ja) + 9 * (size_t)e;
+    const double* Jq = (u_is_b ? w.Ja : w.Jb) + 9 * (size_t)e;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) h[3 * i + j] += Ju[i] * Jq[j] + Ju[3 + i] * Jq[3 + j] + Ju[6 + i] * Jq[6 + j];
+  }
+  const int cu = col_of(v, u), cq = col_of(v, q);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      w.H[(size_t)(cu + i) * v.n + cq + j] = h[3 * i + j];
+      w.H[(size_t)(cq + j) * v.n + cu + i] = h[3 * i + j];
+    }
+}
+
+struct PGCounts {
+  int ge, gn;  // partial rows of edge and node kernels
+};
+
+__global__ __launch_bounds__(kPgThreads) void k_pg_begin(PGCounts c, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  LMState* st = w.state;
+  if (st->done) return;
+  double cs = 0.0, g2 = 0.0, gm = 0.0;
+  for (int i = threadIdx.x; i < c.ge; i += kPgThreads) cs += w.part_cost[i];
+  for (int i = threadIdx.x; i < c.gn; i += kPgThreads) {
+    g2 += w.part_grad[2 * i];
+    gm = fmax(gm, w.part_grad[2 * i + 1]);
+  }
+  const double cost = block_sum<kPgThreads>(cs, red);
+  const double gnorm2 = block_sum<kPgThreads>(g2, red);
+  const double gmax = block_max<kPgThreads>(gm, red);
+  if (threadIdx.x == 0) lm_begin_update(st, cost, gnorm2, gmax, w.records);
+}
+
+// x' = x + dx on the free nodes; |dx|^2, |x'|^2 over all nodes, non-finite count
+__global__ __launch_bounds__(kPgThreads) void k_pg_step(PGView v, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int cur = st->cur;
+  const double* x = w.x + (size_t)cur * 3 * v.N;
+  double* xn = w.x + (size_t)(1 - cur) * 3 * v.N;
+  const int k = blockIdx.x * kPgThreads + threadIdx.x;
+  double dx2 = 0.0, x2 = 0.0, bad = 0.0;
+  if (k < v.N) {
+    const int c = col_of(v, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double d = (k == v.fixed) ? 0.0 : w.dx[c + i];
+      const double nv = x[3 * k + i] + d;
+      xn[3 * k + i] = nv;
+      dx2 += d * d;
+      x2 += nv * nv;
+      if (!isfinite(d)) bad = 1.0;
+    }
+  }
+  const double a0 = block_sum<kPgThreads>(dx2, red);
+  const double a1 = block_sum<kPgThreads>(x2, red);
+  const double a2 = block_sum<kPgThreads>(bad, red);
+  if (threadIdx.x == 0) {
+    w.part_step[3 * blockIdx.x] = a0;
+    w.part_step[3 * blockIdx.x + 1] = a1;
+    w.part_step[3 * blockIdx.x + 2] = a2;
+  }
+}
+
+// candidate cost at x' and the model decrease -r.u - u.u/2, u = Ja dx_a + Jb dx_b
+__global__ __launch_bounds__(kPgThreads) void k_pg_cand(PGView v, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const double* xn = w.x + (size_t)(1 - st->cur) * 3 * v.N;
+  const int e = blockIdx.x * kPgThreads + threadIdx.x;
+  double cost = 0.0, md = 0.0;
+  if (e < v.E) {
+    const int a = v.edges[2 * e], b = v.edges[2 * e + 1];
+    double da[3] = {0.0, 0.0, 0.0}, db[3] = {0.0, 0.0, 0.0};
+    if (a != v.fixed) {
+      const int c = col_of(v, a);
+      da[0] = w.dx[c];
+      da[1] = w.dx[c + 1];
+      da[2] = w.dx[c + 2];
+    }
+    if (b != v.fixed) {
+      const int c = col_of(v, b);
+      db[0] = w.dx[c];
+      db[1] = w.dx[c + 1];
+      db[2] = w.dx[c + 2];
+    }
+    const double* Ja = w.Ja + 9 * (size_t)e;
+    const double* Jb = w.Jb + 9 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double u = Ja[3 * k] * da[0] + Ja[3 * k + 1] * da[1] + Ja[3 * k + 2] * da[2] + Jb[3 * k] * db[0] +
+                       Jb[3 * k + 1] * db[1] + Jb[3 * k + 2] * db[2];
+      md += -w.r[3 * e + k] * u - 0.5 * u * u;
+    }
+    double r[3];
+    pose_residual(xn + 3 * a, xn + 3 * b, v.meas + 3 * e, r, nullptr, nullptr, nullptr, nullptr);
+    cost = robust_cost3(v.loss, r);
+  }
+  const double a0 = block_sum<kPgThreads>(cost, red);
+  const double a1 = block_sum<kPgThreads>(md, red);
+  if (threadIdx.x == 0) {
+    w.part_cand[2 * blockIdx.x] = a0;
+    w.part_cand[2 * blockIdx.x + 1] = a1;
+  }
+}
+
+__global__ __launch_bounds__(kPgThreads) void k_pg_end(PGCounts c, PGWork w) {
+  __shared__ double red[kPgThreads / kWave];
+  LMState* st = w.state;
+  const int done = st->done;
+  const int skip = st->skip_step;
+  __syncthreads();
+  if (done) return;
+  if (skip) {
+    if (threadIdx.x == 0) lm_skip_update(st);
+    return;
+  }
+  double a[5] = {0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < c.ge; i += kPgThreads) {
+    a[0] += w.part_cand[2 * i];
+    a[1] += w.part_cand[2 * i + 1];
+  }
+  for (int i = threadIdx.x; i < c.gn; i += kPgThreads) {
+    a[2] += w.part_step[3 * i];
+    a[3] += w.part_step[3 * i + 1];
+    a[4] += w.part_step[3 * i + 2];
+  }
+  double s[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s[k] = block_sum<kPgThreads>(a[k], red);
+  if (threadIdx.x == 0) lm_end_update(st, s[0], s[1], s[2], s[3], s[4] != 0.0, w.records);
+}
+
+int blocks_of(int n) { return n > 0 ? (n + kPgThreads - 1) / kPgThreads : 1; }
+
+struct PGLayout {
+  PGWork w{};
+  size_t bytes = 0;
+};
+
+PGLayout pg_carve(const ct_pose2d_problem* p, const ct_lm_options* o, void* base) {
+  PGLayout L;
+  Carver cv(base);
+  const size_t N = p->num_nodes, E = p->num_edges;
+  const int n = 3 * (p->num_nodes - (p->fixed_node >= 0 ? 1 : 0));
+  L.w.x = cv.take<double>(2 * 3 * N);
+  L.w.r = cv.take<double>(3 * E);
+  L.w.Ja = cv.take<double>(9 * E);
+  L.w.Jb = cv.take<double>(9 * E);
+  L.w.H = cv.take<double>((size_t)n * n);
+  L.w.rhs = cv.take<double>((size_t)n);
+  L.w.dx = cv.take<double>((size_t)n);
+  L.w.part_cost = cv.take<double>((size_t)blocks_of((int)E) + 1);
+  L.w.part_grad = cv.take<double>(2 * (size_t)blocks_of((int)N) + 2);
+  L.w.part_step = cv.take<double>(3 * (size_t)blocks_of((int)N) + 3);
+  L.w.part_cand = cv.take<double>(2 * (size_t)blocks_of((int)E) + 2);
+  L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
+  L.w.state = cv.take<LMState>(1);
+  L.w.solve_tmp = cv.take<double>(rcs_cholesky_tmp_doubles(n) + 1);
+  L.bytes = cv.offset + 256;
+  return L;
+}
+
+int validate_pg(const ct_pose2d_problem* p) {
+  CT_CHECK_ARG(p != nullptr, "problem is null");
+  CT_CHECK_ARG(p->num_nodes >= 1 && p->num_edges >= 0 && p->num_pairs >= 0, "bad problem size");
+  CT_CHECK_ARG(p->fixed_node >= -1 && p->fixed_node < p->num_nodes, "fixed_node out of range");
+  CT_CHECK_ARG((int64_t)p->num_nodes * 3 <= 24576, "too many nodes for the dense normal equations");
+  CT_CHECK_ARG(p->loss.type >= CT_LOSS_TRIVIAL && p->loss.type <= CT_LOSS_TUKEY, "unknown loss type");
+  if (p->num_edges > 0) {
+    CT_CHECK_ARG(p->edges && p->measurements && p->node_offsets && p->node_edges, "edge arrays are null");
+    CT_CHECK_ARG(p->num_pairs == 0 || (p->pair_nodes && p->pair_offsets && p->pair_edges), "pair arrays are null");
+  }
+  return CT_OK;
+}
+
+template <typename T>
+int copy_back(std::vector<T>& h, const T* d, size_t count) {
+  h.resize(count);
+  if (count) CT_HIP(hipMemcpy(h.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost));
+  return CT_OK;
+}
+
+// every index the kernels follow, checked once on the host
+int validate_pg_tables(const ct_pose2d_problem& p) {
+  const int N = p.num_nodes, E = p.num_edges, P = p.num_pairs;
+  if (E == 0) return CT_OK;
+  std::vector<int32_t> ed, no, ne, pn, po, pe;
+  CT_TRY(copy_back(ed, p.edges, 2 * (size_t)E));
+  CT_TRY(copy_back(no, p.node_offsets, (size_t)N + 1));
+  CT_TRY(copy_back(ne, p.node_edges, 2 * (size_t)E));
+  CT_TRY(copy_back(pn, p.pair_nodes, 2 * (size_t)P));
+  CT_TRY(copy_back(po, p.pair_offsets, (size_t)P + 1));
+  CT_TRY(copy_back(pe, p.pair_edges, (size_t)E));
+  for (int e = 0; e < E; ++e)
+    CT_CHECK_ARG(ed[2 * e] >= 0 && ed[2 * e] < N && ed[2 * e + 1] >= 0 && ed[2 * e + 1] < N &&
+                     ed[2 * e] != ed[2 * e + 1],
+                 "edges must join two distinct valid nodes");
+  CT_CHECK_ARG(no[0] == 0 && no[N] == 2 * E, "node_offsets must cover 2 num_edges entries");
+  for (int k = 0; k < N; ++k) CT_CHECK_ARG(no[k] <= no[k + 1], "node_offsets must be non-decreasing");
+  for (int k = 0; k < N; ++k)
+    for (int t = no[k]; t < no[k + 1]; ++t) {
+      const int e = ne[t] >> 1;
+      CT_CHECK_ARG(ne[t] >= 0 && e < E && ed[2 * e + (ne[t] & 1)] == k, "node_edges must list incident edges");
+    }
+  CT_CHECK_ARG(P == 0 || (po[0] == 0 && po[P] == E), "pair_offsets must cover num_edges entries");
+  for (int q = 0; q < P; ++q) {
+    CT_CHECK_ARG(po[q] <= po[q + 1], "pair_offsets must be non-decreasing");
+    const int u = pn[2 * q], w = pn[2 * q + 1];
+    CT_CHECK_ARG(u >= 0 && w < N && u < w, "pair_nodes must hold u < v");
+    for (int t = po[q]; t < po[q + 1]; ++t) {
+      const int e = pe[t] >> 1;
+      CT_CHECK_ARG(pe[t] >= 0 && e < E, "pair_edges out of range");
+      const bool u_is_b = pe[t] & 1;
+      CT_CHECK_ARG(ed[2 * e + (u_is_b ? 1 : 0)] == u && ed[2 * e + (u_is_b ? 0 : 1)] == w,
+                   "pair_edges must join the pair's nodes");
+    }
+  }
+  return CT_OK;
+}
+
+}  // namespace
+
+}  // namespace ct
+
+using namespace ct;
+
+struct ct_pose2d_plan {
+  ct_pose2d_problem problem;
+  ct_lm_options options;
+  PGView view;
+  PGLayout layout;
+  PGCounts counts;
+  double* user_poses = nullptr;
+  hipStream_t stream = nullptr;
+  LMDriver driver;
+};
+
+namespace {
+
+int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
+  const PGView& v = pl->view;
+  PGWork& w = pl->layout.w;
+  if (v.E > 0) {
+    hipLaunchKernelGGL(k_pg_eval, dim3(blocks_of(v.E)), dim3(kPgThreads), 0, s, v, w);
+    CT_LAUNCH_CHECK();
+  }
+  if (v.n > 0) {
+    const size_t count = (size_t)v.n * v.n;
+    int grid = (int)((count + kPgThreads - 1) / kPgThreads);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(k_pg_zero, dim3(grid), dim3(kPgThreads), 0, s, w, count);
+    CT_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_pg_node, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w);
+  CT_LAUNCH_CHECK();
+  if (v.P > 0) {
+    hipLaunchKernelGGL(k_pg_pair, dim3(blocks_of(v.P)), dim3(kPgThreads), 0, s, v, w);
+    CT_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_pg_begin, dim3(1), dim3(kPgThreads), 0, s, pl->counts, w);
+  CT_LAUNCH_CHECK();
+  if (!evaluation_only) {
+    if (v.n > 0) CT_TRY(launch_rcs_cholesky_solve(w.H, w.rhs, w.dx, v.n, w.state, w.solve_tmp, s));
+    hipLaunchKernelGGL(k_pg_step, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w);
+    CT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pg_cand, dim3(blocks_of(v.E)), dim3(kPgThreads), 0, s, v, w);
+    CT_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_pg_end, dim3(1), dim3(kPgThreads), 0, s, pl->counts, w);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_pose2d_workspace_bytes(const ct_pose2d_problem* problem, const ct_lm_options* options, size_t* bytes) {
+  clear_error();
+  CT_TRY(validate_pg(problem));
+  CT_CHECK_ARG(options != nullptr && bytes != nullptr, "null argument");
+  *bytes = pg_carve(problem, options, nullptr).bytes;
+  return CT_OK;
+}
+
+int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options* options, double* poses,
+                          void* workspace, size_t workspace_bytes, ct_stream_t stream, ct_pose2d_plan** plan) {
+  clear_error();
+  CT_TRY(validate_pg(problem));
+  CT_CHECK_ARG(options != nullptr && plan != nullptr && poses != nullptr && workspace != nullptr, "null argument");
+  CT_CHECK_ARG(options->max_num_iterations >= 0 && options->initial_trust_region_radius > 0.0, "bad options");
+  CT_CHECK_ARG(options->loop_mode == CT_LOOP_HOST || options->loop_mode == CT_LOOP_DEVICE, "unknown loop mode");
+  const size_t need = pg_carve(problem, options, nullptr).bytes;
+  if (workspace_bytes < need) {
+    set_error("workspace too small: need " + std::to_string(need) + " bytes");
+    return CT_ERR_WORKSPACE;
+  }
+  CT_TRY(validate_pg_tables(*problem));
+  auto pl = std::make_unique<ct_pose2d_plan>();
+  pl->problem = *problem;
+  pl->options = *options;
+  pl->layout = pg_carve(problem, options, workspace);
+  PGView& v = pl->view;
+  v.N = problem->num_nodes;
+  v.E = problem->num_edges;
+  v.P = problem->num_pairs;
+  v.fixed = problem->fixed_node;
+  v.n = 3 * (v.N - (v.fixed >= 0 ? 1 : 0));
+  v.edges = problem->edges;
+  v.meas = problem->measurements;
+  v.node_offsets = problem->node_offsets;
+  v.node_edges = problem->node_edges;
+  v.pair_nodes = problem->pair_nodes;
+  v.pair_offsets = problem->pair_offsets;
+  v.pair_edges = problem->pair_edges;
+  v.loss = problem->loss;
+  pl->counts.ge = blocks_of(v.E);
+  pl->counts.gn = blocks_of(v.N);
+  pl->user_poses = poses;
+  pl->stream = reinterpret_cast<hipStream_t>(stream);
+  if (v.n > 0) {
+    CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, v.n, pl->stream));
+    CT_HIP(hipStreamSynchronize(pl->stream));
+  }
+  ct_pose2d_plan* raw = pl.get();
+  LMDriver& d = pl->driver;
+  d.enqueue_iteration = [raw](hipStream_t s, bool evaluation_only) { return pg_enqueue(raw, s, evaluation_only); };
+  d.launch_init = [raw](hipStream_t s) {
+    hipLaunchKernelGGL(k_pg_init, dim3(blocks_of(3 * raw->view.N)), dim3(kPgThreads), 0, s, raw->view,
+                       raw->layout.w, raw->user_poses, raw->options);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  };
+  d.launch_finalize = [raw](hipStream_t s) {
+    hipLaunchKernelGGL(k_pg_finalize, dim3(blocks_of(3 * raw->view.N)), dim3(kPgThreads), 0, s, raw->view,
+                       raw->layout.w, raw->user_poses);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  };
+  const int st = lm_driver_setup(d, *options, pl->stream, pl->layout.w.state, pl->layout.w.records);
+  if (st != CT_OK) {
+    lm_driver_release(d);
+    return st;
+  }
+  *plan = pl.release();
+  return CT_OK;
+}
+
+int ct_pose2d_plan_run(ct_pose2d_plan* plan, ct_lm_report* report, double* records) {
+  clear_error();
+  CT_CHECK_ARG(plan != nullptr, "plan is null");
+  return lm_driver_run(plan->driver, report, records);
+}
+
+int ct_pose2d_plan_destroy(ct_pose2d_plan* plan) {
+  clear_error();
+  if (!plan) return CT_OK;
+  lm_driver_release(plan->driver);
+  delete plan;
+  return CT_OK;
+}
+
+}  // extern "C"

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 4
+#define CT_ABI_VERSION 5
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -284,6 +284,39 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
 
 /* Host check of offsets (and order) of a batch; synchronous.                */
 int ct_curve_validate(const ct_curve_batch* batch, int32_t* first_bad_problem);
+
+/* ------------------------------------------------------------------------ */
+/* SE2 pose graphs (examples/pose_graph_2d.py:6-46): poses (x, y, theta),
+ * residual relative_pose(a, b) - measurement per edge with a robust loss, one
+ * node held constant, the LM loop of solver.py:35-357 with the exact step of
+ * the damped normal equations (the reference's DENSE_QR), FP64.  Tables
+ * (built by ceres_mi355x.pose2d): per-node incident edges and per-node-pair
+ * edges, each entry tagged 2*edge + (the node is the edge's second end).    */
+/* ------------------------------------------------------------------------ */
+typedef struct ct_pose2d_problem {
+  int32_t num_nodes;
+  int32_t num_edges;
+  int32_t num_pairs;             /* distinct unordered node pairs joined by edges */
+  int32_t fixed_node;            /* the constant node, or -1 */
+  const int32_t* edges;          /* [E][2] (a, b) device */
+  const double* measurements;    /* [E][3] pose of b in a's frame */
+  const int32_t* node_offsets;   /* [N+1] CSR into node_edges */
+  const int32_t* node_edges;     /* [2E] tagged incident edges */
+  const int32_t* pair_nodes;     /* [P][2] u < v */
+  const int32_t* pair_offsets;   /* [P+1] CSR into pair_edges */
+  const int32_t* pair_edges;     /* [E] tagged (tag bit: u is the edge's second end) */
+  ct_loss loss;
+} ct_pose2d_problem;
+
+typedef struct ct_pose2d_plan ct_pose2d_plan;
+
+int ct_pose2d_workspace_bytes(const ct_pose2d_problem* problem, const ct_lm_options* options, size_t* bytes);
+/* poses [N][3] FP64 device: start values in; ct_pose2d_plan_run leaves the
+ * best values there.  Validates every table index on the host.              */
+int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options* options, double* poses,
+                          void* workspace, size_t workspace_bytes, ct_stream_t stream, ct_pose2d_plan** plan);
+int ct_pose2d_plan_run(ct_pose2d_plan* plan, ct_lm_report* report, double* records);
+int ct_pose2d_plan_destroy(ct_pose2d_plan* plan);
 
 /* ------------------------------------------------------------------------ */
 /* Dense damped normal equations for a row-major A [m x n] (leading dimension
