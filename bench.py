@@ -386,6 +386,82 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
     return out
 
 
+def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, baseline_sample=60):
+    """Served request stream: Poisson arrivals at `rate`/s of 60% curve fits (as the batched workload, Cauchy(0.5)),
+    30% small bundle adjustments (2-20 cameras, 100-5000 points, 2-6 observations per point) and 10% SE2 pose
+    graphs (100-2000 nodes); reference default solver options.  Reports solves/s over the stream, p50/p99
+    latency (arrival to result), and the plain per-problem GPU throughput (each request alone, one at a time)."""
+    import numpy as np
+
+    import ceres_mi355x as tc
+    from ceres_mi355x.serving import BARequest, CurveRequest, PoseRequest, SolveServer
+    from oracle import generators as gen
+
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice(3, size=num_requests, p=[0.6, 0.3, 0.1])
+    reqs = []
+    for k in kinds:
+        s = int(rng.integers(1 << 30))
+        if k == 0:
+            cb = gen.make_curve_batch(num_problems=1, seed=s)
+            reqs.append(CurveRequest(cb.x, cb.y, loss=tc.CauchyLoss(0.5)))
+        elif k == 1:
+            C = int(rng.integers(2, 21))
+            P = int(rng.integers(100, 5001))
+            d = gen.make_bundle_adjustment(num_cameras=C, num_points=P, obs_per_point=(2, min(6, C)), seed=s)
+            reqs.append(BARequest(d.cameras, d.points, d.obs_camera, d.obs_point, d.observations))
+        else:
+            d = gen.make_pose_graph(num_nodes=int(rng.integers(100, 2001)), seed=s)
+            reqs.append(PoseRequest(d.poses, d.edges, d.measurements))
+    gaps = rng.exponential(1.0 / rate, size=num_requests)
+    arrivals = np.cumsum(gaps)
+    server = SolveServer(tc.SolverOptions(), num_slots=slots)
+    # warm-up: compile paths, caching allocator
+    for r in (reqs[int(np.argmax(kinds == 0))], reqs[int(np.argmax(kinds == 1))], reqs[int(np.argmax(kinds == 2))]):
+        server.solve_alone(r)
+    t0 = time.perf_counter()
+    futures = []
+    for r, at in zip(reqs, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        futures.append(server.submit(r))
+    results = [f.result() for f in futures]
+    t_end = max(r.finished for r in results)
+    server_wall = t_end - t0
+    lat = np.array([r.latency for r in results])
+    by_kind = {}
+    for kind in ("curve", "ba", "pose2d"):
+        lk = np.array([r.latency for r in results if r.kind == kind])
+        if lk.size:
+            by_kind[kind] = {"count": int(lk.size), "p50_ms": float(np.percentile(lk, 50) * 1e3),
+                             "p99_ms": float(np.percentile(lk, 99) * 1e3)}
+    # plain per-problem GPU throughput: a sample of the same mix, each request alone, one after another
+    sample = reqs[:baseline_sample]
+    t1 = time.perf_counter()
+    for r in sample:
+        server.solve_alone(r)
+    alone_s = (time.perf_counter() - t1) / len(sample)
+    server.close()
+    out = {
+        "workload": f"{num_requests} requests, Poisson arrivals at {rate:g}/s: 60% curve fits (N ~ U[50,2000], "
+                    "Cauchy(0.5)), 30% bundle adjustments (2-20 cameras, 100-5000 points, 2-6 obs/point), 10% SE2 "
+                    "pose graphs (100-2000 nodes); reference default options; "
+                    f"{slots} stream slots + continuous curve batching",
+        "solves_per_s": num_requests / server_wall,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "by_kind": by_kind,
+        "plain_per_problem_solves_per_s": 1.0 / alone_s,
+        "plain_sample": f"first {len(sample)} requests of the mix, solved alone one after another",
+        "offered_rate_per_s": rate,
+    }
+    out["served_vs_plain"] = out["solves_per_s"] / out["plain_per_problem_solves_per_s"]
+    log(f"[bench] serving: {out['solves_per_s']:.1f} solves/s, p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+        f"plain {out['plain_per_problem_solves_per_s']:.1f} solves/s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -448,6 +524,7 @@ def main() -> int:
         log(f"[bench] ba16_huber time-to-tol {extra['time_to_tol_ba16_huber']}")
         sdev.close()
         extra["curve_batch"] = bench_curve_batch()
+        extra["serving"] = bench_serving()
 
     roofline = None
     kernels = None
