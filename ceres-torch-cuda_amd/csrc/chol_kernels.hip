@@ -5,13 +5,14 @@
 //    diagonal block is factored in LDS by all threads, the rows below are
 //    solved one row per thread and the trailing rank-16 update runs on 4x4
 //    register tiles) and runs both triangular solves.
-//  * n > 256: right-looking blocked Cholesky, 32-column panels; per panel one
-//    wavefront factors the diagonal block, one kernel solves the rows below it
-//    and one applies the trailing update on 64x64 tiles; the triangular solves
-//    run in one persistent many-workgroup launch after the 64x64 diagonal
-//    blocks have been inverted.
+//  * n > 256: left-looking Cholesky on 64x64 tiles in one persistent
+//    launch (workgroups take tiles in dependency order and hand them on
+//    through per-tile flags); the diagonal tiles' inverses are formed on the
+//    way and the triangular solves run in a second launch of the same kind.
 //  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning as
 //    one persistent kernel (linear.py:1216-1256, preconditioner :1156-1182).
+#include <utility>
+
 #include "ba_kernels.h"
 
 namespace ct {
@@ -241,134 +242,6 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __re
   }
 }
 
-// ---------------------------------------------------------------------------
-// large n: blocked right-looking factorisation
-// ---------------------------------------------------------------------------
-constexpr int kNB = 32;
-constexpr int kPanelThreads = 256;
-constexpr int kTile = 64;
-
-__device__ __forceinline__ void lds_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// Factor the diagonal block of panel k0 in place with one wavefront: lane l
-// holds row l in registers; column j is broadcast through LDS.  Stores the
-// pivots' reciprocals.
-__global__ __launch_bounds__(kWave) void k_chol_diag(double* __restrict__ A, double* __restrict__ dinv_out, int n,
-                                                     int k0, LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double col[kNB];
-  const int l = threadIdx.x;
-  const int kb = min(kNB, n - k0);
-  const bool own = l < kb;
-  double a[kNB];
-#pragma unroll
-  for (int c = 0; c < kNB; ++c) a[c] = (own && c <= l) ? A[(size_t)(k0 + l) * n + k0 + c] : 0.0;
-  int bad = 0;
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (j < kb) {
-      const double piv = __shfl(a[j], j);
-      if (!(piv > 0.0) || !isfinite(piv)) bad = 1;
-      const double d = sqrt(piv);
-      const double inv = 1.0 / d;
-      const double lj = (l == j) ? d : a[j] * inv;
-      if (l >= j) a[j] = lj;
-      if (l < kNB) col[l] = (l >= j && own) ? lj : 0.0;
-      lds_wave_sync();
-#pragma unroll
-      for (int m = j + 1; m < kNB; ++m)
-        if (m <= l) a[m] -= lj * col[m];
-      if (l == j && own) dinv_out[k0 + j] = inv;
-      lds_wave_sync();
-    }
-  }
-  if (own) {
-#pragma unroll
-    for (int c = 0; c < kNB; ++c)
-      if (c <= l) A[(size_t)(k0 + l) * n + k0 + c] = a[c];
-  }
-  if (l == 0 && bad && st) st->step_invalid = 1;
-}
-
-// Rows below the factored diagonal block of panel k0: L21 = A21 L11^-T, one
-// row per thread.
-__global__ __launch_bounds__(kPanelThreads) void k_chol_panel(double* __restrict__ A,
-                                                              const double* __restrict__ dinv_in, int n, int k0,
-                                                              LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double Ld[kNB][kNB + 1];
-  __shared__ double P[kPanelThreads][kNB + 1];
-  __shared__ double dinv[kNB];
-  const int tid = threadIdx.x;
-  const int kb = min(kNB, n - k0);
-  for (int idx = tid; idx < kNB * kNB; idx += kPanelThreads) {
-    const int r = idx / kNB, c = idx % kNB;
-    Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
-  }
-  if (tid < kNB) dinv[tid] = tid < kb ? dinv_in[k0 + tid] : 0.0;
-  __syncthreads();
-  const int row = k0 + kb + blockIdx.x * kPanelThreads + tid;
-  if (row < n) {
-    double* prow = P[tid];
-    for (int c = 0; c < kb; ++c) prow[c] = A[(size_t)row * n + k0 + c];
-    trsm_row<kNB, kNB + 1>(Ld, dinv, kb, prow);
-    for (int c = 0; c < kb; ++c) A[(size_t)row * n + k0 + c] = prow[c];
-  }
-}
-
-// A[i][j] -= sum_c L[i][k0+c] L[j][k0+c] over 64x64 lower tiles of the
-// trailing matrix (rows/cols >= r0 = k0 + kb).  256 threads x (4x4).
-__global__ __launch_bounds__(256) void k_chol_syrk(double* __restrict__ A, int n, int k0, int kb, LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double Li[kTile][kNB + 1];
-  __shared__ double Lj[kTile][kNB + 1];
-  const int r0 = k0 + kb;
-  const int tt = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-  while (ti * (ti + 1) / 2 > tt) --ti;
-  while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
-  const int tj = tt - ti * (ti + 1) / 2;
-  const int i0 = r0 + ti * kTile, j0 = r0 + tj * kTile;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kTile * kNB; idx += 256) {
-    const int r = idx / kNB, c = idx % kNB;
-    const int gi = i0 + r, gj = j0 + r;
-    Li[r][c] = (gi < n && c < kb) ? A[(size_t)gi * n + k0 + c] : 0.0;
-    Lj[r][c] = (gj < n && c < kb) ? A[(size_t)gj * n + k0 + c] : 0.0;
-  }
-  __syncthreads();
-  const int tr = tid / 16, tc = tid % 16;
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int c = 0; c < kb; ++c) {
-    double pa[4], pb[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      pa[a] = Li[tr + 16 * a][c];
-      pb[a] = Lj[tc + 16 * a][c];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int gi = i0 + tr + 16 * a, gj = j0 + tc + 16 * b;
-      if (gi < n && gj <= gi) A[(size_t)gi * n + gj] -= acc[a][b];
-    }
-  }
-}
-
 // Blocked triangular solves with one workgroup against the factor in A and
 // the pivots' reciprocals tmp[n:2n]; y lives in tmp[0:n].
 constexpr int kTriThreads = 1024;
@@ -433,125 +306,426 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 }
 
 // ---------------------------------------------------------------------------
-// large n: triangular solves with many workgroups
+// large n: one persistent launch for the factorisation, one for the solves
 //
-// The factor is cut into kTB-row blocks.  k_diag_inverse inverts every
-// diagonal block (one workgroup each, all at once).  k_tri_solve_mt then
-// solves L y = b and L^T x = y in one persistent launch: workgroup g owns
-// forward blocks g, g+G, ... (increasing) and backward blocks nt-1-g,
-// nt-1-g-G, ... (decreasing).  Block r of the forward solve subtracts
-// L_rj y_j as each y_j is published and finishes with y_r = L_rr^-1 (...);
-// the backward solve mirrors it with L^T.  Publication is a per-block flag
-// stamped with this launch's epoch (read once at start; the last workgroup
-// to finish advances it), so flags never need clearing.  Each workgroup
-// waits only on blocks that precede its own in the dependency order and all
-// G <= #CU workgroups are resident, so every wait ends; waits are bounded
-// anyway and a timeout flags the step invalid.
+// Both work on kTB x kTB tiles (kTB = 64) and hand tiles between workgroups
+// through per-tile flags stamped with the launch's epoch (read once at the
+// start; the last workgroup to finish advances it), so flags never need
+// clearing.  Workgroups take tiles from an atomic ticket counter in an order
+// in which every tile depends only on tiles of smaller tickets.  A ticket is
+// only taken by a running workgroup, so the oldest unfinished tile always has
+// its inputs ready: every wait ends without any assumption about how many
+// workgroups are resident at once.  Waits are bounded anyway; a timeout
+// raises the launch's abort word, every workgroup stops, and the step is
+// flagged invalid.
+//
+// Publication: every wave completes and writes back its own stores
+// (__threadfence) before the workgroup barrier, then one thread stamps the
+// flag with release semantics.  A waiting workgroup polls with plain atomic
+// loads and takes an acquire fence in every wave once the flag is seen.
 // ---------------------------------------------------------------------------
 constexpr int kTB = 64;
-constexpr int kTriMtThreads = 256;
-constexpr int kTriSlices = kTriMtThreads / kTB;  // column slices per row in a block product
+constexpr int kFThreads = 256;
+constexpr int kFLd = kTB + 2;  // LDS row stride: 16-byte aligned rows, 16 rows on distinct banks
+constexpr long kSpinLimit = 1L << 24;
 
 __host__ __device__ inline size_t tri_blocks(int n) { return ((size_t)n + kTB - 1) / kTB; }
-
-// inverse of the lower-triangular diagonal block b of the factor in A, row-major
-// kTB x kTB (zero outside the block's rows), one lane per column
-__global__ __launch_bounds__(kWave) void k_diag_inverse(const double* __restrict__ A, double* __restrict__ linv,
-                                                        int n, LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double Ls[kTB][kTB + 1];
-  __shared__ double dinv[kTB];
-  const int b = blockIdx.x;
-  const int r0 = b * kTB;
-  const int rb = min(kTB, n - r0);
-  const int c = threadIdx.x;
-  for (int i = 0; i < kTB; ++i) Ls[i][c] = (i < rb && c <= i) ? A[(size_t)(r0 + i) * n + r0 + c] : 0.0;
-  lds_wave_sync();
-  dinv[c] = c < rb ? 1.0 / Ls[c][c] : 0.0;
-  lds_wave_sync();
-  double x[kTB];
-#pragma unroll
-  for (int i = 0; i < kTB; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < i; ++k) s += Ls[i][k] * x[k];
-    x[i] = (i < c) ? 0.0 : (i == c ? dinv[i] : -s * dinv[i]);
-  }
-  double* out = linv + (size_t)b * kTB * kTB;
-#pragma unroll
-  for (int i = 0; i < kTB; ++i) out[i * kTB + c] = (i < rb && c < rb) ? x[i] : 0.0;
+__host__ __device__ inline size_t factor_tiles(int n) {
+  const size_t nt = tri_blocks(n);
+  return nt * (nt + 1) / 2;
 }
+
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// v as held by `lane` (a uniform value: read through scalar registers)
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/sqrt(x): the hardware estimate refined by two Newton steps
+// y <- y (3/2 - x y^2 / 2) (each roughly doubles the correct bits; the result
+// is within a few ulp).  Not positive or not finite x gives NaN or 0.
+__device__ __forceinline__ double pivot_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y *= fma(-h * y, y, 1.5);
+  y *= fma(-h * y, y, 1.5);
+  return y;
+}
+
+// Pivot step JJ of the one-wave factorisation of a 16-column panel held in
+// registers: lane l owns row l of the panel (the panel's first 16 rows are
+// its diagonal block).  Steps are expanded at compile time so that a[] is
+// only ever indexed by constants.  The pivot's reciprocal goes to pinv[JJ]
+// and a bad pivot sets *pbad (LDS stores inside the step keep those results
+// from being carried to the end).
+constexpr int kPW = 16;  // panel width of the diagonal-tile factorisation
+
+template <int JJ>
+__device__ __forceinline__ void panel_step(double (&a)[kPW], double* colb, double* pinv, int* pbad, int l) {
+  const double piv = lane_value(a[JJ], JJ);
+  const double inv = pivot_rsqrt(piv);
+  if (l == JJ) {
+    pinv[JJ] = inv;
+    if (!(piv > 0.0) || !isfinite(piv)) *pbad = 1;
+  }
+  // entries right of a lane's own diagonal are never read: they are scaled
+  // and updated like the others (they stay finite) instead of being masked
+  a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
+  const double f = l > JJ ? a[JJ] : 0.0;
+  colb[l] = f;
+  lds_wave_sync();
+#pragma unroll
+  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, colb[m], a[m]);
+  lds_wave_sync();
+}
+
+template <int... J>
+__device__ __forceinline__ void panel_steps(double (&a)[kPW], double* colb, double* pinv, int* pbad, int l,
+                                            std::integer_sequence<int, J...>) {
+  (panel_step<J>(a, colb, pinv, pbad, l), ...);
+}
+
+// Every thread calls this; thread 0 polls up to two flags.  *status (LDS) is
+// written by thread 0 before the barrier and read by all threads after it;
+// callers keep a barrier between two calls.
+__device__ __forceinline__ bool wait_flags(const int* f1, const int* f2, int epoch, unsigned int* abort_word,
+                                           int* status) {
+  if (threadIdx.x == 0 && *status == 0) {
+    long spins = 0;
+    for (const int* f : {f1, f2}) {
+      if (f == nullptr) continue;
+      while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned int)epoch ||
+            ++spins > kSpinLimit) {
+          *status = 1;
+          __hip_atomic_store(abort_word, (unsigned int)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (*status) break;
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *status == 0;
+}
+
+__device__ __forceinline__ void publish_flag(int* flag, int epoch) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Next ticket for this workgroup (every thread calls it), or `stop` once the
+// launch is aborted.  The barrier in front keeps the previous tile's LDS use
+// and the previous ticket's readers ahead of the overwrite.
+__device__ __forceinline__ int take_ticket(unsigned int* counter, int stop, const int* status, int* slot) {
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = *status ? stop : (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  return *slot;
+}
+
+// The last workgroup of the launch to finish resets the ticket counter and
+// advances the epoch for the next launch on the stream.
+__device__ __forceinline__ void finish_launch(unsigned int* ctl, int epoch) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int done = atomicAdd(&ctl[1], 1u);
+    if (done == gridDim.x - 1) {
+      ctl[1] = 0u;
+      ctl[2] = 0u;
+      __threadfence();
+      __hip_atomic_store(&ctl[0], (unsigned int)epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+struct FactorWork {
+  double* dinv;        // [n] reciprocals of the pivots
+  double* linv;        // [nt][kTB][kTB] inverses of the diagonal tiles of L (row-major, zero outside)
+  int* ready;          // [nt (nt + 1) / 2] epoch stamp of each finished tile of L
+  unsigned int* ctl;   // [0] completed launches, [1] finished workgroups, [2] next ticket, [3] abort epoch
+};
+
+// Lower tiles in column-major order: column j holds tiles (j..nt-1, j).
+__device__ __forceinline__ int tile_index(int i, int j, int nt) { return j * nt - j * (j - 1) / 2 + (i - j); }
+
+// Left-looking tiled Cholesky, S = L L^T in place (lower triangle), in one
+// launch.  The workgroup holding tile (i, j) keeps it in registers (4x4 per
+// thread) and subtracts L_ik L_jk^T for k = 0..j-1 as those tiles are
+// published, always in increasing k, so the arithmetic does not depend on
+// which workgroup takes which tile.  Then:
+//  * diagonal tile: wave 0 factors it in registers (lane l owns row l); the
+//    inverse of the factor is formed blockwise (16x16 diagonal blocks by
+//    substitution, then two doubling steps X21 = -V22 (L21 V11)) and kept for
+//    the off-diagonal tiles of the column and for the triangular solves;
+//  * off-diagonal tile: L_ij = A_ij V_jj^T with the published inverse.
+// Rows past n in the last tile row are zero; the diagonal tile pads them with
+// an identity block.  A pivot that is not positive and finite flags the step.
+__global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A, int n, FactorWork fw,
+                                                          LMState* st) {
+  if (skip_solve(st)) return;
+  __shared__ double lds[2][kTB][kFLd];
+  __shared__ double colb[kTB], pinv[kTB];
+  __shared__ int s_status, s_ticket, s_bad;
+  const int nt = (int)tri_blocks(n);
+  const int ntiles = nt * (nt + 1) / 2;
+  const int epoch = (int)__hip_atomic_load(&fw.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+  if (threadIdx.x == 0) s_status = 0;
+  for (;;) {
+    const int q = take_ticket(&fw.ctl[2], ntiles, &s_status, &s_ticket);
+    if (q >= ntiles) break;
+    // an opaque copy of the thread index per tile: nothing derived from it
+    // (LDS and global addresses) is computed ahead and kept live across tiles
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int tr = tid / 16, tc = tid % 16;
+    int i = 0, j = 0;
+    {
+      int rest = q;
+      while (rest >= nt - j) {
+        rest -= nt - j;
+        ++j;
+      }
+      i = j + rest;
+    }
+    const int r0 = i * kTB, c0 = j * kTB;
+    const int ri = min(kTB, n - r0), cj = min(kTB, n - c0);
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int row = tr + 16 * a, col = tc + 16 * b;
+        acc[a][b] = (row < ri && col < cj) ? A[(size_t)(r0 + row) * n + c0 + col] : 0.0;
+      }
+    double(*Li)[kFLd] = lds[0];
+    double(*Lj)[kFLd] = (i == j) ? lds[0] : lds[1];
+    for (int k = 0; k < j; ++k) {
+      if (!wait_flags(&fw.ready[tile_index(i, k, nt)], i == j ? nullptr : &fw.ready[tile_index(j, k, nt)], epoch,
+                      &fw.ctl[3], &s_status))
+        break;
+      const int kk = k * kTB;  // tiles left of the diagonal are full width
+      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
+        const int r = idx / kTB, c = idx % kTB;
+        Li[r][c] = r < ri ? A[(size_t)(r0 + r) * n + kk + c] : 0.0;
+        if (i != j) Lj[r][c] = A[(size_t)(c0 + r) * n + kk + c];
+      }
+      __syncthreads();
+      for (int c = 0; c < kTB; ++c) {
+        double pa[4], pb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          pa[a] = Li[tr + 16 * a][c];
+          pb[a] = Lj[tc + 16 * a][c];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = fma(-pa[a], pb[b], acc[a][b]);
+      }
+      __syncthreads();
+    }
+    if (s_status) break;
+    if (i == j) {
+      double(*D)[kFLd] = lds[0];
+      double(*V)[kFLd] = lds[1];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int row = tr + 16 * a, col = tc + 16 * b;
+          D[row][col] = col > row ? 0.0 : (row < cj ? acc[a][b] : (row == col ? 1.0 : 0.0));
+          V[row][col] = 0.0;
+        }
+      __syncthreads();
+      // right-looking over four 16-column panels: wave 0 factors the panel
+      // (rows base..63) in registers, then all threads apply its rank-16
+      // update to the trailing lower triangle in LDS
+      if (tid == 0) s_bad = 0;
+      for (int base = 0; base < kTB; base += kPW) {
+        if (tid < kWave) {
+          const int l = tid, row = base + l;
+          double a[kPW];
+#pragma unroll
+          for (int q = 0; q < kPW; ++q) a[q] = row < kTB ? D[row][base + q] : 0.0;
+          lds_wave_sync();
+          panel_steps(a, colb, pinv + base, &s_bad, l, std::make_integer_sequence<int, kPW>{});
+          if (row < kTB) {
+#pragma unroll
+            for (int q = 0; q < kPW; ++q) D[row][base + q] = a[q];
+          }
+        }
+        __syncthreads();
+        const int t0 = base + kPW, tn = kTB - t0;
+        for (int e = tid; e < tn * tn; e += kFThreads) {
+          const int r = t0 + e / tn, c = t0 + e % tn;
+          if (c <= r) {
+            double acc2 = D[r][c];
+#pragma unroll
+            for (int q = 0; q < kPW; ++q) acc2 = fma(-D[r][base + q], D[c][base + q], acc2);
+            D[r][c] = acc2;
+          }
+        }
+        __syncthreads();
+      }
+      if (tid < cj) fw.dinv[c0 + tid] = pinv[tid];
+      if (tid == 0 && s_bad && st) st->step_invalid = 1;
+      // inverse, 16x16 diagonal blocks: wave w inverts block w, lane c < 16 one column
+      {
+        const int w = tid / kWave, c = tid % kWave, b0 = 16 * w;
+        if (c < 16) {
+          double x[16];
+#pragma unroll
+          for (int ii = 0; ii < 16; ++ii) {
+            double s = ii == c ? 1.0 : 0.0;
+#pragma unroll
+            for (int kq = 0; kq < ii; ++kq) s = fma(-D[b0 + ii][b0 + kq], x[kq], s);
+            x[ii] = ii < c ? 0.0 : s / D[b0 + ii][b0 + ii];
+          }
+#pragma unroll
+          for (int ii = 0; ii < 16; ++ii) V[b0 + ii][b0 + c] = x[ii];
+        }
+      }
+      __syncthreads();
+      // doubling steps: for blocks of width h at base b, X21 = -V22 (L21 V11);
+      // T = L21 V11 is staged in the (unused) upper triangle of D at D[b + r][b + h + c]
+      for (int h = 16; h < kTB; h *= 2) {
+        const int per = (kTB / (2 * h)) * h * h;  // entries of all blocks at this level
+        for (int e = tid; e < per; e += kFThreads) {
+          const int blk = e / (h * h), r = (e / h) % h, c = e % h;
+          const int b = blk * 2 * h;
+          double s = 0.0;
+          for (int m = c; m < h; ++m) s = fma(D[b + h + r][b + m], V[b + m][b + c], s);
+          D[b + r][b + h + c] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < per; e += kFThreads) {
+          const int blk = e / (h * h), r = (e / h) % h, c = e % h;
+          const int b = blk * 2 * h;
+          double s = 0.0;
+          for (int m = 0; m <= r; ++m) s = fma(V[b + h + r][b + h + m], D[b + m][b + h + c], s);
+          V[b + h + r][b + c] = -s;
+        }
+        __syncthreads();
+      }
+      double* vout = fw.linv + (size_t)j * kTB * kTB;
+      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
+        const int r = idx / kTB, c = idx % kTB;
+        vout[idx] = (r < cj && c < cj) ? V[r][c] : 0.0;
+        if (r < cj && c <= r) A[(size_t)(c0 + r) * n + c0 + c] = D[r][c];
+      }
+    } else {
+      if (!wait_flags(&fw.ready[tile_index(j, j, nt)], nullptr, epoch, &fw.ctl[3], &s_status)) break;
+      double(*X)[kFLd] = lds[0];
+      double(*Vt)[kFLd] = lds[1];
+      const double* vin = fw.linv + (size_t)j * kTB * kTB;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) X[tr + 16 * a][tc + 16 * b] = acc[a][b];
+      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) Vt[idx / kTB][idx % kTB] = vin[idx];
+      __syncthreads();
+      double out[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
+      for (int m = 0; m < kTB; ++m) {
+        double pa[4], pb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          pa[a] = X[tr + 16 * a][m];
+          pb[a] = Vt[tc + 16 * a][m];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) out[a][b] = fma(pa[a], pb[b], out[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int row = tr + 16 * a, col = tc + 16 * b;
+          if (row < ri) A[(size_t)(r0 + row) * n + c0 + col] = out[a][b];
+        }
+    }
+    publish_flag(&fw.ready[tile_index(i, j, nt)], epoch);
+  }
+  if (threadIdx.x == 0 && s_status && st) st->step_invalid = 1;
+  finish_launch(fw.ctl, epoch);
+}
+
+// L y = b and L^T x = y in one launch.  Tickets 0..nt-1 are the forward
+// blocks in increasing order, tickets nt..2nt-1 the backward blocks in
+// decreasing order.  Forward block r subtracts L_rj y_j as each y_j is
+// published and finishes with y_r = V_rr (...); the backward block mirrors
+// it with L^T and V_rr^T.
+constexpr int kTriMtThreads = 256;
+constexpr int kTriSlices = kTriMtThreads / kTB;  // column slices per row in a block product
 
 struct TriWork {
   double* y;             // [n] forward solution
   const double* linv;    // [nt][kTB][kTB] inverses of the diagonal blocks
   int* flags;            // [2 nt] epoch stamps: y block ready, x block ready
-  unsigned int* epoch;   // [2] completed launches, finished workgroups of this launch
+  unsigned int* ctl;     // [0] completed launches, [1] finished workgroups, [2] next ticket, [3] abort epoch
 };
-
-__device__ __forceinline__ bool wait_stamp(const int* flag, int epoch, int* timed_out) {
-  if (threadIdx.x == 0 && !*timed_out) {
-    long spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-      if (++spins > (1L << 24)) {
-        *timed_out = 1;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-  __syncthreads();
-  return *timed_out == 0;
-}
-
-__device__ __forceinline__ void publish_stamp(int* flag, int epoch) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
 
 __global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __restrict__ A,
                                                                 const double* __restrict__ rhs,
                                                                 double* __restrict__ x, int n, TriWork tw,
-                                                                LMState* st, int G) {
+                                                                LMState* st) {
   if (skip_solve(st)) return;
   __shared__ double acc[kTB];
   __shared__ double part[kTriSlices][kTB];
-  __shared__ int timed_out;
+  __shared__ int s_status, s_ticket;
   const int tid = threadIdx.x;
   const int row = tid % kTB, sl = tid / kTB;
-  const int g = blockIdx.x;
   const int nt = (int)tri_blocks(n);
-  const int epoch = (int)__hip_atomic_load(&tw.epoch[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
-  if (tid == 0) timed_out = 0;
-  __syncthreads();
-  // forward: y_r = L_rr^-1 (b_r - sum_{j<r} L_rj y_j)
-  for (int r = g; r < nt; r += G) {
-    const int r0 = r * kTB, rb = min(kTB, n - r0);
-    if (tid < kTB) acc[tid] = tid < rb ? rhs[r0 + tid] : 0.0;
-    __syncthreads();
-    for (int j = 0; j < r; ++j) {
-      if (!wait_stamp(&tw.flags[j], epoch, &timed_out)) break;
-      double p = 0.0;
-      if (row < rb) {
-        const double* Lrow = A + (size_t)(r0 + row) * n + j * kTB + sl * (kTB / kTriSlices);
-        const double* yj = tw.y + j * kTB + sl * (kTB / kTriSlices);
-#pragma unroll
-        for (int c = 0; c < kTB / kTriSlices; ++c) p += Lrow[c] * yj[c];
-      }
-      part[sl][row] = p;
+  const int epoch = (int)__hip_atomic_load(&tw.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+  int my_bad = 0;
+  if (tid == 0) s_status = 0;
+  for (;;) {
+    const int q = take_ticket(&tw.ctl[2], 2 * nt, &s_status, &s_ticket);
+    if (q >= 2 * nt) break;
+    if (q < nt) {
+      // forward: y_r = L_rr^-1 (b_r - sum_{j<r} L_rj y_j)
+      const int r = q;
+      const int r0 = r * kTB, rb = min(kTB, n - r0);
+      if (tid < kTB) acc[tid] = tid < rb ? rhs[r0 + tid] : 0.0;
       __syncthreads();
-      if (tid < kTB) {
-        double t = 0.0;
+      for (int j = 0; j < r; ++j) {
+        if (!wait_flags(&tw.flags[j], nullptr, epoch, &tw.ctl[3], &s_status)) break;
+        double p = 0.0;
+        if (row < rb) {
+          const double* Lrow = A + (size_t)(r0 + row) * n + j * kTB + sl * (kTB / kTriSlices);
+          const double* yj = tw.y + j * kTB + sl * (kTB / kTriSlices);
 #pragma unroll
-        for (int q = 0; q < kTriSlices; ++q) t += part[q][tid];
-        acc[tid] -= t;
+          for (int c = 0; c < kTB / kTriSlices; ++c) p += Lrow[c] * yj[c];
+        }
+        part[sl][row] = p;
+        __syncthreads();
+        if (tid < kTB) {
+          double t = 0.0;
+#pragma unroll
+          for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
+          acc[tid] -= t;
+        }
+        __syncthreads();
       }
-      __syncthreads();
-    }
-    {
+      if (s_status) break;
       const double* Li = tw.linv + (size_t)r * kTB * kTB + row * kTB + sl * (kTB / kTriSlices);
       double p = 0.0;
 #pragma unroll
@@ -561,38 +735,36 @@ __global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __
       if (tid < rb) {
         double t = 0.0;
 #pragma unroll
-        for (int q = 0; q < kTriSlices; ++q) t += part[q][tid];
+        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
         tw.y[r0 + tid] = t;
       }
-    }
-    publish_stamp(&tw.flags[r], epoch);
-  }
-  // backward: x_r = L_rr^-T (y_r - sum_{j>r} L_jr^T x_j)
-  for (int q0 = g; q0 < nt; q0 += G) {
-    const int r = nt - 1 - q0;
-    const int r0 = r * kTB, rb = min(kTB, n - r0);
-    if (!wait_stamp(&tw.flags[r], epoch, &timed_out)) break;
-    if (tid < kTB) acc[tid] = tid < rb ? tw.y[r0 + tid] : 0.0;
-    __syncthreads();
-    for (int j = nt - 1; j > r; --j) {
-      if (!wait_stamp(&tw.flags[nt + j], epoch, &timed_out)) break;
-      const int j0 = j * kTB, jb = min(kTB, n - j0);
-      double p = 0.0;
-      if (row < rb) {
-        for (int k = sl * (kTB / kTriSlices); k < (sl + 1) * (kTB / kTriSlices); ++k)
-          if (k < jb) p += A[(size_t)(j0 + k) * n + r0 + row] * x[j0 + k];
-      }
-      part[sl][row] = p;
+      publish_flag(&tw.flags[r], epoch);
+    } else {
+      // backward: x_r = L_rr^-T (y_r - sum_{j>r} L_jr^T x_j)
+      const int r = 2 * nt - 1 - q;
+      const int r0 = r * kTB, rb = min(kTB, n - r0);
+      if (!wait_flags(&tw.flags[r], nullptr, epoch, &tw.ctl[3], &s_status)) break;
+      if (tid < kTB) acc[tid] = tid < rb ? tw.y[r0 + tid] : 0.0;
       __syncthreads();
-      if (tid < kTB) {
-        double t = 0.0;
+      for (int j = nt - 1; j > r; --j) {
+        if (!wait_flags(&tw.flags[nt + j], nullptr, epoch, &tw.ctl[3], &s_status)) break;
+        const int j0 = j * kTB, jb = min(kTB, n - j0);
+        double p = 0.0;
+        if (row < rb) {
+          for (int k = sl * (kTB / kTriSlices); k < (sl + 1) * (kTB / kTriSlices); ++k)
+            if (k < jb) p += A[(size_t)(j0 + k) * n + r0 + row] * x[j0 + k];
+        }
+        part[sl][row] = p;
+        __syncthreads();
+        if (tid < kTB) {
+          double t = 0.0;
 #pragma unroll
-        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
-        acc[tid] -= t;
+          for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
+          acc[tid] -= t;
+        }
+        __syncthreads();
       }
-      __syncthreads();
-    }
-    {
+      if (s_status) break;
       const double* Lb = tw.linv + (size_t)r * kTB * kTB;
       double p = 0.0;
       for (int c = sl * (kTB / kTriSlices); c < (sl + 1) * (kTB / kTriSlices); ++c) p += Lb[c * kTB + row] * acc[c];
@@ -603,29 +775,15 @@ __global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __
 #pragma unroll
         for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
         x[r0 + tid] = t;
+        if (!isfinite(t)) my_bad = 1;
       }
-    }
-    publish_stamp(&tw.flags[nt + r], epoch);
-  }
-  // non-finite solution or a timed-out wait invalidates the step
-  int my_bad = 0;
-  for (int q0 = g; q0 < nt; q0 += G) {
-    const int r0 = (nt - 1 - q0) * kTB;
-    if (tid < kTB && r0 + tid < n && !isfinite(x[r0 + tid])) my_bad = 1;
-  }
-  if (st && (my_bad || timed_out)) st->step_invalid = 1;
-  if (st && g == 0 && tid == 0) st->linear_iterations = 1;
-  // the last workgroup to finish advances the epoch for the next launch
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    const unsigned int done = atomicAdd(&tw.epoch[1], 1u);
-    if (done == (unsigned int)G - 1) {
-      tw.epoch[1] = 0u;
-      __threadfence();
-      __hip_atomic_store(&tw.epoch[0], (unsigned int)epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      publish_flag(&tw.flags[nt + r], epoch);
     }
   }
+  // a non-finite solution or an aborted launch invalidates the step
+  if (st && (my_bad || (tid == 0 && s_status))) st->step_invalid = 1;
+  if (st && blockIdx.x == 0 && tid == 0) st->linear_iterations = 1;
+  finish_launch(tw.ctl, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,6 +813,7 @@ struct PcgWork {
 };
 
 __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int target, int* timed_out) {
+  __threadfence();  // every wave's stores complete before the arrival is counted
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -670,6 +829,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int tar
     __threadfence();
   }
   __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *timed_out == 0;
 }
 
@@ -936,11 +1096,18 @@ fail:
 }  // namespace
 
 // solve scratch layout (doubles): [0,n) y, [n,2n) pivot reciprocals,
-// [2n, 2n + nt kTB^2) diagonal-block inverses, then 2 nt + 2 ints of flags
-size_t rcs_cholesky_tmp_doubles(int n) {
+// [2n, 2n + nt kTB^2) diagonal-block inverses, then the control words:
+// triangular-solve flags (2 nt ints) and counters (4 uints), factor-tile
+// flags (nt (nt + 1) / 2 ints) and counters (4 uints).
+namespace {
+size_t control_doubles(int n) {
   const size_t nt = tri_blocks(n);
-  return 2 * (size_t)n + nt * kTB * kTB + nt + 2;
+  return nt + 2 + (factor_tiles(n) + 1) / 2 + 2;
 }
+int* control_words(double* tmp, int n) { return reinterpret_cast<int*>(tmp + 2 * (size_t)n + tri_blocks(n) * kTB * kTB); }
+}  // namespace
+
+size_t rcs_cholesky_tmp_doubles(int n) { return 2 * (size_t)n + tri_blocks(n) * kTB * kTB + control_doubles(n); }
 
 namespace {
 TriWork tri_work(double* tmp, int n) {
@@ -948,31 +1115,39 @@ TriWork tri_work(double* tmp, int n) {
   TriWork tw;
   tw.y = tmp;
   tw.linv = tmp + 2 * (size_t)n;
-  tw.flags = reinterpret_cast<int*>(tmp + 2 * (size_t)n + nt * kTB * kTB);
-  tw.epoch = reinterpret_cast<unsigned int*>(tw.flags + 2 * nt);
+  tw.flags = control_words(tmp, n);
+  tw.ctl = reinterpret_cast<unsigned int*>(tw.flags + 2 * nt);
   return tw;
 }
 
-int tri_groups(int n, int* groups) {
-  int dev = 0, cus = 0;
+FactorWork factor_work(double* tmp, int n) {
+  const size_t nt = tri_blocks(n);
+  FactorWork fw;
+  fw.dinv = tmp + n;
+  fw.linv = tmp + 2 * (size_t)n;
+  fw.ready = control_words(tmp, n) + 2 * nt + 4;
+  fw.ctl = reinterpret_cast<unsigned int*>(fw.ready + factor_tiles(n) + (factor_tiles(n) & 1));
+  return fw;
+}
+
+int device_cus(int* cus) {
+  int dev = 0;
   CT_HIP(hipGetDevice(&dev));
-  CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int nt = (int)tri_blocks(n);
-  *groups = nt < cus ? nt : cus;
-  if (*groups < 1) *groups = 1;
+  CT_HIP(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (*cus < 1) *cus = 1;
   return CT_OK;
 }
 
 int launch_tri_mt(const double* L, const double* rhs, double* x, int n, LMState* st, double* tmp, hipStream_t s) {
-  int G = 0;
-  CT_TRY(tri_groups(n, &G));
-  hipLaunchKernelGGL(k_tri_solve_mt, dim3(G), dim3(kTriMtThreads), 0, s, L, rhs, x, n, tri_work(tmp, n), st, G);
+  int cus = 0;
+  CT_TRY(device_cus(&cus));
+  const int nt = (int)tri_blocks(n);
+  const int G = nt < cus ? nt : cus;
+  hipLaunchKernelGGL(k_tri_solve_mt, dim3(G), dim3(kTriMtThreads), 0, s, L, rhs, x, n, tri_work(tmp, n), st);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
-}  // namespace
 
-namespace {
 __global__ void k_zero_words(double* p, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
     p[i] = 0.0;
@@ -980,10 +1155,13 @@ __global__ void k_zero_words(double* p, size_t count) {
 }  // namespace
 
 int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s) {
-  // zero the flags and epoch of the many-workgroup triangular solves (a fill
+  // zero the flags and counters of the many-workgroup launches (a fill
   // kernel, so the call may sit inside a captured graph)
-  const size_t nt = tri_blocks(n);
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, s, tmp + 2 * (size_t)n + nt * kTB * kTB, nt + 2);
+  if (n <= kSmallMax) return CT_OK;
+  const size_t words = control_doubles(n);
+  const unsigned int grid = (unsigned int)std::min<size_t>((words + 255) / 256, 64);
+  hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s, reinterpret_cast<double*>(control_words(tmp, n)),
+                     words);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -996,22 +1174,11 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
-  for (int k0 = 0; k0 < n; k0 += kNB) {
-    const int kb = n - k0 < kNB ? n - k0 : kNB;
-    const int rows = n - k0 - kb;
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(kWave), 0, s, S, tmp + n, n, k0, st);
-    CT_LAUNCH_CHECK();
-    if (rows > 0) {
-      const int grid = (rows + kPanelThreads - 1) / kPanelThreads;
-      hipLaunchKernelGGL(k_chol_panel, dim3(grid), dim3(kPanelThreads), 0, s, S, tmp + n, n, k0, st);
-      CT_LAUNCH_CHECK();
-      const int T = (rows + kTile - 1) / kTile;
-      hipLaunchKernelGGL(k_chol_syrk, dim3(T * (T + 1) / 2), dim3(256), 0, s, S, n, k0, kb, st);
-      CT_LAUNCH_CHECK();
-    }
-  }
-  hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned int)tri_blocks(n)), dim3(kWave), 0, s, S,
-                     tmp + 2 * (size_t)n, n, st);
+  int cus = 0;
+  CT_TRY(device_cus(&cus));
+  const size_t tiles = factor_tiles(n);
+  const int G = (int)std::min<size_t>(tiles, 2 * (size_t)cus);  // two workgroups fit a CU (LDS)
+  hipLaunchKernelGGL(k_chol_tiles, dim3(G), dim3(kFThreads), 0, s, S, n, factor_work(tmp, n), st);
   CT_LAUNCH_CHECK();
   if (rhs == nullptr) return CT_OK;  // factorisation only
   return launch_tri_mt(S, rhs, x, n, st, tmp, s);
