@@ -96,17 +96,18 @@ class _Ticket:
 
 
 def estimate_bytes(req) -> int:
-    """Device working set of a request (inputs, Jacobians, normal matrices, scratch), generously rounded."""
+    """Device working set of a request, generously rounded: inputs, Jacobians, the dense normal matrix, its
+    factor's inverse and scratch."""
     if isinstance(req, CurveRequest):
         return 64 * int(np.asarray(req.x).size) + 4096
     if isinstance(req, BARequest):
         C, P, N = len(req.cameras), len(req.points), len(req.obs_camera)
         n = 9 * C
         pairs = int(np.sum(np.bincount(np.asarray(req.obs_point), minlength=P) ** 2))
-        return 400 * N + 200 * P + 24 * n * n + 16 * pairs + (1 << 20)
+        return 400 * N + 200 * P + 32 * n * n + 16 * pairs + (1 << 20)
     if isinstance(req, PoseRequest):
         n = 3 * len(req.poses)
-        return 200 * len(req.edges) + 100 * len(req.poses) + 24 * n * n + (1 << 20)
+        return 200 * len(req.edges) + 100 * len(req.poses) + 32 * n * n + (1 << 20)
     raise TypeError(f"unknown request type {type(req).__name__}")
 
 

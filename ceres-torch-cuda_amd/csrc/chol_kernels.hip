@@ -7,8 +7,8 @@
 //    register tiles) and runs both triangular solves.
 //  * n > 256: left-looking Cholesky on 64x64 tiles in one persistent
 //    launch (workgroups take tiles in dependency order and hand them on
-//    through per-tile flags); the diagonal tiles' inverses are formed on the
-//    way and the triangular solves run in a second launch of the same kind.
+//    through per-tile flags); the same launch forms V = L^-1 tile by tile, so
+//    every solve with the factor is two bandwidth-bound products with V.
 //  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning as
 //    one persistent kernel (linear.py:1216-1256, preconditioner :1156-1182).
 #include <utility>
@@ -306,9 +306,9 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 }
 
 // ---------------------------------------------------------------------------
-// large n: one persistent launch for the factorisation, one for the solves
+// large n: one persistent launch for the factorisation and the inverse
 //
-// Both work on kTB x kTB tiles (kTB = 64) and hand tiles between workgroups
+// It works on kTB x kTB tiles (kTB = 64) and hands tiles between workgroups
 // through per-tile flags stamped with the launch's epoch (read once at the
 // start; the last workgroup to finish advances it), so flags never need
 // clearing.  Workgroups take tiles from an atomic ticket counter in an order
@@ -319,10 +319,13 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 // raises the launch's abort word, every workgroup stops, and the step is
 // flagged invalid.
 //
-// Publication: every wave completes and writes back its own stores
-// (__threadfence) before the workgroup barrier, then one thread stamps the
-// flag with release semantics.  A waiting workgroup polls with plain atomic
-// loads and takes an acquire fence in every wave once the flag is seen.
+// Publication: every wave waits until its own stores have reached L2
+// (s_waitcnt: a workgroup barrier alone does not wait for other waves'
+// stores), then after the barrier one thread writes L2 back for the other
+// XCDs (agent-scope fence) and stamps the flag with release semantics.  A
+// waiting workgroup's polling thread uses plain atomic loads and one
+// agent-scope acquire once the flag is seen; the caches it invalidates (L1
+// per CU, L2 per XCD) are the ones every wave of the workgroup reads through.
 // ---------------------------------------------------------------------------
 constexpr int kTB = 64;
 constexpr int kFThreads = 256;
@@ -334,6 +337,9 @@ __host__ __device__ inline size_t factor_tiles(int n) {
   const size_t nt = tri_blocks(n);
   return nt * (nt + 1) / 2;
 }
+
+// This wave's outstanding memory operations (stores included) are complete.
+__device__ __forceinline__ void own_stores_done() { __builtin_amdgcn_s_waitcnt(0); }
 
 __device__ __forceinline__ void lds_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -363,13 +369,15 @@ __device__ __forceinline__ double pivot_rsqrt(double x) {
 // Pivot step JJ of the one-wave factorisation of a 16-column panel held in
 // registers: lane l owns row l of the panel (the panel's first 16 rows are
 // its diagonal block).  Steps are expanded at compile time so that a[] is
-// only ever indexed by constants.  The pivot's reciprocal goes to pinv[JJ]
-// and a bad pivot sets *pbad (LDS stores inside the step keep those results
-// from being carried to the end).
+// only ever indexed by constants.  The pivot and the column below it reach
+// every lane through scalar registers (v_readlane), so a step needs no LDS
+// round trip.  The pivot's reciprocal goes to pinv[JJ] and a bad pivot sets
+// *pbad (LDS stores inside the step keep those results from being carried to
+// the end).
 constexpr int kPW = 16;  // panel width of the diagonal-tile factorisation
 
 template <int JJ>
-__device__ __forceinline__ void panel_step(double (&a)[kPW], double* colb, double* pinv, int* pbad, int l) {
+__device__ __forceinline__ void panel_step(double (&a)[kPW], double* pinv, int* pbad, int l) {
   const double piv = lane_value(a[JJ], JJ);
   const double inv = pivot_rsqrt(piv);
   if (l == JJ) {
@@ -380,17 +388,14 @@ __device__ __forceinline__ void panel_step(double (&a)[kPW], double* colb, doubl
   // and updated like the others (they stay finite) instead of being masked
   a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
   const double f = l > JJ ? a[JJ] : 0.0;
-  colb[l] = f;
-  lds_wave_sync();
 #pragma unroll
-  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, colb[m], a[m]);
-  lds_wave_sync();
+  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
 }
 
 template <int... J>
-__device__ __forceinline__ void panel_steps(double (&a)[kPW], double* colb, double* pinv, int* pbad, int l,
+__device__ __forceinline__ void panel_steps(double (&a)[kPW], double* pinv, int* pbad, int l,
                                             std::integer_sequence<int, J...>) {
-  (panel_step<J>(a, colb, pinv, pbad, l), ...);
+  (panel_step<J>(a, pinv, pbad, l), ...);
 }
 
 // Every thread calls this; thread 0 polls up to two flags.  *status (LDS) is
@@ -413,16 +418,19 @@ __device__ __forceinline__ bool wait_flags(const int* f1, const int* f2, int epo
       }
       if (*status) break;
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *status == 0;
 }
 
 __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
-  __threadfence();
+  own_stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Next ticket for this workgroup (every thread calls it), or `stop` once the
@@ -451,15 +459,138 @@ __device__ __forceinline__ void finish_launch(unsigned int* ctl, int epoch) {
   }
 }
 
+// Copy a kTB x kTB tile (rows r0.., columns c0.. of a row-major matrix with
+// leading dimension ld; rows >= rows_valid read as zero) into LDS.  All 16
+// loads of a thread are issued before the first store.
+__device__ __forceinline__ void stage_tile(double (*dst)[kFLd], const double* __restrict__ src, int ld, int r0,
+                                           int c0, int rows_valid, int tid) {
+  constexpr int kPer = kTB * kTB / kFThreads;
+  double v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int r = u * (kFThreads / kTB) + tid / kTB, c = tid % kTB;
+    v[u] = r < rows_valid ? src[(size_t)(r0 + r) * ld + c0 + c] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) dst[u * (kFThreads / kTB) + tid / kTB][tid % kTB] = v[u];
+}
+
+// One doubling step of the blocked inverse of the lower-triangular factor in
+// D (V holds the inverses of the diagonal blocks of width H, zero above the
+// diagonal): for every 2H-block at base b, X21 = -V22 (L21 V11).  The
+// products run over full block widths (the zero upper triangles make the
+// triangular bounds implicit), with T = L21 V11 staged in D's unused upper
+// triangle at D[b + r][b + H + c].
+template <int H>
+__device__ __forceinline__ void inverse_doubling(double (*D)[kFLd], double (*V)[kFLd], int tid) {
+  constexpr int kPerLevel = (kTB / (2 * H)) * H * H;
+  for (int e = tid; e < kPerLevel; e += kFThreads) {
+    const int b = (e / (H * H)) * 2 * H, r = (e / H) % H, c = e % H;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < H; ++m) s = fma(D[b + H + r][b + m], V[b + m][b + c], s);
+    D[b + r][b + H + c] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < kPerLevel; e += kFThreads) {
+    const int b = (e / (H * H)) * 2 * H, r = (e / H) % H, c = e % H;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < H; ++m) s = fma(V[b + H + r][b + H + m], D[b + m][b + H + c], s);
+    V[b + H + r][b + c] = -s;
+  }
+  __syncthreads();
+}
+
 struct FactorWork {
   double* dinv;        // [n] reciprocals of the pivots
   double* linv;        // [nt][kTB][kTB] inverses of the diagonal tiles of L (row-major, zero outside)
+  double* vinv;        // [n][n] V = L^-1: tiles below the diagonal (row-major, leading dimension n)
   int* ready;          // [nt (nt + 1) / 2] epoch stamp of each finished tile of L
+  int* vready;         // [nt (nt - 1) / 2] epoch stamp of each finished tile of V below the diagonal
   unsigned int* ctl;   // [0] completed launches, [1] finished workgroups, [2] next ticket, [3] abort epoch
 };
 
 // Lower tiles in column-major order: column j holds tiles (j..nt-1, j).
 __device__ __forceinline__ int tile_index(int i, int j, int nt) { return j * nt - j * (j - 1) / 2 + (i - j); }
+// Tiles strictly below the diagonal in column-major order: column j holds (j+1..nt-1, j).
+__device__ __forceinline__ int vtile_index(int i, int j, int nt) { return j * (nt - 1) - j * (j - 1) / 2 + (i - j - 1); }
+
+// Tile (i, j), i > j, of V = L^-1: acc = sum_{k=j}^{i-1} L_ik V_kj, then
+// V_ij = -V_ii acc.  V_jj and V_ii are the diagonal tiles' inverses.  Returns
+// false once the launch is aborted.  Every thread calls it.
+__device__ bool inverse_tile(const double* __restrict__ A, int n, int nt, int i, int j, const FactorWork& fw,
+                             int epoch, double (*lds)[kTB][kFLd], int* status) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int tr = tid / 16, tc = tid % 16;
+  const int r0 = i * kTB, ri = min(kTB, n - r0);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  double(*Li)[kFLd] = lds[0];
+  double(*Vk)[kFLd] = lds[1];
+  for (int k = j; k < i; ++k) {
+    if (!wait_flags(&fw.ready[tile_index(i, k, nt)],
+                    k == j ? &fw.ready[tile_index(j, j, nt)] : &fw.vready[vtile_index(k, j, nt)], epoch, &fw.ctl[3],
+                    status))
+      return false;
+    stage_tile(Li, A, n, r0, k * kTB, ri, tid);  // L_ik (k < i: full width)
+    if (k == j) stage_tile(Vk, fw.linv + (size_t)j * kTB * kTB, kTB, 0, 0, kTB, tid);
+    else stage_tile(Vk, fw.vinv, n, k * kTB, j * kTB, kTB, tid);  // V_kj, j < k < i: full tile
+    __syncthreads();
+    for (int m = 0; m < kTB; ++m) {
+      double pa[4], pb[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        pa[a] = Li[tr + 16 * a][m];
+        pb[a] = Vk[m][tc + 16 * a];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(pa[a], pb[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  if (!wait_flags(&fw.ready[tile_index(i, i, nt)], nullptr, epoch, &fw.ctl[3], status)) return false;
+  double(*X)[kFLd] = lds[0];
+  double(*Vi)[kFLd] = lds[1];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) X[tr + 16 * a][tc + 16 * b] = acc[a][b];
+  stage_tile(Vi, fw.linv + (size_t)i * kTB * kTB, kTB, 0, 0, kTB, tid);
+  __syncthreads();
+  double out[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
+  for (int m = 0; m < kTB; ++m) {
+    double pa[4], pb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      pa[a] = Vi[tr + 16 * a][m];
+      pb[a] = X[m][tc + 16 * a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) out[a][b] = fma(pa[a], pb[b], out[a][b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int row = tr + 16 * a, col = tc + 16 * b;
+      if (row < ri) fw.vinv[(size_t)(r0 + row) * n + j * kTB + col] = -out[a][b];
+    }
+  publish_flag(&fw.vready[vtile_index(i, j, nt)], epoch);
+  return true;
+}
 
 // Left-looking tiled Cholesky, S = L L^T in place (lower triangle), in one
 // launch.  The workgroup holding tile (i, j) keeps it in registers (4x4 per
@@ -473,33 +604,37 @@ __device__ __forceinline__ int tile_index(int i, int j, int nt) { return j * nt 
 //  * off-diagonal tile: L_ij = A_ij V_jj^T with the published inverse.
 // Rows past n in the last tile row are zero; the diagonal tile pads them with
 // an identity block.  A pivot that is not positive and finite flags the step.
+// Row c of V = L^-1 below the diagonal follows column c of L in the ticket
+// order (workgroups that would otherwise idle on the factor's critical path
+// take those tiles): V_ij = -V_ii sum_{k=j}^{i-1} L_ik V_kj, k in increasing
+// order.  Each later solve is then two products with V.
 __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A, int n, FactorWork fw,
                                                           LMState* st) {
   if (skip_solve(st)) return;
   __shared__ double lds[2][kTB][kFLd];
-  __shared__ double colb[kTB], pinv[kTB];
+  __shared__ double pinv[kTB];
   __shared__ int s_status, s_ticket, s_bad;
   const int nt = (int)tri_blocks(n);
-  const int ntiles = nt * (nt + 1) / 2;
+  const int nall = nt * nt;
   const int epoch = (int)__hip_atomic_load(&fw.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
   if (threadIdx.x == 0) s_status = 0;
   for (;;) {
-    const int q = take_ticket(&fw.ctl[2], ntiles, &s_status, &s_ticket);
-    if (q >= ntiles) break;
+    // tickets c nt .. c nt + nt - 1: column c of L (nt - c tiles), then row c
+    // of V below the diagonal (c tiles); everything a tile reads comes from
+    // a smaller ticket
+    const int q = take_ticket(&fw.ctl[2], nall, &s_status, &s_ticket);
+    if (q >= nall) break;
+    const int seg = q / nt, pos = q % nt;
+    if (pos >= nt - seg) {
+      if (!inverse_tile(A, n, nt, seg, pos - (nt - seg), fw, epoch, lds, &s_status)) break;
+      continue;
+    }
     // an opaque copy of the thread index per tile: nothing derived from it
     // (LDS and global addresses) is computed ahead and kept live across tiles
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int tr = tid / 16, tc = tid % 16;
-    int i = 0, j = 0;
-    {
-      int rest = q;
-      while (rest >= nt - j) {
-        rest -= nt - j;
-        ++j;
-      }
-      i = j + rest;
-    }
+    const int i = seg + pos, j = seg;
     const int r0 = i * kTB, c0 = j * kTB;
     const int ri = min(kTB, n - r0), cj = min(kTB, n - c0);
     double acc[4][4];
@@ -517,11 +652,8 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
                       &fw.ctl[3], &s_status))
         break;
       const int kk = k * kTB;  // tiles left of the diagonal are full width
-      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
-        const int r = idx / kTB, c = idx % kTB;
-        Li[r][c] = r < ri ? A[(size_t)(r0 + r) * n + kk + c] : 0.0;
-        if (i != j) Lj[r][c] = A[(size_t)(c0 + r) * n + kk + c];
-      }
+      stage_tile(Li, A, n, r0, kk, ri, tid);
+      if (i != j) stage_tile(Lj, A, n, c0, kk, kTB, tid);
       __syncthreads();
       for (int c = 0; c < kTB; ++c) {
         double pa[4], pb[4];
@@ -561,7 +693,7 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
 #pragma unroll
           for (int q = 0; q < kPW; ++q) a[q] = row < kTB ? D[row][base + q] : 0.0;
           lds_wave_sync();
-          panel_steps(a, colb, pinv + base, &s_bad, l, std::make_integer_sequence<int, kPW>{});
+          panel_steps(a, pinv + base, &s_bad, l, std::make_integer_sequence<int, kPW>{});
           if (row < kTB) {
 #pragma unroll
             for (int q = 0; q < kPW; ++q) D[row][base + q] = a[q];
@@ -592,7 +724,7 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
             double s = ii == c ? 1.0 : 0.0;
 #pragma unroll
             for (int kq = 0; kq < ii; ++kq) s = fma(-D[b0 + ii][b0 + kq], x[kq], s);
-            x[ii] = ii < c ? 0.0 : s / D[b0 + ii][b0 + ii];
+            x[ii] = ii < c ? 0.0 : s * pinv[b0 + ii];
           }
 #pragma unroll
           for (int ii = 0; ii < 16; ++ii) V[b0 + ii][b0 + c] = x[ii];
@@ -601,25 +733,8 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       __syncthreads();
       // doubling steps: for blocks of width h at base b, X21 = -V22 (L21 V11);
       // T = L21 V11 is staged in the (unused) upper triangle of D at D[b + r][b + h + c]
-      for (int h = 16; h < kTB; h *= 2) {
-        const int per = (kTB / (2 * h)) * h * h;  // entries of all blocks at this level
-        for (int e = tid; e < per; e += kFThreads) {
-          const int blk = e / (h * h), r = (e / h) % h, c = e % h;
-          const int b = blk * 2 * h;
-          double s = 0.0;
-          for (int m = c; m < h; ++m) s = fma(D[b + h + r][b + m], V[b + m][b + c], s);
-          D[b + r][b + h + c] = s;
-        }
-        __syncthreads();
-        for (int e = tid; e < per; e += kFThreads) {
-          const int blk = e / (h * h), r = (e / h) % h, c = e % h;
-          const int b = blk * 2 * h;
-          double s = 0.0;
-          for (int m = 0; m <= r; ++m) s = fma(V[b + h + r][b + h + m], D[b + m][b + h + c], s);
-          V[b + h + r][b + c] = -s;
-        }
-        __syncthreads();
-      }
+      inverse_doubling<16>(D, V, tid);
+      inverse_doubling<32>(D, V, tid);
       double* vout = fw.linv + (size_t)j * kTB * kTB;
       for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
         const int r = idx / kTB, c = idx % kTB;
@@ -630,12 +745,11 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       if (!wait_flags(&fw.ready[tile_index(j, j, nt)], nullptr, epoch, &fw.ctl[3], &s_status)) break;
       double(*X)[kFLd] = lds[0];
       double(*Vt)[kFLd] = lds[1];
-      const double* vin = fw.linv + (size_t)j * kTB * kTB;
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) X[tr + 16 * a][tc + 16 * b] = acc[a][b];
-      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) Vt[idx / kTB][idx % kTB] = vin[idx];
+      stage_tile(Vt, fw.linv + (size_t)j * kTB * kTB, kTB, 0, 0, kTB, tid);
       __syncthreads();
       double out[4][4];
 #pragma unroll
@@ -668,122 +782,87 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
   finish_launch(fw.ctl, epoch);
 }
 
-// L y = b and L^T x = y in one launch.  Tickets 0..nt-1 are the forward
-// blocks in increasing order, tickets nt..2nt-1 the backward blocks in
-// decreasing order.  Forward block r subtracts L_rj y_j as each y_j is
-// published and finishes with y_r = V_rr (...); the backward block mirrors
-// it with L^T and V_rr^T.
-constexpr int kTriMtThreads = 256;
-constexpr int kTriSlices = kTriMtThreads / kTB;  // column slices per row in a block product
+// Solves with the inverse: y = V b, x = V^T y.  Each lower tile (i, j) of V
+// (the diagonal tiles from linv) writes its partial product to
+// part[block][row]; a second kernel sums the partials of each row in one
+// fixed order, so the solve is deterministic and reads V once per product.
+constexpr int kApplyThreads = 256;
 
-struct TriWork {
-  double* y;             // [n] forward solution
-  const double* linv;    // [nt][kTB][kTB] inverses of the diagonal blocks
-  int* flags;            // [2 nt] epoch stamps: y block ready, x block ready
-  unsigned int* ctl;     // [0] completed launches, [1] finished workgroups, [2] next ticket, [3] abort epoch
-};
-
-__global__ __launch_bounds__(kTriMtThreads) void k_tri_solve_mt(const double* __restrict__ A,
-                                                                const double* __restrict__ rhs,
-                                                                double* __restrict__ x, int n, TriWork tw,
-                                                                LMState* st) {
+// part[j][i kTB + r] = sum_c V_ij[r][c] vec[j kTB + c]            (transpose = false)
+// part[i][j kTB + c] = sum_r V_ij[r][c] vec[i kTB + r]            (transpose = true)
+template <bool kTranspose>
+__global__ __launch_bounds__(kApplyThreads) void k_inv_partials(const double* __restrict__ vinv,
+                                                                 const double* __restrict__ linv,
+                                                                 const double* __restrict__ vec,
+                                                                 double* __restrict__ part, int n, LMState* st) {
   if (skip_solve(st)) return;
-  __shared__ double acc[kTB];
-  __shared__ double part[kTriSlices][kTB];
-  __shared__ int s_status, s_ticket;
-  const int tid = threadIdx.x;
-  const int row = tid % kTB, sl = tid / kTB;
+  __shared__ double red[kApplyThreads / kTB][kTB];
   const int nt = (int)tri_blocks(n);
-  const int epoch = (int)__hip_atomic_load(&tw.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
-  int my_bad = 0;
-  if (tid == 0) s_status = 0;
-  for (;;) {
-    const int q = take_ticket(&tw.ctl[2], 2 * nt, &s_status, &s_ticket);
-    if (q >= 2 * nt) break;
-    if (q < nt) {
-      // forward: y_r = L_rr^-1 (b_r - sum_{j<r} L_rj y_j)
-      const int r = q;
-      const int r0 = r * kTB, rb = min(kTB, n - r0);
-      if (tid < kTB) acc[tid] = tid < rb ? rhs[r0 + tid] : 0.0;
-      __syncthreads();
-      for (int j = 0; j < r; ++j) {
-        if (!wait_flags(&tw.flags[j], nullptr, epoch, &tw.ctl[3], &s_status)) break;
-        double p = 0.0;
-        if (row < rb) {
-          const double* Lrow = A + (size_t)(r0 + row) * n + j * kTB + sl * (kTB / kTriSlices);
-          const double* yj = tw.y + j * kTB + sl * (kTB / kTriSlices);
+  const int q = blockIdx.x;
+  int j = 0, rest = q;
+  while (rest >= nt - j) {
+    rest -= nt - j;
+    ++j;
+  }
+  const int i = j + rest;
+  const int tid = threadIdx.x;
+  const int r0 = i * kTB, c0 = j * kTB;
+  const int ri = min(kTB, n - r0), cj = min(kTB, n - c0);
+  const double* tile = (i == j) ? linv + (size_t)j * kTB * kTB : vinv + (size_t)r0 * n + c0;
+  const int ld = (i == j) ? kTB : n;
+  if (!kTranspose) {
+    // thread (row r, 16-column slice s): contiguous 128-byte reads
+    const int r = tid / 4, sl = tid % 4;
+    double sum = 0.0;
+    if (r < ri) {
 #pragma unroll
-          for (int c = 0; c < kTB / kTriSlices; ++c) p += Lrow[c] * yj[c];
-        }
-        part[sl][row] = p;
-        __syncthreads();
-        if (tid < kTB) {
-          double t = 0.0;
-#pragma unroll
-          for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
-          acc[tid] -= t;
-        }
-        __syncthreads();
+      for (int u = 0; u < 16; ++u) {
+        const int c = sl * 16 + u;
+        if (c < cj) sum = fma(tile[(size_t)r * ld + c], vec[c0 + c], sum);
       }
-      if (s_status) break;
-      const double* Li = tw.linv + (size_t)r * kTB * kTB + row * kTB + sl * (kTB / kTriSlices);
-      double p = 0.0;
+    }
+    sum += __shfl_xor(sum, 1, kWave);
+    sum += __shfl_xor(sum, 2, kWave);
+    if (sl == 0 && r < ri) part[(size_t)j * n + r0 + r] = sum;
+  } else {
+    // thread (16-row slice s, column c): coalesced along c
+    const int sl = tid / kTB, c = tid % kTB;
+    double sum = 0.0;
+    if (c < cj) {
 #pragma unroll
-      for (int c = 0; c < kTB / kTriSlices; ++c) p += Li[c] * acc[sl * (kTB / kTriSlices) + c];
-      part[sl][row] = p;
-      __syncthreads();
-      if (tid < rb) {
-        double t = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
-        tw.y[r0 + tid] = t;
+      for (int u = 0; u < 16; ++u) {
+        const int r = sl * 16 + u;
+        if (r < ri) sum = fma(tile[(size_t)r * ld + c], vec[r0 + r], sum);
       }
-      publish_flag(&tw.flags[r], epoch);
-    } else {
-      // backward: x_r = L_rr^-T (y_r - sum_{j>r} L_jr^T x_j)
-      const int r = 2 * nt - 1 - q;
-      const int r0 = r * kTB, rb = min(kTB, n - r0);
-      if (!wait_flags(&tw.flags[r], nullptr, epoch, &tw.ctl[3], &s_status)) break;
-      if (tid < kTB) acc[tid] = tid < rb ? tw.y[r0 + tid] : 0.0;
-      __syncthreads();
-      for (int j = nt - 1; j > r; --j) {
-        if (!wait_flags(&tw.flags[nt + j], nullptr, epoch, &tw.ctl[3], &s_status)) break;
-        const int j0 = j * kTB, jb = min(kTB, n - j0);
-        double p = 0.0;
-        if (row < rb) {
-          for (int k = sl * (kTB / kTriSlices); k < (sl + 1) * (kTB / kTriSlices); ++k)
-            if (k < jb) p += A[(size_t)(j0 + k) * n + r0 + row] * x[j0 + k];
-        }
-        part[sl][row] = p;
-        __syncthreads();
-        if (tid < kTB) {
-          double t = 0.0;
+    }
+    red[sl][c] = sum;
+    __syncthreads();
+    if (tid < cj) {
+      double t = 0.0;
 #pragma unroll
-          for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
-          acc[tid] -= t;
-        }
-        __syncthreads();
-      }
-      if (s_status) break;
-      const double* Lb = tw.linv + (size_t)r * kTB * kTB;
-      double p = 0.0;
-      for (int c = sl * (kTB / kTriSlices); c < (sl + 1) * (kTB / kTriSlices); ++c) p += Lb[c * kTB + row] * acc[c];
-      part[sl][row] = p;
-      __syncthreads();
-      if (tid < rb) {
-        double t = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < kTriSlices; ++qq) t += part[qq][tid];
-        x[r0 + tid] = t;
-        if (!isfinite(t)) my_bad = 1;
-      }
-      publish_flag(&tw.flags[nt + r], epoch);
+      for (int qq = 0; qq < kApplyThreads / kTB; ++qq) t += red[qq][tid];
+      part[(size_t)i * n + c0 + tid] = t;
     }
   }
-  // a non-finite solution or an aborted launch invalidates the step
-  if (st && (my_bad || (tid == 0 && s_status))) st->step_invalid = 1;
-  if (st && blockIdx.x == 0 && tid == 0) st->linear_iterations = 1;
-  finish_launch(tw.ctl, epoch);
+}
+
+// out[e] = sum over the partial blocks of e in increasing block order: blocks
+// 0..b(e) for V b, blocks b(e)..nt-1 for V^T y (b(e) = e / kTB).
+template <bool kTranspose>
+__global__ void k_inv_reduce(const double* __restrict__ part, double* __restrict__ out, int n, LMState* st) {
+  if (skip_solve(st)) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int nt = (int)tri_blocks(n), be = e / kTB;
+  double t = 0.0;
+  if (!kTranspose) {
+    for (int b = 0; b <= be; ++b) t += part[(size_t)b * n + e];
+  } else {
+    for (int b = be; b < nt; ++b) t += part[(size_t)b * n + e];
+    if (!isfinite(t) && st) st->step_invalid = 1;
+  }
+  out[e] = t;
+  if (kTranspose && e == 0 && st) st->linear_iterations = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -813,7 +892,7 @@ struct PcgWork {
 };
 
 __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int target, int* timed_out) {
-  __threadfence();  // every wave's stores complete before the arrival is counted
+  own_stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -829,7 +908,6 @@ __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int tar
     __threadfence();
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *timed_out == 0;
 }
 
@@ -1096,37 +1174,35 @@ fail:
 }  // namespace
 
 // solve scratch layout (doubles): [0,n) y, [n,2n) pivot reciprocals,
-// [2n, 2n + nt kTB^2) diagonal-block inverses, then the control words:
-// triangular-solve flags (2 nt ints) and counters (4 uints), factor-tile
-// flags (nt (nt + 1) / 2 ints) and counters (4 uints).
+// [2n, 2n + nt kTB^2) diagonal-tile inverses, then V = L^-1 (n x n), the
+// solve partials (nt x n) and the control words of the factor launch: tile
+// flags of L (nt (nt + 1) / 2 ints) and of V (nt (nt - 1) / 2 ints), then 4
+// counters.  Only n > kSmallMax uses more than the first 2n.
 namespace {
+size_t vinv_offset(int n) { return 2 * (size_t)n + tri_blocks(n) * kTB * kTB; }
+size_t part_offset(int n) { return vinv_offset(n) + (size_t)n * n; }
+size_t control_offset(int n) { return part_offset(n) + tri_blocks(n) * (size_t)n; }
 size_t control_doubles(int n) {
   const size_t nt = tri_blocks(n);
-  return nt + 2 + (factor_tiles(n) + 1) / 2 + 2;
+  return (nt * nt + 1) / 2 + 2;  // nt (nt + 1) / 2 + nt (nt - 1) / 2 = nt^2 ints, 4 uints
 }
-int* control_words(double* tmp, int n) { return reinterpret_cast<int*>(tmp + 2 * (size_t)n + tri_blocks(n) * kTB * kTB); }
 }  // namespace
 
-size_t rcs_cholesky_tmp_doubles(int n) { return 2 * (size_t)n + tri_blocks(n) * kTB * kTB + control_doubles(n); }
-
-namespace {
-TriWork tri_work(double* tmp, int n) {
-  const size_t nt = tri_blocks(n);
-  TriWork tw;
-  tw.y = tmp;
-  tw.linv = tmp + 2 * (size_t)n;
-  tw.flags = control_words(tmp, n);
-  tw.ctl = reinterpret_cast<unsigned int*>(tw.flags + 2 * nt);
-  return tw;
+size_t rcs_cholesky_tmp_doubles(int n) {
+  if (n <= kSmallMax) return 2 * (size_t)n;
+  return control_offset(n) + control_doubles(n);
 }
 
+namespace {
 FactorWork factor_work(double* tmp, int n) {
   const size_t nt = tri_blocks(n);
   FactorWork fw;
   fw.dinv = tmp + n;
   fw.linv = tmp + 2 * (size_t)n;
-  fw.ready = control_words(tmp, n) + 2 * nt + 4;
-  fw.ctl = reinterpret_cast<unsigned int*>(fw.ready + factor_tiles(n) + (factor_tiles(n) & 1));
+  fw.vinv = tmp + vinv_offset(n);
+  fw.ready = reinterpret_cast<int*>(tmp + control_offset(n));
+  fw.vready = fw.ready + factor_tiles(n);
+  fw.ctl = reinterpret_cast<unsigned int*>(fw.ready + nt * nt + (nt & 1));
   return fw;
 }
 
@@ -1138,12 +1214,21 @@ int device_cus(int* cus) {
   return CT_OK;
 }
 
-int launch_tri_mt(const double* L, const double* rhs, double* x, int n, LMState* st, double* tmp, hipStream_t s) {
-  int cus = 0;
-  CT_TRY(device_cus(&cus));
-  const int nt = (int)tri_blocks(n);
-  const int G = nt < cus ? nt : cus;
-  hipLaunchKernelGGL(k_tri_solve_mt, dim3(G), dim3(kTriMtThreads), 0, s, L, rhs, x, n, tri_work(tmp, n), st);
+// x = V^T (V rhs) with the inverse the factor launch left in tmp
+int launch_inverse_apply(const double* rhs, double* x, int n, LMState* st, double* tmp, hipStream_t s) {
+  const unsigned int tiles = (unsigned int)factor_tiles(n);
+  const unsigned int rgrid = (unsigned int)((n + 255) / 256);
+  double* y = tmp;
+  double* part = tmp + part_offset(n);
+  const double* vinv = tmp + vinv_offset(n);
+  const double* linv = tmp + 2 * (size_t)n;
+  hipLaunchKernelGGL(k_inv_partials<false>, dim3(tiles), dim3(kApplyThreads), 0, s, vinv, linv, rhs, part, n, st);
+  CT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_inv_reduce<false>, dim3(rgrid), dim3(256), 0, s, part, y, n, st);
+  CT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_inv_partials<true>, dim3(tiles), dim3(kApplyThreads), 0, s, vinv, linv, y, part, n, st);
+  CT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_inv_reduce<true>, dim3(rgrid), dim3(256), 0, s, part, x, n, st);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -1155,13 +1240,12 @@ __global__ void k_zero_words(double* p, size_t count) {
 }  // namespace
 
 int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s) {
-  // zero the flags and counters of the many-workgroup launches (a fill
-  // kernel, so the call may sit inside a captured graph)
+  // zero the flags and counters of the factor launch (a fill kernel, so the
+  // call may sit inside a captured graph)
   if (n <= kSmallMax) return CT_OK;
   const size_t words = control_doubles(n);
   const unsigned int grid = (unsigned int)std::min<size_t>((words + 255) / 256, 64);
-  hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s, reinterpret_cast<double*>(control_words(tmp, n)),
-                     words);
+  hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s, tmp + control_offset(n), words);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -1176,18 +1260,18 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
   }
   int cus = 0;
   CT_TRY(device_cus(&cus));
-  const size_t tiles = factor_tiles(n);
+  const size_t tiles = tri_blocks(n) * tri_blocks(n);  // factor and inverse tiles
   const int G = (int)std::min<size_t>(tiles, 2 * (size_t)cus);  // two workgroups fit a CU (LDS)
   hipLaunchKernelGGL(k_chol_tiles, dim3(G), dim3(kFThreads), 0, s, S, n, factor_work(tmp, n), st);
   CT_LAUNCH_CHECK();
   if (rhs == nullptr) return CT_OK;  // factorisation only
-  return launch_tri_mt(S, rhs, x, n, st, tmp, s);
+  return launch_inverse_apply(rhs, x, n, st, tmp, s);
 }
 
 int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, LMState* st, double* tmp,
                          hipStream_t s) {
   if (n <= 0) return CT_OK;
-  if (n > kSmallMax) return launch_tri_mt(L, rhs, x, n, st, tmp, s);
+  if (n > kSmallMax) return launch_inverse_apply(rhs, x, n, st, tmp, s);  // the factor launch left V in tmp
   hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(kTriThreads), 0, s, L, rhs, x, tmp, n, st);
   CT_LAUNCH_CHECK();
   return CT_OK;
