@@ -33,11 +33,15 @@ COMMON = [
 ]
 
 
-# Kernels may not spill: no VGPR spills, no AGPRs used as spill space (no
-# kernel here issues MFMA), no scratch.  A register-allocator spill copy placed
-# inside divergent control flow only saves the lanes active there, so a value
-# that is live in other lanes comes back as garbage when restored under a wider
-# exec mask; kernels are kept small enough that the allocator never spills.
+# Kernels may not spill: no VGPR spills, no AGPRs used as spill space, no
+# scratch.  A register-allocator spill copy placed inside divergent control
+# flow only saves the lanes active there, so a value that is live in other
+# lanes comes back as garbage when restored under a wider exec mask; kernels
+# are kept small enough that the allocator never spills.  The one kernel that
+# issues MFMA may hold its matrix-core accumulators in AGPRs: it is allowed
+# exactly that many (4 blocks of v_mfma_f64_16x16x4f64 results, 4 doubles
+# each = 32 registers) and no more, so any spill into AGPRs still fails.
+MFMA_ACCUMULATOR_AGPRS = {"k_chol_tiles": 32}
 RESOURCE_RE = re.compile(r"remark: .*?(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|VGPRs Spill|SGPRs Spill): (\S+)")
 
 
@@ -47,7 +51,9 @@ def check_resources(stderr: str, src: str) -> list[str]:
         if key == "Function Name":
             name = val
         elif key == "AGPRs" and int(val) > 0:
-            problems.append(f"{src}:{name} uses {val} AGPRs")
+            allowed = next((v for k, v in MFMA_ACCUMULATOR_AGPRS.items() if name and k in name), 0)
+            if int(val) > allowed:
+                problems.append(f"{src}:{name} uses {val} AGPRs (allowed {allowed})")
         elif key == "VGPRs Spill" and int(val) > 0:
             problems.append(f"{src}:{name} spills {val} VGPRs")
         elif key.startswith("ScratchSize") and int(val) > 0:

@@ -516,20 +516,55 @@ __device__ __forceinline__ int tile_index(int i, int j, int nt) { return j * nt 
 // Tiles strictly below the diagonal in column-major order: column j holds (j+1..nt-1, j).
 __device__ __forceinline__ int vtile_index(int i, int j, int nt) { return j * (nt - 1) - j * (j - 1) / 2 + (i - j - 1); }
 
+// 64x64 tile products on the matrix cores.  Wave w of the workgroup owns the
+// 32x32 quadrant (w / 2, w % 2) as 2x2 blocks of v_mfma_f64_16x16x4f64: in
+// lane l, block b = 2 bi + bj holds in item v the element
+//   (32 (w / 2) + 16 bi + 4 v + l / 16,  32 (w % 2) + 16 bj + l % 16).
+// The A operand of one instruction is a 16x4 slice (lane l: row l % 16,
+// column l / 16), the B operand a 4x16 slice (lane l: row l / 16, column
+// l % 16).
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int acc_row(int w, int b, int v, int lane) {
+  return 32 * (w >> 1) + 16 * (b >> 1) + 4 * v + (lane >> 4);
+}
+__device__ __forceinline__ int acc_col(int w, int b, int lane) { return 32 * (w & 1) + 16 * (b & 1) + (lane & 15); }
+
+__device__ __forceinline__ void acc_zero(dx4 (&acc)[4]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) acc[b] = dx4{0.0, 0.0, 0.0, 0.0};
+}
+
+// acc += s P Q over k = 0..kTB-1, P[r][k] = Pm[r][k] and Q[k][c] = Qm[c][k]
+// (kQT) or Qm[k][c]; Pm and Qm are kTB x kTB LDS tiles, s is +1 or -1.
+template <bool kQT>
+__device__ __forceinline__ void tile_mma(dx4 (&acc)[4], const double (*Pm)[kFLd], const double (*Qm)[kFLd], double s,
+                                         int w, int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+  const int pr = 32 * (w >> 1) + li, qc = 32 * (w & 1) + li;
+#pragma unroll 4
+  for (int k0 = 0; k0 < kTB; k0 += 4) {
+    const int k = k0 + lk;
+    const double a0 = s * Pm[pr][k], a1 = s * Pm[pr + 16][k];
+    const double b0 = kQT ? Qm[qc][k] : Qm[k][qc];
+    const double b1 = kQT ? Qm[qc + 16][k] : Qm[k][qc + 16];
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+  }
+}
+
 // Tile (i, j), i > j, of V = L^-1: acc = sum_{k=j}^{i-1} L_ik V_kj, then
-// V_ij = -V_ii acc.  V_jj and V_ii are the diagonal tiles' inverses.  Returns
-// false once the launch is aborted.  Every thread calls it.
-__device__ bool inverse_tile(const double* __restrict__ A, int n, int nt, int i, int j, const FactorWork& fw,
+// V_ij = -V_ii acc.  V_jj and V_ii are the diagonal tiles' inverses.  tid is
+// the caller's thread index.  Returns false once the launch is aborted.  Every
+// thread calls it.
+__device__ __forceinline__ bool inverse_tile(const double* __restrict__ A, int n, int nt, int i, int j, int tid, const FactorWork& fw,
                              int epoch, double (*lds)[kTB][kFLd], int* status) {
-  int tid = threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  const int tr = tid / 16, tc = tid % 16;
+  const int w = tid / kWave, lane = tid % kWave;
   const int r0 = i * kTB, ri = min(kTB, n - r0);
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  dx4 acc[4];
+  acc_zero(acc);
   double(*Li)[kFLd] = lds[0];
   double(*Vk)[kFLd] = lds[1];
   for (int k = j; k < i; ++k) {
@@ -541,61 +576,36 @@ __device__ bool inverse_tile(const double* __restrict__ A, int n, int nt, int i,
     if (k == j) stage_tile(Vk, fw.linv + (size_t)j * kTB * kTB, kTB, 0, 0, kTB, tid);
     else stage_tile(Vk, fw.vinv, n, k * kTB, j * kTB, kTB, tid);  // V_kj, j < k < i: full tile
     __syncthreads();
-    for (int m = 0; m < kTB; ++m) {
-      double pa[4], pb[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        pa[a] = Li[tr + 16 * a][m];
-        pb[a] = Vk[m][tc + 16 * a];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = fma(pa[a], pb[b], acc[a][b]);
-    }
+    tile_mma<false>(acc, Li, Vk, 1.0, w, lane);
     __syncthreads();
   }
   if (!wait_flags(&fw.ready[tile_index(i, i, nt)], nullptr, epoch, &fw.ctl[3], status)) return false;
   double(*X)[kFLd] = lds[0];
   double(*Vi)[kFLd] = lds[1];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) X[tr + 16 * a][tc + 16 * b] = acc[a][b];
+    for (int v = 0; v < 4; ++v) X[acc_row(w, b, v, lane)][acc_col(w, b, lane)] = acc[b][v];
   stage_tile(Vi, fw.linv + (size_t)i * kTB * kTB, kTB, 0, 0, kTB, tid);
   __syncthreads();
-  double out[4][4];
+  dx4 out[4];
+  acc_zero(out);
+  tile_mma<false>(out, Vi, X, -1.0, w, lane);
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
-  for (int m = 0; m < kTB; ++m) {
-    double pa[4], pb[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      pa[a] = Vi[tr + 16 * a][m];
-      pb[a] = X[m][tc + 16 * a];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) out[a][b] = fma(pa[a], pb[b], out[a][b]);
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int row = tr + 16 * a, col = tc + 16 * b;
-      if (row < ri) fw.vinv[(size_t)(r0 + row) * n + j * kTB + col] = -out[a][b];
+    for (int v = 0; v < 4; ++v) {
+      const int row = acc_row(w, b, v, lane), col = acc_col(w, b, lane);
+      if (row < ri) fw.vinv[(size_t)(r0 + row) * n + j * kTB + col] = out[b][v];
     }
   publish_flag(&fw.vready[vtile_index(i, j, nt)], epoch);
   return true;
 }
 
 // Left-looking tiled Cholesky, S = L L^T in place (lower triangle), in one
-// launch.  The workgroup holding tile (i, j) keeps it in registers (4x4 per
-// thread) and subtracts L_ik L_jk^T for k = 0..j-1 as those tiles are
-// published, always in increasing k, so the arithmetic does not depend on
+// launch.  The workgroup holding tile (i, j) keeps it in registers (matrix-
+// core accumulators) and subtracts L_ik L_jk^T for k = 0..j-1 as those tiles
+// are published, always in increasing k, so the arithmetic does not depend on
 // which workgroup takes which tile.  Then:
 //  * diagonal tile: wave 0 factors it in registers (lane l owns row l); the
 //    inverse of the factor is formed blockwise (16x16 diagonal blocks by
@@ -626,24 +636,25 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
     if (q >= nall) break;
     const int seg = q / nt, pos = q % nt;
     if (pos >= nt - seg) {
-      if (!inverse_tile(A, n, nt, seg, pos - (nt - seg), fw, epoch, lds, &s_status)) break;
+      if (!inverse_tile(A, n, nt, seg, pos - (nt - seg), (int)threadIdx.x ^ (q >> 30), fw, epoch, lds, &s_status))
+        break;
       continue;
     }
-    // an opaque copy of the thread index per tile: nothing derived from it
-    // (LDS and global addresses) is computed ahead and kept live across tiles
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    const int tr = tid / 16, tc = tid % 16;
+    // a copy of the thread index the compiler cannot see through (q < 2^30,
+    // so q >> 30 is zero): nothing derived from it (LDS and global addresses)
+    // is computed ahead and kept live across tiles
+    const int tid = (int)threadIdx.x ^ (q >> 30);
+    const int w = tid / kWave, lane = tid % kWave;
     const int i = seg + pos, j = seg;
     const int r0 = i * kTB, c0 = j * kTB;
     const int ri = min(kTB, n - r0), cj = min(kTB, n - c0);
-    double acc[4][4];
+    dx4 acc[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int row = tr + 16 * a, col = tc + 16 * b;
-        acc[a][b] = (row < ri && col < cj) ? A[(size_t)(r0 + row) * n + c0 + col] : 0.0;
+      for (int v = 0; v < 4; ++v) {
+        const int row = acc_row(w, b, v, lane), col = acc_col(w, b, lane);
+        acc[b][v] = (row < ri && col < cj) ? A[(size_t)(r0 + row) * n + c0 + col] : 0.0;
       }
     double(*Li)[kFLd] = lds[0];
     double(*Lj)[kFLd] = (i == j) ? lds[0] : lds[1];
@@ -655,18 +666,7 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       stage_tile(Li, A, n, r0, kk, ri, tid);
       if (i != j) stage_tile(Lj, A, n, c0, kk, kTB, tid);
       __syncthreads();
-      for (int c = 0; c < kTB; ++c) {
-        double pa[4], pb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          pa[a] = Li[tr + 16 * a][c];
-          pb[a] = Lj[tc + 16 * a][c];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = fma(-pa[a], pb[b], acc[a][b]);
-      }
+      tile_mma<true>(acc, Li, Lj, -1.0, w, lane);
       __syncthreads();
     }
     if (s_status) break;
@@ -674,11 +674,11 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       double(*D)[kFLd] = lds[0];
       double(*V)[kFLd] = lds[1];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int row = tr + 16 * a, col = tc + 16 * b;
-          D[row][col] = col > row ? 0.0 : (row < cj ? acc[a][b] : (row == col ? 1.0 : 0.0));
+        for (int v = 0; v < 4; ++v) {
+          const int row = acc_row(w, b, v, lane), col = acc_col(w, b, lane);
+          D[row][col] = col > row ? 0.0 : (row < cj ? acc[b][v] : (row == col ? 1.0 : 0.0));
           V[row][col] = 0.0;
         }
       __syncthreads();
@@ -746,34 +746,20 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       double(*X)[kFLd] = lds[0];
       double(*Vt)[kFLd] = lds[1];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) X[tr + 16 * a][tc + 16 * b] = acc[a][b];
+        for (int v = 0; v < 4; ++v) X[acc_row(w, b, v, lane)][acc_col(w, b, lane)] = acc[b][v];
       stage_tile(Vt, fw.linv + (size_t)j * kTB * kTB, kTB, 0, 0, kTB, tid);
       __syncthreads();
-      double out[4][4];
+      dx4 out[4];
+      acc_zero(out);
+      tile_mma<true>(out, X, Vt, 1.0, w, lane);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
-      for (int m = 0; m < kTB; ++m) {
-        double pa[4], pb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          pa[a] = X[tr + 16 * a][m];
-          pb[a] = Vt[tc + 16 * a][m];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) out[a][b] = fma(pa[a], pb[b], out[a][b]);
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int row = tr + 16 * a, col = tc + 16 * b;
-          if (row < ri) A[(size_t)(r0 + row) * n + c0 + col] = out[a][b];
+        for (int v = 0; v < 4; ++v) {
+          const int row = acc_row(w, b, v, lane), col = acc_col(w, b, lane);
+          if (row < ri) A[(size_t)(r0 + row) * n + c0 + col] = out[b][v];
         }
     }
     publish_flag(&fw.ready[tile_index(i, j, nt)], epoch);
