@@ -229,7 +229,30 @@ def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0
     return None, None
 
 
-def roofline_entry(kernels, data, precision, mean_lin, items):
+KERNEL_SYMBOL = {"eval_obs": "k_eval_obs", "eval_points": "k_eval_points", "camera_items": "k_camera_items",
+                 "camera_reduce": "k_camera_reduce", "rcs_solve": "k_pcg", "step": "k_step_points",
+                 "candidate": "k_candidate_obs"}
+COUNTER_SUMMARY = Path(__file__).resolve().parent / "profiles" / "ba200_fp64_pcg_hbm_counters.md"
+
+
+def measured_traffic(group, name, precision, linear):
+    """HBM bytes per dispatch of a kernel group from the committed rocprofv3 counter summary.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE per dispatch for the ba200 FP64
+    ITERATIVE_SCHUR run, scaled by what the counters read for a 256 MiB copy
+    in the same run.  Other workloads and forms have no summary: None.
+    """
+    if name != "ba200" or precision != "fp64" or linear != "iterative_schur" or not COUNTER_SUMMARY.exists():
+        return None
+    sym = KERNEL_SYMBOL.get(group)
+    for row in COUNTER_SUMMARY.read_text().splitlines():
+        cells = [c.strip() for c in row.strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`").split("<")[0] == sym:
+            return float(cells[4])
+    return None
+
+
+def roofline_entry(kernels, data, precision, mean_lin, items, linear="iterative_schur", workload="ba200"):
     """Roofline of the kernel group with the largest time per launch (lm_begin/lm_end/refine excluded)."""
     cands = {k: v for k, v in kernels.items() if k not in ("lm_begin", "lm_end", "refine") and v[1] > 0}
     dominant = max(cands, key=lambda k: cands[k][0])
@@ -237,13 +260,39 @@ def roofline_entry(kernels, data, precision, mean_lin, items):
     sec = kernels[dominant][0] * 1e-3
     if bound == "hbm":
         ach = work / sec / 1e9
-        return dict(kernel=dominant, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=ach / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=work, ms_per_launch=kernels[dominant][0],
-                    launches=kernels[dominant][1], linear_iterations_per_launch=mean_lin if dominant == "rcs_solve" else None)
+        traffic = measured_traffic(dominant, workload, precision, linear)
+        out = dict(kernel=dominant, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=ach / HBM_PEAK_GBS, traffic=traffic, algorithmic_bytes=work, ms_per_launch=kernels[dominant][0],
+                   launches=kernels[dominant][1], linear_iterations_per_launch=mean_lin if dominant == "rcs_solve" else None)
+        if traffic:
+            out["traffic_source"] = ("profiles/ba200_fp64_pcg_hbm_counters.md: rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
+                                     "(scales from a 256 MiB copy in the same run), bytes per dispatch")
+            out["achieved_on_traffic"] = traffic / sec / 1e9
+        return out
     ach = work / sec / 1e12
     return dict(kernel=dominant, bound="mfma", achieved=ach, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                 frac=ach / FP64_PEAK_TFLOPS, traffic=None, algorithmic_flops=work, ms_per_launch=kernels[dominant][0],
                 launches=kernels[dominant][1])
+
+
+def dense_cholesky_entry(dev, data, iters=5):
+    """Matrix-core rate of the dense FP64 RCS solve (DENSE_SCHUR) on this workload.
+
+    One RCS solve factors S = L L^T on 64x64 tiles with MFMA FP64 tile
+    products, forms V = L^-1 in the same launch and solves with two products
+    with V: about n^3/3 + n^3/3 + 2 n^2 flops for n = 9 x cameras.  The time is
+    the HIP-event time of that launch group per LM iteration.  Its critical
+    path is the sequential chain of 64-column diagonal tiles, so the rate sits
+    far below the peak at this size.
+    """
+    kernels, _ = profile_kernels(dev, data, precision="fp64", linear="dense_schur", iters=iters)
+    n = 9 * data.num_cameras
+    ms = kernels["rcs_solve"][0]
+    flops = 2.0 * n ** 3 / 3.0 + 2.0 * n * n
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"n": n, "ms_per_solve": ms, "algorithmic_flops": flops, "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+            "counters": "profiles/ba200_cholesky_mfma_counters.md"}
 
 
 def cpu_baseline(data, iterations=2, sample_points=20_000):
@@ -386,11 +435,15 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
     return out
 
 
-def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, baseline_sample=60):
-    """Served request stream: Poisson arrivals at `rate`/s of 60% curve fits (as the batched workload, Cauchy(0.5)),
-    30% small bundle adjustments (2-20 cameras, 100-5000 points, 2-6 observations per point) and 10% SE2 pose
-    graphs (100-2000 nodes); reference default solver options.  Reports solves/s over the stream, p50/p99
-    latency (arrival to result), and the plain per-problem GPU throughput (each request alone, one at a time)."""
+def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=(4, 8), baseline_sample=60):
+    """Served request stream of 60% curve fits (as the batched workload, Cauchy(0.5)), 30% small bundle
+    adjustments (2-20 cameras, 100-5000 points, 2-6 observations per point) and 10% SE2 pose graphs (100-2000
+    nodes); reference default solver options.
+
+    Two runs of the same requests: Poisson arrivals at `rate`/s (latency at a light load), and all requests
+    arriving at once (saturation: the server's sustained solves/s, with p50/p99 latency at that load), for each
+    slot count in `saturated_slots`.  Latency is arrival to result.  The plain per-problem GPU throughput solves a
+    sample of the mix alone, one request after another; `served_vs_plain` is the best saturated rate over it."""
     import numpy as np
 
     import ceres_mi355x as tc
@@ -413,29 +466,36 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, baseline_sample=
         else:
             d = gen.make_pose_graph(num_nodes=int(rng.integers(100, 2001)), seed=s)
             reqs.append(PoseRequest(d.poses, d.edges, d.measurements))
-    gaps = rng.exponential(1.0 / rate, size=num_requests)
-    arrivals = np.cumsum(gaps)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+
+    def run(server, at):
+        t0 = time.perf_counter()
+        futures = []
+        for r, a in zip(reqs, at):
+            delay = t0 + a - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futures.append(server.submit(r))
+        results = [f.result() for f in futures]
+        wall = max(r.finished for r in results) - t0
+        lat = np.array([r.latency for r in results])
+        by_kind = {}
+        for kind in ("curve", "ba", "pose2d"):
+            lk = np.array([r.latency for r in results if r.kind == kind])
+            if lk.size:
+                by_kind[kind] = {"count": int(lk.size), "p50_ms": float(np.percentile(lk, 50) * 1e3),
+                                 "p99_ms": float(np.percentile(lk, 99) * 1e3)}
+        return {"solves_per_s": num_requests / wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "by_kind": by_kind}
+
+    def warm(server):
+        # compile paths, caching allocator
+        for r in (reqs[int(np.argmax(kinds == 0))], reqs[int(np.argmax(kinds == 1))], reqs[int(np.argmax(kinds == 2))]):
+            server.solve_alone(r)
+
     server = SolveServer(tc.SolverOptions(), num_slots=slots)
-    # warm-up: compile paths, caching allocator
-    for r in (reqs[int(np.argmax(kinds == 0))], reqs[int(np.argmax(kinds == 1))], reqs[int(np.argmax(kinds == 2))]):
-        server.solve_alone(r)
-    t0 = time.perf_counter()
-    futures = []
-    for r, at in zip(reqs, arrivals):
-        delay = t0 + at - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        futures.append(server.submit(r))
-    results = [f.result() for f in futures]
-    t_end = max(r.finished for r in results)
-    server_wall = t_end - t0
-    lat = np.array([r.latency for r in results])
-    by_kind = {}
-    for kind in ("curve", "ba", "pose2d"):
-        lk = np.array([r.latency for r in results if r.kind == kind])
-        if lk.size:
-            by_kind[kind] = {"count": int(lk.size), "p50_ms": float(np.percentile(lk, 50) * 1e3),
-                             "p99_ms": float(np.percentile(lk, 99) * 1e3)}
+    warm(server)
+    light = run(server, arrivals)
     # plain per-problem GPU throughput: a sample of the same mix, each request alone, one after another
     sample = reqs[:baseline_sample]
     t1 = time.perf_counter()
@@ -443,22 +503,33 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, baseline_sample=
         server.solve_alone(r)
     alone_s = (time.perf_counter() - t1) / len(sample)
     server.close()
+    saturated = {}
+    for ns in saturated_slots:
+        server = SolveServer(tc.SolverOptions(), num_slots=ns)
+        warm(server)
+        saturated[f"{ns}_slots"] = run(server, np.zeros(num_requests))
+        server.close()
+    best = max(saturated, key=lambda k: saturated[k]["solves_per_s"])
+    plain = 1.0 / alone_s
     out = {
-        "workload": f"{num_requests} requests, Poisson arrivals at {rate:g}/s: 60% curve fits (N ~ U[50,2000], "
-                    "Cauchy(0.5)), 30% bundle adjustments (2-20 cameras, 100-5000 points, 2-6 obs/point), 10% SE2 "
-                    "pose graphs (100-2000 nodes); reference default options; "
-                    f"{slots} stream slots + continuous curve batching",
-        "solves_per_s": num_requests / server_wall,
-        "p50_ms": float(np.percentile(lat, 50) * 1e3),
-        "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "by_kind": by_kind,
-        "plain_per_problem_solves_per_s": 1.0 / alone_s,
+        "workload": f"{num_requests} requests: 60% curve fits (N ~ U[50,2000], Cauchy(0.5)), 30% bundle adjustments "
+                    "(2-20 cameras, 100-5000 points, 2-6 obs/point), 10% SE2 pose graphs (100-2000 nodes); reference "
+                    "default options; stream slots + continuous curve batching",
+        "light_load": {"offered_rate_per_s": rate, "slots": slots, **light},
+        "saturated": saturated,
+        "solves_per_s": saturated[best]["solves_per_s"],
+        "p50_ms": saturated[best]["p50_ms"],
+        "p99_ms": saturated[best]["p99_ms"],
+        "saturated_config": best,
+        "plain_per_problem_solves_per_s": plain,
         "plain_sample": f"first {len(sample)} requests of the mix, solved alone one after another",
-        "offered_rate_per_s": rate,
+        "served_vs_plain": saturated[best]["solves_per_s"] / plain,
     }
-    out["served_vs_plain"] = out["solves_per_s"] / out["plain_per_problem_solves_per_s"]
-    log(f"[bench] serving: {out['solves_per_s']:.1f} solves/s, p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
-        f"plain {out['plain_per_problem_solves_per_s']:.1f} solves/s")
+    log(f"[bench] serving light load ({rate:g}/s): {light['solves_per_s']:.1f} solves/s, p50 {light['p50_ms']:.1f} ms, "
+        f"p99 {light['p99_ms']:.1f} ms; saturated: "
+        + ", ".join(f"{k} {v['solves_per_s']:.1f}/s p50 {v['p50_ms']:.0f} ms p99 {v['p99_ms']:.0f} ms"
+                    for k, v in saturated.items())
+        + f"; plain {plain:.1f} solves/s")
     return out
 
 
@@ -537,12 +608,15 @@ def main() -> int:
             mean_lin = 0.0
         log(f"[bench] {prec} {lin_solver} kernel groups (ms/launch): "
             + ", ".join(f"{k}={v[0]:.4f}" for k, v in kernels.items()) + f"; mean linear iterations {mean_lin:.2f}")
-        roofline = roofline_entry(kernels, data, prec, mean_lin, items)
+        roofline = roofline_entry(kernels, data, prec, mean_lin, items, linear=lin_solver, workload=name)
         per_kernel = {}
         for k, (ms, cnt) in kernels.items():
             bound, work = algorithmic_work(k, data, prec, linear_iterations=mean_lin, items=items)
             if bound == "hbm" and cnt:
                 per_kernel[k] = {"ms": ms, "GB/s": work / (ms * 1e-3) / 1e9, "algorithmic_bytes": work}
+        if not (args.only_value or args.profile_only):
+            extra["dense_cholesky"] = dense_cholesky_entry(dev, data)
+            log(f"[bench] dense RCS factorisation + inverse + solve: {extra['dense_cholesky']}")
         if args.profile_only:
             # calibration for hardware byte counters: an elementwise copy of exactly 256 MiB read + 256 MiB written
             src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
