@@ -461,6 +461,10 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));
   }
+  if (options->linear_solver == CT_ITERATIVE_SCHUR && problem->num_cameras > 0) {
+    CT_TRY(rcs_pcg_prepare(pl->layout.pcg_work, 9 * problem->num_cameras, pl->stream));
+    CT_HIP(hipStreamSynchronize(pl->stream));
+  }
   ct_ba_plan* raw = pl.get();
   LMDriver& d = pl->driver;
   d.enqueue_iteration = [raw](hipStream_t s, bool evaluation_only) {

@@ -115,7 +115,7 @@ struct BAWork {
   double* item_vec;   // [items][27] rhs, g_c, column norms (kind 1 items)
   // dense solve scratch
   double* solve_tmp;  // [2n]
-  unsigned int* pcg_bar;  // PCG grid-barrier counter (null for dense solves), cleared by lm_begin
+  unsigned int* pcg_bar;  // PCG solve epoch (null for dense solves), advanced by lm_begin
   int32_t* status;    // [4] device status words
 };
 

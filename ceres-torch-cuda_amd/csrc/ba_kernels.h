@@ -54,7 +54,8 @@ int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, L
 int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState* state, double* work,
                    size_t work_doubles, hipStream_t s);
 size_t rcs_pcg_work_doubles(int n);
-unsigned int* rcs_pcg_barrier(double* work, int n);  // arrival counter inside the PCG workspace
+unsigned int* rcs_pcg_barrier(double* work, int n);  // the PCG solve-epoch word inside its workspace
+int rcs_pcg_prepare(double* work, int n, hipStream_t s);  // zero the epoch and barrier flags (once per plan)
 size_t rcs_cholesky_tmp_doubles(int n);
 // Clears the synchronisation words inside the solve scratch; once per plan, before the first solve.
 int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s);

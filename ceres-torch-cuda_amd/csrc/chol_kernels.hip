@@ -860,8 +860,12 @@ __global__ void k_inv_reduce(const double* __restrict__ part, double* __restrict
 // shared z, so an iteration needs two grid barriers (after p.Ap and after
 // r.z).  Cross-workgroup sums read the per-workgroup partials in one fixed
 // order in every workgroup, so all of them take identical decisions.  The
-// grid barrier is a monotonic arrival counter (cleared by the LM begin
-// kernel) with a bounded spin; on timeout the step is flagged invalid.
+// grid barrier is a flag array: each workgroup stores the barrier's number
+// (the solve's epoch in the high word, the barrier index in the low word) in
+// its own slot and one wave polls all slots, so no two workgroups contend for
+// one address (a single arrival counter serialised 200 atomics, 13-18 us per
+// barrier).  The LM begin kernel advances the epoch before each solve; waits
+// are bounded and a timeout flags the step invalid.
 // ---------------------------------------------------------------------------
 constexpr int kBlk = 9;
 constexpr int kPcgWaves = kBlk;  // one wavefront per row of a camera block
@@ -872,37 +876,63 @@ constexpr int kPcgMaxN = 12288;  // p is held in LDS
 constexpr double kEps = 2.220446049250313e-16;
 
 struct PcgWork {
-  double* z;          // [n]  preconditioned residual, read by every workgroup
-  double* part;       // [3][kPcgMaxGroups] partials: r.r, r.z, p.Ap
-  unsigned int* bar;  // arrival counter
+  double* z;                  // [n]  preconditioned residual, read by every workgroup
+  double* part;               // [3][kPcgMaxGroups] partials: r.r, r.z, p.Ap
+  unsigned int* epoch;        // solves so far (advanced by the LM begin kernel)
+  unsigned long long* flags;  // [kPcgMaxGroups] last barrier passed by each workgroup
 };
 
-__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int target, int* timed_out) {
+__device__ __forceinline__ bool grid_barrier(unsigned long long* flags, unsigned long long target, unsigned int G,
+                                             int* timed_out) {
   own_stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    atomicAdd(bar, 1u);
+    __hip_atomic_store(&flags[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < kWave) {
     long spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++spins > (1L << 26)) {
-        *timed_out = 1;
+    for (;;) {
+      bool ok = true;
+      for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
+        unsigned long long fv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const unsigned int i = i0 + t * kWave;
+          fv[t] = i < G ? __hip_atomic_load(&flags[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ok = ok && fv[t] >= target;
+      }
+      if (__all(ok)) break;
+      if (++spins > (1L << 24)) {
+        if (threadIdx.x == 0) *timed_out = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return *timed_out == 0;
 }
 
 // Sum of part[0:G] in one fixed order (identical in every workgroup); valid
-// in every thread after the call.
+// in every thread after the call.  A lane's loads are issued four at a time
+// before their adds.
 __device__ __forceinline__ double group_total(const double* part, unsigned int G, double* slot) {
   if (threadIdx.x < kWave) {
     double v = 0.0;
-    for (unsigned int i = threadIdx.x; i < G; i += kWave) v += part[i];
+    for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
+      double pv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const unsigned int i = i0 + t * kWave;
+        pv[t] = i < G ? part[i] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v += pv[t];
+    }
     v = wave_sum(v);
     if (threadIdx.x == 0) *slot = v;
   }
@@ -910,6 +940,46 @@ __device__ __forceinline__ double group_total(const double* part, unsigned int G
   const double t = *slot;
   __syncthreads();
   return t;
+}
+
+// One row of S p: lane l sums S[row][c] p[c] over c = l, l + kWave, ... in
+// that order, with the loads of kRowBatch terms issued before their adds.
+constexpr int kRowBatch = 16;
+__device__ __forceinline__ double row_dot(const double* __restrict__ Srow, const double* p, int n, int lane) {
+  double acc = 0.0;
+  for (int c0 = lane; c0 < n; c0 += kRowBatch * kWave) {
+    double sv[kRowBatch];
+#pragma unroll
+    for (int t = 0; t < kRowBatch; ++t) {
+      const int c = c0 + t * kWave;
+      sv[t] = c < n ? Srow[c] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kRowBatch; ++t) {
+      const int c = c0 + t * kWave;
+      if (c < n) acc += sv[t] * p[c];
+    }
+  }
+  return acc;
+}
+
+// p = z + beta p over the whole vector (every workgroup), loads issued first.
+constexpr int kVecBatch = 8;
+__device__ __forceinline__ void update_direction(double* p, const double* __restrict__ z, double beta, int n,
+                                                 bool copy) {
+  for (int c0 = threadIdx.x; c0 < n; c0 += kVecBatch * kPcgThreads) {
+    double zv[kVecBatch];
+#pragma unroll
+    for (int t = 0; t < kVecBatch; ++t) {
+      const int c = c0 + t * kPcgThreads;
+      zv[t] = c < n ? z[c] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kVecBatch; ++t) {
+      const int c = c0 + t * kPcgThreads;
+      if (c < n) p[c] = copy ? zv[t] : zv[t] + beta * p[c];
+    }
+  }
 }
 
 // Cholesky of a 9x9 block in place (lower triangle), reciprocal pivots in dinv.
@@ -1028,6 +1098,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   double* part_rr = wk.part;
   double* part_rz = wk.part + kPcgMaxGroups;
   double* part_pap = wk.part + 2 * kPcgMaxGroups;
+  // barrier numbers of this solve: epoch in the high word, index in the low word
+  const unsigned long long bar_base =
+      (unsigned long long)__hip_atomic_load(wk.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 32;
   unsigned int barriers = 0;
   int iterations = 0;
   if (tid == 0) {
@@ -1077,21 +1150,19 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
       part_rz[g] = rz;
     }
   }
-  if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+  if (!grid_barrier(wk.flags, bar_base + ++barriers, G, &timed_out)) goto fail;
   {
     const double rr0 = group_total(part_rr, G, &slot);
     double rz_old = group_total(part_rz, G, &slot);
     const double rhs_norm = fmax(sqrt(rr0), kEps);
-    for (int c = tid; c < n; c += kPcgThreads) p[c] = wk.z[c];
+    update_direction(p, wk.z, 0.0, n, true);
     __syncthreads();
     if (!(min_iter <= 0 && sqrt(rr0) <= tol * rhs_norm)) {
       for (int k = 0; k < max_iter; ++k) {
         // Ap on the owned rows; wavefront w computes row w of each owned block
         for (int b = 0; b < nown; ++b) {
           const size_t row = (size_t)kBlk * (g + b * G) + wave;
-          const double* Srow = S + row * n;
-          double acc = 0.0;
-          for (int c = lane; c < n; c += kWave) acc += Srow[c] * p[c];
+          double acc = row_dot(S + row * n, p, n, lane);
           acc = wave_sum(acc);
           if (lane == 0) apo[b][wave] = acc;
         }
@@ -1105,7 +1176,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
           pap = wave_sum(pap);
           if (lane == 0) part_pap[g] = pap;
         }
-        if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+        if (!grid_barrier(wk.flags, bar_base + ++barriers, G, &timed_out)) goto fail;
         const double denom = group_total(part_pap, G, &slot);
         if (fabs(denom) <= kEps) break;
         const double alpha = rz_old / denom;
@@ -1134,13 +1205,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
             part_rz[g] = rz;
           }
         }
-        if (!grid_barrier(wk.bar, ++barriers * G, &timed_out)) goto fail;
+        if (!grid_barrier(wk.flags, bar_base + ++barriers, G, &timed_out)) goto fail;
         const double rr = group_total(part_rr, G, &slot);
         const double rz_new = group_total(part_rz, G, &slot);
         if (iterations >= min_iter && sqrt(rr) <= tol * rhs_norm) break;
         if (fabs(rz_old) <= kEps) break;
         const double beta = rz_new / rz_old;
-        for (int c = tid; c < n; c += kPcgThreads) p[c] = wk.z[c] + beta * p[c];
+        update_direction(p, wk.z, beta, n, false);
         __syncthreads();
         rz_old = rz_new;
       }
@@ -1263,7 +1334,23 @@ int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, L
   return CT_OK;
 }
 
-size_t rcs_pcg_work_doubles(int n) { return (size_t)n + 3 * (size_t)kPcgMaxGroups + 8; }
+// PCG work layout (doubles): z [n], partials [3 kPcgMaxGroups], the epoch
+// word (8 doubles reserved), barrier flags [kPcgMaxGroups] (64-bit each).
+size_t rcs_pcg_work_doubles(int n) { return (size_t)n + 4 * (size_t)kPcgMaxGroups + 8; }
+
+namespace {
+unsigned long long* rcs_pcg_flags(double* work, int n) {
+  return reinterpret_cast<unsigned long long*>(work + n + 3 * (size_t)kPcgMaxGroups + 8);
+}
+}  // namespace
+
+int rcs_pcg_prepare(double* work, int n, hipStream_t s) {
+  // zero the epoch word and the barrier flags once per plan (a fill kernel)
+  hipLaunchKernelGGL(k_zero_words, dim3(4), dim3(256), 0, s, work + n + 3 * (size_t)kPcgMaxGroups,
+                     8 + (size_t)kPcgMaxGroups);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
 
 unsigned int* rcs_pcg_barrier(double* work, int n) {
   return reinterpret_cast<unsigned int*>(work + n + 3 * (size_t)kPcgMaxGroups);
@@ -1309,7 +1396,8 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
   PcgWork wk;
   wk.z = work;
   wk.part = work + n;
-  wk.bar = rcs_pcg_barrier(work, n);
+  wk.epoch = rcs_pcg_barrier(work, n);
+  wk.flags = rcs_pcg_flags(work, n);
   hipLaunchKernelGGL(k_pcg, dim3(G), dim3(kPcgThreads), 0, s, S, rhs, x, n, wk, st, (unsigned int)G);
   CT_LAUNCH_CHECK();
   return CT_OK;

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   const double gnorm2 = block_sum<kLMThreads>(g2, red);
   const double gmax = block_max<kLMThreads>(gm, red);
   if (threadIdx.x != 0) return;
-  if (w.pcg_bar) *w.pcg_bar = 0u;  // the PCG kernel of this iteration counts arrivals from 0
+  if (w.pcg_bar) *w.pcg_bar += 1u;  // a new epoch for the barrier numbers of this iteration's PCG
   lm_begin_update(st, cost, gnorm2, gmax, w.records);
 }
 
