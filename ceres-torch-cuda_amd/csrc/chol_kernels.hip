@@ -329,6 +329,7 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 // ---------------------------------------------------------------------------
 constexpr int kTB = 64;
 constexpr int kFThreads = 256;
+typedef double dx4 __attribute__((ext_vector_type(4)));  // one lane's v_mfma_f64_16x16x4f64 result
 constexpr int kFLd = kTB + 2;  // LDS row stride: 16-byte aligned rows, 16 rows on distinct banks
 constexpr long kSpinLimit = 1L << 24;
 
@@ -478,26 +479,34 @@ __device__ __forceinline__ void stage_tile(double (*dst)[kFLd], const double* __
 // One doubling step of the blocked inverse of the lower-triangular factor in
 // D (V holds the inverses of the diagonal blocks of width H, zero above the
 // diagonal): for every 2H-block at base b, X21 = -V22 (L21 V11).  The
-// products run over full block widths (the zero upper triangles make the
-// triangular bounds implicit), with T = L21 V11 staged in D's unused upper
-// triangle at D[b + r][b + H + c].
+// products run on the matrix cores as 16x16 output blocks, one per wave,
+// over full block widths (the zero upper triangles make the triangular
+// bounds implicit), with T = L21 V11 staged in D's unused upper triangle at
+// D[b + r][b + H + c].
 template <int H>
-__device__ __forceinline__ void inverse_doubling(double (*D)[kFLd], double (*V)[kFLd], int tid) {
-  constexpr int kPerLevel = (kTB / (2 * H)) * H * H;
-  for (int e = tid; e < kPerLevel; e += kFThreads) {
-    const int b = (e / (H * H)) * 2 * H, r = (e / H) % H, c = e % H;
-    double s = 0.0;
+__device__ __forceinline__ void inverse_doubling(double (*D)[kFLd], double (*V)[kFLd], int w, int lane) {
+  constexpr int kSub = H / 16, kTasks = (kTB / (2 * H)) * kSub * kSub;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int t = w; t < kTasks; t += kFThreads / kWave) {
+    const int b = (t / (kSub * kSub)) * 2 * H, r0 = 16 * ((t / kSub) % kSub), c0 = 16 * (t % kSub);
+    dx4 c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int m = 0; m < H; ++m) s = fma(D[b + H + r][b + m], V[b + m][b + c], s);
-    D[b + r][b + H + c] = s;
+    for (int k0 = 0; k0 < H; k0 += 4)
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(D[b + H + r0 + li][b + k0 + lk], V[b + k0 + lk][b + c0 + li], c, 0,
+                                               0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) D[b + r0 + 4 * v + lk][b + H + c0 + li] = c[v];
   }
   __syncthreads();
-  for (int e = tid; e < kPerLevel; e += kFThreads) {
-    const int b = (e / (H * H)) * 2 * H, r = (e / H) % H, c = e % H;
-    double s = 0.0;
+  for (int t = w; t < kTasks; t += kFThreads / kWave) {
+    const int b = (t / (kSub * kSub)) * 2 * H, r0 = 16 * ((t / kSub) % kSub), c0 = 16 * (t % kSub);
+    dx4 c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int m = 0; m < H; ++m) s = fma(V[b + H + r][b + H + m], D[b + m][b + H + c], s);
-    V[b + H + r][b + c] = -s;
+    for (int k0 = 0; k0 < H; k0 += 4)
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(-V[b + H + r0 + li][b + H + k0 + lk], D[b + k0 + lk][b + H + c0 + li],
+                                               c, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) V[b + H + r0 + 4 * v + lk][b + c0 + li] = c[v];
   }
   __syncthreads();
 }
@@ -523,8 +532,6 @@ __device__ __forceinline__ int vtile_index(int i, int j, int nt) { return j * (n
 // The A operand of one instruction is a 16x4 slice (lane l: row l % 16,
 // column l / 16), the B operand a 4x16 slice (lane l: row l / 16, column
 // l % 16).
-typedef double dx4 __attribute__((ext_vector_type(4)));
-
 __device__ __forceinline__ int acc_row(int w, int b, int v, int lane) {
   return 32 * (w >> 1) + 16 * (b >> 1) + 4 * v + (lane >> 4);
 }
@@ -552,6 +559,32 @@ __device__ __forceinline__ void tile_mma(dx4 (&acc)[4], const double (*Pm)[kFLd]
     acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
     acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
     acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+  }
+}
+
+// Rank-16 update of the diagonal tile's trailing lower triangle after the
+// panel at column `base`: D[r][c] -= sum_q D[r][base+q] D[c][base+q] for
+// base+16 <= c <= r, as 16x16 blocks on the matrix cores (block (bi, bj),
+// bi >= bj, of the trailing part goes to wave (block index mod 4)).
+__device__ __forceinline__ void panel_trailing_update(double (*D)[kFLd], int base, int w, int lane) {
+  const int t0 = base + kPW, tb = (kTB - t0) / kPW;  // trailing 16-blocks per side
+  const int li = lane & 15, lk = lane >> 4;
+  for (int blk = w; blk < tb * (tb + 1) / 2; blk += kFThreads / kWave) {
+    int bi = 0;
+    while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
+    const int bj = blk - bi * (bi + 1) / 2;
+    const int r0 = t0 + kPW * bi, c0 = t0 + kPW * bj;
+    dx4 c;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) c[v] = D[r0 + 4 * v + lk][c0 + li];
+#pragma unroll
+    for (int k0 = 0; k0 < kPW; k0 += 4) {
+      const double a = -D[r0 + li][base + k0 + lk];
+      const double b = D[c0 + li][base + k0 + lk];
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) D[r0 + 4 * v + lk][c0 + li] = c[v];
   }
 }
 
@@ -700,16 +733,7 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
           }
         }
         __syncthreads();
-        const int t0 = base + kPW, tn = kTB - t0;
-        for (int e = tid; e < tn * tn; e += kFThreads) {
-          const int r = t0 + e / tn, c = t0 + e % tn;
-          if (c <= r) {
-            double acc2 = D[r][c];
-#pragma unroll
-            for (int q = 0; q < kPW; ++q) acc2 = fma(-D[r][base + q], D[c][base + q], acc2);
-            D[r][c] = acc2;
-          }
-        }
+        panel_trailing_update(D, base, w, lane);
         __syncthreads();
       }
       if (tid < cj) fw.dinv[c0 + tid] = pinv[tid];
@@ -733,8 +757,8 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       __syncthreads();
       // doubling steps: for blocks of width h at base b, X21 = -V22 (L21 V11);
       // T = L21 V11 is staged in the (unused) upper triangle of D at D[b + r][b + h + c]
-      inverse_doubling<16>(D, V, tid);
-      inverse_doubling<32>(D, V, tid);
+      inverse_doubling<16>(D, V, w, lane);
+      inverse_doubling<32>(D, V, w, lane);
       double* vout = fw.linv + (size_t)j * kTB * kTB;
       for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
         const int r = idx / kTB, c = idx % kTB;
