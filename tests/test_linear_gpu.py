@@ -73,7 +73,7 @@ def test_registry_contract_declines_host_tensors_and_dispatches():
     assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
 
 
-@pytest.mark.parametrize("n", [9, 144, 300, 1800])
+@pytest.mark.parametrize("n", [9, 144, 257, 300, 320, 1000, 1800])
 def test_standalone_potrf_potrs(n):
     rng = np.random.default_rng(n)
     M = rng.normal(size=(n, n))
@@ -106,3 +106,74 @@ def test_potrf_reports_not_positive_definite():
     St = torch.tensor(S, device="cuda")
     assert lib.ct_rcs_potrf_f64(nat.ptr(St), n, nat.ptr(ws), ws.numel(), nat.current_stream_handle()) == \
         nat.CT_ERR_NOT_SPD
+
+
+def _factor(S, ws=None):
+    """ct_rcs_potrf_f64 of a host SPD matrix; returns (status, factor tensor, workspace)."""
+    n = S.shape[0]
+    lib = nat.load()
+    if ws is None:
+        nbytes = ctypes.c_size_t(0)
+        assert lib.ct_rcs_workspace_bytes(n, ctypes.byref(nbytes)) == 0
+        ws = torch.zeros(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    St = torch.tensor(S, device="cuda")
+    rc = lib.ct_rcs_potrf_f64(nat.ptr(St), n, nat.ptr(ws), ws.numel(), nat.current_stream_handle())
+    return rc, St, ws
+
+
+def _solve(St, ws, rhs):
+    n = St.shape[0]
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    r = torch.tensor(rhs, device="cuda")
+    assert nat.load().ct_rcs_potrs_f64(nat.ptr(St), n, nat.ptr(r), nat.ptr(x), nat.ptr(ws), ws.numel(),
+                                       nat.current_stream_handle()) == 0
+    return x.cpu().numpy()
+
+
+def test_potrf_tiled_reports_not_positive_definite():
+    # n > 256 takes the tiled factorisation; the bad pivot sits in the fourth 64-column tile
+    n = 300
+    S = np.eye(n)
+    S[200, 200] = -1.0
+    rc, _, _ = _factor(S)
+    assert rc == nat.CT_ERR_NOT_SPD
+
+
+def test_refactor_on_one_workspace_is_bit_identical():
+    # consecutive factorisations reuse the workspace's tile flags (a new epoch each launch)
+    rng = np.random.default_rng(11)
+    n = 700
+    mats, rhss = [], []
+    for _ in range(3):
+        M = rng.normal(size=(n, n))
+        mats.append(M @ M.T + n * np.eye(n))
+        rhss.append(rng.normal(size=n))
+    fresh = []
+    for S, b in zip(mats, rhss):
+        rc, St, ws = _factor(S)
+        assert rc == 0
+        fresh.append((np.tril(St.cpu().numpy()), _solve(St, ws, b)))
+    ws = None
+    for (S, b), (Lf, xf) in zip(zip(mats, rhss), fresh):
+        rc, St, ws = _factor(S, ws)
+        assert rc == 0
+        np.testing.assert_array_equal(np.tril(St.cpu().numpy()), Lf)
+        np.testing.assert_array_equal(_solve(St, ws, b), xf)
+        np.testing.assert_allclose(_solve(St, ws, b), np.linalg.solve(S, b), rtol=1e-9, atol=1e-12)
+
+
+def test_inverse_based_solve_on_ill_conditioned_system():
+    # solves with the factor go through V = L^-1; condition number 1e8
+    rng = np.random.default_rng(5)
+    n = 500
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    S = (Q * np.logspace(0.0, 8.0, n)) @ Q.T
+    S = 0.5 * (S + S.T)
+    b = rng.normal(size=n)
+    rc, St, ws = _factor(S)
+    assert rc == 0
+    x = _solve(St, ws, b)
+    backward = np.linalg.norm(S @ x - b) / (np.linalg.norm(S, 2) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert backward < 1e-12
+    ref = np.linalg.solve(S, b)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-6
