@@ -65,6 +65,7 @@ struct Layout {
   BlockItem* items = nullptr;
   int32_t* block_item_offsets = nullptr;
   int32_t* item_perm = nullptr;
+  int32_t* point_groups = nullptr;
   BABuffers<float> bf{};
   BABuffers<double> bd{};
   double* pcg_work = nullptr;
@@ -83,8 +84,10 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.rhs = cv.take<double>(n);
   L.w.dxc = cv.take<double>(n);
   L.w.dxp = cv.take<double>(3 * P);
-  L.w.part_cost = cv.take<double>((size_t)obs_blocks(p) + 1);
-  L.w.part_eval = cv.take<double>(2 * (size_t)eval_blocks(p) + 2);
+  // cost and gradient partials: per observation / point block, or per fused-evaluation workgroup
+  const size_t fused_blocks = ((size_t)P + kFusedThreads / kWave - 1) / (kFusedThreads / kWave);
+  L.w.part_cost = cv.take<double>(std::max((size_t)obs_blocks(p), fused_blocks) + 1);
+  L.w.part_eval = cv.take<double>(2 * std::max((size_t)eval_blocks(p), fused_blocks) + 2);
   L.w.part_cam = cv.take<double>(2 * C + 2);
   L.w.part_step = cv.take<double>(3 * (size_t)step_blocks(p) + 3);
   L.w.part_cand = cv.take<double>(2 * (size_t)obs_blocks(p) + 2);
@@ -107,6 +110,8 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.block_item_offsets = L.block_item_offsets;
   L.w.item_perm = L.item_perm;
   L.w.item_vec = cv.take<double>(27 * I);
+  L.point_groups = cv.take<int32_t>(P + 1);
+  L.w.point_groups = L.point_groups;
   if (o->precision == CT_MIXED) {
     L.bf.JP = cv.take<float>(kJPStride * N);
     L.bf.Jp = cv.take<float>(6 * N);
@@ -175,6 +180,7 @@ struct ct_ba_plan {
   hipStream_t stream = nullptr;
   LMDriver driver;
   int num_items = 0;
+  int num_groups = 0;       // point groups of the fused evaluation (0: two-kernel evaluation)
 };
 
 namespace {
@@ -192,8 +198,12 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
   const bool mixed = pl->options.precision == CT_MIXED;
   switch (group) {
     case CT_KERNEL_EVAL_OBS:
+      if (pl->num_groups > 0)
+        return mixed ? launch_eval_fused<float>(v, L.bf, L.w, pl->num_groups, s)
+                     : launch_eval_fused<double>(v, L.bd, L.w, pl->num_groups, s);
       return mixed ? launch_eval_obs<float>(v, L.bf, L.w, s) : launch_eval_obs<double>(v, L.bd, L.w, s);
     case CT_KERNEL_EVAL_POINTS:
+      if (pl->num_groups > 0) return CT_OK;  // the fused evaluation covered the points
       return mixed ? launch_eval_points<float>(v, L.bf, L.w, s) : launch_eval_points<double>(v, L.bd, L.w, s);
     case CT_KERNEL_CAMERA_ITEMS:
       return mixed ? launch_camera_items<float>(v, L.bf, L.w, pl->num_items, s)
@@ -359,6 +369,33 @@ void split_range(std::vector<BlockItem>& out, int block, int kind, int begin, in
 // Split every camera-pair block into work items of at most kItemMaxWork terms
 // or observations (balanced within a block) and upload the list.  Reads the
 // block and camera offsets back from the device once.
+// Pack whole points, in order, into groups of at most kWave observations for
+// the fused evaluation (one wavefront per group).  A point with more
+// observations than a wavefront has lanes keeps the plan on the two-kernel
+// evaluation (num_groups = 0).
+int build_point_groups(ct_ba_plan* pl) {
+  const ct_ba_problem& p = pl->problem;
+  pl->num_groups = 0;
+  if (p.num_points <= 0 || p.num_observations <= 0) return CT_OK;
+  std::vector<int32_t> po((size_t)p.num_points + 1);
+  CT_HIP(hipMemcpy(po.data(), p.point_offsets, sizeof(int32_t) * po.size(), hipMemcpyDeviceToHost));
+  std::vector<int32_t> starts;
+  int used = 0;
+  for (int q = 0; q < p.num_points; ++q) {
+    const int k = po[q + 1] - po[q];
+    if (k > kWave) return CT_OK;
+    if (starts.empty() || used + k > kWave) {
+      starts.push_back(q);
+      used = 0;
+    }
+    used += k;
+  }
+  starts.push_back(p.num_points);
+  CT_HIP(hipMemcpy(pl->layout.point_groups, starts.data(), sizeof(int32_t) * starts.size(), hipMemcpyHostToDevice));
+  pl->num_groups = (int)starts.size() - 1;
+  return CT_OK;
+}
+
 int build_items(ct_ba_plan* pl) {
   const ct_ba_problem& p = pl->problem;
   const int B = p.num_blocks, C = p.num_cameras;
@@ -451,12 +488,19 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->stream = reinterpret_cast<hipStream_t>(stream);
   pl->init.opt = *options;
   pl->init.num_obs_blocks = obs_blocks(problem);
+  pl->init.num_cost_blocks = obs_blocks(problem);
   pl->init.num_eval_blocks = eval_blocks(problem);
   pl->init.num_step_blocks = step_blocks(problem);
   pl->init.num_cameras = problem->num_cameras;
   pl->init.n_cam_params = 9 * problem->num_cameras;
   pl->init.n_pt_params = 3 * problem->num_points;
   CT_TRY(build_items(pl.get()));
+  CT_TRY(build_point_groups(pl.get()));
+  if (pl->num_groups > 0) {
+    const int fused_blocks = (pl->num_groups + kFusedThreads / kWave - 1) / (kFusedThreads / kWave);
+    pl->init.num_cost_blocks = fused_blocks;
+    pl->init.num_eval_blocks = fused_blocks;
+  }
   if (options->linear_solver == CT_DENSE_SCHUR && problem->num_cameras > 0) {
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));

@@ -113,6 +113,8 @@ struct BAWork {
   const int32_t* block_item_offsets; // [B+1]
   const int32_t* item_perm;          // [num_items] launch slot -> item (locality order)
   double* item_vec;   // [items][27] rhs, g_c, column norms (kind 1 items)
+  // whole points packed into waves of at most 64 observations (fused evaluation)
+  const int32_t* point_groups;       // [num_groups + 1] first point of each group
   // dense solve scratch
   double* solve_tmp;  // [2n]
   unsigned int* pcg_bar;  // PCG solve epoch (null for dense solves), advanced by lm_begin

@@ -17,6 +17,11 @@ template <typename real>
 int launch_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
 template <typename real>
 int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+// Fused evaluation over point groups (one wavefront per group of whole
+// points, one lane per observation); replaces k_eval_obs + k_eval_points.
+constexpr int kFusedThreads = 256;
+template <typename real>
+int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, hipStream_t s);
 template <typename real>
 int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
@@ -27,8 +32,9 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 // LM state machine (lm_kernels.hip)
 struct LMInit {
   ct_lm_options opt;
-  int num_obs_blocks;    // partial rows written by observation-parallel kernels
-  int num_eval_blocks;   // partial rows written by k_eval_points
+  int num_obs_blocks;    // partial rows written by the candidate kernel
+  int num_cost_blocks;   // cost partial rows written by the evaluation
+  int num_eval_blocks;   // gradient partial rows written by the evaluation
   int num_step_blocks;   // partial rows written by k_step_points
   int num_cameras;
   int n_cam_params;

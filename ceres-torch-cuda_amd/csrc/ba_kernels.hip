@@ -328,6 +328,185 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
 }
 
 // ---------------------------------------------------------------------------
+// K1 (fused): one wavefront per group of whole points (at most kWave
+// observations, built at plan creation), one lane per observation.
+//   lane:      residual, Jacobian, loss correction (as k_eval_obs);
+//   point:     the point's first lane sums its observations' J_p^T J_p and
+//              J_p^T r terms from LDS in observation order and forms the
+//              damping, the 3x3 factor, L^-1 and q (as k_eval_points);
+//   lane:      P = Jp L^-T and e = P q - r, then the whole JP record
+//              (Jc and P) is written at once.
+// Same arithmetic as the two-kernel form; J_p and r stay in registers and
+// no record is written in two parts.  Block partials: cost, |g_p|^2 and
+// max|g_p| per workgroup.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffers<real> b, BAWork w,
+                                                              int num_groups) {
+  constexpr int kWaves = kFusedThreads / kWave;
+  __shared__ double red[kWaves];
+  __shared__ double term[kWaves][9][kWave];   // per-observation V (6, assembly precision) and g (3) terms
+  __shared__ double pdat[kWaves][9][kWave];   // at the point's first lane: L^-1 (6) and q (3)
+  const LMState* st = w.state;
+  if (st->done) return;
+  const int cur = st->cur;
+  const double radius = st->radius;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int grp = blockIdx.x * kWaves + wave;
+  int o = 0, p = 0, head = 0, count = 0;
+  bool active = false;
+  if (grp < num_groups) {
+    const int o0 = v.point_offsets[w.point_groups[grp]], o1 = v.point_offsets[w.point_groups[grp + 1]];
+    o = o0 + lane;
+    active = o < o1;
+  }
+  double cost = 0.0, g2 = 0.0, gmax = 0.0;
+  double r[2] = {0.0, 0.0}, Jc[2][9];
+  real jq[2][3];
+  if (active) {
+    const int ci = v.obs_camera[o];
+    p = v.obs_point[o];
+    const int po0 = v.point_offsets[p];
+    head = lane - (o - po0);
+    count = v.point_offsets[p + 1] - po0;
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
+    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+    double Jp[2][3];
+    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    if (v.camera_mask) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double m = v.camera_mask[9 * ci + j] ? 1.0 : 0.0;
+        Jc[0][j] *= m;
+        Jc[1][j] *= m;
+      }
+    }
+    if (v.point_mask) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double m = v.point_mask[3 * p + j] ? 1.0 : 0.0;
+        Jp[0][j] *= m;
+        Jp[1][j] *= m;
+      }
+    }
+    cost = robustify2(v.loss, r, Jc, Jp);
+    b.r[2 * (size_t)o] = r[0];
+    b.r[2 * (size_t)o + 1] = r[1];
+    real* jp = b.Jp + 6 * (size_t)o;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        jq[k][j] = (real)Jp[k][j];
+        jp[3 * k + j] = jq[k][j];
+      }
+    const real a0 = jq[0][0], a1 = jq[0][1], a2 = jq[0][2], c0 = jq[1][0], c1 = jq[1][1], c2 = jq[1][2];
+    term[wave][0][lane] = (double)(real)(a0 * a0 + c0 * c0);
+    term[wave][1][lane] = (double)(real)(a1 * a0 + c1 * c0);
+    term[wave][2][lane] = (double)(real)(a1 * a1 + c1 * c1);
+    term[wave][3][lane] = (double)(real)(a2 * a0 + c2 * c0);
+    term[wave][4][lane] = (double)(real)(a2 * a1 + c2 * c1);
+    term[wave][5][lane] = (double)(real)(a2 * a2 + c2 * c2);
+    term[wave][6][lane] = (double)a0 * r[0] + (double)c0 * r[1];
+    term[wave][7][lane] = (double)a1 * r[0] + (double)c1 * r[1];
+    term[wave][8][lane] = (double)a2 * r[0] + (double)c2 * r[1];
+  }
+  __syncthreads();
+  if (active && head == lane) {
+    real V00 = 0, V10 = 0, V11 = 0, V20 = 0, V21 = 0, V22 = 0;
+    double g0 = 0.0, g1 = 0.0, g2v = 0.0;
+    for (int k = lane; k < lane + count; ++k) {
+      V00 += (real)term[wave][0][k];
+      V10 += (real)term[wave][1][k];
+      V11 += (real)term[wave][2][k];
+      V20 += (real)term[wave][3][k];
+      V21 += (real)term[wave][4][k];
+      V22 += (real)term[wave][5][k];
+      g0 += term[wave][6][k];
+      g1 += term[wave][7][k];
+      g2v += term[wave][8][k];
+    }
+    // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
+    const double D0 = damping_of((double)V00, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D1 = damping_of((double)V11, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D2 = damping_of((double)V22, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    w.Dp[3 * p] = D0;
+    w.Dp[3 * p + 1] = D1;
+    w.Dp[3 * p + 2] = D2;
+    V00 += (real)D0;
+    V11 += (real)D1;
+    V22 += (real)D2;
+    const real l00 = sqrt(V00);
+    const real l10 = V10 / l00, l20 = V20 / l00;
+    const real l11 = sqrt(V11 - l10 * l10);
+    const real l21 = (V21 - l20 * l10) / l11;
+    const real l22 = sqrt(V22 - l20 * l20 - l21 * l21);
+    const real i00 = real(1) / l00, i11 = real(1) / l11, i22 = real(1) / l22;
+    const real i10 = -l10 * i00 * i11;
+    const real i21 = -l21 * i11 * i22;
+    const real i20 = -(l20 * i00 + l21 * i10) * i22;
+    const real q0 = i00 * (real)g0;
+    const real q1 = i10 * (real)g0 + i11 * (real)g1;
+    const real q2 = i20 * (real)g0 + i21 * (real)g1 + i22 * (real)g2v;
+    real* Li = b.Linv + 6 * (size_t)p;
+    Li[0] = i00; Li[1] = i10; Li[2] = i11; Li[3] = i20; Li[4] = i21; Li[5] = i22;
+    b.q[3 * p] = q0;
+    b.q[3 * p + 1] = q1;
+    b.q[3 * p + 2] = q2;
+    b.gp[3 * p] = g0;
+    b.gp[3 * p + 1] = g1;
+    b.gp[3 * p + 2] = g2v;
+    pdat[wave][0][lane] = i00; pdat[wave][1][lane] = i10; pdat[wave][2][lane] = i11;
+    pdat[wave][3][lane] = i20; pdat[wave][4][lane] = i21; pdat[wave][5][lane] = i22;
+    pdat[wave][6][lane] = q0; pdat[wave][7][lane] = q1; pdat[wave][8][lane] = q2;
+    g2 = g0 * g0 + g1 * g1 + g2v * g2v;
+    gmax = fmax(fabs(g0), fmax(fabs(g1), fabs(g2v)));
+  }
+  __syncthreads();
+  if (active) {
+    const real i00 = (real)pdat[wave][0][head], i10 = (real)pdat[wave][1][head], i11 = (real)pdat[wave][2][head];
+    const real i20 = (real)pdat[wave][3][head], i21 = (real)pdat[wave][4][head], i22 = (real)pdat[wave][5][head];
+    const real q0 = (real)pdat[wave][6][head], q1 = (real)pdat[wave][7][head], q2 = (real)pdat[wave][8][head];
+    real rec[kJPStride];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 9; ++j) rec[9 * k + j] = (real)Jc[k][j];
+    double e[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const real j0 = jq[k][0], j1 = jq[k][1], j2 = jq[k][2];
+      // P = Jp L^-T : P[k][m] = sum_{l<=m} Jp[k][l] Linv[m][l]
+      const real p0 = j0 * i00;
+      const real p1 = j0 * i10 + j1 * i11;
+      const real p2 = j0 * i20 + j1 * i21 + j2 * i22;
+      rec[18 + 3 * k] = p0;
+      rec[18 + 3 * k + 1] = p1;
+      rec[18 + 3 * k + 2] = p2;
+      e[k] = (double)(p0 * q0 + p1 * q1 + p2 * q2) - r[k];
+    }
+    real* dst = b.JP + kJPStride * (size_t)o;
+#pragma unroll
+    for (int k = 0; k < kJPStride; ++k) dst[k] = rec[k];
+    b.e[2 * (size_t)o] = (real)e[0];
+    b.e[2 * (size_t)o + 1] = (real)e[1];
+  }
+  const double tc = block_sum<kFusedThreads>(cost, red);
+  const double tg = block_sum<kFusedThreads>(g2, red);
+  const double tm = block_max<kFusedThreads>(gmax, red);
+  if (threadIdx.x == 0) {
+    w.part_cost[blockIdx.x] = tc;
+    w.part_eval[2 * blockIdx.x] = tg;
+    w.part_eval[2 * blockIdx.x + 1] = tm;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K2a: one wavefront per work item (a bounded slice of one camera-pair block
 // (i, j), i <= j):
 //   kind 0 (Schur terms):  - sum Jc_a^T (P_a P_b^T) Jc_b
@@ -861,6 +1040,17 @@ int launch_eval_obs(const BAView& v, const BABuffers<real>& b, const BAWork& w, 
 }
 
 template <typename real>
+int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, hipStream_t s) {
+  if (num_groups > 0) {
+    const int waves = kFusedThreads / kWave;
+    hipLaunchKernelGGL(k_eval_fused<real>, dim3((num_groups + waves - 1) / waves), dim3(kFusedThreads), 0, s, v, b, w,
+                       num_groups);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+template <typename real>
 int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.P > 0) {
     hipLaunchKernelGGL(k_eval_points<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
@@ -958,6 +1148,8 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 CT_INSTANTIATE(float)
 CT_INSTANTIATE(double)
 #undef CT_INSTANTIATE
+template int launch_eval_fused<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
+template int launch_eval_fused<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 template int launch_refine_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);

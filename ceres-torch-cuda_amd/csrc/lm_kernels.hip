@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   LMState* st = w.state;
   if (st->done) return;
   double c = 0.0, g2 = 0.0, gm = 0.0;
-  for (int i = threadIdx.x; i < init.num_obs_blocks; i += kLMThreads) c += w.part_cost[i];
+  for (int i = threadIdx.x; i < init.num_cost_blocks; i += kLMThreads) c += w.part_cost[i];
   for (int i = threadIdx.x; i < init.num_eval_blocks; i += kLMThreads) {
     g2 += w.part_eval[2 * i];
     gm = fmax(gm, w.part_eval[2 * i + 1]);
