@@ -89,8 +89,8 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.part_cost = cv.take<double>(std::max((size_t)obs_blocks(p), fused_blocks) + 1);
   L.w.part_eval = cv.take<double>(2 * std::max((size_t)eval_blocks(p), fused_blocks) + 2);
   L.w.part_cam = cv.take<double>(2 * C + 2);
-  L.w.part_step = cv.take<double>(3 * (size_t)step_blocks(p) + 3);
-  L.w.part_cand = cv.take<double>(2 * (size_t)obs_blocks(p) + 2);
+  L.w.part_step = cv.take<double>(3 * std::max((size_t)step_blocks(p), fused_blocks) + 3);
+  L.w.part_cand = cv.take<double>(2 * std::max((size_t)obs_blocks(p), fused_blocks) + 2);
   L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
   L.w.state = cv.take<LMState>(1);
   L.w.Dp = cv.take<double>(3 * P);
@@ -223,14 +223,26 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
         CT_TRY(launch_refine_step<float>(v, L.bf, L.w, s));
       return CT_OK;
     case CT_KERNEL_STEP:
+      if (pl->num_groups > 0)
+        return mixed ? launch_step_candidate<float>(v, L.bf, L.w, pl->num_groups, !refining(pl), s)
+                     : launch_step_candidate<double>(v, L.bd, L.w, pl->num_groups, true, s);
       return mixed ? launch_step<float>(v, L.bf, L.w, !refining(pl), s) : launch_step<double>(v, L.bd, L.w, true, s);
     case CT_KERNEL_CANDIDATE:
+      if (pl->num_groups > 0) return CT_OK;  // the fused step computed the candidate
       return mixed ? launch_candidate<float>(v, L.bf, L.w, s) : launch_candidate<double>(v, L.bd, L.w, s);
     case CT_KERNEL_LM_END:
       return launch_lm_end(pl->init, L.w, s);
     default:
       return CT_OK;
   }
+}
+
+// Whether a group launches any kernel in this plan (the point evaluation is
+// part of the fused evaluation; the refinement only runs in the mixed form).
+bool group_launches(const ct_ba_plan* pl, int group) {
+  if (group == CT_KERNEL_EVAL_POINTS || group == CT_KERNEL_CANDIDATE) return pl->num_groups == 0;
+  if (group == CT_KERNEL_REFINE) return refining(pl);
+  return true;
 }
 
 // Groups that compute and test a step; iteration 0 only evaluates.
@@ -500,6 +512,8 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
     const int fused_blocks = (pl->num_groups + kFusedThreads / kWave - 1) / (kFusedThreads / kWave);
     pl->init.num_cost_blocks = fused_blocks;
     pl->init.num_eval_blocks = fused_blocks;
+    pl->init.num_step_blocks = fused_blocks;
+    pl->init.num_obs_blocks = fused_blocks;
   }
   if (options->linear_solver == CT_DENSE_SCHUR && problem->num_cameras > 0) {
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
@@ -583,6 +597,7 @@ int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_
     for (int it = 0; it < completed; ++it) {
       hipEvent_t* e = &ev[(size_t)it * per];
       for (int k = 0; k < CT_KERNEL_COUNT; ++k) {
+        if (!group_launches(pl, k)) continue;  // an empty group only times its events
         float ms = 0.0f;
         hipEventElapsedTime(&ms, e[k], e[k + 1]);
         kernel_ms[k] += ms;

@@ -22,6 +22,11 @@ int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w,
 constexpr int kFusedThreads = 256;
 template <typename real>
 int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, hipStream_t s);
+// Fused step + candidate over the same point groups (replaces k_step_points +
+// k_candidate_obs); backsub = false takes dx_p from the refinement.
+template <typename real>
+int launch_step_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, bool backsub,
+                          hipStream_t s);
 template <typename real>
 int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
@@ -32,10 +37,10 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 // LM state machine (lm_kernels.hip)
 struct LMInit {
   ct_lm_options opt;
-  int num_obs_blocks;    // partial rows written by the candidate kernel
+  int num_obs_blocks;    // candidate partial rows
   int num_cost_blocks;   // cost partial rows written by the evaluation
   int num_eval_blocks;   // gradient partial rows written by the evaluation
-  int num_step_blocks;   // partial rows written by k_step_points
+  int num_step_blocks;   // step partial rows
   int num_cameras;
   int n_cam_params;
   int n_pt_params;

@@ -994,6 +994,145 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_candidate_obs(BAView v, BA
   }
 }
 
+// ---------------------------------------------------------------------------
+// K5 (fused): step and candidate over the point groups of the fused
+// evaluation (one wavefront per group, one lane per observation).
+//   lane:  y = Jc dx_c and its back-substitution term P^T y;
+//   point: the first lane forms dx_p = -L^-T (q + sum P^T y) in observation
+//          order (or reads it when the refinement computed it), x'_p, and
+//          its |dx|^2, |x'|^2 terms (as k_step_points);
+//   lane:  u = Jp dx_p + Jc dx_c, the model-decrease term -r.u - u.u/2 and
+//          the candidate cost at x' (as k_candidate_obs).
+// The first 9C threads of the grid also step the cameras.  Each JP record is
+// read once.  Block partials: |dx|^2, |x'|^2, non-finite count; candidate
+// cost and model decrease, per workgroup.
+// ---------------------------------------------------------------------------
+template <typename real, bool kBacksub>
+__global__ __launch_bounds__(kFusedThreads) void k_step_candidate(BAView v, BABuffers<real> b, BAWork w,
+                                                                  int num_groups) {
+  constexpr int kWaves = kFusedThreads / kWave;
+  __shared__ double red[kWaves];
+  __shared__ double term[kWaves][3][kWave];  // per observation: P^T (Jc dx_c)
+  __shared__ double pnew[kWaves][6][kWave];  // at the point's first lane: dx_p (3), x'_p (3)
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int cur = st->cur;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
+  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int grp = blockIdx.x * kWaves + wave;
+  double dx2 = 0.0, x2 = 0.0, bad = 0.0, cost = 0.0, md = 0.0;
+  // cameras: x'_c = x_c + dx_c
+  for (int t = blockIdx.x * kFusedThreads + threadIdx.x; t < 9 * v.C; t += gridDim.x * kFusedThreads) {
+    const double dd = w.dxc[t];
+    const double nx = cams[t] + dd;
+    ncams[t] = nx;
+    dx2 += dd * dd;
+    x2 += nx * nx;
+    if (!isfinite(dd)) bad = 1.0;
+  }
+  int o = 0, p = 0, head = 0, count = 0, ci = 0;
+  bool active = false;
+  if (grp < num_groups) {
+    const int o0 = v.point_offsets[w.point_groups[grp]], o1 = v.point_offsets[w.point_groups[grp + 1]];
+    o = o0 + lane;
+    active = o < o1;
+  }
+  real J[18];
+  if (active) {
+    ci = v.obs_camera[o];
+    p = v.obs_point[o];
+    const int po0 = v.point_offsets[p];
+    head = lane - (o - po0);
+    count = v.point_offsets[p + 1] - po0;
+    const real* rec = b.JP + kJPStride * (size_t)o;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) J[k] = rec[k];
+    if (kBacksub) {
+      double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double dd = w.dxc[9 * ci + j];
+        y0 += (double)J[j] * dd;
+        y1 += (double)J[9 + j] * dd;
+      }
+      const real* P = rec + 18;
+      term[wave][0][lane] = (double)P[0] * y0 + (double)P[3] * y1;
+      term[wave][1][lane] = (double)P[1] * y0 + (double)P[4] * y1;
+      term[wave][2][lane] = (double)P[2] * y0 + (double)P[5] * y1;
+    }
+  }
+  __syncthreads();
+  if (active && head == lane) {
+    double d0, d1, d2;
+    if (kBacksub) {
+      // dx_p = -L^-T (q + sum_o P_o^T Jc_o dx_c)   (schur_solve_dense's xa, linear.py:698)
+      const real* Li = b.Linv + 6 * (size_t)p;
+      double s0 = (double)b.q[3 * p], s1 = (double)b.q[3 * p + 1], s2 = (double)b.q[3 * p + 2];
+      for (int k = lane; k < lane + count; ++k) {
+        s0 += term[wave][0][k];
+        s1 += term[wave][1][k];
+        s2 += term[wave][2][k];
+      }
+      d0 = -((double)Li[0] * s0 + (double)Li[1] * s1 + (double)Li[3] * s2);
+      d1 = -((double)Li[2] * s1 + (double)Li[4] * s2);
+      d2 = -((double)Li[5] * s2);
+      w.dxp[3 * p] = d0;
+      w.dxp[3 * p + 1] = d1;
+      w.dxp[3 * p + 2] = d2;
+    } else {
+      d0 = w.dxp[3 * p];
+      d1 = w.dxp[3 * p + 1];
+      d2 = w.dxp[3 * p + 2];
+    }
+    const double n0 = pts[3 * p] + d0, n1 = pts[3 * p + 1] + d1, n2 = pts[3 * p + 2] + d2;
+    npts[3 * p] = n0;
+    npts[3 * p + 1] = n1;
+    npts[3 * p + 2] = n2;
+    dx2 += d0 * d0 + d1 * d1 + d2 * d2;
+    x2 += n0 * n0 + n1 * n1 + n2 * n2;
+    if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) bad = 1.0;
+    pnew[wave][0][lane] = d0; pnew[wave][1][lane] = d1; pnew[wave][2][lane] = d2;
+    pnew[wave][3][lane] = n0; pnew[wave][4][lane] = n1; pnew[wave][5][lane] = n2;
+  }
+  __syncthreads();
+  if (active) {
+    const real* Jq = b.Jp + 6 * (size_t)o;
+    const double d0 = pnew[wave][0][head], d1 = pnew[wave][1][head], d2 = pnew[wave][2][head];
+    double u0 = (double)Jq[0] * d0 + (double)Jq[1] * d1 + (double)Jq[2] * d2;
+    double u1 = (double)Jq[3] * d0 + (double)Jq[4] * d1 + (double)Jq[5] * d2;
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const double dd = w.dxc[9 * ci + j];
+      u0 += (double)J[j] * dd;
+      u1 += (double)J[9 + j] * dd;
+      cam[j] = cams[9 * ci + j] + dd;
+    }
+    const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
+    md = -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
+    const double X[3] = {pnew[wave][3][head], pnew[wave][4][head], pnew[wave][5][head]};
+    const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+    double rr[2];
+    snavely_residual(cam, X, obs, rr);
+    cost = robust_cost2(v.loss, rr);
+  }
+  const double a0 = block_sum<kFusedThreads>(dx2, red);
+  const double a1 = block_sum<kFusedThreads>(x2, red);
+  const double a2 = block_sum<kFusedThreads>(bad, red);
+  const double a3 = block_sum<kFusedThreads>(cost, red);
+  const double a4 = block_sum<kFusedThreads>(md, red);
+  if (threadIdx.x == 0) {
+    w.part_step[3 * blockIdx.x] = a0;
+    w.part_step[3 * blockIdx.x + 1] = a1;
+    w.part_step[3 * blockIdx.x + 2] = a2;
+    w.part_cand[2 * blockIdx.x] = a3;
+    w.part_cand[2 * blockIdx.x + 1] = a4;
+  }
+}
+
 // Plain evaluation (cost + corrected residuals) at given parameters.
 __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const double* cams, const double* pts,
                                                                   double* residuals, double* part) {
@@ -1092,6 +1231,19 @@ int launch_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool
 }
 
 template <typename real>
+int launch_step_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, bool backsub,
+                          hipStream_t s) {
+  const int waves = kFusedThreads / kWave;
+  const int grid = std::max((num_groups + waves - 1) / waves, 1);
+  if (backsub)
+    hipLaunchKernelGGL((k_step_candidate<real, true>), dim3(grid), dim3(kFusedThreads), 0, s, v, b, w, num_groups);
+  else
+    hipLaunchKernelGGL((k_step_candidate<real, false>), dim3(grid), dim3(kFusedThreads), 0, s, v, b, w, num_groups);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+template <typename real>
 int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.N > 0) {
     hipLaunchKernelGGL(k_candidate_obs<real>, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, b, w);
@@ -1148,6 +1300,10 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
 CT_INSTANTIATE(float)
 CT_INSTANTIATE(double)
 #undef CT_INSTANTIATE
+template int launch_step_candidate<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, bool,
+                                         hipStream_t);
+template int launch_step_candidate<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, bool,
+                                          hipStream_t);
 template int launch_eval_fused<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
 template int launch_eval_fused<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);

@@ -214,8 +214,9 @@ int ct_ba_plan_destroy(ct_ba_plan* plan);
 
 /* Kernel groups of one LM iteration, in launch order. */
 enum {
-  CT_KERNEL_EVAL_OBS = 0,        /* residual, Jacobian, loss correction per observation */
-  CT_KERNEL_EVAL_POINTS = 1,     /* point elimination: V, its Cholesky, q, P, e */
+  CT_KERNEL_EVAL_OBS = 0,        /* residual, Jacobian, loss correction per observation; with points of at
+                                    most 64 observations also the point elimination (one fused kernel) */
+  CT_KERNEL_EVAL_POINTS = 1,     /* point elimination: V, its Cholesky, q, P, e (empty when fused) */
   CT_KERNEL_CAMERA_ITEMS = 2,    /* reduced camera system: partial camera-pair blocks */
   CT_KERNEL_CAMERA_REDUCE = 3,   /* reduced camera system: block sums, damping, rhs */
   CT_KERNEL_LM_BEGIN = 4,        /* cost/gradient reduction + checks */
@@ -231,7 +232,7 @@ enum {
  * if LM terminates) host-driven, with HIP events around each kernel group
  * on the plan's stream; kernel_ms[k] and kernel_launches[k] (host,
  * CT_KERNEL_COUNT entries) receive the summed milliseconds and the number
- * of timed launches.  The parameters end at the
+ * of timed launches (0 for a group that launches nothing in this plan).  The parameters end at the
  * state reached, as after ct_ba_plan_run.  No reference counterpart: the
  * reference times whole calls on the host (benchmarking.py:32-69).         */
 int ct_ba_plan_profile(ct_ba_plan* plan, int iterations, double* kernel_ms, int32_t* kernel_launches);
