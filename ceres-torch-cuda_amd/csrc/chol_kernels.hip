@@ -321,8 +321,8 @@ __global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restr
 //
 // Publication: every wave waits until its own stores have reached L2
 // (s_waitcnt: a workgroup barrier alone does not wait for other waves'
-// stores), then after the barrier one thread writes L2 back for the other
-// XCDs (agent-scope fence) and stamps the flag with release semantics.  A
+// stores), then after the barrier one thread stamps the flag with an
+// agent-scope release store, which writes L2 back for the other XCDs first.  A
 // waiting workgroup's polling thread uses plain atomic loads and one
 // agent-scope acquire once the flag is seen; the caches it invalidates (L1
 // per CU, L2 per XCD) are the ones every wave of the workgroup reads through.
@@ -428,10 +428,8 @@ __device__ __forceinline__ bool wait_flags(const int* f1, const int* f2, int epo
 __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
   own_stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  // the release store writes L2 back (for the other XCDs) before the flag
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Next ticket for this workgroup (every thread calls it), or `stop` once the
@@ -910,10 +908,8 @@ __device__ __forceinline__ bool grid_barrier(unsigned long long* flags, unsigned
                                              int* timed_out) {
   own_stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(&flags[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  // the release store writes L2 back (for the other XCDs) before the flag
+  if (threadIdx.x == 0) __hip_atomic_store(&flags[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   if (threadIdx.x < kWave) {
     long spins = 0;
     for (;;) {
