@@ -1,11 +1,11 @@
 // FP64 solves of the reduced camera system S x = rhs (S symmetric positive
 // definite, n x n row-major, lower triangle referenced).
 //
-//  * n <= 256: one workgroup factors S blockwise (16-column panels: the
-//    diagonal block is factored in LDS by all threads, the rows below are
-//    solved one row per thread and the trailing rank-16 update runs on 4x4
-//    register tiles) and runs both triangular solves.
-//  * n > 256: left-looking Cholesky on 64x64 tiles in one persistent
+//  * n <= 192: one workgroup of 8 waves keeps the matrix as 16x16 tiles in
+//    matrix-core accumulator registers and the factor in LDS; the right-hand
+//    side is factored along as an extra row (forward substitution), the
+//    backward substitution runs from LDS.
+//  * n > 192: left-looking Cholesky on 64x64 tiles in one persistent
 //    launch (workgroups take tiles in dependency order and hand them on
 //    through per-tile flags); the same launch forms V = L^-1 tile by tile, so
 //    every solve with the factor is two bandwidth-bound products with V.
@@ -20,290 +20,6 @@ namespace ct {
 namespace {
 
 __device__ __forceinline__ bool skip_solve(const LMState* st) { return st != nullptr && (st->done || st->skip_step); }
-
-// Factor the kb x kb lower block held in LDS (row stride NBS) in place with
-// every thread of the workgroup: per column, all threads read the pivot, the
-// column below it is scaled by 1/sqrt(pivot) and each trailing lower entry
-// gets its rank-1 update from one thread.  dinv[j] = 1/L[j][j].  A pivot that
-// is not positive and finite sets *bad.
-template <int NB, int NBS, int THREADS>
-__device__ void factor_diag_block(double (*Ld)[NBS], double* dinv, int kb, int* bad) {
-  const int tid = threadIdx.x;
-  for (int j = 0; j < kb; ++j) {
-    const double piv = Ld[j][j];
-    const double d = sqrt(piv);
-    const double inv = 1.0 / d;
-    __syncthreads();  // the pivot is read by all threads before it is replaced
-    if (tid == 0) {
-      if (!(piv > 0.0 && isfinite(piv))) *bad = 1;
-      Ld[j][j] = d;
-      dinv[j] = inv;
-    }
-    for (int i = j + 1 + tid; i < kb; i += THREADS) Ld[i][j] *= inv;
-    __syncthreads();
-    for (int idx = tid; idx < NB * NB; idx += THREADS) {
-      const int i = idx / NB, k = idx % NB;
-      if (k > j && k <= i && i < kb) Ld[i][k] -= Ld[i][j] * Ld[k][j];
-    }
-    __syncthreads();
-  }
-}
-
-// One row of the panel solve v = a L11^-T, i.e.
-//   v[c] = (a[c] - sum_{l<c} v[l] L[c][l]) / L[c][c],
-// in chunks of 8 columns held in registers.  prow (the thread's own LDS row)
-// holds a on entry and v on exit.  Ld rows >= kb must be zero.
-template <int NB, int NBS>
-__device__ __forceinline__ void trsm_row(const double (*Ld)[NBS], const double* dinv, int kb, double* prow) {
-  static_assert(NB % 8 == 0, "panel width must be a multiple of 8");
-  for (int c0 = 0; c0 < kb; c0 += 8) {
-    double v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = c0 + k < kb ? prow[c0 + k] : 0.0;
-    for (int l = 0; l < c0; ++l) {
-      const double pl = prow[l];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] -= pl * Ld[c0 + k][l];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double s = v[k];
-#pragma unroll
-      for (int l = 0; l < k; ++l) s -= v[l] * Ld[c0 + k][c0 + l];
-      v[k] = c0 + k < kb ? s * dinv[c0 + k] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (c0 + k < kb) prow[c0 + k] = v[k];
-  }
-}
-
-// Forward substitution of one kb-row block, wave 0 only (lane l owns row l):
-// solves L[k0:k0+kb, k0:k0+kb] yb = y[k0:k0+kb] with the pivots' reciprocals.
-__device__ __forceinline__ double tri_forward_block(const double* __restrict__ A, const double* __restrict__ dinv,
-                                                    int n, int k0, int kb, double yl) {
-  const int l = threadIdx.x;
-  for (int j = 0; j < kb; ++j) {
-    const double yj = __shfl(yl, j) * dinv[k0 + j];
-    if (l == j) yl = yj;
-    else if (l > j && l < kb) yl -= A[(size_t)(k0 + l) * n + k0 + j] * yj;
-  }
-  return yl;
-}
-
-// Backward substitution of one block with L^T, wave 0 only.
-__device__ __forceinline__ double tri_backward_block(const double* __restrict__ A, const double* __restrict__ dinv,
-                                                     int n, int k0, int kb, double yl) {
-  const int l = threadIdx.x;
-  for (int j = kb - 1; j >= 0; --j) {
-    const double yj = __shfl(yl, j) * dinv[k0 + j];
-    if (l == j) yl = yj;
-    else if (l < j) yl -= A[(size_t)(k0 + j) * n + k0 + l] * yj;
-  }
-  return yl;
-}
-
-// ---------------------------------------------------------------------------
-// small n: one workgroup
-// ---------------------------------------------------------------------------
-constexpr int kSmallNB = 16;
-constexpr int kSmallMax = 256;
-constexpr int kSmallThreads = 256;
-
-__global__ __launch_bounds__(kSmallThreads) void k_chol_solve_small(double* __restrict__ A,
-                                                                    const double* __restrict__ rhs,
-                                                                    double* __restrict__ x, double* __restrict__ tmp,
-                                                                    int n, LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double Ld[kSmallNB][kSmallNB + 1];
-  __shared__ double Pn[kSmallMax][kSmallNB + 1];
-  __shared__ double dinv[kSmallMax];
-  __shared__ double y[kSmallMax];
-  __shared__ double yb[kSmallNB];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
-  if (tid == 0) bad = 0;
-  __syncthreads();
-  for (int k0 = 0; k0 < n; k0 += kSmallNB) {
-    const int kb = min(kSmallNB, n - k0);
-    for (int idx = tid; idx < kSmallNB * kSmallNB; idx += kSmallThreads) {
-      const int r = idx / kSmallNB, c = idx % kSmallNB;
-      Ld[r][c] = (r < kb && c <= r) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
-    }
-    __syncthreads();
-    factor_diag_block<kSmallNB, kSmallNB + 1, kSmallThreads>(Ld, dinv + k0, kb, &bad);
-    for (int idx = tid; idx < kSmallNB * kSmallNB; idx += kSmallThreads) {
-      const int r = idx / kSmallNB, c = idx % kSmallNB;
-      if (r < kb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = Ld[r][c];
-    }
-    // rows below the block: L21 = A21 L11^-T
-    const int rows = n - k0 - kb;
-    for (int i = tid; i < rows; i += kSmallThreads) {
-      const int row = k0 + kb + i;
-      for (int c = 0; c < kb; ++c) Pn[i][c] = A[(size_t)row * n + k0 + c];
-      trsm_row<kSmallNB, kSmallNB + 1>(Ld, dinv + k0, kb, Pn[i]);
-      for (int c = 0; c < kb; ++c) A[(size_t)row * n + k0 + c] = Pn[i][c];
-    }
-    __syncthreads();
-    // trailing update A22 -= L21 L21^T on 4x4 register tiles of the lower triangle
-    const int T = (rows + 3) / 4;
-    const int ntiles = T * (T + 1) / 2;
-    for (int tt = tid; tt < ntiles; tt += kSmallThreads) {
-      int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-      while (ti * (ti + 1) / 2 > tt) --ti;
-      while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
-      const int tj = tt - ti * (ti + 1) / 2;
-      double acc[4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-      for (int c = 0; c < kb; ++c) {
-        double pa[4], pb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = 4 * ti + a, j = 4 * tj + a;
-          pa[a] = i < rows ? Pn[i][c] : 0.0;
-          pb[a] = j < rows ? Pn[j][c] : 0.0;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int i = 4 * ti + a, j = 4 * tj + b;
-          if (i < rows && j <= i) A[(size_t)(k0 + kb + i) * n + k0 + kb + j] -= acc[a][b];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // keep the pivots' reciprocals for later triangular solves with this factor
-  for (int i = tid; i < n; i += kSmallThreads) {
-    tmp[n + i] = dinv[i];
-    y[i] = rhs ? rhs[i] : 0.0;
-  }
-  __syncthreads();
-  if (rhs == nullptr) {  // factorisation only
-    if (tid == 0 && st && bad) st->step_invalid = 1;
-    return;
-  }
-  // forward substitution L y = rhs
-  for (int k0 = 0; k0 < n; k0 += kSmallNB) {
-    const int kb = min(kSmallNB, n - k0);
-    if (tid < kWave) {
-      const double yl = tri_forward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
-      if (tid < kb) {
-        y[k0 + tid] = yl;
-        yb[tid] = yl;
-      }
-    }
-    __syncthreads();
-    for (int i = k0 + kb + tid; i < n; i += kSmallThreads) {
-      double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)i * n + k0 + c] * yb[c];
-      y[i] -= s;
-    }
-    __syncthreads();
-  }
-  // backward substitution L^T x = y
-  const int last = ((n - 1) / kSmallNB) * kSmallNB;
-  for (int k0 = last; k0 >= 0; k0 -= kSmallNB) {
-    const int kb = min(kSmallNB, n - k0);
-    if (tid < kWave) {
-      const double yl = tri_backward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
-      if (tid < kb) {
-        y[k0 + tid] = yl;
-        yb[tid] = yl;
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < k0; i += kSmallThreads) {
-      double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)(k0 + c) * n + i] * yb[c];
-      y[i] -= s;
-    }
-    __syncthreads();
-  }
-  int my_bad = 0;
-  for (int i = tid; i < n; i += kSmallThreads) {
-    x[i] = y[i];
-    if (!isfinite(y[i])) my_bad = 1;
-  }
-  if (my_bad) bad = 1;
-  __syncthreads();
-  if (tid == 0 && st) {
-    if (bad) st->step_invalid = 1;
-    st->linear_iterations = 1;
-  }
-}
-
-// Blocked triangular solves with one workgroup against the factor in A and
-// the pivots' reciprocals tmp[n:2n]; y lives in tmp[0:n].
-constexpr int kTriThreads = 1024;
-__global__ __launch_bounds__(kTriThreads) void k_tri_solve(const double* __restrict__ A,
-                                                            const double* __restrict__ rhs, double* __restrict__ x,
-                                                            double* __restrict__ tmp, int n, LMState* st) {
-  if (skip_solve(st)) return;
-  __shared__ double yb[kSmallNB];
-  __shared__ int bad;
-  double* __restrict__ y = tmp;
-  const double* __restrict__ dinv = tmp + n;
-  const int tid = threadIdx.x;
-  if (tid == 0) bad = 0;
-  for (int i = tid; i < n; i += kTriThreads) y[i] = rhs[i];
-  __syncthreads();
-  for (int k0 = 0; k0 < n; k0 += kSmallNB) {
-    const int kb = min(kSmallNB, n - k0);
-    if (tid < kWave) {
-      const double yl = tri_forward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
-      if (tid < kb) {
-        y[k0 + tid] = yl;
-        yb[tid] = yl;
-      }
-    }
-    __syncthreads();
-    for (int i = k0 + kb + tid; i < n; i += kTriThreads) {
-      double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)i * n + k0 + c] * yb[c];
-      y[i] -= s;
-    }
-    __syncthreads();
-  }
-  const int last = ((n - 1) / kSmallNB) * kSmallNB;
-  for (int k0 = last; k0 >= 0; k0 -= kSmallNB) {
-    const int kb = min(kSmallNB, n - k0);
-    if (tid < kWave) {
-      const double yl = tri_backward_block(A, dinv, n, k0, kb, tid < kb ? y[k0 + tid] : 0.0);
-      if (tid < kb) {
-        y[k0 + tid] = yl;
-        yb[tid] = yl;
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < k0; i += kTriThreads) {
-      double s = 0.0;
-      for (int c = 0; c < kb; ++c) s += A[(size_t)(k0 + c) * n + i] * yb[c];
-      y[i] -= s;
-    }
-    __syncthreads();
-  }
-  int my_bad = 0;
-  for (int i = tid; i < n; i += kTriThreads) {
-    x[i] = y[i];
-    if (!isfinite(y[i])) my_bad = 1;
-  }
-  if (my_bad) bad = 1;
-  __syncthreads();
-  if (tid == 0 && st) {
-    if (bad) st->step_invalid = 1;
-    st->linear_iterations = 1;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // large n: one persistent launch for the factorisation and the inverse
@@ -397,6 +113,328 @@ template <int... J>
 __device__ __forceinline__ void panel_steps(double (&a)[kPW], double* pinv, int* pbad, int l,
                                             std::integer_sequence<int, J...>) {
   (panel_step<J>(a, pinv, pbad, l), ...);
+}
+
+// ---------------------------------------------------------------------------
+// small n (n <= kSmallMax): one workgroup of 8 waves
+//
+// The matrix is cut into 16x16 tiles; lower tile t (row-major order) lives in
+// the matrix-core accumulator registers of wave t % 8, slot t / 8, for the
+// whole factorisation.  The right-hand side rides along as row n, so the
+// factorisation of the augmented matrix leaves y = L^-1 rhs in that row (the
+// forward substitution costs nothing extra).  Per 16-column panel p:
+//  1. the owners of the panel's tiles write them into the packed lower
+//     triangle in LDS (row i starts at i (i + 1) / 2);
+//  2. 1-4 waves factor the panel in registers, each holding the 16x16
+//     diagonal block in lanes 0-15 (every wave factors it identically) and
+//     48 rows below it in lanes 16-63; pivots and columns of the diagonal block
+//     reach every lane through scalar registers (v_readlane), so the 16 pivot
+//     steps need no barrier; the factored rows go back to LDS;
+//  3. every wave subtracts L_ip L_jp^T from its tiles right of the panel with
+//     v_mfma_f64_16x16x4f64.
+// The packed L then stays in LDS for the backward substitution L^T x = y
+// (16-row blocks: one wave solves the block's triangle, then all threads
+// update the rows above) and is copied to global memory for later solves.
+// ---------------------------------------------------------------------------
+constexpr int kSmallMax = 192;
+constexpr int kSmallThreads = 512;
+constexpr int kSmallWaves = kSmallThreads / kWave;
+constexpr int kSmallTileRows = (kSmallMax + 1 + 15) / 16;                                   // with the rhs row
+constexpr int kSmallSlots = (kSmallTileRows * (kSmallTileRows + 1) / 2 + kSmallWaves - 1) / kSmallWaves;
+constexpr int kSmallPacked = (kSmallMax + 1) * (kSmallMax + 2) / 2;                          // rows 0..n
+constexpr int kBelowPerWave = kWave - kPW;  // panel rows below the diagonal block per factoring wave
+
+__device__ __forceinline__ int packed_index(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Pivot step JJ of a panel whose first `valid` columns are real (the others
+// are padding: their pivot reciprocal is taken as 0, so they change nothing).
+// `writer`: this wave records the pivots' reciprocals and a bad pivot.
+template <int JJ>
+__device__ __forceinline__ void small_panel_step(double (&a)[kPW], double* pinv, int* pbad, int l, int valid,
+                                                 bool writer) {
+  const double piv = lane_value(a[JJ], JJ);
+  const double inv = JJ < valid ? pivot_rsqrt(piv) : 0.0;
+  if (writer && l == JJ && JJ < valid) {
+    pinv[JJ] = inv;
+    if (!(piv > 0.0) || !isfinite(piv)) *pbad = 1;
+  }
+  a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
+  const double f = l > JJ ? a[JJ] : 0.0;
+#pragma unroll
+  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
+}
+
+template <int... J>
+__device__ __forceinline__ void small_panel_steps(double (&a)[kPW], double* pinv, int* pbad, int l, int valid,
+                                                  bool writer, std::integer_sequence<int, J...>) {
+  (small_panel_step<J>(a, pinv, pbad, l, valid, writer), ...);
+}
+
+// Lower tile t (row-major enumeration of the tiles with bj <= bi) -> (bi, bj).
+__device__ __forceinline__ void small_tile_rc(int t, int& bi, int& bj) {
+  int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > t) --i;
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  bi = i;
+  bj = t - i * (i + 1) / 2;
+}
+
+// Backward substitution L^T x = y on the packed factor in LDS, in place in
+// y[0:n] (every thread calls it).  dinv holds the pivots' reciprocals.  Per
+// 16-row block, wave 0 solves the block's triangle with the factor entries
+// and pivots preloaded into registers and the solved values broadcast through
+// scalar registers; then every thread updates one row above the block.
+__device__ void small_backward(const double* Lp, const double* dinv, double* y, int n) {
+  const int tid = threadIdx.x;
+  const int nb = (n + kPW - 1) / kPW;
+  for (int p = nb - 1; p >= 0; --p) {
+    const int b0 = kPW * p, kb = min(kPW, n - b0);
+    if (tid < kWave) {
+      double lc[kPW], dj[kPW];
+#pragma unroll
+      for (int j = 0; j < kPW; ++j) {
+        lc[j] = (j < kb && tid < j) ? Lp[packed_index(b0 + j, b0 + tid)] : 0.0;  // L(b0 + j, b0 + tid)
+        dj[j] = j < kb ? dinv[b0 + j] : 0.0;
+      }
+      double yl = tid < kb ? y[b0 + tid] : 0.0;
+#pragma unroll
+      for (int j = kPW - 1; j >= 0; --j) {
+        if (j < kb) {
+          const double xj = lane_value(yl, j) * dj[j];
+          yl = tid == j ? xj : fma(-lc[j], xj, yl);
+        }
+      }
+      if (tid < kb) y[b0 + tid] = yl;
+    }
+    __syncthreads();
+    for (int c = tid; c < b0; c += kSmallThreads) {
+      double s = 0.0;
+      for (int j = 0; j < kb; ++j) s = fma(Lp[packed_index(b0 + j, c)], y[b0 + j], s);
+      y[c] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+// Forward substitution L y = b in place in y[0:n] (every thread calls it).
+__device__ void small_forward(const double* Lp, const double* dinv, double* y, int n) {
+  const int tid = threadIdx.x;
+  for (int b0 = 0; b0 < n; b0 += kPW) {
+    const int kb = min(kPW, n - b0);
+    if (tid < kWave) {
+      double lr[kPW], dj[kPW];
+#pragma unroll
+      for (int j = 0; j < kPW; ++j) {
+        lr[j] = (j < tid && tid < kb) ? Lp[packed_index(b0 + tid, b0 + j)] : 0.0;  // L(b0 + tid, b0 + j)
+        dj[j] = j < kb ? dinv[b0 + j] : 0.0;
+      }
+      double yl = tid < kb ? y[b0 + tid] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kPW; ++j) {
+        if (j < kb) {
+          const double yj = lane_value(yl, j) * dj[j];
+          yl = tid == j ? yj : fma(-lr[j], yj, yl);
+        }
+      }
+      if (tid < kb) y[b0 + tid] = yl;
+    }
+    __syncthreads();
+    for (int i = b0 + kb + tid; i < n; i += kSmallThreads) {
+      double s = 0.0;
+      for (int j = 0; j < kb; ++j) s = fma(Lp[packed_index(i, b0 + j)], y[b0 + j], s);
+      y[i] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+// Copies between the lower triangle of a row-major n x n matrix in global
+// memory and the packed triangle in LDS: wave w takes rows w, w + 8, ...,
+// its lanes the row's columns (no index division; each thread's loads of a
+// row batch are issued before its LDS stores).
+__device__ __forceinline__ void small_store_lower(double* __restrict__ A, const double* Lp, int n) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int r = w; r < n; r += kSmallWaves)
+    for (int c = lane; c <= r; c += kWave) A[(size_t)r * n + c] = Lp[packed_index(r, c)];
+}
+
+__device__ __forceinline__ void small_load_lower(double* Lp, const double* __restrict__ A, int n) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  constexpr int kRows = 4;  // rows in flight per wave
+  for (int r0 = w * kRows; r0 < n; r0 += kSmallWaves * kRows) {
+    double v[kRows][(kSmallMax + kWave - 1) / kWave];
+#pragma unroll
+    for (int q = 0; q < kRows; ++q)
+#pragma unroll
+      for (int u = 0; u < (kSmallMax + kWave - 1) / kWave; ++u) {
+        const int r = r0 + q, c = lane + kWave * u;
+        v[q][u] = (r < n && c <= r) ? A[(size_t)r * n + c] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < kRows; ++q)
+#pragma unroll
+      for (int u = 0; u < (kSmallMax + kWave - 1) / kWave; ++u) {
+        const int r = r0 + q, c = lane + kWave * u;
+        if (r < n && c <= r) Lp[packed_index(r, c)] = v[q][u];
+      }
+  }
+}
+
+// x = S^-1 rhs (rhs == nullptr: factor only); the factor overwrites the lower
+// triangle of A and the pivots' reciprocals go to tmp[n:2n].
+__global__ __launch_bounds__(kSmallThreads) void k_chol_small(double* __restrict__ A, const double* __restrict__ rhs,
+                                                              double* __restrict__ x, double* __restrict__ tmp, int n,
+                                                              LMState* st) {
+  if (skip_solve(st)) return;
+  __shared__ double Lp[kSmallPacked];
+  __shared__ double dinv[kSmallMax];
+  __shared__ int tile_rc[kSmallWaves][kSmallSlots][2];  // (bi, bj) of each wave's slots, -1 if unused
+  __shared__ int bad;
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nt = (n + 1 + kPW - 1) / kPW;          // tile rows, the rhs row included
+  const int ntiles = nt * (nt + 1) / 2;
+  const int npanels = (n + kPW - 1) / kPW;
+  if (tid == 0) bad = 0;
+  // load the owned tiles: lower triangle of A, the rhs as row n, zeros elsewhere.
+  // The slots' tile coordinates are read back from LDS at every use, so no
+  // address derived from them stays live across the panel loop.
+  dx4 acc[kSmallSlots];
+#pragma unroll
+  for (int s = 0; s < kSmallSlots; ++s) {
+    const int t = w + kSmallWaves * s;
+    int bi = -1, bj = -1;
+    acc[s] = dx4{0.0, 0.0, 0.0, 0.0};
+    if (t < ntiles) small_tile_rc(t, bi, bj);
+    if (lane == 0) {
+      tile_rc[w][s][0] = bi;
+      tile_rc[w][s][1] = bj;
+    }
+    if (t < ntiles) {
+      const int c = kPW * bj + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = kPW * bi + 4 * v + lk;
+        double val = 0.0;
+        if (c < n && c <= r) {
+          if (r < n) val = A[(size_t)r * n + c];
+          else if (r == n && rhs != nullptr) val = rhs[c];
+        }
+        acc[s][v] = val;
+      }
+    }
+  }
+  for (int p = 0; p < npanels; ++p) {
+    const int c0 = kPW * p;
+    const int valid = min(kPW, n - c0);
+    // 1. the panel's tiles into LDS (rows c0..n, columns c0..c0+15 below the diagonal)
+#pragma unroll
+    for (int s = 0; s < kSmallSlots; ++s) {
+      if (tile_rc[w][s][1] == p) {
+        const int c = c0 + li, r0 = kPW * tile_rc[w][s][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + 4 * v + lk;
+          if (r <= n && c <= r && c < n) Lp[packed_index(r, c)] = acc[s][v];
+        }
+      }
+    }
+    __syncthreads();
+    // 2. factor the panel: wave k holds the diagonal block and rows c0 + 16 + 48k + (lane - 16)
+    const int below = n - (c0 + kPW) + 1;  // rows below the diagonal block, the rhs row included
+    const int fwaves = below > 0 ? (below + kBelowPerWave - 1) / kBelowPerWave : 1;
+    double a[kPW];
+    int row = -1;
+    if (w < fwaves) {
+      row = lane < kPW ? c0 + lane : c0 + kPW + kBelowPerWave * w + (lane - kPW);
+#pragma unroll
+      for (int q = 0; q < kPW; ++q) {
+        const int c = c0 + q;
+        a[q] = (row <= n && c < n && c <= row) ? Lp[packed_index(row, c)] : 0.0;
+      }
+    }
+    __syncthreads();  // every factoring wave has read the raw diagonal block
+    if (w < fwaves) {
+      small_panel_steps(a, dinv + c0, &bad, lane, valid, w == 0, std::make_integer_sequence<int, kPW>{});
+      if (row <= n && (lane >= kPW || w == 0)) {
+#pragma unroll
+        for (int q = 0; q < kPW; ++q) {
+          const int c = c0 + q;
+          if (c < n && c <= row) Lp[packed_index(row, c)] = a[q];
+        }
+      }
+    }
+    __syncthreads();
+    // 3. trailing update of the tiles right of the panel
+#pragma unroll
+    for (int s = 0; s < kSmallSlots; ++s) {
+      const int bj = tile_rc[w][s][1];
+      if (bj > p) {
+        const int ra = kPW * tile_rc[w][s][0] + li, rb = kPW * bj + li;
+#pragma unroll
+        for (int k0 = 0; k0 < kPW; k0 += 4) {
+          const int c = c0 + k0 + lk;
+          const double av = (ra <= n && c < n) ? -Lp[packed_index(ra, c)] : 0.0;
+          const double bv = (rb <= n && c < n) ? Lp[packed_index(rb, c)] : 0.0;
+          acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[s], 0, 0, 0);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // the factor and the pivots' reciprocals for later solves with it
+  small_store_lower(A, Lp, n);
+  for (int i = tid; i < n; i += kSmallThreads) tmp[n + i] = dinv[i];
+  if (rhs == nullptr) {
+    if (tid == 0 && st && bad) st->step_invalid = 1;
+    return;
+  }
+  double* y = Lp + packed_index(n, 0);  // row n: y = L^-1 rhs
+  small_backward(Lp, dinv, y, n);
+  int my_bad = 0;
+  for (int i = tid; i < n; i += kSmallThreads) {
+    x[i] = y[i];
+    if (!isfinite(y[i])) my_bad = 1;
+  }
+  if (my_bad) bad = 1;
+  __syncthreads();
+  if (tid == 0 && st) {
+    if (bad) st->step_invalid = 1;
+    st->linear_iterations = 1;
+  }
+}
+
+// Both triangular solves with a factor left by k_chol_small (lower triangle
+// of L, pivots' reciprocals in tmp[n:2n]): the factor is loaded into LDS once.
+__global__ __launch_bounds__(kSmallThreads) void k_tri_small(const double* __restrict__ L,
+                                                             const double* __restrict__ rhs, double* __restrict__ x,
+                                                             const double* __restrict__ tmp, int n, LMState* st) {
+  if (skip_solve(st)) return;
+  __shared__ double Lp[kSmallPacked];
+  __shared__ double dinv[kSmallMax];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  small_load_lower(Lp, L, n);
+  double* y = Lp + packed_index(n, 0);
+  for (int i = tid; i < n; i += kSmallThreads) {
+    dinv[i] = tmp[n + i];
+    y[i] = rhs[i];
+  }
+  __syncthreads();
+  small_forward(Lp, dinv, y, n);
+  small_backward(Lp, dinv, y, n);
+  int my_bad = 0;
+  for (int i = tid; i < n; i += kSmallThreads) {
+    x[i] = y[i];
+    if (!isfinite(y[i])) my_bad = 1;
+  }
+  if (my_bad) bad = 1;
+  __syncthreads();
+  if (tid == 0 && st) {
+    if (bad) st->step_invalid = 1;
+    st->linear_iterations = 1;
+  }
 }
 
 // Every thread calls this; thread 0 polls up to two flags.  *status (LDS) is
@@ -1331,7 +1369,7 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
                               hipStream_t s) {
   if (n <= 0) return CT_OK;
   if (n <= kSmallMax) {
-    hipLaunchKernelGGL(k_chol_solve_small, dim3(1), dim3(kSmallThreads), 0, s, S, rhs, x, tmp, n, st);
+    hipLaunchKernelGGL(k_chol_small, dim3(1), dim3(kSmallThreads), 0, s, S, rhs, x, tmp, n, st);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -1349,7 +1387,7 @@ int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, L
                          hipStream_t s) {
   if (n <= 0) return CT_OK;
   if (n > kSmallMax) return launch_inverse_apply(rhs, x, n, st, tmp, s);  // the factor launch left V in tmp
-  hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(kTriThreads), 0, s, L, rhs, x, tmp, n, st);
+  hipLaunchKernelGGL(k_tri_small, dim3(1), dim3(kSmallThreads), 0, s, L, rhs, x, tmp, n, st);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

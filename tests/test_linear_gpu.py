@@ -73,7 +73,9 @@ def test_registry_contract_declines_host_tensors_and_dispatches():
     assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
 
 
-@pytest.mark.parametrize("n", [9, 144, 257, 300, 320, 1000, 1800])
+# n <= 192: one-workgroup factorisation (tile boundaries at multiples of 16, the
+# right-hand-side row in a tile of its own at n = 176 and 192); above: tiled
+@pytest.mark.parametrize("n", [1, 9, 16, 17, 144, 175, 176, 191, 192, 193, 256, 257, 300, 320, 1000, 1800])
 def test_standalone_potrf_potrs(n):
     rng = np.random.default_rng(n)
     M = rng.normal(size=(n, n))
@@ -130,8 +132,38 @@ def _solve(St, ws, rhs):
     return x.cpu().numpy()
 
 
+def test_potrf_small_reports_not_positive_definite_in_a_later_panel():
+    # n <= 192: the bad pivot sits in the seventh 16-column panel
+    n = 150
+    S = np.eye(n)
+    S[100, 100] = -1.0
+    rc, _, _ = _factor(S)
+    assert rc == nat.CT_ERR_NOT_SPD
+
+
+def test_small_factor_ill_conditioned_and_repeatable():
+    # one-workgroup path: condition number 1e8, repeated factorisations bit-identical
+    rng = np.random.default_rng(6)
+    n = 150
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    S = (Q * np.logspace(0.0, 8.0, n)) @ Q.T
+    S = 0.5 * (S + S.T)
+    b = rng.normal(size=n)
+    rc, St, ws = _factor(S)
+    assert rc == 0
+    x = _solve(St, ws, b)
+    backward = np.linalg.norm(S @ x - b) / (np.linalg.norm(S, 2) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert backward < 1e-12
+    ref = np.linalg.solve(S, b)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-6
+    rc2, St2, ws2 = _factor(S, ws)
+    assert rc2 == 0
+    np.testing.assert_array_equal(np.tril(St2.cpu().numpy()), np.tril(St.cpu().numpy()))
+    np.testing.assert_array_equal(_solve(St2, ws2, b), x)
+
+
 def test_potrf_tiled_reports_not_positive_definite():
-    # n > 256 takes the tiled factorisation; the bad pivot sits in the fourth 64-column tile
+    # n > 192 takes the tiled factorisation; the bad pivot sits in the fourth 64-column tile
     n = 300
     S = np.eye(n)
     S[200, 200] = -1.0
