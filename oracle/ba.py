@@ -468,6 +468,36 @@ class BAModel:
         after = r + jd.reshape(-1)
         return max(0.0, float(0.5 * (r @ r) - 0.5 * (after @ after)))
 
+    # -- dogleg ----------------------------------------------------------------
+    def jmul(self, ev, v: np.ndarray) -> np.ndarray:
+        """J v for a vector in the active column order (flattened per residual)."""
+        dp, dc = split_columns(self.struct, v)
+        jd = np.einsum("nij,nj->ni", ev.Jc, dc[self.struct.obs_camera]) + np.einsum(
+            "nij,nj->ni", ev.Jp, dp[self.struct.obs_point]
+        )
+        return jd.reshape(-1)
+
+    def gauss_newton_step(self, ev) -> np.ndarray:
+        """solve(J^T J, -g) (linear.py:570-571) by eliminating the points with no
+        damping: the exact solution whenever J^T J is nonsingular (which needs
+        the gauge of the reconstruction fixed, e.g. by constant cameras)."""
+        if self.dense_faithful:
+            from .lm import _solve_square
+
+            J = self.dense_jacobian(ev)
+            return _solve_square(J.T @ J, -(J.T @ ev.r.reshape(-1)))
+        st = self.struct
+        zero_p = np.zeros((st.num_points, 3))
+        zero_c = np.zeros((st.num_cameras, 9))
+        S, rhs, Vinv, W, ga = schur_system(st, ev, zero_p, zero_c)
+        act = np.repeat(st.camera_active, 9)
+        dxc = np.zeros(9 * st.num_cameras)
+        if act.any():
+            dxc[act] = np.linalg.solve(S[np.ix_(act, act)], rhs[act])
+        dxc = dxc.reshape(-1, 9)
+        dxp = schur_back_substitute(st, Vinv, W, ga, dxc)
+        return column_vector(st, dxp, dxc)
+
     # -- linear solve ----------------------------------------------------------
     def solve(self, ev, damping: np.ndarray, opts) -> tuple[np.ndarray, int]:
         if self.dense_faithful:

@@ -6,6 +6,9 @@ inner iterations and callbacks, which the measured workloads do not use, are
 left out).  Helpers restate:
 
 * ``_LevenbergMarquardtRadiusState``      solver.py:805-820
+* ``_accepted/_rejected_trust_region_radius``, ``_updated_trust_region_radius``
+  (DOGLEG)                                 solver.py:864-883, 1051-1061
+* ``dogleg_step`` (traditional and subspace) linear.py:554-664
 * ``_TrustRegionStepEvaluator``           solver.py:823-861
 * ``_model_decrease``                     solver.py:798-802 (in each model)
 * ``jacobi_damping_from_jacobian``        linear.py:539-551
@@ -52,6 +55,8 @@ class OracleOptions:
     max_consecutive_nonmonotonic_steps: int = 5
     use_mixed_precision_solves: bool = False
     max_num_refinement_iterations: int = 0
+    trust_region_strategy_type: str = "levenberg_marquardt"  # or "dogleg"
+    dogleg_type: str = "traditional"                          # or "subspace"
 
 
 class LMRadius:
@@ -127,6 +132,131 @@ class OracleSummary:
     num_linear_solves: int = 0
 
 
+def updated_trust_region_radius(radius: float, rho: float, step_norm: float, max_radius: float) -> float:
+    """solver.py:1051-1061 (the DOGLEG radius after an accepted step)."""
+    if rho > 0.75:
+        return min(max_radius, max(2.0 * radius, 3.0 * max(step_norm, _EPS)))
+    if rho < 0.25:
+        return max(radius * 0.5, np.finfo(np.float64).tiny)
+    return radius
+
+
+def _dogleg(opts: OracleOptions) -> bool:
+    return opts.trust_region_strategy_type == "dogleg"
+
+
+def _rejected_radius(opts: OracleOptions, radius: float, lm: "LMRadius") -> float:
+    """solver.py:876-883."""
+    return radius * 0.25 if _dogleg(opts) else lm.step_rejected()
+
+
+def steepest_descent_step(jmul, g: np.ndarray) -> np.ndarray:
+    """linear.py:611-616."""
+    Jg = jmul(g)
+    denom = max(float(Jg @ Jg), _EPS)
+    alpha = float(g @ g) / denom
+    return -alpha * g
+
+
+def traditional_dogleg_step(jmul, g: np.ndarray, gn: np.ndarray, radius: float) -> np.ndarray:
+    """linear.py:568-583 with the Gauss-Newton step gn = solve(J^T J, -g) given."""
+    if np.linalg.norm(gn) <= radius:
+        return gn
+    sd = steepest_descent_step(jmul, g)
+    sd_norm = float(np.linalg.norm(sd))
+    if sd_norm >= radius:
+        return (radius / max(sd_norm, _EPS)) * sd
+    diff = gn - sd
+    a = float(diff @ diff)
+    b = 2.0 * float(sd @ diff)
+    c = float(sd @ sd) - radius * radius
+    tau = (-b + np.sqrt(max(b * b - 4.0 * a * c, 0.0))) / max(2.0 * a, _EPS)
+    return sd + tau * diff
+
+
+def orthonormal_basis(vectors) -> np.ndarray:
+    """linear.py:619-633 (Gram-Schmidt, dropping nearly dependent vectors)."""
+    basis = []
+    for v in vectors:
+        cand = np.array(v, dtype=np.float64).reshape(-1)
+        cand_norm = float(np.linalg.norm(cand))
+        for q in basis:
+            cand = cand - float(q @ cand) * q
+        nrm = float(np.linalg.norm(cand))
+        if nrm > 100.0 * _EPS * max(cand_norm, 1.0):
+            basis.append(cand / nrm)
+    if not basis:
+        return np.zeros((np.asarray(vectors[0]).size, 0))
+    return np.stack(basis, axis=1)
+
+
+def trust_region_boundary_step(H: np.ndarray, g: np.ndarray, radius: float) -> np.ndarray:
+    """linear.py:636-664: minimiser of the quadratic model inside the ball, by bisection on the shift."""
+    unconstrained = _solve_square(H, -g)
+    if np.linalg.norm(unconstrained) <= radius:
+        return unconstrained
+    eye = np.eye(H.shape[0])
+    radius_value = max(float(radius), np.finfo(np.float64).tiny)
+
+    def solve_shifted(shift):
+        return _solve_square(H + shift * eye, -g)
+
+    low, high = 0.0, 1.0
+    y = solve_shifted(high)
+    for _ in range(80):
+        if np.linalg.norm(y) <= radius_value:
+            break
+        high *= 2.0
+        y = solve_shifted(high)
+    for _ in range(80):
+        mid = 0.5 * (low + high)
+        y = solve_shifted(mid)
+        if np.linalg.norm(y) > radius_value:
+            low = mid
+        else:
+            high = mid
+    y = solve_shifted(high)
+    yn = float(np.linalg.norm(y))
+    if yn > radius_value:
+        return (radius / max(yn, _EPS)) * y
+    return y
+
+
+def subspace_dogleg_step(jmul, r: np.ndarray, g: np.ndarray, gn: np.ndarray, radius: float) -> np.ndarray:
+    """linear.py:586-608."""
+    if np.linalg.norm(gn) <= radius:
+        return gn
+    sd = steepest_descent_step(jmul, g)
+    sd_norm = float(np.linalg.norm(sd))
+    if sd_norm >= radius:
+        return (radius / max(sd_norm, _EPS)) * sd
+    basis = orthonormal_basis((-g, gn))
+    if basis.shape[1] == 0:
+        return np.zeros_like(g)
+    if basis.shape[1] < 2:
+        return traditional_dogleg_step(jmul, g, gn, radius)
+    RJ = np.stack([jmul(basis[:, 0]), jmul(basis[:, 1])], axis=1)
+    RH = RJ.T @ RJ
+    rg = RJ.T @ r.reshape(-1)
+    return basis @ trust_region_boundary_step(RH, rg, radius)
+
+
+def dogleg_step(model, ev, g: np.ndarray, radius: float, opts: OracleOptions) -> np.ndarray:
+    """linear.py:554-565: dogleg_step(J, r, radius) for a model that supplies
+    ``gauss_newton_step(ev)`` (solve(J^T J, -g)), ``jmul(ev, v)`` (J v) and the
+    residual vector ``ev.r``."""
+    if radius <= 0.0 or g.size == 0:
+        return np.zeros(g.size)
+    gn = model.gauss_newton_step(ev)
+
+    def jmul(v):
+        return model.jmul(ev, v)
+
+    if opts.dogleg_type == "subspace":
+        return subspace_dogleg_step(jmul, np.asarray(ev.r).reshape(-1), g, gn, radius)
+    return traditional_dogleg_step(jmul, g, gn, radius)
+
+
 def damping_from_column_sq(col_sq: np.ndarray, radius: float, opts: OracleOptions) -> np.ndarray:
     """linear.py:539-551."""
     diag = col_sq if opts.jacobi_scaling else np.ones_like(col_sq)
@@ -195,12 +325,17 @@ def trust_region_solve(model, opts: OracleOptions):
             s.final_cost = cost
             finished = True
             break
-        damping = damping_from_column_sq(model.column_sq(ev), radius, opts)
-        step, lin_iters = model.solve(ev, damping, opts)
-        s.num_linear_solves += 1
+        if _dogleg(opts):
+            # solver.py:129-133: no damping, one "linear iteration", not a linear solve
+            step = dogleg_step(model, ev, g, radius, opts)
+            lin_iters = 1
+        else:
+            damping = damping_from_column_sq(model.column_sq(ev), radius, opts)
+            step, lin_iters = model.solve(ev, damping, opts)
+            s.num_linear_solves += 1
         if not np.all(np.isfinite(step)):
             consecutive_invalid += 1
-            radius = lm.step_rejected()
+            radius = _rejected_radius(opts, radius, lm)
             rec.update(step_is_valid=False, step_is_successful=False, linear_solver_iterations=lin_iters,
                        trust_region_radius=radius)
             s.num_unsuccessful_steps += 1
@@ -229,11 +364,14 @@ def trust_region_solve(model, opts: OracleOptions):
                 best_cost = cand_cost
                 best_x = copy.deepcopy(x)
             s.final_cost = best_cost
-            radius = lm.step_accepted(rho)
+            if _dogleg(opts):
+                radius = updated_trust_region_radius(radius, rho, rec["step_norm"], opts.max_trust_region_radius)
+            else:
+                radius = lm.step_accepted(rho)
             rec.update(trust_region_radius=radius, step_is_successful=True)
         else:
             s.num_unsuccessful_steps += 1
-            radius = lm.step_rejected()
+            radius = _rejected_radius(opts, radius, lm)
             rec.update(trust_region_radius=radius, step_is_successful=False)
         s.iterations.append(rec)
         if accepted:

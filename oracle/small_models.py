@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .lm import dense_linear_solve
+from .lm import _solve_square, dense_linear_solve
 from .losses import robustify
 
 
@@ -69,6 +69,13 @@ class CurveModel:
     def model_decrease(self, ev, step) -> float:
         after = ev.r + ev.J @ step
         return max(0.0, float(0.5 * (ev.r @ ev.r) - 0.5 * (after @ after)))
+
+    def gauss_newton_step(self, ev) -> np.ndarray:
+        """linear.py:570-571: solve(J^T J, -J^T r)."""
+        return _solve_square(ev.J.T @ ev.J, -(ev.J.T @ ev.r))
+
+    def jmul(self, ev, v) -> np.ndarray:
+        return ev.J @ v
 
 
 def relative_pose_residual(a: np.ndarray, b: np.ndarray, meas: np.ndarray, jacobian: bool):
@@ -144,3 +151,10 @@ class PoseGraphModel:
     def model_decrease(self, ev, step) -> float:
         after = ev.r + ev.J @ step
         return max(0.0, float(0.5 * (ev.r @ ev.r) - 0.5 * (after @ after)))
+
+    def gauss_newton_step(self, ev) -> np.ndarray:
+        """linear.py:570-571: solve(J^T J, -J^T r)."""
+        return _solve_square(ev.J.T @ ev.J, -(ev.J.T @ ev.r))
+
+    def jmul(self, ev, v) -> np.ndarray:
+        return ev.J @ v

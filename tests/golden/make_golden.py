@@ -90,7 +90,9 @@ def _loss(tc, loss):
     raise ValueError(loss)
 
 
-def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int, store_inputs: bool):
+def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int, store_inputs: bool,
+           strategy: str = "levenberg_marquardt", dogleg_type: str = "traditional", constant_cameras=(),
+           initial_radius: float | None = None):
     torch, tc = _import_reference()
     from oracle.generators import ba_config
 
@@ -112,6 +114,8 @@ def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int,
     pt_blocks = [pts[i] for i in range(pts.shape[0])]
     for c in cam_blocks:
         problem.AddParameterBlock(c)
+    for i in constant_cameras:
+        problem.SetParameterBlockConstant(cam_blocks[i])
     for p in pt_blocks:
         problem.AddParameterBlock(p)
         problem.SetParameterBlockOrderingGroup(p, 0)
@@ -124,6 +128,12 @@ def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int,
             [cam_blocks[int(d.obs_camera[k])], pt_blocks[int(d.obs_point[k])]],
         )
     kwargs = dict(max_num_iterations=max_iters)
+    if initial_radius is not None:
+        kwargs["initial_trust_region_radius"] = initial_radius
+    if strategy == "dogleg":
+        kwargs["trust_region_strategy_type"] = tc.TrustRegionStrategyType.DOGLEG
+        kwargs["dogleg_type"] = (tc.DoglegType.SUBSPACE_DOGLEG if dogleg_type == "subspace"
+                                 else tc.DoglegType.TRADITIONAL_DOGLEG)
     if tolerances == "zero":
         kwargs.update(function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
     if solver == "dense_schur":
@@ -142,6 +152,10 @@ def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int,
         tolerances=tolerances,
         max_num_iterations=max_iters,
         loss=list(loss) if loss else None,
+        trust_region_strategy=strategy,
+        dogleg_type=dogleg_type,
+        constant_cameras=list(constant_cameras),
+        initial_trust_region_radius=initial_radius,
         input_sha256=array_digest(d.cameras, d.points, d.observations, d.obs_camera, d.obs_point),
         reference_seconds=elapsed,
         summary=_summary_dict(summary),
@@ -247,6 +261,13 @@ CASES = {
     "ba6_huber_dense_schur": lambda: run_ba("ba6_huber_dense_schur", "ba6_huber", "dense_schur", "default", 50, True),
     "curve_cauchy": lambda: run_curve("curve_cauchy", 3, [0, 5, 11, 17, 23, 31], 30),
     "pose2d_40": lambda: run_pose("pose2d_40", 40, 4, 50),
+    "ba4_dogleg_zero_tol": lambda: run_ba("ba4_dogleg_zero_tol", "ba4", "dense_schur", "zero", 25, True,
+                                          strategy="dogleg", constant_cameras=(0, 1), initial_radius=0.05),
+    "ba4_subspace_dogleg_zero_tol": lambda: run_ba("ba4_subspace_dogleg_zero_tol", "ba4", "dense_schur", "zero", 25,
+                                                   True, strategy="dogleg", dogleg_type="subspace",
+                                                   constant_cameras=(0, 1), initial_radius=0.05),
+    "ba4_dogleg_default_tol": lambda: run_ba("ba4_dogleg_default_tol", "ba4", "dense_schur", "default", 50, True,
+                                             strategy="dogleg", constant_cameras=(0, 1), initial_radius=0.05),
     "ba16_dense_schur_zero_tol": lambda: run_ba("ba16_dense_schur_zero_tol", "ba16", "dense_schur", "zero", 25, False),
 }
 

@@ -12,6 +12,9 @@ formula:
 | ba.schur_system / back-substitution     | damped least squares [J; sqrt D] (textbook identity), reference 6x4 KAT (tests/test_linear_schur.py:109-134) |
 | ba.block_jacobi_pcg                     | reference KAT (tests/test_linear_schur.py:137-174), exact CG termination |
 | lm.LMRadius / StepEvaluator             | reference KATs (tests/test_problem_solver.py:551-577) |
+| lm.updated_trust_region_radius (DOGLEG) | known answers of each branch of solver.py:1051-1061 |
+| lm.traditional/subspace_dogleg_step     | brute force: the boundary crossing of the dogleg path on a fine grid; the model minimum over a polar grid of the 2-D subspace disc; normal equations for the interior step |
+| ba.BAModel.gauss_newton_step / jmul     | dense J^T J solve and dense J v on the same evaluation |
 | lm.trust_region_solve (BA, curve, SE2)  | golden trajectories written by running the reference (tests/golden/) |
 | generators                              | sha256 of the inputs recorded with each golden trajectory |
 """
@@ -23,7 +26,9 @@ import torch
 
 from oracle import ba as oba
 from oracle import generators as gen
-from oracle.lm import LMRadius, OracleOptions, StepEvaluator, dense_linear_solve, schur_solve_dense, trust_region_solve
+from oracle.lm import (LMRadius, OracleOptions, StepEvaluator, dense_linear_solve, orthonormal_basis,
+                       schur_solve_dense, subspace_dogleg_step, traditional_dogleg_step, trust_region_solve,
+                       updated_trust_region_radius)
 from oracle.losses import loss_rho, robustify
 from oracle.small_models import CurveModel, PoseGraphModel
 
@@ -383,3 +388,119 @@ def test_generator_inputs_match_golden_digest(case):
     for a in (d.cameras, d.points, d.observations, d.obs_camera, d.obs_point):
         h.update(np.ascontiguousarray(a).tobytes())
     assert h.hexdigest() == g["input_sha256"]
+
+
+# ---------------------------------------------------------------------------
+# dogleg (linear.py:554-664, solver.py:864-883, 1051-1061)
+# ---------------------------------------------------------------------------
+
+
+def test_dogleg_radius_known_answers():
+    assert updated_trust_region_radius(1.0, 0.8, 0.1, 1e16) == 2.0      # expand to 2 radius
+    assert updated_trust_region_radius(1.0, 0.8, 1.0, 1e16) == 3.0      # ... or 3 |step|
+    assert updated_trust_region_radius(1.0, 0.8, 1.0, 2.5) == 2.5       # capped
+    assert updated_trust_region_radius(1.0, 0.5, 1.0, 1e16) == 1.0      # unchanged
+    assert updated_trust_region_radius(1.0, 0.1, 1.0, 1e16) == 0.5      # shrink
+    assert updated_trust_region_radius(1e-320, 0.1, 1.0, 1e16) == np.finfo(np.float64).tiny
+
+
+def _dogleg_case(seed, m=14, n=5):
+    rng = np.random.default_rng(seed)
+    J = rng.normal(size=(m, n)) * np.linspace(0.3, 3.0, n)
+    r = rng.normal(size=m)
+    g = J.T @ r
+    gn = np.linalg.solve(J.T @ J, -g)
+    return J, r, g, gn
+
+
+def test_traditional_dogleg_branches_by_brute_force():
+    for seed in range(6):
+        J, r, g, gn = _dogleg_case(seed)
+        jmul = lambda v: J @ v  # noqa: E731
+        # interior: the Gauss-Newton step solves the normal equations
+        s = traditional_dogleg_step(jmul, g, gn, 2.0 * np.linalg.norm(gn))
+        np.testing.assert_allclose(J.T @ (J @ s), -g, rtol=1e-10, atol=1e-12)
+        # the Cauchy point, by minimising the model along -g on a fine grid
+        ts = np.linspace(0.0, 10.0, 200001) * np.linalg.norm(g) ** 2 / max(np.linalg.norm(J @ g) ** 2, 1e-300)
+        model = [0.5 * np.sum((r - t * (J @ g)) ** 2) for t in ts[::100]]
+        t_star = ts[::100][int(np.argmin(model))]
+        sd = -t_star * g
+        # small radius: the step is -g scaled to the boundary
+        rad = 0.25 * np.linalg.norm(sd)
+        s = traditional_dogleg_step(jmul, g, gn, rad)
+        assert np.linalg.norm(s) == pytest.approx(rad, rel=1e-12)
+        assert float(s @ g) < 0 and abs(float(s @ g)) == pytest.approx(np.linalg.norm(s) * np.linalg.norm(g), rel=1e-12)
+        # between: the boundary crossing of the path sd -> gn, found on a grid of the segment
+        sd_exact = -(g @ g) / np.sum((J @ g) ** 2) * g
+        np.testing.assert_allclose(sd_exact, sd, rtol=2e-3)
+        lo, hi = np.linalg.norm(sd_exact), np.linalg.norm(gn)
+        if hi <= lo * 1.01:
+            continue
+        rad = 0.5 * (lo + hi)
+        s = traditional_dogleg_step(jmul, g, gn, rad)
+        taus = np.linspace(0.0, 1.0, 100001)
+        pts = sd_exact[None, :] + taus[:, None] * (gn - sd_exact)[None, :]
+        k = int(np.argmax(np.linalg.norm(pts, axis=1) >= rad))
+        np.testing.assert_allclose(s, pts[k], atol=2e-4 * np.linalg.norm(gn))
+        assert np.linalg.norm(s) == pytest.approx(rad, rel=1e-12)
+
+
+def test_subspace_dogleg_minimises_the_model_on_the_subspace_disc():
+    for seed in range(4):
+        J, r, g, gn = _dogleg_case(10 + seed)
+        jmul = lambda v: J @ v  # noqa: E731
+        sd = -(g @ g) / np.sum((J @ g) ** 2) * g
+        rad = 0.5 * (np.linalg.norm(sd) + np.linalg.norm(gn))
+        if np.linalg.norm(gn) <= np.linalg.norm(sd) * 1.01:
+            continue
+        s = subspace_dogleg_step(jmul, r, g, gn, rad)
+        B = orthonormal_basis((-g, gn))
+        assert B.shape[1] == 2
+        np.testing.assert_allclose(B.T @ B, np.eye(2), atol=1e-12)
+        # brute force over the disc |y| <= rad in span(B) (the minimum is on the boundary)
+        ang = np.linspace(0.0, 2.0 * np.pi, 20001)
+        ys = rad * np.stack([np.cos(ang), np.sin(ang)], axis=1)
+        vals = 0.5 * np.sum((r[None, :] + (J @ B @ ys.T).T) ** 2, axis=1)
+        best = float(vals.min())
+        mine = 0.5 * float(np.sum((r + J @ s) ** 2))
+        assert np.linalg.norm(s) <= rad * (1.0 + 1e-9)
+        assert mine <= best + 1e-9 * abs(best)
+        assert mine >= best - 1e-6 * abs(best)  # and it is that minimum, not a better point off the subspace
+        np.testing.assert_allclose(B @ (B.T @ s), s, atol=1e-12 * max(1.0, np.linalg.norm(s)))
+
+
+def test_ba_gauss_newton_step_and_jmul_match_dense_algebra():
+    d = gen.ba_config("ba4")
+    active = np.ones(d.num_cameras, bool)
+    active[:2] = False
+    model = oba.BAModel(d, camera_active=active)
+    ev, g = model.evaluate(model.x0(), True)
+    J = model.dense_jacobian(ev)
+    np.testing.assert_allclose(model.gauss_newton_step(ev), np.linalg.solve(J.T @ J, -g), rtol=1e-8, atol=1e-12)
+    v = np.random.default_rng(3).normal(size=J.shape[1])
+    np.testing.assert_allclose(model.jmul(ev, v), J @ v, rtol=1e-12, atol=1e-12)
+
+
+def _dogleg_opts(g):
+    o = _ba_opts(g)
+    o.trust_region_strategy_type = g["trust_region_strategy"]
+    o.dogleg_type = g["dogleg_type"]
+    if g.get("initial_trust_region_radius") is not None:
+        o.initial_trust_region_radius = g["initial_trust_region_radius"]
+    return o
+
+
+@pytest.mark.parametrize("case", ["ba4_dogleg_zero_tol", "ba4_subspace_dogleg_zero_tol", "ba4_dogleg_default_tol"])
+@pytest.mark.parametrize("faithful", [False, True])
+def test_ba_dogleg_trajectory_matches_reference(case, faithful):
+    g = load_golden(case)
+    d = gen.ba_config(g["config"])
+    active = np.ones(d.num_cameras, bool)
+    active[g["constant_cameras"]] = False
+    model = oba.BAModel(d, linear_solver=g["linear_solver"], dense_faithful=faithful, camera_active=active)
+    s, _ = trust_region_solve(model, _dogleg_opts(g))
+    assert_trajectory_matches(s, g["summary"])
+    assert s.num_linear_solves == g["summary"]["num_linear_solves"] == 0
+    if g["tolerances"] == "default":
+        assert s.termination_type == g["summary"]["termination_type"]
+        assert len(s.iterations) == len(g["summary"]["iterations"])
