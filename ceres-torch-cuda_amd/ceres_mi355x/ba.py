@@ -240,4 +240,8 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     o.gradient_tolerance = float(options.gradient_tolerance)
     o.parameter_tolerance = float(options.parameter_tolerance)
     o.eta = float(options.eta)
+    dogleg = getattr(options, "trust_region_strategy_type", None)
+    o.trust_region_strategy = 1 if dogleg is not None and dogleg.name == "DOGLEG" else 0
+    dtype = getattr(options, "dogleg_type", None)
+    o.dogleg_type = 1 if dtype is not None and dtype.name == "SUBSPACE_DOGLEG" else 0
     return o

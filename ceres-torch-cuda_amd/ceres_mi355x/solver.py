@@ -58,8 +58,6 @@ def set_device_loop(mode: str, iterations_per_graph: Optional[int] = None) -> No
 def _check_supported(options: SolverOptions) -> None:
     if options.minimizer_type is MinimizerType.LINE_SEARCH:
         raise NotImplementedError("the MI355X device path implements the TRUST_REGION minimizer only")
-    if options.trust_region_strategy_type is TrustRegionStrategyType.DOGLEG:
-        raise NotImplementedError("DOGLEG is not on the MI355X device path (LEVENBERG_MARQUARDT only)")
     if options.use_nonmonotonic_steps:
         raise NotImplementedError("non-monotonic steps are not on the MI355X device path")
     if options.use_inner_iterations:
@@ -215,11 +213,13 @@ def solve(options: SolverOptions, problem: Problem) -> SolverSummary:
     options.validate()
     _check_supported(options)
     start = time.perf_counter()
-    if not torch.cuda.is_available():
-        raise nat.NativeLibraryError("the MI355X solver needs a ROCm GPU (torch.cuda.is_available() is False)")
     kinds = {rb.cost_function.device_kind for rb in problem._residual_blocks}
     if problem.bulk_ba:
         kinds.add("ba")
+    if options.trust_region_strategy_type is TrustRegionStrategyType.DOGLEG and kinds != {"ba"}:
+        raise NotImplementedError("DOGLEG is on the MI355X device path for bundle adjustment only")
+    if not torch.cuda.is_available():
+        raise nat.NativeLibraryError("the MI355X solver needs a ROCm GPU (torch.cuda.is_available() is False)")
     if kinds == {"ba"}:
         return _solve_ba(options, problem, start)
     if kinds == {"curve"}:

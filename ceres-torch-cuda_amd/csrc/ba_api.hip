@@ -42,8 +42,16 @@ int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o->precision == CT_PLAIN_FP64 || o->precision == CT_MIXED, "unknown precision");
   CT_CHECK_ARG(o->loop_mode == CT_LOOP_HOST || o->loop_mode == CT_LOOP_DEVICE, "unknown loop mode");
   CT_CHECK_ARG(o->initial_trust_region_radius > 0.0, "initial_trust_region_radius must be > 0");
+  CT_CHECK_ARG(o->trust_region_strategy == CT_LEVENBERG_MARQUARDT || o->trust_region_strategy == CT_DOGLEG,
+               "unknown trust-region strategy");
+  CT_CHECK_ARG(o->dogleg_type == CT_TRADITIONAL_DOGLEG || o->dogleg_type == CT_SUBSPACE_DOGLEG, "unknown dogleg type");
   return CT_OK;
 }
+
+// DOGLEG solves the undamped Gauss-Newton system exactly (FP64 Cholesky),
+// whatever the linear solver (solver.py:129-133 never calls the linear solvers).
+bool dogleg_options(const ct_lm_options* o) { return o->trust_region_strategy == CT_DOGLEG; }
+bool uses_cholesky(const ct_lm_options* o) { return o->linear_solver == CT_DENSE_SCHUR || dogleg_options(o); }
 
 int obs_blocks(const ct_ba_problem* p) { return (p->num_observations + kPointThreads - 1) / kPointThreads; }
 int eval_blocks(const ct_ba_problem* p) { return (p->num_points + kPointThreads - 1) / kPointThreads; }
@@ -91,6 +99,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   L.w.part_cam = cv.take<double>(2 * C + 2);
   L.w.part_step = cv.take<double>(3 * std::max((size_t)step_blocks(p), fused_blocks) + 3);
   L.w.part_cand = cv.take<double>(2 * std::max((size_t)obs_blocks(p), fused_blocks) + 2);
+  L.w.part_dogleg = cv.take<double>(6 * (size_t)dogleg_blocks((int)P) + 6);
   L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
   L.w.state = cv.take<LMState>(1);
   L.w.Dp = cv.take<double>(3 * P);
@@ -131,7 +140,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
     L.bd.gp = cv.take<double>(3 * P);
     L.bd.item_S = cv.take<double>(81 * I);
   }
-  if (o->linear_solver == CT_ITERATIVE_SCHUR) {
+  if (o->linear_solver == CT_ITERATIVE_SCHUR && !dogleg_options(o)) {
     L.pcg_doubles = rcs_pcg_work_doubles((int)n);
     L.pcg_work = cv.take<double>(L.pcg_doubles);
     L.w.pcg_bar = base ? rcs_pcg_barrier(L.pcg_work, (int)n) : nullptr;
@@ -186,7 +195,7 @@ struct ct_ba_plan {
 namespace {
 
 bool refining(const ct_ba_plan* pl) {
-  return pl->options.precision == CT_MIXED && pl->options.linear_solver == CT_DENSE_SCHUR &&
+  return pl->options.precision == CT_MIXED && pl->options.linear_solver == CT_DENSE_SCHUR && !dogleg_options(&pl->options) &&
          pl->options.max_num_refinement_iterations > 0;
 }
 
@@ -213,7 +222,7 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
     case CT_KERNEL_LM_BEGIN:
       return launch_lm_begin(pl->init, L.w, s);
     case CT_KERNEL_RCS_SOLVE:
-      if (pl->options.linear_solver == CT_ITERATIVE_SCHUR)
+      if (!uses_cholesky(&pl->options))
         return launch_rcs_pcg(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.pcg_work, L.pcg_doubles, s);
       return launch_rcs_cholesky_solve(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.w.solve_tmp, s);
     case CT_KERNEL_REFINE:
@@ -222,11 +231,17 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
       for (int k = 0; k < pl->options.max_num_refinement_iterations; ++k)
         CT_TRY(launch_refine_step<float>(v, L.bf, L.w, s));
       return CT_OK;
-    case CT_KERNEL_STEP:
+    case CT_KERNEL_DOGLEG:
+      if (!dogleg_options(&pl->options)) return CT_OK;
+      return mixed ? launch_dogleg_step<float>(v, L.bf, L.w, s) : launch_dogleg_step<double>(v, L.bd, L.w, s);
+    case CT_KERNEL_STEP: {
+      // the refinement and the dogleg leave the whole step in dxc / dxp
+      const bool backsub = !refining(pl) && !dogleg_options(&pl->options);
       if (pl->num_groups > 0)
-        return mixed ? launch_step_candidate<float>(v, L.bf, L.w, pl->num_groups, !refining(pl), s)
-                     : launch_step_candidate<double>(v, L.bd, L.w, pl->num_groups, true, s);
-      return mixed ? launch_step<float>(v, L.bf, L.w, !refining(pl), s) : launch_step<double>(v, L.bd, L.w, true, s);
+        return mixed ? launch_step_candidate<float>(v, L.bf, L.w, pl->num_groups, backsub, s)
+                     : launch_step_candidate<double>(v, L.bd, L.w, pl->num_groups, backsub, s);
+      return mixed ? launch_step<float>(v, L.bf, L.w, backsub, s) : launch_step<double>(v, L.bd, L.w, backsub, s);
+    }
     case CT_KERNEL_CANDIDATE:
       if (pl->num_groups > 0) return CT_OK;  // the fused step computed the candidate
       return mixed ? launch_candidate<float>(v, L.bf, L.w, s) : launch_candidate<double>(v, L.bd, L.w, s);
@@ -242,13 +257,14 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
 bool group_launches(const ct_ba_plan* pl, int group) {
   if (group == CT_KERNEL_EVAL_POINTS || group == CT_KERNEL_CANDIDATE) return pl->num_groups == 0;
   if (group == CT_KERNEL_REFINE) return refining(pl);
+  if (group == CT_KERNEL_DOGLEG) return dogleg_options(&pl->options);
   return true;
 }
 
 // Groups that compute and test a step; iteration 0 only evaluates.
 bool step_group(int group) {
-  return group == CT_KERNEL_RCS_SOLVE || group == CT_KERNEL_REFINE || group == CT_KERNEL_STEP ||
-         group == CT_KERNEL_CANDIDATE;
+  return group == CT_KERNEL_RCS_SOLVE || group == CT_KERNEL_REFINE || group == CT_KERNEL_DOGLEG ||
+         group == CT_KERNEL_STEP || group == CT_KERNEL_CANDIDATE;
 }
 
 // One LM iteration.  evaluation_only (host-driven iteration 0) leaves out the
@@ -515,11 +531,11 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
     pl->init.num_step_blocks = fused_blocks;
     pl->init.num_obs_blocks = fused_blocks;
   }
-  if (options->linear_solver == CT_DENSE_SCHUR && problem->num_cameras > 0) {
+  if (uses_cholesky(options) && problem->num_cameras > 0) {
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, 9 * problem->num_cameras, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));
   }
-  if (options->linear_solver == CT_ITERATIVE_SCHUR && problem->num_cameras > 0) {
+  if (!uses_cholesky(options) && problem->num_cameras > 0) {
     CT_TRY(rcs_pcg_prepare(pl->layout.pcg_work, 9 * problem->num_cameras, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));
   }

@@ -34,7 +34,10 @@ struct LMState {
   int32_t num_residual_evals, num_jacobian_evals, num_linear_solves;
   int32_t step_invalid;     // set when dx is not finite / factorisation failed
   int32_t linear_iterations;
+  int32_t strategy;         // ct_trust_region_strategy
+  int32_t dogleg_type;      // ct_dogleg_type
   int32_t pad0;
+  double dl_cgn, dl_cg;     // DOGLEG step = dl_cgn * (Gauss-Newton step) + dl_cg * g
   uint64_t t0;              // wall clock at start
   uint64_t t_end;           // wall clock at termination
 };
@@ -97,6 +100,7 @@ struct BAWork {
   double* part_cam;   // [C][2]   |g_c|^2, max|g_c|
   double* part_step;  // [Gstep][3] |dx|^2, |x'|^2, non-finite count
   double* part_cand;  // [Gobs][2] candidate cost, model decrease
+  double* part_dogleg;  // [Gpt][6] DOGLEG dot products (see k_dogleg_partials)
   double* records;    // [(max_iter+2)][CT_REC_FIELDS]
   LMState* state;
   // damping and gradient kept for the refinement residual

@@ -29,6 +29,11 @@ int launch_step_candidate(const BAView& v, const BABuffers<real>& b, const BAWor
                           hipStream_t s);
 template <typename real>
 int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+// DOGLEG step (back-substitution of the Gauss-Newton step, dot products,
+// coefficients, dx <- cgn gn + cg g); partial rows: dogleg_blocks(P).
+template <typename real>
+int launch_dogleg_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+inline int dogleg_blocks(int num_points) { return num_points > 0 ? (num_points + kPointThreads - 1) / kPointThreads : 1; }
 template <typename real>
 int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,

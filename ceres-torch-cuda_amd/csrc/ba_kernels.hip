@@ -3,7 +3,10 @@
 // per-observation candidate evaluation.  Templated on the assembly precision (float: mixed,
 // double: plain FP64); parameters, residuals, costs and the model decrease are
 // always FP64.
+#include <algorithm>
+
 #include "ba_kernels.h"
+#include "lm_logic.h"
 
 namespace ct {
 
@@ -177,6 +180,15 @@ __device__ __forceinline__ double damping_of(double colsq, double min_diag, doub
   return fmin(fmax(dg, min_diag), max_diag) / fmax(radius, 2.220446049250313e-16);
 }
 
+// Diagonal added to the normal equations of one column: the LM damping
+// (linear.py:539-551), or for DOGLEG none (the Gauss-Newton system J^T J),
+// except a unit diagonal on the columns of constant parameters, which keeps
+// their (zero) rows of the system nonsingular without touching any other.
+__device__ __forceinline__ double column_damping(const LMState* st, double colsq, bool free_column) {
+  if (st->strategy == CT_DOGLEG) return free_column ? 0.0 : 1.0;
+  return damping_of(colsq, st->min_diag, st->max_diag, st->radius, st->jacobi_scaling);
+}
+
 // ---------------------------------------------------------------------------
 // K1a: one thread per observation - residual, Jacobian, loss correction.
 // Writes Jc [2x9], Jp [2x3] (assembly precision) and r (FP64); block partial
@@ -246,7 +258,6 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
   __shared__ double red[kPointThreads / kWave];
   const LMState* st = w.state;
   if (st->done) return;
-  const double radius = st->radius;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
   double g2 = 0.0, gmax = 0.0;
   if (p < v.P) {
@@ -268,9 +279,9 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
       g2v += (double)a2 * r0 + (double)c2 * r1;
     }
     // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
-    const double D0 = damping_of((double)V00, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    const double D1 = damping_of((double)V11, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    const double D2 = damping_of((double)V22, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D0 = column_damping(st, (double)V00, !v.point_mask || v.point_mask[3 * p + 0]);
+    const double D1 = column_damping(st, (double)V11, !v.point_mask || v.point_mask[3 * p + 1]);
+    const double D2 = column_damping(st, (double)V22, !v.point_mask || v.point_mask[3 * p + 2]);
     w.Dp[3 * p] = D0;
     w.Dp[3 * p + 1] = D1;
     w.Dp[3 * p + 2] = D2;
@@ -350,7 +361,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
   const LMState* st = w.state;
   if (st->done) return;
   const int cur = st->cur;
-  const double radius = st->radius;
   const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -432,9 +442,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
       g2v += term[wave][8][k];
     }
     // damping D = clamp(colsum J^2)/radius (linear.py:539-551)
-    const double D0 = damping_of((double)V00, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    const double D1 = damping_of((double)V11, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
-    const double D2 = damping_of((double)V22, st->min_diag, st->max_diag, radius, st->jacobi_scaling);
+    const double D0 = column_damping(st, (double)V00, !v.point_mask || v.point_mask[3 * p + 0]);
+    const double D1 = column_damping(st, (double)V11, !v.point_mask || v.point_mask[3 * p + 1]);
+    const double D2 = column_damping(st, (double)V22, !v.point_mask || v.point_mask[3 * p + 2]);
     w.Dp[3 * p] = D0;
     w.Dp[3 * p + 1] = D1;
     w.Dp[3 * p + 2] = D2;
@@ -707,7 +717,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_camera_reduce(BAView v, BABu
   __syncthreads();
   if (diag && tid < 9) {
     const int k = 9 * ci + tid;
-    const double d = damping_of(svec[18 + tid], st->min_diag, st->max_diag, st->radius, st->jacobi_scaling);
+    const double d = column_damping(st, svec[18 + tid], !v.camera_mask || v.camera_mask[k]);
     w.rhs[k] = svec[tid];
     w.gc[k] = svec[9 + tid];
     w.Dc[k] = d;
@@ -1133,6 +1143,88 @@ __global__ __launch_bounds__(kFusedThreads) void k_step_candidate(BAView v, BABu
   }
 }
 
+// ---------------------------------------------------------------------------
+// DOGLEG step (linear.py:554-608).  With the Gauss-Newton step gn = (dxc, dxp)
+// of the undamped Schur system and the gradient g = (gc, gp):
+//   k_dogleg_partials: per workgroup |g|^2, |gn|^2, g.gn (cameras: the first 9C
+//     threads of the grid; points: one thread each) and, over each point's
+//     observations, |J g|^2, |J gn|^2, (J g).(J gn);
+//   k_dogleg_coefficients: the fixed-order sum and the step coefficients;
+//   k_dogleg_combine: dx <- cgn gn + cg g, in place.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(kPointThreads, 2) void k_dogleg_partials(BAView v, BABuffers<real> b, BAWork w) {
+  __shared__ double red[kPointThreads / kWave];
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < 9 * v.C; t += gridDim.x * kPointThreads) {
+    const double gg = w.gc[t], nn = w.dxc[t];
+    d[0] += gg * gg;
+    d[1] += nn * nn;
+    d[2] += gg * nn;
+  }
+  const int p = blockIdx.x * kPointThreads + threadIdx.x;
+  if (p < v.P) {
+    const double g0 = b.gp[3 * p], g1 = b.gp[3 * p + 1], g2 = b.gp[3 * p + 2];
+    const double n0 = w.dxp[3 * p], n1 = w.dxp[3 * p + 1], n2 = w.dxp[3 * p + 2];
+    d[0] += g0 * g0 + g1 * g1 + g2 * g2;
+    d[1] += n0 * n0 + n1 * n1 + n2 * n2;
+    d[2] += g0 * n0 + g1 * n1 + g2 * n2;
+    const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
+    for (int o = o0; o < o1; ++o) {
+      const int ci = v.obs_camera[o];
+      const real* J = b.JP + kJPStride * (size_t)o;
+      const real* Jq = b.Jp + 6 * (size_t)o;
+      double jg0 = (double)Jq[0] * g0 + (double)Jq[1] * g1 + (double)Jq[2] * g2;
+      double jg1 = (double)Jq[3] * g0 + (double)Jq[4] * g1 + (double)Jq[5] * g2;
+      double jn0 = (double)Jq[0] * n0 + (double)Jq[1] * n1 + (double)Jq[2] * n2;
+      double jn1 = (double)Jq[3] * n0 + (double)Jq[4] * n1 + (double)Jq[5] * n2;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double gc = w.gc[9 * ci + j], nc = w.dxc[9 * ci + j];
+        jg0 += (double)J[j] * gc;
+        jg1 += (double)J[9 + j] * gc;
+        jn0 += (double)J[j] * nc;
+        jn1 += (double)J[9 + j] * nc;
+      }
+      d[3] += jg0 * jg0 + jg1 * jg1;
+      d[4] += jn0 * jn0 + jn1 * jn1;
+      d[5] += jg0 * jn0 + jg1 * jn1;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double t = block_sum<kPointThreads>(d[k], red);
+    if (threadIdx.x == 0) w.part_dogleg[6 * (size_t)blockIdx.x + k] = t;
+  }
+}
+
+constexpr int kDoglegThreads = 256;
+__global__ __launch_bounds__(kDoglegThreads) void k_dogleg_coefficients(BAWork w, int num_blocks) {
+  __shared__ double red[kDoglegThreads / kWave];
+  LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < num_blocks; i += kDoglegThreads)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a[k] += w.part_dogleg[6 * (size_t)i + k];
+  double d[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) d[k] = block_sum<kDoglegThreads>(a[k], red);
+  if (threadIdx.x == 0) lm_dogleg_coefficients(st, d);
+}
+
+template <typename real>
+__global__ __launch_bounds__(kPointThreads) void k_dogleg_combine(BAView v, BABuffers<real> b, BAWork w) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const double cgn = st->dl_cgn, cg = st->dl_cg;
+  const int stride = gridDim.x * kPointThreads;
+  for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < 9 * v.C; t += stride) w.dxc[t] = cgn * w.dxc[t] + cg * w.gc[t];
+  for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < 3 * v.P; t += stride) w.dxp[t] = cgn * w.dxp[t] + cg * b.gp[t];
+}
+
 // Plain evaluation (cost + corrected residuals) at given parameters.
 __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const double* cams, const double* pts,
                                                                   double* residuals, double* part) {
@@ -1261,6 +1353,23 @@ int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, h
   return CT_OK;
 }
 
+// DOGLEG: gn's point part by back-substitution, the dot products, the
+// coefficients and the combined step in dxc / dxp.
+template <typename real>
+int launch_dogleg_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+  CT_TRY(launch_backsub<real>(v, b, w, s));
+  const int blocks = dogleg_blocks(v.P);
+  hipLaunchKernelGGL(k_dogleg_partials<real>, dim3(blocks), dim3(kPointThreads), 0, s, v, b, w);
+  CT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dogleg_coefficients, dim3(1), dim3(kDoglegThreads), 0, s, w, blocks);
+  CT_LAUNCH_CHECK();
+  const size_t work = std::max<size_t>(9 * (size_t)v.C, 3 * (size_t)v.P);
+  const unsigned int grid = (unsigned int)std::min<size_t>(std::max<size_t>((work + kPointThreads - 1) / kPointThreads, 1), 4096);
+  hipLaunchKernelGGL(k_dogleg_combine<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
 template <typename real>
 int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
@@ -1308,6 +1417,8 @@ template int launch_eval_fused<float>(const BAView&, const BABuffers<float>&, co
 template int launch_eval_fused<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_dogleg_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
+template int launch_dogleg_step<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 template int launch_refine_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_refine_step<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 

@@ -261,6 +261,10 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
   CT_CHECK_ARG(options->max_num_iterations >= 0, "max_num_iterations must be >= 0");
   CT_CHECK_ARG(options->precision == CT_PLAIN_FP64 || options->precision == CT_MIXED, "unknown precision");
   CT_CHECK_ARG(options->initial_trust_region_radius > 0.0, "initial_trust_region_radius must be > 0");
+  if (options->trust_region_strategy != CT_LEVENBERG_MARQUARDT) {
+    ::ct::set_error("DOGLEG is on the device path for bundle adjustment only");
+    return CT_ERR_UNSUPPORTED;
+  }
   if (batch->num_problems == 0) return CT_OK;
   CT_CHECK_ARG(batch->offsets != nullptr && params != nullptr, "offsets / params are null");
   CT_CHECK_ARG(batch->num_samples == 0 || (batch->x != nullptr && batch->y != nullptr), "x / y are null");

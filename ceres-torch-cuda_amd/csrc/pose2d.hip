@@ -505,6 +505,10 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
   CT_CHECK_ARG(options != nullptr && plan != nullptr && poses != nullptr && workspace != nullptr, "null argument");
   CT_CHECK_ARG(options->max_num_iterations >= 0 && options->initial_trust_region_radius > 0.0, "bad options");
   CT_CHECK_ARG(options->loop_mode == CT_LOOP_HOST || options->loop_mode == CT_LOOP_DEVICE, "unknown loop mode");
+  if (options->trust_region_strategy != CT_LEVENBERG_MARQUARDT) {
+    ::ct::set_error("DOGLEG is on the device path for bundle adjustment only");
+    return CT_ERR_UNSUPPORTED;
+  }
   const size_t need = pg_carve(problem, options, nullptr).bytes;
   if (workspace_bytes < need) {
     set_error("workspace too small: need " + std::to_string(need) + " bytes");

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 5
+#define CT_ABI_VERSION 6
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -89,6 +89,24 @@ typedef enum ct_loop_mode {
   CT_LOOP_DEVICE = 1  /* device-resident LM state, graph-captured iterations */
 } ct_loop_mode;
 
+/* Trust-region strategy (TrustRegionStrategyType, types.py:88-91) and dogleg
+ * variant (DoglegType, types.py:93-95).  DOGLEG takes the step of
+ * dogleg_step (linear.py:554-664) from the undamped Gauss-Newton step (the
+ * Schur complement with no damping, factored by the FP64 Cholesky whatever
+ * the linear solver) and the gradient, and updates the radius by
+ * solver.py:864-883, 1051-1061.  J^T J must be nonsingular (for bundle
+ * adjustment: the gauge fixed, e.g. by constant cameras); a factorisation that
+ * meets a non-positive pivot gives an invalid step.                         */
+typedef enum ct_trust_region_strategy {
+  CT_LEVENBERG_MARQUARDT = 0,
+  CT_DOGLEG = 1
+} ct_trust_region_strategy;
+
+typedef enum ct_dogleg_type {
+  CT_TRADITIONAL_DOGLEG = 0,
+  CT_SUBSPACE_DOGLEG = 1
+} ct_dogleg_type;
+
 typedef struct ct_lm_options {
   int32_t max_num_iterations;
   int32_t max_num_consecutive_invalid_steps;
@@ -110,6 +128,8 @@ typedef struct ct_lm_options {
   double gradient_tolerance;
   double parameter_tolerance;
   double eta;
+  int32_t trust_region_strategy;        /* ct_trust_region_strategy (bundle adjustment only) */
+  int32_t dogleg_type;                  /* ct_dogleg_type */
 } ct_lm_options;
 
 /* Termination codes match TerminationType (types.py:98-103). */
@@ -222,10 +242,11 @@ enum {
   CT_KERNEL_LM_BEGIN = 4,        /* cost/gradient reduction + checks */
   CT_KERNEL_RCS_SOLVE = 5,       /* Cholesky (or PCG) of S */
   CT_KERNEL_REFINE = 6,          /* mixed: back-substitution + iterative refinement */
-  CT_KERNEL_STEP = 7,            /* step application (+ point back-substitution) */
-  CT_KERNEL_CANDIDATE = 8,       /* candidate cost + model decrease per observation */
-  CT_KERNEL_LM_END = 9,          /* gain ratio, radius, accept/reject */
-  CT_KERNEL_COUNT = 10
+  CT_KERNEL_DOGLEG = 7,          /* DOGLEG: back-substitution, dot products, step coefficients, combined step */
+  CT_KERNEL_STEP = 8,            /* step application (+ point back-substitution) */
+  CT_KERNEL_CANDIDATE = 9,       /* candidate cost + model decrease per observation */
+  CT_KERNEL_LM_END = 10,         /* gain ratio, radius, accept/reject */
+  CT_KERNEL_COUNT = 11
 };
 
 /* Run up to `iterations` LM iterations (after an untimed iteration 0; fewer

@@ -112,6 +112,38 @@ def test_per_block_api_matches_bulk_api():
     np.testing.assert_allclose(pts.numpy(), np.array(g["final_points"]), atol=1e-6)
 
 
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("case", ["ba4_dogleg_zero_tol", "ba4_subspace_dogleg_zero_tol", "ba4_dogleg_default_tol"])
+def test_ba_dogleg_matches_reference_trajectory(case, loop):
+    """DOGLEG (traditional and subspace), gauge fixed by two constant cameras.
+
+    The small initial radius (0.05) makes the first steps the scaled Cauchy
+    point, then points on the dogleg path / the subspace boundary, then full
+    Gauss-Newton steps.  Same bar as the LM trajectories.
+    """
+    print(f"case={case} loop={loop}", flush=True)
+    tc.set_device_loop(loop)
+    g = load_golden(case)
+    data = gen.ba_config(g["config"])
+    problem, cams, pts = ba_problem(data)
+    for i in g["constant_cameras"]:
+        problem.SetParameterBlockConstant(problem.parameter_blocks[i])
+    o = options_for(g, trust_region_strategy_type=tc.TrustRegionStrategyType.DOGLEG,
+                    dogleg_type=(tc.DoglegType.SUBSPACE_DOGLEG if g["dogleg_type"] == "subspace"
+                                 else tc.DoglegType.TRADITIONAL_DOGLEG),
+                    initial_trust_region_radius=g["initial_trust_region_radius"])
+    cams0 = cams.clone()
+    summary = tc.solve(o, problem)
+    assert_trajectory_matches(summary, g["summary"])
+    assert summary.num_linear_solves == 0
+    if g["tolerances"] == "default":
+        assert summary.termination_type.value == g["summary"]["termination_type"]
+        assert summary.message == g["summary"]["message"]
+        assert len(summary.iterations) == len(g["summary"]["iterations"])
+    np.testing.assert_allclose(pts.cpu().numpy(), np.array(g["final_points"]), atol=1e-6)
+    torch.testing.assert_close(cams[:2], cams0[:2], rtol=0, atol=0)
+
+
 def test_tiny_bal_with_constant_cameras():
     # examples/simple_bundle_adjuster.py:142-176; tests/test_examples.py:151-165
     true_cams = np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 800.0, 0.0, 0.0],

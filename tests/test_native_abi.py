@@ -62,6 +62,13 @@ def test_workspace_query_and_argument_errors():
     assert b"loss" in lib.ct_last_error()
     opts.linear_solver = 7
     assert lib.ct_ba_workspace_bytes(ctypes.byref(_problem_struct()), ctypes.byref(opts), ctypes.byref(nbytes)) == nat.CT_ERR_ARGUMENT
+    opts.linear_solver = 0
+    opts.trust_region_strategy = 9
+    assert lib.ct_ba_workspace_bytes(ctypes.byref(_problem_struct()), ctypes.byref(opts), ctypes.byref(nbytes)) == nat.CT_ERR_ARGUMENT
+    assert b"strategy" in lib.ct_last_error()
+    opts.trust_region_strategy = 1  # DOGLEG sizes the Cholesky workspace even for ITERATIVE_SCHUR
+    opts.linear_solver = 1
+    assert lib.ct_ba_workspace_bytes(ctypes.byref(_problem_struct()), ctypes.byref(opts), ctypes.byref(nbytes)) == 0
 
 
 def test_plan_create_rejects_small_workspace():
@@ -132,6 +139,7 @@ def test_solve_refuses_unsupported_problems_without_fallback():
     problem.AddResidualBlock(tc.AutoDiffCostFunction(lambda x: 10.0 - x, [1], 1), None, [x])
     with pytest.raises((NotImplementedError, tc.NativeLibraryError)):
         tc.solve(tc.SolverOptions(), problem)
+    # DOGLEG runs on the device for bundle adjustment only
     with pytest.raises(NotImplementedError, match="DOGLEG"):
         tc.solve(tc.SolverOptions(trust_region_strategy_type=tc.TrustRegionStrategyType.DOGLEG), problem)
 
