@@ -168,6 +168,25 @@ def test_tiny_bal_with_constant_cameras():
     np.testing.assert_array_equal(cams.numpy(), true_cams)
 
 
+def test_tiny_bal_file_solve_recovers_points(tmp_path):
+    # tests/test_examples.py:151-165: the example's BAL file, cameras fixed
+    from ceres_mi355x.bal import BALProblem, build_problem
+    from test_bal import tiny_bal_text
+
+    text, true_cams, true_pts, _ = tiny_bal_text()
+    path = tmp_path / "tiny.bal"
+    path.write_text(text)
+    bal = BALProblem.from_file(path)
+    problem, cams, pts = build_problem(bal, fix_cameras=True, device="cuda")
+    summary = tc.solve(tc.SolverOptions(max_num_iterations=40, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
+                                        gradient_tolerance=1e-12, function_tolerance=1e-12,
+                                        parameter_tolerance=1e-12), problem)
+    assert summary.IsSolutionUsable()
+    assert summary.final_cost < 1e-12
+    np.testing.assert_allclose(pts.cpu().numpy(), true_pts, atol=1e-5, rtol=1e-5)
+    np.testing.assert_array_equal(cams.cpu().numpy(), true_cams)
+
+
 def test_evaluate_matches_oracle_cost_and_residuals():
     data = gen.ba_config("ba6_huber")
     problem, cams, pts = ba_problem(data)
