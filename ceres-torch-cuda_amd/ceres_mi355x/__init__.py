@@ -32,7 +32,7 @@ from .losses import (
     TukeyLoss,
 )
 from .manifolds import EuclideanManifold, Manifold, SubsetManifold
-from .problem import EvaluationResult, ParameterBlock, Problem, ResidualBlock
+from .problem import CRSMatrix, EvaluateOptions, EvaluationResult, ParameterBlock, Problem, ResidualBlock
 from .bal import BALProblem
 from .rotation import angle_axis_rotate_point, angle_axis_to_quaternion, unit_quaternion_rotate_point
 from .solver import set_device_loop, solve

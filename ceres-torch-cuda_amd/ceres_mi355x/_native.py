@@ -169,6 +169,9 @@ SIGNATURES = {
     "ct_ba_plan_profile": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "ct_ba_evaluate": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), _vp, _vp, ctypes.POINTER(ctypes.c_double), _vp,
                                       _vp, ctypes.c_size_t, _vp]),
+    "ct_ba_evaluate_jacobian": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), _vp, _vp, _vp, _vp, _vp,
+                                               ctypes.c_int64, ctypes.c_int32, _vp, _vp,
+                                               ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
     "ct_curve_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_size_t)]),
     "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
                                       ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),

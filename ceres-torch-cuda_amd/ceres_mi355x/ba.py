@@ -203,6 +203,31 @@ class BADevice:
         out[torch.from_numpy(self.tables.order).to(self.device)] = res
         return cost.value, out
 
+    def evaluate_jacobian(self, camera_columns: np.ndarray, point_columns: np.ndarray, num_columns: int,
+                          apply_loss: bool = True):
+        """(cost, residuals [2N], dense Jacobian [2N, num_columns]) in caller observation order.
+
+        camera_columns [C, 9] / point_columns [P, 3]: each coordinate's tangent
+        column, -1 for none.
+        """
+        lib = nat.load()
+        prob = self.problem_struct()
+        N = self.num_observations
+        jac = torch.zeros((2 * N, num_columns), dtype=torch.float64, device=self.device)
+        res = torch.empty(2 * N, dtype=torch.float64, device=self.device)
+        rows = torch.from_numpy(np.asarray(self.tables.order, dtype=np.int32)).to(self.device)
+        cc = torch.from_numpy(np.asarray(camera_columns, dtype=np.int64).reshape(-1)).to(self.device)
+        pc = torch.from_numpy(np.asarray(point_columns, dtype=np.int64).reshape(-1)).to(self.device)
+        part = torch.zeros(N // 256 + 2, dtype=torch.float64, device=self.device)
+        cost = ctypes.c_double(0.0)
+        nat.check(lib.ct_ba_evaluate_jacobian(ctypes.byref(prob), nat.ptr(self.cameras), nat.ptr(self.points),
+                                              nat.ptr(rows), nat.ptr(cc), nat.ptr(pc), int(num_columns),
+                                              1 if apply_loss else 0, nat.ptr(jac), nat.ptr(res),
+                                              ctypes.byref(cost), nat.ptr(part), part.numel() * 8,
+                                              nat.current_stream_handle()),
+                  "ct_ba_evaluate_jacobian")
+        return cost.value, res, jac
+
     def close(self) -> None:
         lib = nat.load()
         for handle, _ws, _p in self._plans.values():

@@ -75,11 +75,52 @@ class BulkBundleAdjustment:
 
 
 @dataclass
+class CRSMatrix:
+    """Compressed-row copy of a dense matrix (``problem.py:15-41`` of the reference): the nonzero
+    entries row by row, ``rows`` the row starts."""
+
+    num_rows: int
+    num_cols: int
+    rows: list[int]
+    cols: list[int]
+    values: list[float]
+
+    @classmethod
+    def from_dense(cls, matrix: torch.Tensor) -> "CRSMatrix":
+        m = matrix.detach()
+        nz = torch.nonzero(m != 0, as_tuple=False)          # row-major order
+        counts = torch.bincount(nz[:, 0], minlength=m.shape[0]) if nz.numel() else torch.zeros(
+            m.shape[0], dtype=torch.int64, device=m.device)
+        rows = [0] + torch.cumsum(counts, 0).cpu().tolist()
+        vals = m[nz[:, 0], nz[:, 1]] if nz.numel() else m.new_zeros(0)
+        return cls(int(m.shape[0]), int(m.shape[1]), rows, nz[:, 1].cpu().tolist(), vals.cpu().tolist())
+
+    def to_dense(self, *, dtype: torch.dtype = torch.float64, device: torch.device | str = "cpu") -> torch.Tensor:
+        matrix = torch.zeros((self.num_rows, self.num_cols), dtype=dtype)
+        if self.cols:
+            r = torch.repeat_interleave(torch.arange(self.num_rows), torch.diff(torch.tensor(self.rows)))
+            matrix[r, torch.tensor(self.cols)] = torch.tensor(self.values, dtype=dtype)
+        return matrix.to(device)
+
+
+@dataclass
+class EvaluateOptions:
+    """``problem.py:100-105`` of the reference."""
+
+    parameter_blocks: Optional[Sequence] = None
+    residual_blocks: Optional[Sequence] = None
+    apply_loss_function: bool = True
+    num_threads: int = 1
+    new_evaluation_point: bool = True
+
+
+@dataclass
 class EvaluationResult:
     cost: torch.Tensor
     residuals: torch.Tensor
     gradient: Optional[torch.Tensor] = None
     jacobian: Optional[torch.Tensor] = None
+    crs_jacobian: Optional[CRSMatrix] = None
 
 
 class Problem:
@@ -326,12 +367,12 @@ class Problem:
             for b, v in zip(self.parameter_blocks, snapshot):
                 b.tensor.reshape(-1).copy_(v.to(dtype=b.tensor.dtype, device=b.tensor.device).reshape(-1))
 
-    def evaluate(self, options=None, *, compute_jacobian: bool = False) -> EvaluationResult:
-        """Cost and loss-corrected residuals at the current parameters, on the device."""
+    def evaluate(self, options: Optional[EvaluateOptions] = None, *, compute_jacobian: bool = True) -> EvaluationResult:
+        """``Problem.evaluate`` (``problem.py:377-413``) on the device: cost, loss-corrected residuals in
+        residual-block order and, with ``compute_jacobian``, the dense Jacobian over the active tangent
+        columns (blocks in problem order, or ``options.parameter_blocks``), ``g = J^T r`` and its CRS copy."""
         from .solver import evaluate_problem
 
-        if compute_jacobian:
-            raise NotImplementedError("the device path does not materialise the dense Jacobian")
-        return evaluate_problem(self)
+        return evaluate_problem(self, options or EvaluateOptions(), compute_jacobian)
 
     Evaluate = evaluate

@@ -172,14 +172,17 @@ def _device_for(problem: Problem) -> BADevice:
     cached = getattr(problem, "_device_cache", None)
     ex = _BAExtract(problem)
     cams, pts = ex.stacked()
-    if cached is not None and cached[0] == (len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss):
+    cm, pm = ex.masks()
+    # constant blocks and manifolds may change between solves: the masks are part of the key
+    key = (len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss,
+           None if cm is None else cm.tobytes(), None if pm is None else pm.tobytes())
+    if cached is not None and cached[0] == key:
         dev = cached[1]
         dev.set_parameters(cams, pts)
         return dev, ex
-    cm, pm = ex.masks()
     dev = BADevice(cams, pts, ex.obs_camera, ex.obs_point, ex.observations, loss_native=ex.loss,
                    camera_mask=cm, point_mask=pm)
-    problem._device_cache = ((len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss), dev)
+    problem._device_cache = (key, dev)
     return dev, ex
 
 
@@ -292,12 +295,64 @@ def _solve_ba(options: SolverOptions, problem: Problem, start: float) -> SolverS
     return s
 
 
-def evaluate_problem(problem: Problem) -> EvaluationResult:
+# a dense Jacobian above this size is refused (the reference would build it on the host)
+MAX_DENSE_JACOBIAN_BYTES = 8 << 30
+
+
+def _column_maps(problem: Problem, ex: "_BAExtract", blocks):
+    """Tangent column of every camera / point coordinate (-1: none), as _active_column_map
+    (problem.py:522-535) orders them: the selected blocks in order, constant blocks skipped,
+    a block's free coordinates in order."""
+    from .problem import ParameterBlock
+
+    selected = [problem._coerce(b) if not isinstance(b, ParameterBlock) else b for b in blocks]
+    column = {}
+    offset = 0
+    for b in selected:
+        if b.tangent_size == 0:
+            continue
+        mask = b.free_mask()
+        cols = np.full(len(mask), -1, dtype=np.int64)
+        free = np.nonzero(mask)[0]
+        cols[free] = offset + np.arange(free.size)
+        column[id(b)] = cols
+        offset += b.tangent_size
+    cam = np.stack([column.get(id(b), np.full(9, -1, np.int64)) for b in ex.cam_blocks]) if ex.cam_blocks else \
+        np.zeros((0, 9), np.int64)
+    pt = np.stack([column.get(id(b), np.full(3, -1, np.int64)) for b in ex.pt_blocks]) if ex.pt_blocks else \
+        np.zeros((0, 3), np.int64)
+    return cam, pt, offset
+
+
+def evaluate_problem(problem: Problem, options=None, compute_jacobian: bool = False) -> EvaluationResult:
+    from .problem import CRSMatrix, EvaluateOptions
+
+    options = options or EvaluateOptions()
     kinds = {rb.cost_function.device_kind for rb in problem._residual_blocks}
     if problem.bulk_ba:
         kinds.add("ba")
     if kinds != {"ba"}:
         raise NotImplementedError("device evaluation is implemented for bundle-adjustment problems")
+    if options.residual_blocks is not None:
+        raise NotImplementedError("device evaluation covers all residual blocks (EvaluateOptions.residual_blocks)")
     dev, ex = _device_for(problem)
-    cost, res = dev.evaluate()
-    return EvaluationResult(cost=torch.tensor(cost, dtype=torch.float64), residuals=res.reshape(-1))
+    if not compute_jacobian and options.apply_loss_function:
+        cost, res = dev.evaluate()
+        return EvaluationResult(cost=torch.tensor(cost, dtype=torch.float64), residuals=res.reshape(-1))
+    blocks = options.parameter_blocks if options.parameter_blocks is not None else problem.parameter_blocks
+    cam_cols, pt_cols, ncols = _column_maps(problem, ex, blocks)
+    if not compute_jacobian:
+        cam_cols[:] = -1
+        pt_cols[:] = -1
+        ncols = 0
+    need = 2 * dev.num_observations * ncols * 8
+    if need > MAX_DENSE_JACOBIAN_BYTES:
+        raise ValueError(f"the dense Jacobian ({2 * dev.num_observations} x {ncols}) would need {need / 2**30:.1f} GiB; "
+                         "call evaluate(compute_jacobian=False)")
+    cost, res, jac = dev.evaluate_jacobian(cam_cols, pt_cols, ncols, apply_loss=options.apply_loss_function)
+    out = EvaluationResult(cost=torch.tensor(cost, dtype=torch.float64), residuals=res)
+    if compute_jacobian:
+        out.jacobian = jac
+        out.gradient = jac.T @ res
+        out.crs_jacobian = CRSMatrix.from_dense(jac)
+    return out

@@ -652,4 +652,48 @@ int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const do
   return CT_OK;
 }
 
+int ct_ba_evaluate_jacobian(const ct_ba_problem* problem, const double* cameras, const double* points,
+                            const int32_t* row_of_obs, const int64_t* camera_columns, const int64_t* point_columns,
+                            int64_t num_columns, int32_t apply_loss, double* jacobian, double* residuals,
+                            double* cost_out_host, void* workspace, size_t workspace_bytes, ct_stream_t stream) {
+  clear_error();
+  CT_TRY(validate_problem(problem));
+  CT_CHECK_ARG(cost_out_host != nullptr && workspace != nullptr, "null argument");
+  CT_CHECK_ARG(num_columns >= 0, "num_columns must be >= 0");
+  const int64_t N = problem->num_observations;
+  CT_CHECK_ARG(N == 0 || (row_of_obs && residuals && camera_columns && point_columns),
+               "row_of_obs, residuals and the column maps are required");
+  CT_CHECK_ARG(N == 0 || num_columns == 0 || jacobian != nullptr, "jacobian is null");
+  const int grid = obs_blocks(problem);
+  CT_CHECK_ARG(workspace_bytes >= sizeof(double) * (size_t)(grid + 1), "workspace too small for partials");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  CT_TRY(validate_point_indices(*problem));
+  // every row index, and every column index the kernel follows, is checked once here
+  std::vector<int32_t> rows;
+  CT_TRY(copy_back(rows, row_of_obs, N));
+  std::vector<char> seen(N, 0);
+  for (int64_t o = 0; o < N; ++o) {
+    CT_CHECK_ARG(rows[o] >= 0 && rows[o] < N && !seen[rows[o]], "row_of_obs must be a permutation of 0..N-1");
+    seen[rows[o]] = 1;
+  }
+  std::vector<int64_t> cc, pc;
+  CT_TRY(copy_back(cc, camera_columns, 9 * (size_t)problem->num_cameras));
+  CT_TRY(copy_back(pc, point_columns, 3 * (size_t)problem->num_points));
+  for (int64_t c : cc) CT_CHECK_ARG(c >= -1 && c < num_columns, "camera column out of range");
+  for (int64_t c : pc) CT_CHECK_ARG(c >= -1 && c < num_columns, "point column out of range");
+  BAView v = make_view(problem);
+  double* part = static_cast<double*>(workspace);
+  CT_TRY(launch_evaluate_jacobian(v, cameras, points, row_of_obs, camera_columns, point_columns, num_columns,
+                                  apply_loss, jacobian, residuals, part, s));
+  std::vector<double> h(grid > 0 ? grid : 1, 0.0);
+  if (grid > 0) {
+    CT_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * grid, hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+  }
+  double c = 0.0;
+  for (int i = 0; i < grid; ++i) c += h[i];
+  *cost_out_host = c;
+  return CT_OK;
+}
+
 }  // extern "C"

@@ -38,6 +38,9 @@ template <typename real>
 int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,
                          hipStream_t s);
+int launch_evaluate_jacobian(const BAView& v, const double* cams, const double* pts, const int32_t* row_of_obs,
+                             const int64_t* cam_cols, const int64_t* pt_cols, int64_t ncols, int apply_loss,
+                             double* jac, double* residuals, double* part, hipStream_t s);
 
 // LM state machine (lm_kernels.hip)
 struct LMInit {

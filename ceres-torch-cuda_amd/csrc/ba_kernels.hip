@@ -1254,6 +1254,60 @@ __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const
   if (threadIdx.x == 0) part[blockIdx.x] = tc;
 }
 
+// Evaluation with the Jacobian (Problem.evaluate with compute_jacobian=True,
+// problem.py:377-413, 486-530): one thread per observation writes its
+// residual block's corrected residuals and Jacobian rows straight into the
+// caller's dense matrix - rows 2 row_of_obs[o] and 2 row_of_obs[o] + 1,
+// columns from the per-coordinate column maps (-1: no column, a constant
+// block or coordinate).  Block partials of the cost.
+__global__ __launch_bounds__(kPointThreads) void k_evaluate_jacobian(BAView v, const double* __restrict__ cams,
+                                                                      const double* __restrict__ pts,
+                                                                      const int32_t* __restrict__ row_of_obs,
+                                                                      const int64_t* __restrict__ cam_cols,
+                                                                      const int64_t* __restrict__ pt_cols,
+                                                                      int64_t ncols, int apply_loss,
+                                                                      double* __restrict__ jac,
+                                                                      double* __restrict__ residuals,
+                                                                      double* __restrict__ part) {
+  __shared__ double red[kPointThreads / kWave];
+  const int o = blockIdx.x * kPointThreads + threadIdx.x;
+  double cost = 0.0;
+  if (o < v.N) {
+    const int ci = v.obs_camera[o], p = v.obs_point[o];
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
+    const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
+    double r[2], Jc[2][9], Jp[2][3];
+    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    if (apply_loss) {
+      cost = robustify2(v.loss, r, Jc, Jp);
+    } else {
+      cost = 0.5 * (r[0] * r[0] + r[1] * r[1]);
+    }
+    const size_t row = 2 * (size_t)row_of_obs[o];
+    residuals[row] = r[0];
+    residuals[row + 1] = r[1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double* out = jac + (row + k) * (size_t)ncols;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const int64_t c = cam_cols[9 * ci + j];
+        if (c >= 0) out[c] = Jc[k][j];
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int64_t c = pt_cols[3 * p + j];
+        if (c >= 0) out[c] = Jp[k][j];
+      }
+    }
+  }
+  const double tc = block_sum<kPointThreads>(cost, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = tc;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1394,6 +1448,17 @@ int launch_evaluate_only(const BAView& v, const double* cams, const double* pts,
   if (v.P > 0) {
     hipLaunchKernelGGL(k_evaluate_only, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, cams, pts, residuals,
                        part);
+    CT_LAUNCH_CHECK();
+  }
+  return CT_OK;
+}
+
+int launch_evaluate_jacobian(const BAView& v, const double* cams, const double* pts, const int32_t* row_of_obs,
+                             const int64_t* cam_cols, const int64_t* pt_cols, int64_t ncols, int apply_loss,
+                             double* jac, double* residuals, double* part, hipStream_t s) {
+  if (v.N > 0) {
+    hipLaunchKernelGGL(k_evaluate_jacobian, dim3(blocks_for(v.N)), dim3(kPointThreads), 0, s, v, cams, pts, row_of_obs,
+                       cam_cols, pt_cols, ncols, apply_loss, jac, residuals, part);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;

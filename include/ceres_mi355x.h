@@ -265,6 +265,21 @@ int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const do
                    double* cost_out_host, double* residuals, void* workspace, size_t workspace_bytes,
                    ct_stream_t stream);
 
+/* Evaluation with the Jacobian: Problem.evaluate(compute_jacobian=True)
+ * (problem.py:377-413) with _evaluate_residual_block_internal
+ * (problem.py:486-520) and _active_column_map (problem.py:522-530).  The
+ * residual block of observation o fills rows 2*row_of_obs[o] and
+ * 2*row_of_obs[o]+1 of residuals [N*2] and of jacobian [N*2][num_columns]
+ * (row-major; the caller zero-fills it); camera_columns [C*9] and
+ * point_columns [P*3] give each coordinate's tangent column or -1 (constant
+ * block or coordinate).  apply_loss = 0 gives the uncorrected residuals and
+ * the cost 1/2 |r|^2 (EvaluateOptions.apply_loss_function).  Indices are
+ * checked on the host.                                                      */
+int ct_ba_evaluate_jacobian(const ct_ba_problem* problem, const double* cameras, const double* points,
+                            const int32_t* row_of_obs, const int64_t* camera_columns, const int64_t* point_columns,
+                            int64_t num_columns, int32_t apply_loss, double* jacobian, double* residuals,
+                            double* cost_out_host, void* workspace, size_t workspace_bytes, ct_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched curve fits: y = exp(m x + c), one residual block per sample with a
  * robust loss (examples/robust_curve_fitting.py:80-97; parameter blocks m and

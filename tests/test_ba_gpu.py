@@ -197,6 +197,50 @@ def test_evaluate_matches_oracle_cost_and_residuals():
     np.testing.assert_allclose(res.residuals.cpu().numpy(), ev.r.reshape(-1), rtol=1e-12, atol=1e-9)
 
 
+def test_evaluate_with_jacobian_matches_oracle():
+    """Problem.evaluate(compute_jacobian=True) (problem.py:377-413): columns of the active tangent
+    coordinates in problem order (cameras, then points, as registered), a constant camera and a
+    subset manifold dropping columns; gradient J^T r; the CRS copy; apply_loss_function=False;
+    an explicit parameter-block selection.  Oracle: the float64 evaluation of the same inputs."""
+    data = gen.ba_config("ba6_huber")
+    problem, cams, pts = ba_problem(data)
+    C, P, N = data.num_cameras, data.num_points, data.num_observations
+    problem.SetParameterBlockConstant(problem.parameter_blocks[0])
+    problem.set_manifold(problem.parameter_blocks[1], tc.SubsetManifold(9, [6, 7, 8]))
+    m = oba.BAModel(data)
+    ev, _ = m.evaluate(m.x0(), True)
+    cam_cols = [None] + [list(range(6))] + [list(range(9))] * (C - 2)
+    cols = []  # (camera or point, index, coordinate) per tangent column
+    for c in range(1, C):
+        cols += [("c", c, j) for j in cam_cols[c]]
+    cols += [("p", p, j) for p in range(P) for j in range(3)]
+    J = np.zeros((2 * N, len(cols)))
+    for k, (kind, idx, j) in enumerate(cols):
+        sel = (data.obs_camera == idx) if kind == "c" else (data.obs_point == idx)
+        blk = ev.Jc if kind == "c" else ev.Jp
+        for i in range(2):
+            J[2 * np.nonzero(sel)[0] + i, k] = blk[sel, i, j]
+    r = ev.r.reshape(-1)
+    res = problem.evaluate()
+    assert float(res.cost) == pytest.approx(ev.cost, rel=1e-13)
+    np.testing.assert_allclose(res.residuals.cpu().numpy(), r, rtol=1e-12, atol=1e-9)
+    scale = np.abs(J).max()
+    np.testing.assert_allclose(res.jacobian.cpu().numpy(), J, rtol=1e-9, atol=1e-11 * scale)
+    np.testing.assert_allclose(res.gradient.cpu().numpy(), J.T @ r, rtol=1e-8, atol=1e-9 * np.abs(J.T @ r).max())
+    torch.testing.assert_close(res.crs_jacobian.to_dense(), res.jacobian.cpu(), rtol=0, atol=0)
+    assert res.crs_jacobian.rows[-1] == int((res.jacobian != 0).sum())
+    # no loss: raw residuals, cost 1/2 |r|^2
+    raw = oba.project(data.cameras[data.obs_camera], data.points[data.obs_point]) - data.observations
+    res2 = problem.evaluate(tc.EvaluateOptions(apply_loss_function=False), compute_jacobian=False)
+    np.testing.assert_allclose(res2.residuals.cpu().numpy(), raw.reshape(-1), rtol=1e-12, atol=1e-9)
+    assert float(res2.cost) == pytest.approx(0.5 * float(np.sum(raw * raw)), rel=1e-13)
+    # the points alone, in reverse order
+    sel = [problem.parameter_blocks[C + p] for p in reversed(range(P))]
+    res3 = problem.evaluate(tc.EvaluateOptions(parameter_blocks=sel))
+    perm = [len(cols) - 3 * P + 3 * p + j for p in reversed(range(P)) for j in range(3)]
+    np.testing.assert_allclose(res3.jacobian.cpu().numpy(), J[:, perm], rtol=1e-9, atol=1e-11 * scale)
+
+
 def test_mixed_precision_reaches_fp64_final_cost():
     """Mixed form (FP32 J / J^T J / Schur, FP64 Cholesky, cost, gain ratio, 2 refinement steps) vs plain FP64.
 

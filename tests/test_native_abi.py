@@ -185,3 +185,18 @@ def test_registry_declines_host_tensors_like_the_reference():
     assert L.get_optional_backend("block_schur")(A, b) == "mine"
     L.unregister_rocm_backends()
     assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
+
+
+def test_crs_matrix_round_trip_and_layout():
+    # problem.py:15-41 of the reference: nonzeros row by row, rows = row starts
+    import torch
+
+    m = torch.tensor([[0.0, 2.0, 0.0], [0.0, 0.0, 0.0], [1.5, 0.0, -3.0]], dtype=torch.float64)
+    crs = tc.CRSMatrix.from_dense(m)
+    assert (crs.num_rows, crs.num_cols) == (3, 3)
+    assert crs.rows == [0, 1, 1, 3]
+    assert crs.cols == [1, 0, 2]
+    assert crs.values == [2.0, 1.5, -3.0]
+    assert torch.equal(crs.to_dense(), m)
+    empty = tc.CRSMatrix.from_dense(torch.zeros(2, 4, dtype=torch.float64))
+    assert empty.rows == [0, 0, 0] and empty.cols == [] and torch.equal(empty.to_dense(), torch.zeros(2, 4, dtype=torch.float64))
