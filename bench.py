@@ -88,7 +88,7 @@ def timed_iters(dev, data, *, precision, loop, linear, steps, warmup, refine=2):
 
     def opts_for(n):
         o = tc.SolverOptions(max_num_iterations=n, function_tolerance=0.0, gradient_tolerance=0.0,
-                             parameter_tolerance=0.0,
+                             parameter_tolerance=0.0, preconditioner_type=tc.PreconditionerType.SCHUR_JACOBI,
                              max_num_refinement_iterations=refine if precision == "mixed" else 0)
         return lm_options(o, precision=precision, loop=loop, linear=linear)
 
@@ -127,7 +127,8 @@ def time_to_tol(dev, data, *, precision, loop, linear, max_iters, target):
     from ceres_mi355x.ba import lm_options
 
     o = tc.SolverOptions(max_num_iterations=max_iters, function_tolerance=0.0, gradient_tolerance=0.0,
-                         parameter_tolerance=0.0, max_num_refinement_iterations=2 if precision == "mixed" else 0)
+                         parameter_tolerance=0.0, max_num_refinement_iterations=2 if precision == "mixed" else 0,
+                         preconditioner_type=tc.PreconditionerType.SCHUR_JACOBI)
     opts = lm_options(o, precision=precision, loop=loop, linear=linear)
     run_solve(dev, data, opts)
     rep, rec, wall = run_solve(dev, data, opts)
@@ -157,7 +158,8 @@ def profile_kernels(dev, data, *, precision, linear, iters):
     from ceres_mi355x.ba import lm_options
 
     o = tc.SolverOptions(max_num_iterations=iters, function_tolerance=0.0, gradient_tolerance=0.0,
-                         parameter_tolerance=0.0, max_num_refinement_iterations=2 if precision == "mixed" else 0)
+                         parameter_tolerance=0.0, max_num_refinement_iterations=2 if precision == "mixed" else 0,
+                         preconditioner_type=tc.PreconditionerType.SCHUR_JACOBI)
     opts = lm_options(o, precision=precision, loop="host", linear=linear)
     run_solve(dev, data, opts)  # builds the plan
     handle = dev._plans[tuple(getattr(opts, f) for f, _ in nat.ct_lm_options._fields_)][0]
@@ -337,7 +339,7 @@ def cpu_baseline(data, iterations=2, sample_points=20_000):
     with threadpool_limits(limits=1):
         model = BAModel(sample, linear_solver="iterative_schur")
         opts = OracleOptions(max_num_iterations=iterations, function_tolerance=0.0, gradient_tolerance=0.0,
-                             parameter_tolerance=0.0)
+                             parameter_tolerance=0.0, preconditioner_type="schur_jacobi")
         t0 = time.perf_counter()
         trust_region_solve(model, opts)
         el = time.perf_counter() - t0

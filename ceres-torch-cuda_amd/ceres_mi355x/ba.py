@@ -269,4 +269,10 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     o.trust_region_strategy = 1 if dogleg is not None and dogleg.name == "DOGLEG" else 0
     dtype = getattr(options, "dogleg_type", None)
     o.dogleg_type = 1 if dtype is not None and dtype.name == "SUBSPACE_DOGLEG" else 0
+    pc = getattr(options, "preconditioner_type", None)
+    o.preconditioner = PRECONDITIONERS.get(pc.name, 0) if pc is not None else 0
     return o
+
+
+# ITERATIVE_SCHUR preconditioners on the device (ct_preconditioner)
+PRECONDITIONERS = {"SCHUR_JACOBI": 0, "JACOBI": 1, "IDENTITY": 2}

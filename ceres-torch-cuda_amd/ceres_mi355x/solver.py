@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .ba import BADevice, lm_options
+from .ba import PRECONDITIONERS, BADevice, lm_options
 from .losses import native_loss
 from .problem import EvaluationResult, Problem
 from .types import (
@@ -68,6 +68,10 @@ def _check_supported(options: SolverOptions) -> None:
         raise NotImplementedError("check_gradients is not on the MI355X device path")
     if options.linear_solver_type is LinearSolverType.CGNR:
         raise NotImplementedError("CGNR is not on the MI355X device path")
+    if (options.linear_solver_type is LinearSolverType.ITERATIVE_SCHUR
+            and options.preconditioner_type.name not in PRECONDITIONERS):
+        raise NotImplementedError(f"ITERATIVE_SCHUR on the MI355X device path takes the SCHUR_JACOBI, JACOBI or "
+                                  f"IDENTITY preconditioner, not {options.preconditioner_type.name}")
 
 
 def _linear_kind(options: SolverOptions) -> str:

@@ -45,6 +45,8 @@ int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o->trust_region_strategy == CT_LEVENBERG_MARQUARDT || o->trust_region_strategy == CT_DOGLEG,
                "unknown trust-region strategy");
   CT_CHECK_ARG(o->dogleg_type == CT_TRADITIONAL_DOGLEG || o->dogleg_type == CT_SUBSPACE_DOGLEG, "unknown dogleg type");
+  CT_CHECK_ARG(o->preconditioner >= CT_PRECONDITIONER_SCHUR_JACOBI && o->preconditioner <= CT_PRECONDITIONER_IDENTITY,
+               "unknown preconditioner");
   return CT_OK;
 }
 

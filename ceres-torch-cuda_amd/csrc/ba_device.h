@@ -36,7 +36,7 @@ struct LMState {
   int32_t linear_iterations;
   int32_t strategy;         // ct_trust_region_strategy
   int32_t dogleg_type;      // ct_dogleg_type
-  int32_t pad0;
+  int32_t precond;          // ct_preconditioner
   double dl_cgn, dl_cg;     // DOGLEG step = dl_cgn * (Gauss-Newton step) + dl_cg * g
   uint64_t t0;              // wall clock at start
   uint64_t t_end;           // wall clock at termination

@@ -9,8 +9,9 @@
 //    launch (workgroups take tiles in dependency order and hand them on
 //    through per-tile flags); the same launch forms V = L^-1 tile by tile, so
 //    every solve with the factor is two bandwidth-bound products with V.
-//  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI) preconditioning as
-//    one persistent kernel (linear.py:1216-1256, preconditioner :1156-1182).
+//  * explicit-S PCG with 9x9 block-Jacobi (SCHUR_JACOBI), diagonal (JACOBI) or
+//    no (IDENTITY) preconditioning as one persistent kernel
+//    (linear.py:1216-1256, preconditioners :462-537, 1156-1182).
 #include <utility>
 
 #include "ba_kernels.h"
@@ -912,9 +913,9 @@ __global__ void k_inv_reduce(const double* __restrict__ part, double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// explicit-S PCG with the SCHUR_JACOBI preconditioner, one persistent kernel
-// (linear.py:1216-1256; preconditioner linear.py:1156-1182, applied as in
-// linear.py:69-77).  Workgroup g owns the 9-row camera blocks g, g+G, ...;
+// explicit-S PCG, one persistent kernel (linear.py:1216-1256); preconditioner
+// SCHUR_JACOBI (linear.py:1156-1182, applied as in linear.py:69-77), JACOBI
+// (the inverse diagonal, linear.py:531-537) or IDENTITY (linear.py:54-55).  Workgroup g owns the 9-row camera blocks g, g+G, ...;
 // its 9 wavefronts compute one row each of S p.  Every workgroup keeps the
 // whole search direction p in LDS and updates it identically from the
 // shared z, so an iteration needs two grid barriers (after p.Ap and after
@@ -1091,10 +1092,16 @@ __device__ __forceinline__ bool lu9(double* A, int* piv, double* dinv) {
   return true;
 }
 
-// z = B^-1 r for one block: Cholesky solve (kind 1, in registers) or LU
-// solve (kind 0, rare: worked in place in z, which must not alias r).
+// z = M^-1 r for one block: Cholesky solve (kind 1, in registers), LU solve
+// (kind 0, rare: worked in place in z, which must not alias r), the inverse
+// diagonal (kind 2, JACOBI) or the identity (kind 3).
 __device__ __forceinline__ void block_solve(const double* A, const double* dinv, const int* piv, int kind, const double* r,
                             double* z) {
+  if (kind == 2 || kind == 3) {
+#pragma unroll
+    for (int i = 0; i < kBlk; ++i) z[i] = kind == 2 ? dinv[i] * r[i] : r[i];
+    return;
+  }
   if (kind == 1) {
     double t[kBlk];
 #pragma unroll
@@ -1179,8 +1186,14 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   }
   __syncthreads();
   if (tid < nown) {
-    // Cholesky when the block is SPD, else an LU solve of the block
-    if (chol9(Lf[tid], dv[tid])) {
+    if (st->precond == CT_PRECONDITIONER_JACOBI) {
+      // 1 / max(S_kk, eps) (linear.py:531-537)
+      for (int k = 0; k < kBlk; ++k) dv[tid][k] = 1.0 / fmax(Sd[tid][k * kBlk + k], kEps);
+      kind[tid] = 2;
+    } else if (st->precond == CT_PRECONDITIONER_IDENTITY) {
+      kind[tid] = 3;
+    } else if (chol9(Lf[tid], dv[tid])) {
+      // Cholesky when the block is SPD, else an LU solve of the block
       kind[tid] = 1;
     } else {
       for (int e = 0; e < kBlk * kBlk; ++e) Lf[tid][e] = Sd[tid][e];

@@ -228,6 +228,7 @@ __device__ __forceinline__ void lm_state_init(LMState* st, const ct_lm_options& 
   st->refine_iters = o.max_num_refinement_iterations;
   st->strategy = o.trust_region_strategy;
   st->dogleg_type = o.dogleg_type;
+  st->precond = o.preconditioner;
   st->dl_cgn = 1.0;
   st->dl_cg = 0.0;
   st->radius = o.initial_trust_region_radius;

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 6
+#define CT_ABI_VERSION 7
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -107,6 +107,15 @@ typedef enum ct_dogleg_type {
   CT_SUBSPACE_DOGLEG = 1
 } ct_dogleg_type;
 
+/* ITERATIVE_SCHUR preconditioner (build_schur_complement_preconditioner,
+ * linear.py:462-537): the Cholesky (LU if not SPD) of the 9x9 diagonal camera
+ * blocks of S, the inverse diagonal 1 / max(S_kk, eps), or none.            */
+typedef enum ct_preconditioner {
+  CT_PRECONDITIONER_SCHUR_JACOBI = 0,
+  CT_PRECONDITIONER_JACOBI = 1,
+  CT_PRECONDITIONER_IDENTITY = 2
+} ct_preconditioner;
+
 typedef struct ct_lm_options {
   int32_t max_num_iterations;
   int32_t max_num_consecutive_invalid_steps;
@@ -130,6 +139,7 @@ typedef struct ct_lm_options {
   double eta;
   int32_t trust_region_strategy;        /* ct_trust_region_strategy (bundle adjustment only) */
   int32_t dogleg_type;                  /* ct_dogleg_type */
+  int32_t preconditioner;               /* ct_preconditioner (ITERATIVE_SCHUR) */
 } ct_lm_options;
 
 /* Termination codes match TerminationType (types.py:98-103). */

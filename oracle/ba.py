@@ -319,10 +319,32 @@ def schur_back_substitute(struct: BAStructure, Vinv, W, ga, dxc: np.ndarray) -> 
     return np.einsum("pij,pj->pi", Vinv, ga - tp)
 
 
-def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0):
-    """_preconditioned_conjugate_gradient with SCHUR_JACOBI blocks of ``block``."""
+def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0, preconditioner="schur_jacobi"):
+    """_preconditioned_conjugate_gradient (linear.py:1216-1256) on the explicit Schur complement.
+
+    ``preconditioner`` follows build_schur_complement_preconditioner (linear.py:462-537):
+    "schur_jacobi" - Cholesky (LU if not SPD) of the diagonal blocks of ``block``;
+    "jacobi" - the inverse diagonal, 1 / max(S_kk, eps) (linear.py:531-537);
+    "identity" - none (linear.py:480-481, 54-55).
+    """
     n = rhs.shape[0]
     nb = n // block
+    if preconditioner == "identity":
+        def apply(r):
+            return r
+    elif preconditioner == "jacobi":
+        dinv = 1.0 / np.maximum(np.diag(S), _EPS)
+
+        def apply(r):
+            return dinv * r
+    elif preconditioner == "schur_jacobi":
+        apply = _block_apply(S, nb, block)
+    else:
+        raise ValueError(preconditioner)
+    return _pcg(S, rhs, apply, tol=tol, max_iter=max_iter, min_iter=min_iter)
+
+
+def _block_apply(S, nb, block):
     blocks = S.reshape(nb, block, nb, block)[np.arange(nb), :, np.arange(nb), :]
     factors = []
     for i in range(nb):
@@ -342,6 +364,11 @@ def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0):
                 z[i * block : (i + 1) * block] = np.linalg.solve(f, ri)
         return z
 
+    return apply
+
+
+def _pcg(S, rhs, apply, *, tol, max_iter, min_iter):
+    n = rhs.shape[0]
     x = np.zeros(n)
     r = rhs - S @ x
     z = apply(r)
@@ -540,6 +567,7 @@ class BAModel:
         iters = 1
         if self.linear_solver == "iterative_schur":
             xb, iters, _ = block_jacobi_pcg(S_a, rhs_a, 9, tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
+                                            preconditioner=opts.preconditioner_type,
                                             min_iter=opts.min_linear_solver_iterations)
         else:
             xb = np.linalg.solve(S_a, rhs_a)

@@ -55,6 +55,7 @@ class OracleOptions:
     max_consecutive_nonmonotonic_steps: int = 5
     use_mixed_precision_solves: bool = False
     max_num_refinement_iterations: int = 0
+    preconditioner_type: str = "jacobi"                      # ITERATIVE_SCHUR: "jacobi", "schur_jacobi", "identity"
     trust_region_strategy_type: str = "levenberg_marquardt"  # or "dogleg"
     dogleg_type: str = "traditional"                          # or "subspace"
 
@@ -476,6 +477,7 @@ def dense_linear_solve(J, b, damping, num_eliminate, solver, opts, block_sizes=N
         S = H[e:, e:] - H[e:, :e] @ Haa_inv_Hab
         rhs_s = rhs[e:] - H[e:, :e] @ Haa_inv_ga
         xb, iters, _ = block_jacobi_pcg(S, rhs_s, 9, tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
+                                        preconditioner=opts.preconditioner_type,
                                         min_iter=opts.min_linear_solver_iterations)
         return np.concatenate([Haa_inv_ga - Haa_inv_Hab @ xb, xb]), iters
     # DENSE_QR: damped least squares (linear.py:359-371)

@@ -92,7 +92,7 @@ def _loss(tc, loss):
 
 def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int, store_inputs: bool,
            strategy: str = "levenberg_marquardt", dogleg_type: str = "traditional", constant_cameras=(),
-           initial_radius: float | None = None):
+           initial_radius: float | None = None, preconditioner: str = "schur_jacobi"):
     torch, tc = _import_reference()
     from oracle.generators import ba_config
 
@@ -140,7 +140,7 @@ def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int,
         kwargs["linear_solver_type"] = tc.LinearSolverType.DENSE_SCHUR
     else:
         kwargs["linear_solver_type"] = tc.LinearSolverType.ITERATIVE_SCHUR
-        kwargs["preconditioner_type"] = tc.PreconditionerType.SCHUR_JACOBI
+        kwargs["preconditioner_type"] = tc.PreconditionerType[preconditioner.upper()]
     t0 = time.perf_counter()
     summary = tc.solve(tc.SolverOptions(**kwargs), problem)
     elapsed = time.perf_counter() - t0
@@ -156,6 +156,7 @@ def run_ba(name: str, config: str, solver: str, tolerances: str, max_iters: int,
         dogleg_type=dogleg_type,
         constant_cameras=list(constant_cameras),
         initial_trust_region_radius=initial_radius,
+        preconditioner=preconditioner if solver == "iterative_schur" else None,
         input_sha256=array_digest(d.cameras, d.points, d.observations, d.obs_camera, d.obs_point),
         reference_seconds=elapsed,
         summary=_summary_dict(summary),
@@ -261,6 +262,10 @@ CASES = {
     "ba6_huber_dense_schur": lambda: run_ba("ba6_huber_dense_schur", "ba6_huber", "dense_schur", "default", 50, True),
     "curve_cauchy": lambda: run_curve("curve_cauchy", 3, [0, 5, 11, 17, 23, 31], 30),
     "pose2d_40": lambda: run_pose("pose2d_40", 40, 4, 50),
+    "ba4_iterative_jacobi_zero_tol": lambda: run_ba("ba4_iterative_jacobi_zero_tol", "ba4", "iterative_schur", "zero",
+                                                    25, True, preconditioner="jacobi"),
+    "ba4_iterative_identity_zero_tol": lambda: run_ba("ba4_iterative_identity_zero_tol", "ba4", "iterative_schur",
+                                                      "zero", 25, True, preconditioner="identity"),
     "ba4_dogleg_zero_tol": lambda: run_ba("ba4_dogleg_zero_tol", "ba4", "dense_schur", "zero", 25, True,
                                           strategy="dogleg", constant_cameras=(0, 1), initial_radius=0.05),
     "ba4_subspace_dogleg_zero_tol": lambda: run_ba("ba4_subspace_dogleg_zero_tol", "ba4", "dense_schur", "zero", 25,

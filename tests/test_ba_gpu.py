@@ -19,7 +19,7 @@ from oracle import generators as gen
 from oracle.lm import OracleOptions, trust_region_solve
 
 from conftest import GOLDEN, load_golden
-from test_oracle import assert_trajectory_matches
+from test_oracle import assert_trajectory_matches, long_pcg_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,7 @@ def options_for(golden, **kw):
     if golden.get("tolerances") == "zero":
         o.function_tolerance = o.gradient_tolerance = o.parameter_tolerance = 0.0
     if golden["linear_solver"] == "iterative_schur":
-        o.preconditioner_type = tc.PreconditionerType.SCHUR_JACOBI
+        o.preconditioner_type = tc.PreconditionerType[(golden.get("preconditioner") or "schur_jacobi").upper()]
     return o
 
 
@@ -57,7 +57,8 @@ def _loop_mode_reset():
 
 @pytest.mark.parametrize("loop", ["host", "device"])
 @pytest.mark.parametrize("case", ["ba4_dense_schur_zero_tol", "ba4_dense_schur_default_tol", "ba6_huber_dense_schur",
-                                  "ba4_iterative_schur_zero_tol"])
+                                  "ba4_iterative_schur_zero_tol", "ba4_iterative_jacobi_zero_tol",
+                                  "ba4_iterative_identity_zero_tol"])
 def test_ba_matches_reference_trajectory(case, loop):
     print(f"case={case} loop={loop}", flush=True)
     tc.set_device_loop(loop)
@@ -66,7 +67,7 @@ def test_ba_matches_reference_trajectory(case, loop):
     problem, cams, pts = ba_problem(data)
     summary = tc.solve(options_for(g), problem)
     final_rtol = 1e-7 if "huber" in case else 1e-10
-    assert_trajectory_matches(summary, g["summary"], final_rtol=final_rtol)
+    assert_trajectory_matches(summary, g["summary"], final_rtol=final_rtol, max_exact_pcg=long_pcg_bound(g))
     if g["tolerances"] == "default":
         assert summary.termination_type.value == g["summary"]["termination_type"]
         assert summary.message == g["summary"]["message"]
@@ -74,7 +75,7 @@ def test_ba_matches_reference_trajectory(case, loop):
         assert summary.num_successful_steps == g["summary"]["num_successful_steps"]
         assert summary.num_jacobian_evaluations == g["summary"]["num_jacobian_evaluations"]
         assert summary.num_residual_evaluations == g["summary"]["num_residual_evaluations"]
-    if "huber" not in case:
+    if "huber" not in case and long_pcg_bound(g) is None:
         np.testing.assert_allclose(pts.cpu().numpy(), np.array(g["final_points"]), atol=1e-6)
 
 

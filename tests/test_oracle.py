@@ -10,7 +10,7 @@ formula:
 | losses.loss_rho                         | reference KATs (tests/test_losses.py:6-19), finite differences of rho0 |
 | losses.robustify                        | Triggs identity J^T r = rho1 J0^T r0 (tests/test_losses.py:31-46) |
 | ba.schur_system / back-substitution     | damped least squares [J; sqrt D] (textbook identity), reference 6x4 KAT (tests/test_linear_schur.py:109-134) |
-| ba.block_jacobi_pcg                     | reference KAT (tests/test_linear_schur.py:137-174), exact CG termination |
+| ba.block_jacobi_pcg                     | reference KAT (tests/test_linear_schur.py:137-174), exact CG termination; JACOBI on a diagonal system in one step; IDENTITY in at most (distinct eigenvalues) steps |
 | lm.LMRadius / StepEvaluator             | reference KATs (tests/test_problem_solver.py:551-577) |
 | lm.updated_trust_region_radius (DOGLEG) | known answers of each branch of solver.py:1051-1061 |
 | lm.traditional/subspace_dogleg_step     | brute force: the boundary crossing of the dogleg path on a fine grid; the model minimum over a polar grid of the 2-D subspace disc; normal equations for the interior step |
@@ -291,16 +291,25 @@ def test_tiny_bal_known_answer():
 # ---------------------------------------------------------------------------
 
 
-def assert_trajectory_matches(summary, golden_summary, *, final_rtol=1e-10, cost_rtol=1e-9):
+def assert_trajectory_matches(summary, golden_summary, *, final_rtol=1e-10, cost_rtol=1e-9, max_exact_pcg=None):
     """Accept/reject sequence and costs agree until the cost has converged.
 
     Once the reference's cost is within 1e-11 (relative) of its final value,
     every remaining gain ratio is rounding noise in both implementations, so
     the comparison stops there; the final cost must agree to ``final_rtol``.
+
+    ``max_exact_pcg``: preconditioned CG that runs many iterations (the
+    JACOBI and IDENTITY preconditioners) amplifies rounding differences in the
+    Krylov recurrence - the numpy restatement of the reference's own dense
+    algebra lands 2.5e-6 away in the cost after a 19-iteration solve - so the
+    comparison also stops after the first PCG solve longer than this.
     """
     gits = golden_summary["iterations"]
     final = golden_summary["final_cost"]
     conv = next((k for k, it in enumerate(gits) if abs(it["cost"] - final) <= 1e-11 * abs(final)), len(gits))
+    if max_exact_pcg is not None:
+        long_solve = next((k for k, it in enumerate(gits) if it["linear_solver_iterations"] > max_exact_pcg), len(gits))
+        conv = min(conv, long_solve + 1)
     mine = summary.iterations if isinstance(summary.iterations[0], dict) else [vars(i) for i in summary.iterations]
     assert len(mine) >= min(conv, len(gits)), (len(mine), conv)
     for k in range(min(conv, len(gits))):
@@ -309,22 +318,31 @@ def assert_trajectory_matches(summary, golden_summary, *, final_rtol=1e-10, cost
         assert bool(a["step_is_successful"]) == b["step_is_successful"], k
         assert a["cost"] == pytest.approx(b["cost"], rel=cost_rtol), k
         assert a["trust_region_radius"] == pytest.approx(b["trust_region_radius"], rel=1e-3), k
-        if "linear_solver_iterations" in b and "linear_solver_iterations" in a:
+        if "linear_solver_iterations" in b and "linear_solver_iterations" in a and (
+                max_exact_pcg is None or b["linear_solver_iterations"] <= max_exact_pcg):
             # PCG iteration counts (1 for direct solves) are part of the trajectory
             assert int(a["linear_solver_iterations"]) == int(b["linear_solver_iterations"]), k
     assert summary.final_cost == pytest.approx(final, rel=final_rtol)
     assert summary.initial_cost == pytest.approx(golden_summary["initial_cost"], rel=1e-12)
 
 
+def long_pcg_bound(g):
+    """PCG length up to which trajectories are compared exactly (None: all) - see assert_trajectory_matches."""
+    return 10 if g.get("preconditioner") in ("jacobi", "identity") else None
+
+
 def _ba_opts(g):
     o = OracleOptions(max_num_iterations=g["max_num_iterations"])
+    if g["linear_solver"] == "iterative_schur":
+        o.preconditioner_type = g.get("preconditioner") or "schur_jacobi"
     if g["tolerances"] == "zero":
         o.function_tolerance = o.gradient_tolerance = o.parameter_tolerance = 0.0
     return o
 
 
 @pytest.mark.parametrize("case", ["ba4_dense_schur_zero_tol", "ba4_dense_schur_default_tol", "ba6_huber_dense_schur",
-                                  "ba4_iterative_schur_zero_tol"])
+                                  "ba4_iterative_schur_zero_tol", "ba4_iterative_jacobi_zero_tol",
+                                  "ba4_iterative_identity_zero_tol"])
 @pytest.mark.parametrize("faithful", [False, True])
 def test_ba_trajectory_matches_reference(case, faithful):
     g = load_golden(case)
@@ -336,12 +354,13 @@ def test_ba_trajectory_matches_reference(case, faithful):
     # as the reference, numpy LAPACK instead of torch) lands 2e-9 away, so the
     # final-cost bound there is 1e-7; trivial-loss cases are held to 1e-10.
     final_rtol = 1e-7 if "huber" in case else 1e-10
-    assert_trajectory_matches(s, g["summary"], final_rtol=final_rtol)
+    assert_trajectory_matches(s, g["summary"], final_rtol=final_rtol, max_exact_pcg=long_pcg_bound(g))
     if g["tolerances"] == "default":
         assert s.termination_type == g["summary"]["termination_type"]
         assert s.message == g["summary"]["message"]
         assert len(s.iterations) == len(g["summary"]["iterations"])
-    if g.get("final_points") is not None and "huber" not in case:
+    if g.get("final_points") is not None and "huber" not in case and long_pcg_bound(g) is None:
+        # (after a trajectory that differs by rounding, the 7-dof gauge of the final state drifts)
         # BA has a 7-dof gauge freedom; compare reprojection-level agreement
         np.testing.assert_allclose(pts, np.array(g["final_points"]), atol=1e-6)
 
@@ -504,3 +523,21 @@ def test_ba_dogleg_trajectory_matches_reference(case, faithful):
     if g["tolerances"] == "default":
         assert s.termination_type == g["summary"]["termination_type"]
         assert len(s.iterations) == len(g["summary"]["iterations"])
+
+
+def test_pcg_jacobi_and_identity_preconditioners():
+    # JACOBI (linear.py:531-537) is exact on a diagonal system: one iteration
+    rng = np.random.default_rng(8)
+    d = rng.uniform(0.5, 4.0, size=18)
+    b = rng.normal(size=18)
+    x, it, ok = oba.block_jacobi_pcg(np.diag(d), b, 9, tol=1e-14, max_iter=50, preconditioner="jacobi")
+    assert ok and it == 1
+    np.testing.assert_allclose(x, b / d, rtol=1e-14)
+    # IDENTITY (plain CG) ends in at most as many steps as the matrix has distinct eigenvalues
+    Q, _ = np.linalg.qr(rng.normal(size=(18, 18)))
+    S = (Q * np.repeat([1.0, 3.0, 7.0], 6)) @ Q.T
+    x, it, ok = oba.block_jacobi_pcg(S, b, 9, tol=1e-12, max_iter=50, preconditioner="identity")
+    assert ok and it <= 3
+    np.testing.assert_allclose(S @ x, b, rtol=1e-10, atol=1e-10)
+    with pytest.raises(ValueError):
+        oba.block_jacobi_pcg(S, b, 9, tol=1e-12, max_iter=5, preconditioner="cluster_jacobi")
