@@ -269,6 +269,7 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     o.trust_region_strategy = 1 if dogleg is not None and dogleg.name == "DOGLEG" else 0
     dtype = getattr(options, "dogleg_type", None)
     o.dogleg_type = 1 if dtype is not None and dtype.name == "SUBSPACE_DOGLEG" else 0
+    o.max_solver_time_in_seconds = float(getattr(options, "max_solver_time_in_seconds", 1e9))
     pc = getattr(options, "preconditioner_type", None)
     o.preconditioner = PRECONDITIONERS.get(pc.name, 0) if pc is not None else 0
     return o

@@ -40,6 +40,7 @@ _MESSAGES = {
     6: "Too many invalid trust-region steps.",
     7: "No active parameters.",
     8: "Invalid problem input.",
+    9: "Maximum solver time reached.",
 }
 
 # Device-path settings with no counterpart in SolverOptions.
@@ -282,8 +283,39 @@ def summary_from_report(options: SolverOptions, problem: Problem, rep, records, 
     return s
 
 
+def _eliminated_blocks(problem: Problem) -> set:
+    """Blocks the reference's Schur solvers eliminate: the first ordering group of the active blocks
+    (_linear_solver_parameter_order and _num_eliminate_for_schur, solver.py:714-741), none when no
+    group is set or all active blocks share one."""
+    blocks = [b for b in problem.parameter_blocks if b.tangent_size > 0]
+    if not blocks or not any(b.ordering_group is not None for b in blocks):
+        return set()
+    max_group = max((b.ordering_group for b in blocks if b.ordering_group is not None), default=0)
+    index = {id(b): i for i, b in enumerate(blocks)}
+    order = sorted(blocks, key=lambda b: (b.ordering_group if b.ordering_group is not None else max_group + 1,
+                                          index[id(b)]))
+    first = order[0].ordering_group
+    if first is None or all(b.ordering_group == first for b in order):
+        return set()
+    return {id(b) for b in order if b.ordering_group == first}
+
+
+def _check_elimination(problem: Problem, ex: "_BAExtract", options: SolverOptions) -> None:
+    """ITERATIVE_SCHUR runs PCG on the Schur complement of the points, the device's one form; the
+    reference eliminates its first ordering group, and with none runs CG on the normal equations
+    (linear.py:191-257), a different iteration.  The exact solvers give the same step in any order."""
+    if (options.linear_solver_type is not LinearSolverType.ITERATIVE_SCHUR
+            or options.trust_region_strategy_type is TrustRegionStrategyType.DOGLEG):
+        return
+    points = {id(b) for b in ex.pt_blocks if b.tangent_size > 0}
+    if not points or _eliminated_blocks(problem) != points:
+        raise NotImplementedError("ITERATIVE_SCHUR on the MI355X device path eliminates exactly the point blocks: "
+                                  "put every variable point (and nothing else) in the first ordering group")
+
+
 def _solve_ba(options: SolverOptions, problem: Problem, start: float) -> SolverSummary:
     dev, ex = _device_for(problem)
+    _check_elimination(problem, ex, options)
     precision = "mixed" if options.use_mixed_precision_solves else "fp64"
     opts = lm_options(options, precision=precision, loop=DEVICE_SETTINGS["loop"], linear=_linear_kind(options),
                       iterations_per_graph=DEVICE_SETTINGS["iterations_per_graph"])

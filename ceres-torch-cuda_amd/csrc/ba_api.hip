@@ -42,6 +42,7 @@ int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o->precision == CT_PLAIN_FP64 || o->precision == CT_MIXED, "unknown precision");
   CT_CHECK_ARG(o->loop_mode == CT_LOOP_HOST || o->loop_mode == CT_LOOP_DEVICE, "unknown loop mode");
   CT_CHECK_ARG(o->initial_trust_region_radius > 0.0, "initial_trust_region_radius must be > 0");
+  CT_CHECK_ARG(o->max_solver_time_in_seconds >= 0.0, "max_solver_time_in_seconds must be >= 0");
   CT_CHECK_ARG(o->trust_region_strategy == CT_LEVENBERG_MARQUARDT || o->trust_region_strategy == CT_DOGLEG,
                "unknown trust-region strategy");
   CT_CHECK_ARG(o->dogleg_type == CT_TRADITIONAL_DOGLEG || o->dogleg_type == CT_SUBSPACE_DOGLEG, "unknown dogleg type");

@@ -38,6 +38,9 @@ struct LMState {
   int32_t dogleg_type;      // ct_dogleg_type
   int32_t precond;          // ct_preconditioner
   double dl_cgn, dl_cg;     // DOGLEG step = dl_cgn * (Gauss-Newton step) + dl_cg * g
+  double max_time;          // seconds of device clock before NO_CONVERGENCE
+  int32_t pending_time;     // the time test after an accepted step runs with the deferred checks
+  int32_t pad1;
   uint64_t t0;              // wall clock at start
   uint64_t t_end;           // wall clock at termination
 };

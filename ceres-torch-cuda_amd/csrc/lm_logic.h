@@ -229,6 +229,8 @@ __device__ __forceinline__ void lm_state_init(LMState* st, const ct_lm_options& 
   st->strategy = o.trust_region_strategy;
   st->dogleg_type = o.dogleg_type;
   st->precond = o.preconditioner;
+  st->max_time = o.max_solver_time_in_seconds;
+  st->pending_time = 0;
   st->dl_cgn = 1.0;
   st->dl_cg = 0.0;
   st->radius = o.initial_trust_region_radius;
@@ -275,6 +277,13 @@ __device__ __forceinline__ void lm_begin_update(LMState* st, double cost, double
     st->pending_radius = 0;
     if (st->radius <= st->min_radius) {  // solver.py:338-341
       lm_terminate(st, CT_TERM_CONVERGENCE, CT_REASON_MIN_RADIUS);
+      return;
+    }
+  }
+  if (st->pending_time) {
+    st->pending_time = 0;
+    if (lm_now(st) >= st->max_time) {  // solver.py:342-345
+      lm_terminate(st, CT_TERM_NO_CONVERGENCE, CT_REASON_MAX_TIME);
       return;
     }
   }
@@ -399,8 +408,12 @@ __device__ __forceinline__ void lm_end_update(LMState* st, double cand, double m
     st->has_prev = 1;
     st->pending_grad = 1;
     st->pending_radius = 1;
+    st->pending_time = 1;
   } else if (st->radius <= st->min_radius) {
     lm_terminate(st, CT_TERM_CONVERGENCE, CT_REASON_MIN_RADIUS);
+    return;
+  } else if (lm_now(st) >= st->max_time) {  // solver.py:342-345
+    lm_terminate(st, CT_TERM_NO_CONVERGENCE, CT_REASON_MAX_TIME);
     return;
   }
   st->iteration += 1;

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 7
+#define CT_ABI_VERSION 8
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -140,6 +140,8 @@ typedef struct ct_lm_options {
   int32_t trust_region_strategy;        /* ct_trust_region_strategy (bundle adjustment only) */
   int32_t dogleg_type;                  /* ct_dogleg_type */
   int32_t preconditioner;               /* ct_preconditioner (ITERATIVE_SCHUR) */
+  double max_solver_time_in_seconds;    /* device clock from the start of the solve (solver.py:342-345);
+                                           SolverOptions' default is 1e9, and 0 stops after the first step */
 } ct_lm_options;
 
 /* Termination codes match TerminationType (types.py:98-103). */
@@ -159,7 +161,8 @@ typedef enum ct_reason {
   CT_REASON_MAX_ITERATIONS = 5,   /* "Maximum iterations reached." */
   CT_REASON_INVALID_STEPS = 6,    /* "Too many invalid trust-region steps." */
   CT_REASON_NO_PARAMETERS = 7,    /* "No active parameters." */
-  CT_REASON_INVALID_INPUT = 8     /* batched problems: malformed sample range */
+  CT_REASON_INVALID_INPUT = 8,    /* batched problems: malformed sample range */
+  CT_REASON_MAX_TIME = 9          /* "Maximum solver time reached." */
 } ct_reason;
 
 /* Per-iteration record layout (doubles), mirroring IterationSummary

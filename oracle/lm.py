@@ -55,6 +55,7 @@ class OracleOptions:
     max_consecutive_nonmonotonic_steps: int = 5
     use_mixed_precision_solves: bool = False
     max_num_refinement_iterations: int = 0
+    max_solver_time_in_seconds: float = 1e9
     preconditioner_type: str = "jacobi"                      # ITERATIVE_SCHUR: "jacobi", "schur_jacobi", "identity"
     trust_region_strategy_type: str = "levenberg_marquardt"  # or "dogleg"
     dogleg_type: str = "traditional"                          # or "subspace"
@@ -270,6 +271,9 @@ def trust_region_solve(model, opts: OracleOptions):
 
     Returns (summary, x_final) where x_final is the best state (solver.py:672).
     """
+    import time
+
+    start = time.perf_counter()
     s = OracleSummary()
     x = model.x0()
     lm = LMRadius(opts.initial_trust_region_radius, opts.max_trust_region_radius)
@@ -395,6 +399,10 @@ def trust_region_solve(model, opts: OracleOptions):
                 break
         if radius <= opts.min_trust_region_radius:
             s.termination_type, s.message = "CONVERGENCE", "Minimum trust-region radius reached."
+            finished = True
+            break
+        if time.perf_counter() - start >= opts.max_solver_time_in_seconds:  # solver.py:342-345
+            s.termination_type, s.message = "NO_CONVERGENCE", "Maximum solver time reached."
             finished = True
             break
     if not finished:

@@ -145,6 +145,23 @@ def test_ba_dogleg_matches_reference_trajectory(case, loop):
     torch.testing.assert_close(cams[:2], cams0[:2], rtol=0, atol=0)
 
 
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_max_solver_time_ends_the_solve_like_the_reference(loop):
+    # solver.py:342-345: the time test follows the radius test of every iteration with a step; with
+    # a zero budget the solve ends after the first step, NO_CONVERGENCE (same record count as the oracle)
+    tc.set_device_loop(loop)
+    data = gen.ba_config("ba4")
+    problem, _, _ = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=25, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
+                         max_solver_time_in_seconds=0.0)
+    summary = tc.solve(o, problem)
+    ref, _ = trust_region_solve(oba.BAModel(data), OracleOptions(max_num_iterations=25, max_solver_time_in_seconds=0.0))
+    assert summary.termination_type == tc.TerminationType.NO_CONVERGENCE
+    assert summary.message == ref.message == "Maximum solver time reached."
+    assert len(summary.iterations) == len(ref.iterations) == 2
+    assert summary.final_cost == pytest.approx(ref.final_cost, rel=1e-10)
+
+
 def test_tiny_bal_with_constant_cameras():
     # examples/simple_bundle_adjuster.py:142-176; tests/test_examples.py:151-165
     true_cams = np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 800.0, 0.0, 0.0],

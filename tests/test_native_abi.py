@@ -200,3 +200,33 @@ def test_crs_matrix_round_trip_and_layout():
     assert torch.equal(crs.to_dense(), m)
     empty = tc.CRSMatrix.from_dense(torch.zeros(2, 4, dtype=torch.float64))
     assert empty.rows == [0, 0, 0] and empty.cols == [] and torch.equal(empty.to_dense(), torch.zeros(2, 4, dtype=torch.float64))
+
+
+def test_iterative_schur_needs_the_points_as_the_eliminated_group():
+    # solver.py:714-741: the reference eliminates the first ordering group; without groups
+    # ITERATIVE_SCHUR is CG on the normal equations (linear.py:248-257), which the device path refuses
+    import torch
+
+    from ceres_mi355x.solver import _BAExtract, _check_elimination, _eliminated_blocks
+
+    cams = torch.zeros(2, 9, dtype=torch.float64)
+    pts = torch.zeros(3, 3, dtype=torch.float64)
+    oc, op = np.array([0, 1, 0, 1, 0, 1]), np.array([0, 0, 1, 1, 2, 2])
+    obs = np.zeros((6, 2))
+    opts = tc.SolverOptions(linear_solver_type=tc.LinearSolverType.ITERATIVE_SCHUR,
+                            preconditioner_type=tc.PreconditionerType.SCHUR_JACOBI)
+    problem = tc.Problem()
+    problem.add_bundle_adjustment_observations(cams, pts, oc, op, obs)
+    ex = _BAExtract(problem)
+    assert _eliminated_blocks(problem) == {id(b) for b in ex.pt_blocks}
+    _check_elimination(problem, ex, opts)
+    problem2 = tc.Problem()
+    problem2.add_bundle_adjustment_observations(cams, pts, oc, op, obs, point_ordering_group=None)
+    assert _eliminated_blocks(problem2) == set()
+    with pytest.raises(NotImplementedError, match="ordering group"):
+        _check_elimination(problem2, _BAExtract(problem2), opts)
+    problem2.SetParameterBlockOrderingGroup(problem2.parameter_blocks[0], 0)  # a camera first
+    with pytest.raises(NotImplementedError, match="ordering group"):
+        _check_elimination(problem2, _BAExtract(problem2), opts)
+    # exact solvers take any order
+    _check_elimination(problem2, _BAExtract(problem2), tc.SolverOptions(linear_solver_type=tc.LinearSolverType.DENSE_SCHUR))
