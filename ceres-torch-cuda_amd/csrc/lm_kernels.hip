@@ -15,7 +15,27 @@ namespace ct {
 
 namespace {
 
-constexpr int kLMThreads = 256;
+constexpr int kLMThreads = 1024;
+constexpr int kLMBatch = 4;  // partials per thread whose loads are issued before their adds
+
+// Fixed-order partial sums over a strided range: thread t takes i = t, t + T, ...
+// in batches of kLMBatch (all loads of a batch in flight together).
+template <int kWidth>
+__device__ __forceinline__ void strided_sums(const double* __restrict__ part, int count, double (&acc)[kWidth]) {
+  for (int i0 = threadIdx.x; i0 < count; i0 += kLMBatch * kLMThreads) {
+    double v[kLMBatch][kWidth];
+#pragma unroll
+    for (int q = 0; q < kLMBatch; ++q) {
+      const int i = i0 + q * kLMThreads;
+#pragma unroll
+      for (int k = 0; k < kWidth; ++k) v[q][k] = i < count ? part[(size_t)kWidth * i + k] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kLMBatch; ++q)
+#pragma unroll
+      for (int k = 0; k < kWidth; ++k) acc[k] += v[q][k];
+  }
+}
 
 __global__ void k_lm_init(LMInit init, BAWork w, const double* __restrict__ ucams, const double* __restrict__ upts) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,17 +49,27 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   __shared__ double red[kLMThreads / kWave];
   LMState* st = w.state;
   if (st->done) return;
-  double c = 0.0, g2 = 0.0, gm = 0.0;
-  for (int i = threadIdx.x; i < init.num_cost_blocks; i += kLMThreads) c += w.part_cost[i];
-  for (int i = threadIdx.x; i < init.num_eval_blocks; i += kLMThreads) {
-    g2 += w.part_eval[2 * i];
-    gm = fmax(gm, w.part_eval[2 * i + 1]);
+  double c[1] = {0.0}, g2 = 0.0, gm = 0.0;
+  strided_sums<1>(w.part_cost, init.num_cost_blocks, c);
+  for (int i0 = threadIdx.x; i0 < init.num_eval_blocks; i0 += kLMBatch * kLMThreads) {
+    double v[kLMBatch][2];
+#pragma unroll
+    for (int q = 0; q < kLMBatch; ++q) {
+      const int i = i0 + q * kLMThreads;
+      v[q][0] = i < init.num_eval_blocks ? w.part_eval[2 * i] : 0.0;
+      v[q][1] = i < init.num_eval_blocks ? w.part_eval[2 * i + 1] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kLMBatch; ++q) {
+      g2 += v[q][0];
+      gm = fmax(gm, v[q][1]);
+    }
   }
   for (int i = threadIdx.x; i < init.num_cameras; i += kLMThreads) {
     g2 += w.part_cam[2 * i];
     gm = fmax(gm, w.part_cam[2 * i + 1]);
   }
-  const double cost = block_sum<kLMThreads>(c, red);
+  const double cost = block_sum<kLMThreads>(c[0], red);
   const double gnorm2 = block_sum<kLMThreads>(g2, red);
   const double gmax = block_max<kLMThreads>(gm, red);
   if (threadIdx.x != 0) return;
@@ -59,16 +89,10 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
     if (threadIdx.x == 0) lm_skip_update(st);
     return;
   }
-  double a[5] = {0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < init.num_obs_blocks; i += kLMThreads) {
-    a[0] += w.part_cand[2 * i];
-    a[1] += w.part_cand[2 * i + 1];
-  }
-  for (int i = threadIdx.x; i < init.num_step_blocks; i += kLMThreads) {
-    a[2] += w.part_step[3 * i];
-    a[3] += w.part_step[3 * i + 1];
-    a[4] += w.part_step[3 * i + 2];
-  }
+  double cand[2] = {0.0, 0.0}, step[3] = {0.0, 0.0, 0.0};
+  strided_sums<2>(w.part_cand, init.num_obs_blocks, cand);
+  strided_sums<3>(w.part_step, init.num_step_blocks, step);
+  const double a[5] = {cand[0], cand[1], step[0], step[1], step[2]};
   double s[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) s[k] = block_sum<kLMThreads>(a[k], red);
