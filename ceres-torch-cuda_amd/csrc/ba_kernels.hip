@@ -523,9 +523,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
 //   kind 1 (observations of camera i, diagonal blocks only):
 //                          sum Jc^T Jc, and rhs = sum Jc^T e, g_c = sum Jc^T r,
 //                          column norms diag(sum Jc^T Jc) in FP64.
-// The slice is processed in rounds of kStageTerms terms.  Staging: lane t gathers term
-// t (its own independent loads, all in flight together), forms
-// T = (P_a P_b^T) Jc_b once, and stores Jc_a and T in LDS.  Accumulation: 63
+// The slice is processed in rounds of kStageTerms terms.  Staging: lanes 2t and
+// 2t+1 gather term t's two records (their own independent loads, all in flight
+// together), swap P through a lane shuffle, and store Jc_a and
+// T = (P_a P_b^T) Jc_b in LDS.  Accumulation: 63
 // lanes form 7 groups of 3x3 sub-block owners; group g takes staged terms
 // g, g+7, ...  The 7 partial blocks are summed in LDS in a fixed order and
 // written as the item's partial.  Splitting blocks into items bounds the
@@ -571,8 +572,8 @@ __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real
   double rv[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
   for (int base = it.begin; base < it.end; base += kStageTerms) {
     const int cnt = min(kStageTerms, it.end - base);
-    if (tid < cnt) {
-      if (diag_obs) {
+    if (diag_obs) {
+      if (tid < cnt) {
         const int o = v.camera_obs[base + tid];
         const real* J = b.JP + kJPStride * (size_t)o;
 #pragma unroll
@@ -585,30 +586,46 @@ __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real
         sV[tid][1] = (double)b.e[2 * (size_t)o + 1];
         sV[tid][2] = b.r[2 * (size_t)o];
         sV[tid][3] = b.r[2 * (size_t)o + 1];
+      }
+    } else {
+      // lane pair (2t, 2t + 1) stages term t: the even lane gathers record a,
+      // the odd lane record b, and the pair swaps P through a lane shuffle
+      const int t = tid >> 1, side = tid & 1;
+      const bool live = t < cnt;
+      real jrec[18], pown[6];
+      if (live) {
+        const int o = v.block_terms[2 * (size_t)(base + t) + side];
+        const real* rec = b.JP + kJPStride * (size_t)o;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) jrec[k] = rec[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pown[k] = rec[18 + k];
       } else {
-        const int oa = v.block_terms[2 * (size_t)(base + tid)];
-        const int ob = v.block_terms[2 * (size_t)(base + tid) + 1];
-        const real* Pa = b.JP + kJPStride * (size_t)oa + 18;
-        const real* Pb = b.JP + kJPStride * (size_t)ob + 18;
-        real pa[6], pb[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          pa[k] = Pa[k];
-          pb[k] = Pb[k];
-        }
-        const real m00 = pa[0] * pb[0] + pa[1] * pb[1] + pa[2] * pb[2];
-        const real m01 = pa[0] * pb[3] + pa[1] * pb[4] + pa[2] * pb[5];
-        const real m10 = pa[3] * pb[0] + pa[4] * pb[1] + pa[5] * pb[2];
-        const real m11 = pa[3] * pb[3] + pa[4] * pb[4] + pa[5] * pb[5];
-        const real* Ja = b.JP + kJPStride * (size_t)oa;
-        const real* Jb = b.JP + kJPStride * (size_t)ob;
+        for (int k = 0; k < 18; ++k) jrec[k] = 0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          const real b0 = Jb[k], b1 = Jb[9 + k];
-          sR[tid][k] = m00 * b0 + m01 * b1;
-          sR[tid][9 + k] = m10 * b0 + m11 * b1;
-          sL[tid][k] = Ja[k];
-          sL[tid][9 + k] = Ja[9 + k];
+        for (int k = 0; k < 6; ++k) pown[k] = 0;
+      }
+      real pother[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) pother[k] = __shfl_xor(pown[k], 1, kWave);
+      if (live) {
+        if (side == 0) {
+#pragma unroll
+          for (int k = 0; k < 18; ++k) sL[t][k] = jrec[k];
+        } else {
+          const real* pa = pother;
+          const real* pb = pown;
+          const real m00 = pa[0] * pb[0] + pa[1] * pb[1] + pa[2] * pb[2];
+          const real m01 = pa[0] * pb[3] + pa[1] * pb[4] + pa[2] * pb[5];
+          const real m10 = pa[3] * pb[0] + pa[4] * pb[1] + pa[5] * pb[2];
+          const real m11 = pa[3] * pb[3] + pa[4] * pb[4] + pa[5] * pb[5];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const real b0 = jrec[k], b1 = jrec[9 + k];
+            sR[t][k] = m00 * b0 + m01 * b1;
+            sR[t][9 + k] = m10 * b0 + m11 * b1;
+          }
         }
       }
     }
