@@ -99,7 +99,6 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   const size_t fused_blocks = ((size_t)P + kFusedThreads / kWave - 1) / (kFusedThreads / kWave);
   L.w.part_cost = cv.take<double>(std::max((size_t)obs_blocks(p), fused_blocks) + 1);
   L.w.part_eval = cv.take<double>(2 * std::max((size_t)eval_blocks(p), fused_blocks) + 2);
-  L.w.part_cam = cv.take<double>(2 * C + 2);
   L.w.part_step = cv.take<double>(3 * std::max((size_t)step_blocks(p), fused_blocks) + 3);
   L.w.part_cand = cv.take<double>(2 * std::max((size_t)obs_blocks(p), fused_blocks) + 2);
   L.w.part_dogleg = cv.take<double>(6 * (size_t)dogleg_blocks((int)P) + 6);
@@ -438,15 +437,22 @@ int build_items(ct_ba_plan* pl) {
   if (C > 0 && p.num_observations > 0)
     CT_HIP(hipMemcpy(co.data(), p.camera_offsets, sizeof(int32_t) * (C + 1), hipMemcpyDeviceToHost));
   CT_TRY(validate_indices(p, co));
+  // the assembly indexes blocks by camera pair: every pair i <= j, row-major
+  if ((long long)B != (long long)C * (C + 1) / 2) {
+    set_error("num_blocks must be C (C + 1) / 2: one block per camera pair i <= j");
+    return CT_ERR_ARGUMENT;
+  }
   std::vector<BlockItem> items;
   std::vector<int32_t> offsets(B + 1, 0);
+  int expect_i = 0, expect_j = 0;
   for (int blk = 0; blk < B; ++blk) {
     offsets[blk] = (int32_t)items.size();
     const int ci = bc[2 * (size_t)blk], cj = bc[2 * (size_t)blk + 1];
-    if (ci < 0 || ci >= C || cj < ci || cj >= C) {
-      set_error("block_cameras must hold pairs i <= j of valid cameras");
+    if (ci != expect_i || cj != expect_j) {
+      set_error("block_cameras must list every camera pair i <= j in row-major order");
       return CT_ERR_ARGUMENT;
     }
+    if (++expect_j == C) expect_j = ++expect_i;
     if (bo[blk + 1] < bo[blk] || bo[blk] < 0 || bo[blk + 1] > p.num_terms) {
       set_error("block_offsets must be non-decreasing within [0, num_terms]");
       return CT_ERR_ARGUMENT;

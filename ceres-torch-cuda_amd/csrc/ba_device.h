@@ -100,7 +100,6 @@ struct BAWork {
   double* dxp;        // [P*3]
   double* part_cost;  // [Gobs]   cost at x
   double* part_eval;  // [Gpt][2] |g_p|^2, max|g_p|
-  double* part_cam;   // [C][2]   |g_c|^2, max|g_c|
   double* part_step;  // [Gstep][3] |dx|^2, |x'|^2, non-finite count
   double* part_cand;  // [Gobs][2] candidate cost, model decrease
   double* part_dogleg;  // [Gpt][6] DOGLEG dot products (see k_dogleg_partials)

@@ -697,72 +697,94 @@ __global__ __launch_bounds__(kWave) void k_camera_items(BAView v, BABuffers<real
 }
 
 // ---------------------------------------------------------------------------
-// K2b: one workgroup per camera-pair block: sum the block's item partials in
-// item order and write
-//   S_ij = d_ij (U_i + D_i) - sum Jc_a^T (P_a P_b^T) Jc_b   (both triangles)
-// and, for diagonal blocks, rhs_i, g_c, D_i (linear.py:539-551) and the
-// camera gradient norm partials.
+// K2b: one thread per element of S, by 9x9 blocks in row-major order over the
+// whole C x C block grid: sum the item partials of the block's camera pair
+// (i, j), i <= j, in item order and write
+//   S_ij = d_ij (U_i + D_i) - sum Jc_a^T (P_a P_b^T) Jc_b
+// (a block below the diagonal takes the transposed element of its mirror),
+// and, for diagonal blocks, rhs_i, g_c and D_i (linear.py:539-551).
+// Consecutive workgroups fill consecutive stretches of the same rows of S,
+// so every store line is written whole.  A diagonal block has one item per
+// kItemMaxWork observations and per kItemMaxWork terms; the item loads are
+// issued in batches ahead of their adds (which stay in item order).
 // ---------------------------------------------------------------------------
-constexpr int kReduceThreads = 128;
+constexpr int kReduceBlocks = 3;     // 9x9 blocks of S per workgroup, 81 threads each
+constexpr int kReduceThreads = 256;  // the last 13 threads idle
+static_assert(81 * kReduceBlocks <= kReduceThreads, "reduce workgroup too small");
+constexpr int kSumBatch = 8;
+
+// Ordered sums over a block's items [i0, i1), loads issued kSumBatch items
+// ahead of the adds: s = sum item_S[81 k + e]; a = sum over the observation
+// items (kind 1) of item_vec[27 k + ao] when ao >= 0.
+template <typename real>
+__device__ __forceinline__ void item_sums(const BABuffers<real>& b, const BAWork& w, int i0, int i1, int e, int ao,
+                                          real& s, double& a) {
+  s = 0;
+  a = 0.0;
+  for (int k0 = i0; k0 < i1; k0 += kSumBatch) {
+    real v[kSumBatch];
+    double va[kSumBatch];
+    int kind[kSumBatch];
+#pragma unroll
+    for (int q = 0; q < kSumBatch; ++q) {
+      const int k = k0 + q;
+      const bool in = k < i1;
+      v[q] = in ? b.item_S[81 * (size_t)k + e] : real(0);
+      // every item has a vector slot; only the observation items' are summed
+      kind[q] = in && ao >= 0 ? w.items[k].kind : 0;
+      va[q] = in && ao >= 0 ? w.item_vec[27 * (size_t)k + ao] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kSumBatch; ++q) {
+      if (k0 + q < i1) {
+        s += v[q];
+        if (kind[q] == 1) a += va[q];
+      }
+    }
+  }
+}
 
 template <typename real>
 __global__ __launch_bounds__(kReduceThreads) void k_camera_reduce(BAView v, BABuffers<real> b, BAWork w) {
-  __shared__ real sblk[81];
-  __shared__ double svec[27];
-  __shared__ double sD[9];
   const LMState* st = w.state;
   if (st->done) return;
-  const int blk = blockIdx.x;
-  const int ci = v.block_cameras[2 * blk];
-  const int cj = v.block_cameras[2 * blk + 1];
-  const bool diag = ci == cj;
-  const int n = 9 * v.C;
+  const int C = v.C;
+  const long long ob = (long long)blockIdx.x * kReduceBlocks + threadIdx.x / 81;  // block of S, row-major
+  if (threadIdx.x >= 81 * kReduceBlocks || ob >= (long long)C * C) return;
+  const int R = (int)(ob / C), Q = (int)(ob - (long long)R * C);
+  const int e = threadIdx.x % 81;
+  const int r = e / 9, c = e % 9;
+  const int n = 9 * C;
+  // camera pair (i, j), i <= j, and the element of its partials this thread sums
+  const int i = R < Q ? R : Q, j = R < Q ? Q : R;
+  const int blk = (int)((long long)i * C - (long long)i * (i - 1) / 2 + (j - i));  // row-major upper triangle
   const int i0 = w.block_item_offsets[blk], i1 = w.block_item_offsets[blk + 1];
-  const int tid = threadIdx.x;
-  if (tid < 81) {
-    real s = 0;
-    for (int k = i0; k < i1; ++k) s += b.item_S[81 * (size_t)k + tid];
-    sblk[tid] = s;
-  } else if (tid < 81 + 27) {
-    double s = 0.0;
-    if (diag) {
-      for (int k = i0; k < i1; ++k)
-        if (w.items[k].kind == 1) s += w.item_vec[27 * (size_t)k + tid - 81];
-    }
-    svec[tid - 81] = s;
+  if (R != Q) {
+    real s;
+    double a;
+    item_sums(b, w, i0, i1, R < Q ? e : 9 * c + r, -1, s, a);
+    w.S[(size_t)(9 * R + r) * n + 9 * Q + c] = (double)s;
+    return;
   }
-  __syncthreads();
-  if (diag && tid < 9) {
-    const int k = 9 * ci + tid;
-    const double d = column_damping(st, svec[18 + tid], !v.camera_mask || v.camera_mask[k]);
-    w.rhs[k] = svec[tid];
-    w.gc[k] = svec[9 + tid];
+  // symmetric: the lower triangle value.  Column r's damping comes from the
+  // diagonal thread, its rhs and g_c from two off-diagonal threads of row r.
+  const int role = r == c ? 0 : c == (r + 1) % 9 ? 1 : c == (r + 2) % 9 ? 2 : -1;
+  const int ao = role == 0 ? 18 + r : role == 1 ? r : role == 2 ? 9 + r : -1;
+  real s;
+  double a;
+  item_sums(b, w, i0, i1, r >= c ? e : 9 * c + r, ao, s, a);
+  double val = (double)s;
+  const int k = 9 * R + r;
+  if (role == 0) {
+    const double d = column_damping(st, a, !v.camera_mask || v.camera_mask[k]);
     w.Dc[k] = d;
-    sD[tid] = d;
+    val += d;
+  } else if (role == 1) {
+    w.rhs[k] = a;
+  } else if (role == 2) {
+    w.gc[k] = a;
   }
-  __syncthreads();
-  for (int idx = tid; idx < 81; idx += kReduceThreads) {
-    const int r = idx / 9, c = idx % 9;
-    if (diag) {
-      const int src = r >= c ? idx : c * 9 + r;  // symmetric: lower triangle value
-      double val = (double)sblk[src];
-      if (r == c) val += sD[r];
-      w.S[(size_t)(9 * ci + r) * n + 9 * ci + c] = val;
-    } else {
-      const double val = (double)sblk[idx];
-      w.S[(size_t)(9 * ci + r) * n + 9 * cj + c] = val;
-      w.S[(size_t)(9 * cj + c) * n + 9 * ci + r] = val;
-    }
-  }
-  if (diag && tid == 0) {
-    double g2 = 0.0, gm = 0.0;
-    for (int j = 0; j < 9; ++j) {
-      g2 += svec[9 + j] * svec[9 + j];
-      gm = fmax(gm, fabs(svec[9 + j]));
-    }
-    w.part_cam[2 * ci] = g2;
-    w.part_cam[2 * ci + 1] = gm;
-  }
+  w.S[(size_t)k * n + 9 * R + c] = val;
 }
 
 // dx_p = -L^-T (q + sum_o P_o^T Jc_o dx_c)   (schur_solve_dense's xa, linear.py:698)
@@ -1373,7 +1395,9 @@ int launch_camera_items(const BAView& v, const BABuffers<real>& b, const BAWork&
 template <typename real>
 int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
   if (v.B > 0) {
-    hipLaunchKernelGGL(k_camera_reduce<real>, dim3(v.B), dim3(kReduceThreads), 0, s, v, b, w);
+    const long long blocks = (long long)v.C * v.C;
+    hipLaunchKernelGGL(k_camera_reduce<real>, dim3((unsigned)((blocks + kReduceBlocks - 1) / kReduceBlocks)),
+                       dim3(kReduceThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;

@@ -1001,6 +1001,39 @@ __device__ __forceinline__ double group_total(const double* part, unsigned int G
   return t;
 }
 
+// group_total of two partial arrays from one round of loads (each summed in
+// group_total's order).
+__device__ __forceinline__ void group_totals2(const double* a, const double* b, unsigned int G, double* slots,
+                                              double* ta, double* tb) {
+  if (threadIdx.x < kWave) {
+    double va = 0.0, vb = 0.0;
+    for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
+      double pa[4], pb[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const unsigned int i = i0 + t * kWave;
+        pa[t] = i < G ? a[i] : 0.0;
+        pb[t] = i < G ? b[i] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        va += pa[t];
+        vb += pb[t];
+      }
+    }
+    va = wave_sum(va);
+    vb = wave_sum(vb);
+    if (threadIdx.x == 0) {
+      slots[0] = va;
+      slots[1] = vb;
+    }
+  }
+  __syncthreads();
+  *ta = slots[0];
+  *tb = slots[1];
+  __syncthreads();
+}
+
 // One row of S p: lane l sums S[row][c] p[c] over c = l, l + kWave, ... in
 // that order, with the loads of kRowBatch terms issued before their adds.
 constexpr int kRowBatch = 16;
@@ -1024,9 +1057,26 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ Srow, const
 
 // p = z + beta p over the whole vector (every workgroup), loads issued first.
 constexpr int kVecBatch = 8;
-__device__ __forceinline__ void update_direction(double* p, const double* __restrict__ z, double beta, int n,
-                                                 bool copy) {
-  for (int c0 = threadIdx.x; c0 < n; c0 += kVecBatch * kPcgThreads) {
+// The first kVecBatch entries of z a thread updates, loaded ahead (issued
+// before the group totals, so both wait out one latency together).
+__device__ __forceinline__ void load_direction_head(double (&zv)[kVecBatch], const double* __restrict__ z, int n) {
+#pragma unroll
+  for (int t = 0; t < kVecBatch; ++t) {
+    const int c = threadIdx.x + t * kPcgThreads;
+    zv[t] = c < n ? z[c] : 0.0;
+  }
+}
+
+// p = z (copy) or p = z + beta p, entries in the same order as a plain pass;
+// zv holds the head from load_direction_head.
+__device__ __forceinline__ void update_direction(double* p, const double (&zv)[kVecBatch],
+                                                 const double* __restrict__ z, double beta, int n, bool copy) {
+#pragma unroll
+  for (int t = 0; t < kVecBatch; ++t) {
+    const int c = threadIdx.x + t * kPcgThreads;
+    if (c < n) p[c] = copy ? zv[t] : zv[t] + beta * p[c];
+  }
+  for (int c0 = threadIdx.x + kVecBatch * kPcgThreads; c0 < n; c0 += kVecBatch * kPcgThreads) {
     double zv[kVecBatch];
 #pragma unroll
     for (int t = 0; t < kVecBatch; ++t) {
@@ -1151,7 +1201,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   __shared__ int piv[kPcgMaxOwn][kBlk];
   __shared__ int kind[kPcgMaxOwn];
   __shared__ double xo[kPcgMaxOwn][kBlk], ro[kPcgMaxOwn][kBlk], zo[kPcgMaxOwn][kBlk], apo[kPcgMaxOwn][kBlk];
-  __shared__ double slot;
+  __shared__ double slot, slots2[2];
   __shared__ int timed_out, singular;
   if (skip_solve(st)) return;
   const int min_iter = st->min_lin_iter, max_iter = st->max_lin_iter;
@@ -1223,10 +1273,12 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
   }
   if (!grid_barrier(wk.flags, bar_base + ++barriers, G, &timed_out)) goto fail;
   {
-    const double rr0 = group_total(part_rr, G, &slot);
-    double rz_old = group_total(part_rz, G, &slot);
+    double zv[kVecBatch];
+    load_direction_head(zv, wk.z, n);
+    double rr0, rz_old;
+    group_totals2(part_rr, part_rz, G, slots2, &rr0, &rz_old);
     const double rhs_norm = fmax(sqrt(rr0), kEps);
-    update_direction(p, wk.z, 0.0, n, true);
+    update_direction(p, zv, wk.z, 0.0, n, true);
     __syncthreads();
     if (!(min_iter <= 0 && sqrt(rr0) <= tol * rhs_norm)) {
       for (int k = 0; k < max_iter; ++k) {
@@ -1277,12 +1329,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
           }
         }
         if (!grid_barrier(wk.flags, bar_base + ++barriers, G, &timed_out)) goto fail;
-        const double rr = group_total(part_rr, G, &slot);
-        const double rz_new = group_total(part_rz, G, &slot);
+        load_direction_head(zv, wk.z, n);
+        double rr, rz_new;
+        group_totals2(part_rr, part_rz, G, slots2, &rr, &rz_new);
         if (iterations >= min_iter && sqrt(rr) <= tol * rhs_norm) break;
         if (fabs(rz_old) <= kEps) break;
         const double beta = rz_new / rz_old;
-        update_direction(p, wk.z, beta, n, false);
+        update_direction(p, zv, wk.z, beta, n, false);
         __syncthreads();
         rz_old = rz_new;
       }

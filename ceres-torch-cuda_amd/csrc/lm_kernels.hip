@@ -65,9 +65,19 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
       gm = fmax(gm, v[q][1]);
     }
   }
+  // camera gradient blocks: |g_c|^2 and max|g_c| per camera, then across cameras
   for (int i = threadIdx.x; i < init.num_cameras; i += kLMThreads) {
-    g2 += w.part_cam[2 * i];
-    gm = fmax(gm, w.part_cam[2 * i + 1]);
+    double gv[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) gv[j] = w.gc[9 * (size_t)i + j];
+    double c2 = 0.0, cm = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      c2 += gv[j] * gv[j];
+      cm = fmax(cm, fabs(gv[j]));
+    }
+    g2 += c2;
+    gm = fmax(gm, cm);
   }
   const double cost = block_sum<kLMThreads>(c[0], red);
   const double gnorm2 = block_sum<kLMThreads>(g2, red);

@@ -208,7 +208,7 @@ typedef struct ct_ba_problem {
   int32_t num_cameras;
   int32_t num_points;
   int32_t num_observations;
-  int32_t num_blocks;              /* camera pair blocks (i <= j) */
+  int32_t num_blocks;              /* camera pair blocks: every pair i <= j, C (C + 1) / 2 */
   int64_t num_terms;               /* observation pairs over all blocks */
   const int32_t* obs_camera;       /* [N] camera of each observation */
   const int32_t* obs_point;        /* [N] point of each observation (non-decreasing) */
@@ -216,7 +216,7 @@ typedef struct ct_ba_problem {
   const double* observations;      /* [N*2] observed pixels */
   const int32_t* camera_offsets;   /* [C+1] CSR of observations by camera */
   const int32_t* camera_obs;       /* [N]   observation ids by camera */
-  const int32_t* block_cameras;    /* [B*2] (i, j), i <= j */
+  const int32_t* block_cameras;    /* [B*2] (i, j), i <= j, row-major: (0,0), (0,1), ..., (1,1), ... */
   const int32_t* block_offsets;    /* [B+1] term ranges */
   const int32_t* block_terms;      /* [T*2] (obs a of camera i, obs b of camera j) */
   const uint8_t* camera_mask;      /* [C*9] 1 = free, 0 = held constant */

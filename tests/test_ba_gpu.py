@@ -279,3 +279,24 @@ def test_mixed_precision_reaches_fp64_final_cost():
     assert s32.termination_type == tc.TerminationType.CONVERGENCE
     assert s32.final_cost == pytest.approx(s64.final_cost, rel=1e-6)
     assert s32.precision == "mixed"
+
+
+def test_plan_rejects_camera_pair_blocks_out_of_row_major_order():
+    """The assembly indexes S's blocks by camera pair, so the C-ABI refuses a block list that is not
+    every pair i <= j in row-major order (here the same pairs reversed, and one pair missing)."""
+    from ceres_mi355x import ba as nba
+    from ceres_mi355x._native import NativeLibraryError
+    data = gen.ba_config("ba4")
+    cams = torch.tensor(data.cameras, dtype=torch.float64, device="cuda")
+    pts = torch.tensor(data.points, dtype=torch.float64, device="cuda")
+    opts = nba.lm_options(tc.SolverOptions(max_num_iterations=2))
+    dev = nba.BADevice(cams, pts, data.obs_camera, data.obs_point, data.observations)
+    good = dev.block_cameras
+    dev.block_cameras = good.flip(0).contiguous()
+    with pytest.raises(NativeLibraryError, match="row-major order"):
+        dev.run(opts)
+    dev.block_cameras = good[:-1].contiguous()
+    dev.block_offsets = dev.block_offsets[:-1].contiguous()
+    with pytest.raises(NativeLibraryError, match=r"C \(C \+ 1\) / 2"):
+        dev.run(opts)
+    dev.close()
