@@ -37,6 +37,22 @@ __device__ __forceinline__ void strided_sums(const double* __restrict__ part, in
   }
 }
 
+// Thread 0 updates an LDS copy of the state, loaded word-parallel at kernel
+// start (in flight during the partial sums; visible to thread 0 after the
+// block reductions' barriers) and stored back once: on the global state the
+// update's branches would wait for their fields one load at a time.
+constexpr int kStateWords = (int)(sizeof(LMState) / sizeof(uint32_t));
+static_assert(sizeof(LMState) % sizeof(uint32_t) == 0 && kStateWords <= kLMThreads, "state copy");
+
+__device__ __forceinline__ void load_state(LMState& dst, const LMState* src) {
+  if (threadIdx.x < kStateWords)
+    reinterpret_cast<uint32_t*>(&dst)[threadIdx.x] = reinterpret_cast<const uint32_t*>(src)[threadIdx.x];
+}
+
+__device__ __forceinline__ void store_state(LMState* dst, const LMState& src) {
+  for (int i = 0; i < kStateWords; ++i) reinterpret_cast<uint32_t*>(dst)[i] = reinterpret_cast<const uint32_t*>(&src)[i];
+}
+
 __global__ void k_lm_init(LMInit init, BAWork w, const double* __restrict__ ucams, const double* __restrict__ upts) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
@@ -49,6 +65,8 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   __shared__ double red[kLMThreads / kWave];
   LMState* st = w.state;
   if (st->done) return;
+  __shared__ LMState ls;
+  load_state(ls, st);
   double c[1] = {0.0}, g2 = 0.0, gm = 0.0;
   strided_sums<1>(w.part_cost, init.num_cost_blocks, c);
   for (int i0 = threadIdx.x; i0 < init.num_eval_blocks; i0 += kLMBatch * kLMThreads) {
@@ -84,7 +102,8 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) 
   const double gmax = block_max<kLMThreads>(gm, red);
   if (threadIdx.x != 0) return;
   if (w.pcg_bar) *w.pcg_bar += 1u;  // a new epoch for the barrier numbers of this iteration's PCG
-  lm_begin_update(st, cost, gnorm2, gmax, w.records);
+  lm_begin_update(&ls, cost, gnorm2, gmax, w.records);
+  store_state(st, ls);
 }
 
 __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
@@ -93,10 +112,15 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
   // every thread reads the flags before thread 0 may change them
   const int done = st->done;
   const int skip = st->skip_step;
+  __shared__ LMState ls;
+  load_state(ls, st);
   __syncthreads();
   if (done) return;
   if (skip) {
-    if (threadIdx.x == 0) lm_skip_update(st);
+    if (threadIdx.x == 0) {
+      lm_skip_update(&ls);
+      store_state(st, ls);
+    }
     return;
   }
   double cand[2] = {0.0, 0.0}, step[3] = {0.0, 0.0, 0.0};
@@ -107,7 +131,8 @@ __global__ __launch_bounds__(kLMThreads) void k_lm_end(LMInit init, BAWork w) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) s[k] = block_sum<kLMThreads>(a[k], red);
   if (threadIdx.x != 0) return;
-  lm_end_update(st, s[0], s[1], s[2], s[3], s[4] != 0.0, w.records);
+  lm_end_update(&ls, s[0], s[1], s[2], s[3], s[4] != 0.0, w.records);
+  store_state(st, ls);
 }
 
 __global__ void k_lm_finalize(LMInit init, BAWork w, double* __restrict__ ucams, double* __restrict__ upts) {
