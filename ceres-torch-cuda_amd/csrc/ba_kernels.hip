@@ -253,6 +253,26 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_obs(BAView v, BABuffe
 //   q = L^-1 g_p, P_o = Jp_o L^-T, e_o = P_o q - r_o.
 // Block partials: |g_p|^2, max|g_p|.
 // ---------------------------------------------------------------------------
+// A point without observations: its block of the system is the damping alone
+// and its gradient is zero, so its step is zero.  L^-1 holds 1/sqrt(D), or 0
+// where D is 0 (a free point under DOGLEG: the Gauss-Newton system carries no
+// information on it, and its step stays zero).
+template <typename real>
+__device__ __forceinline__ void unobserved_point(const BAView& v, const BABuffers<real>& b, const BAWork& w,
+                                                 const LMState* st, int p) {
+  real inv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double d = column_damping(st, 0.0, !v.point_mask || v.point_mask[3 * p + k]);
+    w.Dp[3 * p + k] = d;
+    inv[k] = d > 0.0 ? real(1) / sqrt((real)d) : real(0);
+    b.q[3 * p + k] = 0;
+    b.gp[3 * p + k] = 0.0;
+  }
+  real* Li = b.Linv + 6 * (size_t)p;
+  Li[0] = inv[0]; Li[1] = 0; Li[2] = inv[1]; Li[3] = 0; Li[4] = 0; Li[5] = inv[2];
+}
+
 template <typename real>
 __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ double red[kPointThreads / kWave];
@@ -260,7 +280,9 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
   if (st->done) return;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
   double g2 = 0.0, gmax = 0.0;
-  if (p < v.P) {
+  if (p < v.P && v.point_offsets[p] == v.point_offsets[p + 1]) {
+    unobserved_point(v, b, w, st, p);
+  } else if (p < v.P) {
     const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
     real V00 = 0, V10 = 0, V11 = 0, V20 = 0, V21 = 0, V22 = 0;
     double g0 = 0.0, g1 = 0.0, g2v = 0.0;
@@ -427,6 +449,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
     term[wave][8][lane] = (double)a2 * r[0] + (double)c2 * r[1];
   }
   __syncthreads();
+  if (grp < num_groups) {
+    // the group's points without observations have no lane of their own
+    const int q0 = w.point_groups[grp], q1 = w.point_groups[grp + 1];
+    for (int q = q0 + lane; q < q1; q += kWave)
+      if (v.point_offsets[q] == v.point_offsets[q + 1]) unobserved_point(v, b, w, st, q);
+  }
   if (active && head == lane) {
     real V00 = 0, V10 = 0, V11 = 0, V20 = 0, V21 = 0, V22 = 0;
     double g0 = 0.0, g1 = 0.0, g2v = 0.0;

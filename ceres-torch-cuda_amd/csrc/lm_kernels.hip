@@ -56,8 +56,19 @@ __device__ __forceinline__ void store_state(LMState* dst, const LMState& src) {
 __global__ void k_lm_init(LMInit init, BAWork w, const double* __restrict__ ucams, const double* __restrict__ upts) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
-  for (int i = t; i < init.n_cam_params; i += stride) w.xcam[i] = ucams[i];
-  for (int i = t; i < init.n_pt_params; i += stride) w.xpt[i] = upts[i];
+  // both ping-pong copies: a point without observations is never stepped (the
+  // fused step moves a point from its first observation's lane), so it must
+  // read the same value from either copy
+  for (int i = t; i < init.n_cam_params; i += stride) {
+    const double v = ucams[i];
+    w.xcam[i] = v;
+    w.xcam[init.n_cam_params + i] = v;
+  }
+  for (int i = t; i < init.n_pt_params; i += stride) {
+    const double v = upts[i];
+    w.xpt[i] = v;
+    w.xpt[init.n_pt_params + i] = v;
+  }
   if (t == 0) lm_state_init(w.state, init.opt);
 }
 

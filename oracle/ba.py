@@ -232,8 +232,15 @@ def ba_evaluate(struct: BAStructure, cams, pts, obs, loss, jacobian: bool) -> BA
 
 
 def _segment_sum(values: np.ndarray, offsets: np.ndarray) -> np.ndarray:
-    """Sum rows of ``values`` over CSR segments (every segment non-empty)."""
-    return np.add.reduceat(values, offsets[:-1], axis=0)
+    """Sum rows of ``values`` over CSR segments; an empty segment (a point no
+    camera observes) sums to zero.  ``np.add.reduceat`` alone would return the
+    row at an empty segment's offset instead, so it runs on the non-empty
+    segments only (each then ends where the next non-empty one starts)."""
+    out = np.zeros((offsets.shape[0] - 1,) + values.shape[1:], dtype=values.dtype)
+    nonempty = offsets[1:] > offsets[:-1]
+    if nonempty.any():
+        out[nonempty] = np.add.reduceat(values, offsets[:-1][nonempty], axis=0)
+    return out
 
 
 # ----------------------------------------------------------------------------

@@ -300,3 +300,73 @@ def test_plan_rejects_camera_pair_blocks_out_of_row_major_order():
     with pytest.raises(NativeLibraryError, match=r"C \(C \+ 1\) / 2"):
         dev.run(opts)
     dev.close()
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("linear", ["dense_schur", "iterative_schur"])
+def test_points_with_more_observations_than_a_wavefront_match_oracle(linear, loop):
+    """A point observed by more than 64 cameras cannot be one lane per observation of a wavefront, so
+    the plan evaluates and steps per observation and per point (k_eval_obs + k_eval_points,
+    k_step_points + k_candidate_obs) instead of by point groups.  Oracle: the float64 block-sparse
+    restatement on the same inputs, same tolerances as the golden cases."""
+    print(f"wide points linear={linear} loop={loop}", flush=True)
+    tc.set_device_loop(loop)
+    data = gen.make_bundle_adjustment(num_cameras=72, num_points=80, obs_per_point=(3, 72), seed=21)
+    assert np.bincount(data.obs_point).max() > 64
+    problem, cams, pts = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=10, linear_solver_type=tc.LinearSolverType[linear.upper()])
+    oo = OracleOptions(max_num_iterations=10)
+    if linear == "iterative_schur":
+        o.preconditioner_type = tc.PreconditionerType.SCHUR_JACOBI
+        oo.preconditioner_type = "schur_jacobi"
+    summary = tc.solve(o, problem)
+    ref, (rc, rp) = trust_region_solve(oba.BAModel(data, linear_solver=linear), oo)
+    assert_trajectory_matches(summary, {"iterations": ref.iterations, "final_cost": ref.final_cost,
+                                        "initial_cost": ref.initial_cost})
+    np.testing.assert_allclose(pts.cpu().numpy(), rp, atol=1e-6)
+
+
+def test_points_with_more_observations_than_a_wavefront_mixed_precision():
+    """The per-observation kernels in the mixed form: final cost within 1e-6 of the FP64 oracle's."""
+    data = gen.make_bundle_adjustment(num_cameras=72, num_points=80, obs_per_point=(3, 72), seed=21)
+    problem, cams, pts = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=20, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
+                         use_mixed_precision_solves=True, max_num_refinement_iterations=2)
+    summary = tc.solve(o, problem)
+    ref, _ = trust_region_solve(oba.BAModel(data), OracleOptions(max_num_iterations=20))
+    assert summary.final_cost == pytest.approx(ref.final_cost, rel=1e-6)
+
+
+def _sparse_visibility_problem():
+    """Points seen once (rank-2 V_p without damping), points seen by no camera and a camera that sees
+    nothing: their columns are zero, so only the damping keeps the system nonsingular."""
+    data = gen.make_bundle_adjustment(num_cameras=6, num_points=90, obs_per_point=(1, 4), seed=23)
+    keep = data.obs_camera != 5
+    data.obs_camera, data.obs_point, data.observations = (data.obs_camera[keep], data.obs_point[keep],
+                                                          data.observations[keep])
+    counts = np.bincount(data.obs_point, minlength=data.points.shape[0])
+    assert (counts == 1).any() and (counts == 0).any()
+    return data
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("linear", ["dense_schur", "iterative_schur"])
+def test_unobserved_blocks_and_single_observation_points_match_oracle(linear, loop):
+    print(f"sparse visibility linear={linear} loop={loop}", flush=True)
+    tc.set_device_loop(loop)
+    data = _sparse_visibility_problem()
+    problem, cams, pts = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=10, linear_solver_type=tc.LinearSolverType[linear.upper()])
+    oo = OracleOptions(max_num_iterations=10)
+    if linear == "iterative_schur":
+        o.preconditioner_type = tc.PreconditionerType.SCHUR_JACOBI
+        oo.preconditioner_type = "schur_jacobi"
+    summary = tc.solve(o, problem)
+    ref, (rc, rp) = trust_region_solve(oba.BAModel(data, linear_solver=linear), oo)
+    assert_trajectory_matches(summary, {"iterations": ref.iterations, "final_cost": ref.final_cost,
+                                        "initial_cost": ref.initial_cost})
+    np.testing.assert_allclose(pts.cpu().numpy(), rp, atol=1e-6)
+    np.testing.assert_allclose(cams.cpu().numpy(), rc, atol=1e-6)
+    unseen = np.bincount(data.obs_point, minlength=data.points.shape[0]) == 0
+    np.testing.assert_array_equal(pts.cpu().numpy()[unseen], data.points[unseen])
+    np.testing.assert_array_equal(cams.cpu().numpy()[5], data.cameras[5])

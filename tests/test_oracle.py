@@ -541,3 +541,35 @@ def test_pcg_jacobi_and_identity_preconditioners():
     np.testing.assert_allclose(S @ x, b, rtol=1e-10, atol=1e-10)
     with pytest.raises(ValueError):
         oba.block_jacobi_pcg(S, b, 9, tol=1e-12, max_iter=5, preconditioner="cluster_jacobi")
+
+
+def test_segment_sum_handles_empty_segments():
+    from oracle.ba import _segment_sum
+    rng = np.random.default_rng(3)
+    counts = np.array([0, 2, 0, 0, 3, 1, 0])
+    offsets = np.r_[0, np.cumsum(counts)]
+    values = rng.normal(size=(offsets[-1], 2, 2))
+    want = np.stack([values[offsets[k]: offsets[k + 1]].sum(axis=0) for k in range(len(counts))])
+    got = _segment_sum(values, offsets)
+    np.testing.assert_array_equal(got[counts == 0], 0.0)
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
+def test_unobserved_blocks_stay_fixed_and_match_the_dense_algebra():
+    """Points seen by no camera and a camera that sees nothing keep zero Jacobian columns in the
+    reference (problem.py:522-530 keeps every block with a tangent space in the column map), so the
+    LM damping alone fixes their step at zero.  The block-sparse oracle must agree with the
+    reference's dense algebra on the full Jacobian (dense_faithful) and leave those blocks unchanged."""
+    d = gen.make_bundle_adjustment(num_cameras=6, num_points=90, obs_per_point=(1, 4), seed=23)
+    keep = d.obs_camera != 5
+    d.obs_camera, d.obs_point, d.observations = d.obs_camera[keep], d.obs_point[keep], d.observations[keep]
+    unseen = np.bincount(d.obs_point, minlength=d.points.shape[0]) == 0
+    assert unseen.sum() == 3
+    opts = OracleOptions(max_num_iterations=10)
+    s, (cams, pts) = trust_region_solve(oba.BAModel(d), opts)
+    sd, (cams_d, pts_d) = trust_region_solve(oba.BAModel(d, dense_faithful=True), opts)
+    assert_trajectory_matches(s, {"iterations": sd.iterations, "final_cost": sd.final_cost,
+                                  "initial_cost": sd.initial_cost})
+    np.testing.assert_allclose(pts, pts_d, atol=1e-9)
+    np.testing.assert_array_equal(pts[unseen], d.points[unseen])
+    np.testing.assert_array_equal(cams[5], d.cameras[5])
