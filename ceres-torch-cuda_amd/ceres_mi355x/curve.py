@@ -52,6 +52,9 @@ class CurveBatch:
         self._order_cache: dict = {}
         self._workspace = torch.zeros(256, dtype=torch.uint8, device=self.device)
         bad = ctypes.c_int32(-1)
+        # ct_curve_validate reads the offsets with a synchronous copy, which does not wait for a
+        # non-blocking stream: the uploads above (on the current stream) finish first
+        torch.cuda.current_stream(self.device).synchronize()
         nat.check(nat.load().ct_curve_validate(ctypes.byref(self._struct(None)), ctypes.byref(bad)),
                   "ct_curve_validate")
 

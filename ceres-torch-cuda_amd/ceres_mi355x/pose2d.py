@@ -82,8 +82,7 @@ class PoseGraphDevice:
     def set_poses(self, poses) -> None:
         self.poses.copy_(torch.as_tensor(poses, dtype=torch.float64).to(self.device))
 
-    def run(self, opts: nat.ct_lm_options):
-        """One LM solve in place on self.poses; returns (report, records)."""
+    def _plan(self, opts: nat.ct_lm_options):
         lib = nat.load()
         key = tuple(getattr(opts, f) for f, _ in nat.ct_lm_options._fields_)
         prob = self.problem_struct()
@@ -100,6 +99,19 @@ class PoseGraphDevice:
                       "ct_pose2d_plan_create")
             plan = (handle, ws, prob)
             self._plans[key] = plan
+        return plan
+
+    def system_size(self, opts: nat.ct_lm_options) -> int:
+        """Unknowns of the system the plan for ``opts`` factors densely per iteration."""
+        n = ctypes.c_int32(0)
+        nat.check(nat.load().ct_pose2d_plan_system_size(self._plan(opts)[0], ctypes.byref(n)),
+                  "ct_pose2d_plan_system_size")
+        return int(n.value)
+
+    def run(self, opts: nat.ct_lm_options):
+        """One LM solve in place on self.poses; returns (report, records)."""
+        lib = nat.load()
+        plan = self._plan(opts)
         rep = nat.ct_lm_report()
         rec = (ctypes.c_double * ((opts.max_num_iterations + 2) * nat.CT_REC_FIELDS))()
         nat.check(lib.ct_pose2d_plan_run(plan[0], ctypes.byref(rep), rec), "ct_pose2d_plan_run")

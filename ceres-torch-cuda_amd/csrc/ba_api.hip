@@ -523,6 +523,9 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->user_cams = cameras;
   pl->user_pts = points;
   pl->stream = reinterpret_cast<hipStream_t>(stream);
+  // the tables are read back below with synchronous copies, which do not wait
+  // for a non-blocking stream: the caller's uploads on `stream` finish first
+  CT_HIP(hipStreamSynchronize(pl->stream));
   pl->init.opt = *options;
   pl->init.num_obs_blocks = obs_blocks(problem);
   pl->init.num_cost_blocks = obs_blocks(problem);
@@ -676,6 +679,7 @@ int ct_ba_evaluate_jacobian(const ct_ba_problem* problem, const double* cameras,
   const int grid = obs_blocks(problem);
   CT_CHECK_ARG(workspace_bytes >= sizeof(double) * (size_t)(grid + 1), "workspace too small for partials");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  CT_HIP(hipStreamSynchronize(s));  // the caller's uploads on `stream` precede the host reads below
   CT_TRY(validate_point_indices(*problem));
   // every row index, and every column index the kernel follows, is checked once here
   std::vector<int32_t> rows;

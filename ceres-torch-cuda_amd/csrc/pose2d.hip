@@ -5,12 +5,27 @@
 // relative_pose = (R(theta_a)^T (p_b - p_a), wrap(theta_b - theta_a))
 // (pose_graph_2d.py:10-15), with a robust loss per edge.  One node is held
 // constant (pose_graph_2d.py:34).  The reference's DENSE_QR step is the
-// solution of the damped normal equations, assembled here densely and
-// deterministically: per-node diagonal blocks (incident edges in table order)
-// and per-node-pair off-diagonal blocks (the pair's edges in table order),
-// then the FP64 Cholesky of the reduced-camera-system solver.
+// solution of the damped normal equations, assembled deterministically from
+// per-node diagonal blocks (incident edges in table order) and per-node-pair
+// off-diagonal blocks (the pair's edges in table order), then solved in FP64
+// by one of two exact methods:
+//   dense:     the whole system, factored by the reduced-camera-system
+//              Cholesky;
+//   separator: the nodes touched by a non-chain edge (and the constant node)
+//              form a separator; every other node lies in a run of
+//              consecutive nodes joined only by chain edges (i, i + 1), whose
+//              system is block tridiagonal and touches at most two separator
+//              nodes.  Each run is factored on its own (one thread, block
+//              Cholesky along the chain), its Schur complement onto its two
+//              separator nodes is subtracted from the separator system, that
+//              dense system is factored by the same Cholesky, and each run is
+//              back-substituted.  The plan takes this form whenever some node
+//              lies in a run; it factors 3 x (separator nodes) unknowns
+//              densely instead of 3 x (nodes).
+#include <map>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ba_kernels.h"
@@ -50,6 +65,26 @@ struct PGWork {
   double* records;
   LMState* state;
   double* solve_tmp;
+};
+
+// Separator form (see the file comment).  Blocks are 3x3 row-major.
+struct PGSep {
+  int32_t runs;   // runs of consecutive non-separator nodes
+  int32_t nsep;   // separator nodes other than the constant one
+  int32_t nent;   // off-diagonal separator blocks (joined pairs and runs with two ends)
+  const int32_t* __restrict__ run_nodes;  // [runs][4] first node s, last node t, separator unknowns of the
+                                          // left end L = s - 1 and right end R = t + 1 (-1: no coupling)
+  const int32_t* __restrict__ run_pairs;  // [runs][2] pair of (L, s) and of (t, R), or -1
+  const int32_t* __restrict__ chain_pair; // [N] for a run node k after its run's first: pair of (k - 1, k), or -1
+  const int32_t* __restrict__ sep_nodes;  // [nsep] node of each separator unknown, increasing
+  const int32_t* __restrict__ sep_runs;   // [nsep][2] run whose right end it is, run whose left end it is (-1: none)
+  const int32_t* __restrict__ entries;    // [nent][4] separator unknowns u < v, pair (or -1), run (or -1)
+  double* nodeblk;  // [N][9]   damped diagonal block of each free node
+  double* pairblk;  // [P][9]   H(u, v) of each joined pair u < v (rows u, columns v)
+  double* fac;      // [N][15]  per run node: L_kk^-1 (lower, packed 6) and L_k,k-1 (9)
+  double* contrib;  // [runs][33] Schur terms onto the run's ends: LL (9), LR (9), RR (9), rhs L (3), rhs R (3)
+  double* rhs_s;    // [3 nsep]
+  double* x_s;      // [3 nsep]
 };
 
 __device__ __forceinline__ int col_of(const PGView& v, int k) { return 3 * (v.fixed >= 0 && k > v.fixed ? k - 1 : k); }
@@ -152,8 +187,10 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_zero(PGWork w, size_t count) 
     w.H[i] = 0.0;
 }
 
-// diagonal block, damping and gradient of each free node (incident edges in table order)
-__global__ __launch_bounds__(kPgThreads) void k_pg_node(PGView v, PGWork w) {
+// diagonal block, damping and gradient of each free node (incident edges in
+// table order); into the dense H, or (separator form) into nodeblk
+template <bool kSep>
+__global__ __launch_bounds__(kPgThreads) void k_pg_node(PGView v, PGWork w, PGSep sp) {
   __shared__ double red[kPgThreads / kWave];
   const LMState* st = w.state;
   if (st->done) return;
@@ -180,7 +217,13 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_node(PGView v, PGWork w) {
       const double dg = st->jacobi_scaling ? h[4 * i] : 1.0;
       const double D = fmin(fmax(dg, st->min_diag), st->max_diag) / fmax(st->radius, 2.220446049250313e-16);
 #pragma unroll
-      for (int j = 0; j < 3; ++j) w.H[(size_t)(c + i) * v.n + c + j] = h[3 * i + j] + (i == j ? D : 0.0);
+      for (int j = 0; j < 3; ++j) {
+        const double hv = h[3 * i + j] + (i == j ? D : 0.0);
+        if (kSep)
+          sp.nodeblk[9 * (size_t)k + 3 * i + j] = hv;
+        else
+          w.H[(size_t)(c + i) * v.n + c + j] = hv;
+      }
       w.rhs[c + i] = -g[i];
     }
     g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
@@ -194,8 +237,10 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_node(PGView v, PGWork w) {
   }
 }
 
-// off-diagonal block of each joined node pair u < v (its edges in table order)
-__global__ __launch_bounds__(kPgThreads) void k_pg_pair(PGView v, PGWork w) {
+// off-diagonal block of each joined node pair u < v (its edges in table
+// order); into the dense H, or (separator form) into pairblk
+template <bool kSep>
+__global__ __launch_bounds__(kPgThreads) void k_pg_pair(PGView v, PGWork w, PGSep sp) {
   if (w.state->done) return;
   const int p = blockIdx.x * kPgThreads + threadIdx.x;
   if (p >= v.P) return;
@@ -213,6 +258,11 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_pair(PGView v, PGWork w) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) h[3 * i + j] += Ju[i] * Jq[j] + Ju[3 + i] * Jq[3 + j] + Ju[6 + i] * Jq[6 + j];
   }
+  if (kSep) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sp.pairblk[9 * (size_t)p + i] = h[i];
+    return;
+  }
   const int cu = col_of(v, u), cq = col_of(v, q);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -221,6 +271,267 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_pair(PGView v, PGWork w) {
       w.H[(size_t)(cu + i) * v.n + cq + j] = h[3 * i + j];
       w.H[(size_t)(cq + j) * v.n + cu + i] = h[3 * i + j];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Separator form
+// ---------------------------------------------------------------------------
+
+// 3x3 block Cholesky M = L L^T; Li = L^-1 (lower, packed l00 l10 l11 l20 l21
+// l22).  False unless M is numerically positive definite.
+__device__ __forceinline__ bool chol3_inverse(const double (&M)[9], double (&Li)[6]) {
+  const double d0 = M[0];
+  if (!(d0 > 0.0) || !isfinite(d0)) return false;
+  const double l00 = sqrt(d0);
+  const double l10 = M[3] / l00, l20 = M[6] / l00;
+  const double d1 = M[4] - l10 * l10;
+  if (!(d1 > 0.0) || !isfinite(d1)) return false;
+  const double l11 = sqrt(d1);
+  const double l21 = (M[7] - l20 * l10) / l11;
+  const double d2 = M[8] - l20 * l20 - l21 * l21;
+  if (!(d2 > 0.0) || !isfinite(d2)) return false;
+  const double l22 = sqrt(d2);
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11;
+  const double i21 = -l21 * i11 * i22;
+  const double i20 = -(l20 * i00 + l21 * i10) * i22;
+  Li[0] = i00; Li[1] = i10; Li[2] = i11; Li[3] = i20; Li[4] = i21; Li[5] = i22;
+  return true;
+}
+
+// y = Li x for the packed lower inverse
+__device__ __forceinline__ void lower_apply(const double (&Li)[6], const double* x, double* y) {
+  const double x0 = x[0], x1 = x[1], x2 = x[2];
+  y[0] = Li[0] * x0;
+  y[1] = Li[1] * x0 + Li[2] * x1;
+  y[2] = Li[3] * x0 + Li[4] * x1 + Li[5] * x2;
+}
+
+// y = Li^T x
+__device__ __forceinline__ void lower_apply_t(const double (&Li)[6], const double* x, double* y) {
+  const double x0 = x[0], x1 = x[1], x2 = x[2];
+  y[0] = Li[0] * x0 + Li[1] * x1 + Li[3] * x2;
+  y[1] = Li[2] * x1 + Li[4] * x2;
+  y[2] = Li[5] * x2;
+}
+
+// block of H(k, k - 1) (the transpose of the pair block H(k - 1, k)), or 0
+__device__ __forceinline__ void block_t(const double* __restrict__ pairblk, int q, double (&B)[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) B[3 * i + j] = q >= 0 ? pairblk[9 * (size_t)q + 3 * j + i] : 0.0;
+}
+
+// One thread per run s..t: block Cholesky along the chain,
+//   L_kk L_kk^T = A_k - L_k,k-1 L_k,k-1^T,  L_k,k-1 = H(k, k-1) L_k-1,k-1^-T,
+// forward substitution of the couplings to the run's ends and of its
+// right-hand side, W = L^-1 [H(run, L) | H(run, R) | b], and the Schur terms
+// W^T W onto the ends.  The right coupling is non-zero only at t.
+__global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= sp.runs) return;
+  const int s = sp.run_nodes[4 * r], t = sp.run_nodes[4 * r + 1];
+  const int L = sp.run_nodes[4 * r + 2], R = sp.run_nodes[4 * r + 3];
+  const int pl = sp.run_pairs[2 * r], pr = sp.run_pairs[2 * r + 1];
+  double WL[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, wb[3] = {0, 0, 0};  // this node's rows of W (left columns, b)
+  double GLL[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, GLb[3] = {0, 0, 0};
+  double Li[6] = {0, 0, 0, 0, 0, 0};
+  bool ok = true;
+  for (int k = s; k <= t; ++k) {
+    double Ls[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k > s) {
+      double B[9];
+      block_t(sp.pairblk, sp.chain_pair[k], B);
+      // L_k,k-1 = B Li_prev^T: row i is Li_prev applied to row i of B
+#pragma unroll
+      for (int i = 0; i < 3; ++i) lower_apply(Li, B + 3 * i, Ls + 3 * i);
+    }
+    double M[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        M[3 * i + j] = sp.nodeblk[9 * (size_t)k + 3 * i + j] -
+                       (Ls[3 * i] * Ls[3 * j] + Ls[3 * i + 1] * Ls[3 * j + 1] + Ls[3 * i + 2] * Ls[3 * j + 2]);
+    if (!chol3_inverse(M, Li)) ok = false;
+    double* f = sp.fac + 15 * (size_t)k;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f[i] = Li[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[6 + i] = Ls[i];
+    // T = [H(k, L) | b_k] - L_k,k-1 [W_L | w_b]_(k-1), then W_k = Li T
+    double TL[9], tb[3];
+    double HL[9];
+    block_t(sp.pairblk, k == s && L >= 0 ? pl : -1, HL);
+    const int c = col_of(v, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        TL[3 * i + j] = HL[3 * i + j] - (Ls[3 * i] * WL[j] + Ls[3 * i + 1] * WL[3 + j] + Ls[3 * i + 2] * WL[6 + j]);
+      tb[i] = w.rhs[c + i] - (Ls[3 * i] * wb[0] + Ls[3 * i + 1] * wb[1] + Ls[3 * i + 2] * wb[2]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double col[3] = {TL[j], TL[3 + j], TL[6 + j]};
+      double out[3];
+      lower_apply(Li, col, out);
+      WL[j] = out[0]; WL[3 + j] = out[1]; WL[6 + j] = out[2];
+    }
+    lower_apply(Li, tb, wb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) GLL[3 * i + j] += WL[i] * WL[j] + WL[3 + i] * WL[3 + j] + WL[6 + i] * WL[6 + j];
+      GLb[i] += WL[i] * wb[0] + WL[3 + i] * wb[1] + WL[6 + i] * wb[2];
+    }
+  }
+  double* out = sp.contrib + 33 * (size_t)r;
+  double WR[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (R >= 0) {
+    // W_R at t = Li_t H(t, R); H(t, R) is the pair block itself (t < R)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double col[3] = {sp.pairblk[9 * (size_t)pr + j], sp.pairblk[9 * (size_t)pr + 3 + j],
+                             sp.pairblk[9 * (size_t)pr + 6 + j]};
+      double o3[3];
+      lower_apply(Li, col, o3);
+      WR[j] = o3[0]; WR[3 + j] = o3[1]; WR[6 + j] = o3[2];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      out[3 * i + j] = GLL[3 * i + j];
+      out[9 + 3 * i + j] = WL[i] * WR[j] + WL[3 + i] * WR[3 + j] + WL[6 + i] * WR[6 + j];
+      out[18 + 3 * i + j] = WR[i] * WR[j] + WR[3 + i] * WR[3 + j] + WR[6 + i] * WR[6 + j];
+    }
+    out[27 + i] = GLb[i];
+    out[30 + i] = WR[i] * wb[0] + WR[3 + i] * wb[1] + WR[6 + i] * wb[2];
+  }
+  if (!ok) w.state->step_invalid = 1;
+}
+
+// The separator system: thread per separator unknown (its diagonal block and
+// right-hand side: the node's block minus the Schur terms of the run ending
+// at it and of the run starting at it, in that order), then thread per
+// off-diagonal entry (the pair block minus the Schur term of the run between
+// the two nodes), written with its transpose.  S holds zeros elsewhere.
+__global__ __launch_bounds__(kPgThreads) void k_pg_sep_assemble(PGView v, PGWork w, PGSep sp) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int idx = blockIdx.x * kPgThreads + threadIdx.x;
+  const int ns = 3 * sp.nsep;
+  double* S = w.H;
+  if (idx < sp.nsep) {
+    const int k = sp.sep_nodes[idx];
+    const int rb = sp.sep_runs[2 * idx], ra = sp.sep_runs[2 * idx + 1];
+    const int c = col_of(v, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double h = sp.nodeblk[9 * (size_t)k + 3 * i + j];
+        if (rb >= 0) h -= sp.contrib[33 * (size_t)rb + 18 + 3 * i + j];
+        if (ra >= 0) h -= sp.contrib[33 * (size_t)ra + 3 * i + j];
+        S[(size_t)(3 * idx + i) * ns + 3 * idx + j] = h;
+      }
+      double b = w.rhs[c + i];
+      if (rb >= 0) b -= sp.contrib[33 * (size_t)rb + 30 + i];
+      if (ra >= 0) b -= sp.contrib[33 * (size_t)ra + 27 + i];
+      sp.rhs_s[3 * idx + i] = b;
+    }
+    return;
+  }
+  const int e = idx - sp.nsep;
+  if (e >= sp.nent) return;
+  const int u = sp.entries[4 * e], q = sp.entries[4 * e + 1], pq = sp.entries[4 * e + 2], rr = sp.entries[4 * e + 3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double h = pq >= 0 ? sp.pairblk[9 * (size_t)pq + 3 * i + j] : 0.0;
+      if (rr >= 0) h -= sp.contrib[33 * (size_t)rr + 9 + 3 * i + j];
+      S[(size_t)(3 * u + i) * ns + 3 * q + j] = h;
+      S[(size_t)(3 * q + j) * ns + 3 * u + i] = h;
+    }
+}
+
+// dx of every free node: separator unknowns from x_s; each run (one thread)
+// by forward substitution of b - H(run, ends) x_ends, then backward.
+__global__ __launch_bounds__(64) void k_pg_backsub(PGView v, PGWork w, PGSep sp) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step) return;
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx < sp.nsep) {
+    const int c = col_of(v, sp.sep_nodes[idx]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) w.dx[c + i] = sp.x_s[3 * idx + i];
+    return;
+  }
+  const int r = idx - sp.nsep;
+  if (r >= sp.runs) return;
+  const int s = sp.run_nodes[4 * r], t = sp.run_nodes[4 * r + 1];
+  const int L = sp.run_nodes[4 * r + 2], R = sp.run_nodes[4 * r + 3];
+  const int pl = sp.run_pairs[2 * r], pr = sp.run_pairs[2 * r + 1];
+  // the ends' solution (read from x_s: other threads of this launch write
+  // the separator nodes' dx)
+  double xl[3] = {0, 0, 0}, xr[3] = {0, 0, 0};
+  if (L >= 0)
+    for (int i = 0; i < 3; ++i) xl[i] = sp.x_s[3 * L + i];
+  if (R >= 0)
+    for (int i = 0; i < 3; ++i) xr[i] = sp.x_s[3 * R + i];
+  double vp[3] = {0, 0, 0};
+  for (int k = s; k <= t; ++k) {
+    const double* f = sp.fac + 15 * (size_t)k;
+    double Li[6], Ls[9];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Li[i] = f[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ls[i] = f[6 + i];
+    const int c = col_of(v, k);
+    double tvec[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tvec[i] = w.rhs[c + i] - (Ls[3 * i] * vp[0] + Ls[3 * i + 1] * vp[1] + Ls[3 * i + 2] * vp[2]);
+    if (k == s && L >= 0) {
+      // - H(s, L) x_L, H(s, L) the transpose of the pair block H(L, s)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        tvec[i] -= sp.pairblk[9 * (size_t)pl + i] * xl[0] + sp.pairblk[9 * (size_t)pl + 3 + i] * xl[1] +
+                   sp.pairblk[9 * (size_t)pl + 6 + i] * xl[2];
+    }
+    if (k == t && R >= 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        tvec[i] -= sp.pairblk[9 * (size_t)pr + 3 * i] * xr[0] + sp.pairblk[9 * (size_t)pr + 3 * i + 1] * xr[1] +
+                   sp.pairblk[9 * (size_t)pr + 3 * i + 2] * xr[2];
+    }
+    lower_apply(Li, tvec, vp);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) w.dx[c + i] = vp[i];
+  }
+  // backward: x_k = Li_k^T (v_k - L_k+1,k^T x_k+1)
+  double xn[3] = {0, 0, 0};
+  for (int k = t; k >= s; --k) {
+    const double* f = sp.fac + 15 * (size_t)k;
+    double Li[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Li[i] = f[i];
+    const int c = col_of(v, k);
+    double tvec[3] = {w.dx[c], w.dx[c + 1], w.dx[c + 2]};
+    if (k < t) {
+      const double* Lsn = sp.fac + 15 * (size_t)(k + 1) + 6;  // L_k+1,k
+#pragma unroll
+      for (int i = 0; i < 3; ++i) tvec[i] -= Lsn[i] * xn[0] + Lsn[3 + i] * xn[1] + Lsn[6 + i] * xn[2];
+    }
+    lower_apply_t(Li, tvec, xn);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) w.dx[c + i] = xn[i];
+  }
 }
 
 struct PGCounts {
@@ -349,6 +660,14 @@ int blocks_of(int n) { return n > 0 ? (n + kPgThreads - 1) / kPgThreads : 1; }
 
 struct PGLayout {
   PGWork w{};
+  PGSep sep{};
+  // separator tables, uploaded at plan creation (sizes bounded by N and P)
+  int32_t* run_nodes = nullptr;   // [N][4]
+  int32_t* run_pairs = nullptr;   // [N][2]
+  int32_t* chain_pair = nullptr;  // [N]
+  int32_t* sep_nodes = nullptr;   // [N]
+  int32_t* sep_runs = nullptr;    // [N][2]
+  int32_t* entries = nullptr;     // [P + N][4]
   size_t bytes = 0;
 };
 
@@ -371,8 +690,109 @@ PGLayout pg_carve(const ct_pose2d_problem* p, const ct_lm_options* o, void* base
   L.w.records = cv.take<double>((size_t)(o->max_num_iterations + 2) * CT_REC_FIELDS);
   L.w.state = cv.take<LMState>(1);
   L.w.solve_tmp = cv.take<double>(rcs_cholesky_tmp_doubles(n) + 1);
+  // separator form: the separator system lives in H (3 nsep <= n)
+  const size_t P = p->num_pairs;
+  L.sep.nodeblk = cv.take<double>(9 * N);
+  L.sep.pairblk = cv.take<double>(9 * P + 1);
+  L.sep.fac = cv.take<double>(15 * N);
+  L.sep.contrib = cv.take<double>(33 * N);
+  L.sep.rhs_s = cv.take<double>(3 * N);
+  L.sep.x_s = cv.take<double>(3 * N);
+  L.run_nodes = cv.take<int32_t>(4 * N);
+  L.run_pairs = cv.take<int32_t>(2 * N);
+  L.chain_pair = cv.take<int32_t>(N);
+  L.sep_nodes = cv.take<int32_t>(N);
+  L.sep_runs = cv.take<int32_t>(2 * N);
+  L.entries = cv.take<int32_t>(4 * (P + N));
   L.bytes = cv.offset + 256;
   return L;
+}
+
+// Separator form of the plan (see the file comment): which nodes are
+// separator nodes, the runs between them, and the off-diagonal separator
+// blocks.  *use is false when no node lies in a run (the dense form then).
+int build_separator(const ct_pose2d_problem& p, const std::vector<int32_t>& ed, const std::vector<int32_t>& pn,
+                    PGLayout& L, bool* use, int* nsep_out) {
+  *use = false;
+  const int N = p.num_nodes, E = p.num_edges, P = p.num_pairs, fixed = p.fixed_node;
+  std::vector<char> is_sep(N, 0);
+  if (fixed >= 0) is_sep[fixed] = 1;
+  for (int e = 0; e < E; ++e) {
+    const int a = ed[2 * e], b = ed[2 * e + 1];
+    if (a - b != 1 && b - a != 1) is_sep[a] = is_sep[b] = 1;
+  }
+  int run_nodes_total = 0;
+  for (int k = 0; k < N; ++k) run_nodes_total += !is_sep[k];
+  if (run_nodes_total == 0) return CT_OK;
+  // pair index of each joined pair u < v
+  std::unordered_map<int64_t, int> pair_of;
+  pair_of.reserve(2 * (size_t)P + 1);
+  for (int q = 0; q < P; ++q) pair_of[(int64_t)pn[2 * q] * N + pn[2 * q + 1]] = q;
+  auto find_pair = [&](int u, int v) {
+    const auto it = pair_of.find((int64_t)u * N + v);
+    return it == pair_of.end() ? -1 : it->second;
+  };
+  std::vector<int32_t> sep_index(N, -1), sep_nodes;
+  for (int k = 0; k < N; ++k)
+    if (is_sep[k] && k != fixed) {
+      sep_index[k] = (int32_t)sep_nodes.size();
+      sep_nodes.push_back(k);
+    }
+  const int nsep = (int)sep_nodes.size();
+  std::vector<int32_t> run_nodes, run_pairs, chain_pair(N, -1), sep_runs(2 * (size_t)nsep, -1);
+  // (u, v) separator unknowns -> (pair, run)
+  std::map<std::pair<int, int>, std::pair<int, int>> ent;
+  for (int q = 0; q < P; ++q) {
+    const int u = sep_index[pn[2 * q]], v = sep_index[pn[2 * q + 1]];
+    if (u >= 0 && v >= 0) ent[{u, v}] = {q, -1};
+  }
+  for (int k = 0; k < N;) {
+    if (is_sep[k]) {
+      ++k;
+      continue;
+    }
+    const int s = k;
+    while (k + 1 < N && !is_sep[k + 1]) ++k;
+    const int t = k++;
+    const int r = (int)(run_nodes.size() / 4);
+    for (int j = s + 1; j <= t; ++j) chain_pair[j] = find_pair(j - 1, j);
+    // an end couples when it is a separator unknown joined to the run by an edge
+    int pl = s > 0 && sep_index[s - 1] >= 0 ? find_pair(s - 1, s) : -1;
+    int pr = t + 1 < N && sep_index[t + 1] >= 0 ? find_pair(t, t + 1) : -1;
+    const int sl = pl >= 0 ? sep_index[s - 1] : -1, sr = pr >= 0 ? sep_index[t + 1] : -1;
+    run_nodes.insert(run_nodes.end(), {s, t, sl, sr});
+    run_pairs.insert(run_pairs.end(), {pl, pr});
+    if (sl >= 0) sep_runs[2 * (size_t)sl + 1] = r;
+    if (sr >= 0) sep_runs[2 * (size_t)sr] = r;
+    if (sl >= 0 && sr >= 0) {
+      auto it = ent.find({sl, sr});
+      if (it == ent.end())
+        ent[{sl, sr}] = {-1, r};
+      else
+        it->second.second = r;
+    }
+  }
+  std::vector<int32_t> entries;
+  for (const auto& kv : ent) entries.insert(entries.end(), {kv.first.first, kv.first.second, kv.second.first, kv.second.second});
+  const std::pair<int32_t*, const std::vector<int32_t>*> uploads[] = {
+      {L.run_nodes, &run_nodes}, {L.run_pairs, &run_pairs}, {L.chain_pair, &chain_pair},
+      {L.sep_nodes, &sep_nodes}, {L.sep_runs, &sep_runs},   {L.entries, &entries}};
+  for (const auto& u : uploads)
+    if (!u.second->empty())
+      CT_HIP(hipMemcpy(u.first, u.second->data(), sizeof(int32_t) * u.second->size(), hipMemcpyHostToDevice));
+  PGSep& sp = L.sep;
+  sp.runs = (int32_t)(run_nodes.size() / 4);
+  sp.nsep = nsep;
+  sp.nent = (int32_t)(entries.size() / 4);
+  sp.run_nodes = L.run_nodes;
+  sp.run_pairs = L.run_pairs;
+  sp.chain_pair = L.chain_pair;
+  sp.sep_nodes = L.sep_nodes;
+  sp.sep_runs = L.sep_runs;
+  sp.entries = L.entries;
+  *nsep_out = nsep;
+  *use = true;
+  return CT_OK;
 }
 
 int validate_pg(const ct_pose2d_problem* p) {
@@ -395,11 +815,12 @@ int copy_back(std::vector<T>& h, const T* d, size_t count) {
   return CT_OK;
 }
 
-// every index the kernels follow, checked once on the host
-int validate_pg_tables(const ct_pose2d_problem& p) {
+// every index the kernels follow, checked once on the host; ed and pn keep
+// the host copies of the edges and pair nodes
+int validate_pg_tables(const ct_pose2d_problem& p, std::vector<int32_t>& ed, std::vector<int32_t>& pn) {
   const int N = p.num_nodes, E = p.num_edges, P = p.num_pairs;
   if (E == 0) return CT_OK;
-  std::vector<int32_t> ed, no, ne, pn, po, pe;
+  std::vector<int32_t> no, ne, po, pe;
   CT_TRY(copy_back(ed, p.edges, 2 * (size_t)E));
   CT_TRY(copy_back(no, p.node_offsets, (size_t)N + 1));
   CT_TRY(copy_back(ne, p.node_edges, 2 * (size_t)E));
@@ -445,6 +866,8 @@ struct ct_pose2d_plan {
   PGView view;
   PGLayout layout;
   PGCounts counts;
+  bool separator = false;  // the separator form (else the dense form)
+  int solve_n = 0;         // unknowns of the system factored densely per iteration
   double* user_poses = nullptr;
   hipStream_t stream = nullptr;
   LMDriver driver;
@@ -455,27 +878,49 @@ namespace {
 int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
   const PGView& v = pl->view;
   PGWork& w = pl->layout.w;
+  const PGSep& sp = pl->layout.sep;
+  const bool sep = pl->separator;
   if (v.E > 0) {
     hipLaunchKernelGGL(k_pg_eval, dim3(blocks_of(v.E)), dim3(kPgThreads), 0, s, v, w);
     CT_LAUNCH_CHECK();
   }
-  if (v.n > 0) {
-    const size_t count = (size_t)v.n * v.n;
+  // the dense system, or the separator system, starts from zeros
+  if (pl->solve_n > 0) {
+    const size_t count = (size_t)pl->solve_n * pl->solve_n;
     int grid = (int)((count + kPgThreads - 1) / kPgThreads);
     if (grid > 4096) grid = 4096;
     hipLaunchKernelGGL(k_pg_zero, dim3(grid), dim3(kPgThreads), 0, s, w, count);
     CT_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_pg_node, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w);
+  if (sep)
+    hipLaunchKernelGGL(k_pg_node<true>, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w, sp);
+  else
+    hipLaunchKernelGGL(k_pg_node<false>, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w, sp);
   CT_LAUNCH_CHECK();
   if (v.P > 0) {
-    hipLaunchKernelGGL(k_pg_pair, dim3(blocks_of(v.P)), dim3(kPgThreads), 0, s, v, w);
+    if (sep)
+      hipLaunchKernelGGL(k_pg_pair<true>, dim3(blocks_of(v.P)), dim3(kPgThreads), 0, s, v, w, sp);
+    else
+      hipLaunchKernelGGL(k_pg_pair<false>, dim3(blocks_of(v.P)), dim3(kPgThreads), 0, s, v, w, sp);
     CT_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_pg_begin, dim3(1), dim3(kPgThreads), 0, s, pl->counts, w);
   CT_LAUNCH_CHECK();
+  if (!evaluation_only && sep) {
+    if (sp.runs > 0) {
+      hipLaunchKernelGGL(k_pg_runs, dim3((sp.runs + 63) / 64), dim3(64), 0, s, v, w, sp);
+      CT_LAUNCH_CHECK();
+    }
+    if (sp.nsep > 0) {
+      hipLaunchKernelGGL(k_pg_sep_assemble, dim3(blocks_of(sp.nsep + sp.nent)), dim3(kPgThreads), 0, s, v, w, sp);
+      CT_LAUNCH_CHECK();
+      CT_TRY(launch_rcs_cholesky_solve(w.H, sp.rhs_s, sp.x_s, 3 * sp.nsep, w.state, w.solve_tmp, s));
+    }
+    hipLaunchKernelGGL(k_pg_backsub, dim3((sp.nsep + sp.runs + 63) / 64), dim3(64), 0, s, v, w, sp);
+    CT_LAUNCH_CHECK();
+  }
   if (!evaluation_only) {
-    if (v.n > 0) CT_TRY(launch_rcs_cholesky_solve(w.H, w.rhs, w.dx, v.n, w.state, w.solve_tmp, s));
+    if (!sep && v.n > 0) CT_TRY(launch_rcs_cholesky_solve(w.H, w.rhs, w.dx, v.n, w.state, w.solve_tmp, s));
     hipLaunchKernelGGL(k_pg_step, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w);
     CT_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pg_cand, dim3(blocks_of(v.E)), dim3(kPgThreads), 0, s, v, w);
@@ -514,7 +959,11 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
     set_error("workspace too small: need " + std::to_string(need) + " bytes");
     return CT_ERR_WORKSPACE;
   }
-  CT_TRY(validate_pg_tables(*problem));
+  // the tables are read back with synchronous copies, which do not wait for a
+  // non-blocking stream: the caller's uploads on `stream` finish first
+  CT_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  std::vector<int32_t> ed, pn;
+  CT_TRY(validate_pg_tables(*problem, ed, pn));
   auto pl = std::make_unique<ct_pose2d_plan>();
   pl->problem = *problem;
   pl->options = *options;
@@ -537,8 +986,11 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
   pl->counts.gn = blocks_of(v.N);
   pl->user_poses = poses;
   pl->stream = reinterpret_cast<hipStream_t>(stream);
-  if (v.n > 0) {
-    CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, v.n, pl->stream));
+  int nsep = 0;
+  CT_TRY(build_separator(*problem, ed, pn, pl->layout, &pl->separator, &nsep));
+  pl->solve_n = pl->separator ? 3 * nsep : v.n;
+  if (pl->solve_n > 0) {
+    CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, pl->solve_n, pl->stream));
     CT_HIP(hipStreamSynchronize(pl->stream));
   }
   ct_pose2d_plan* raw = pl.get();
@@ -569,6 +1021,13 @@ int ct_pose2d_plan_run(ct_pose2d_plan* plan, ct_lm_report* report, double* recor
   clear_error();
   CT_CHECK_ARG(plan != nullptr, "plan is null");
   return lm_driver_run(plan->driver, report, records);
+}
+
+int ct_pose2d_plan_system_size(const ct_pose2d_plan* plan, int32_t* unknowns) {
+  clear_error();
+  CT_CHECK_ARG(plan != nullptr && unknowns != nullptr, "null argument");
+  *unknowns = plan->solve_n;
+  return CT_OK;
 }
 
 int ct_pose2d_plan_destroy(ct_pose2d_plan* plan) {

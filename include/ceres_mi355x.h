@@ -234,7 +234,9 @@ typedef struct ct_ba_plan ct_ba_plan; /* opaque, host memory only */
  * and receive the final (best) state, as Problem parameter tensors do in the
  * reference (problem.py:457-460, solver.py:672).  Replaces the per-iteration
  * Python loop of solver.py:77-345 + Problem.evaluate (problem.py:377-413) +
- * solve_linear_system(DENSE_SCHUR / ITERATIVE_SCHUR) (linear.py:141-246).   */
+ * solve_linear_system(DENSE_SCHUR / ITERATIVE_SCHUR) (linear.py:141-246).
+ * Synchronises stream before reading the problem's tables back on the host
+ * (the caller's uploads on it complete first).                             */
 int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options, double* cameras,
                       double* points, void* workspace, size_t workspace_bytes, ct_stream_t stream,
                       ct_ba_plan** plan);
@@ -332,7 +334,9 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
                    double* params, ct_lm_report* results, double* records, void* workspace, size_t workspace_bytes,
                    ct_stream_t stream);
 
-/* Host check of offsets (and order) of a batch; synchronous.                */
+/* Host check of offsets (and order) of a batch; synchronous.  It reads the
+ * arrays with a plain device-to-host copy, which does not wait for work on a
+ * non-blocking stream: the caller's uploads of them must be complete.       */
 int ct_curve_validate(const ct_curve_batch* batch, int32_t* first_bad_problem);
 
 /* ------------------------------------------------------------------------ */
@@ -362,10 +366,19 @@ typedef struct ct_pose2d_plan ct_pose2d_plan;
 
 int ct_pose2d_workspace_bytes(const ct_pose2d_problem* problem, const ct_lm_options* options, size_t* bytes);
 /* poses [N][3] FP64 device: start values in; ct_pose2d_plan_run leaves the
- * best values there.  Validates every table index on the host.              */
+ * best values there.  Validates every table index on the host, after
+ * synchronising stream (the caller's uploads on it complete first).        */
 int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options* options, double* poses,
                           void* workspace, size_t workspace_bytes, ct_stream_t stream, ct_pose2d_plan** plan);
 int ct_pose2d_plan_run(ct_pose2d_plan* plan, ct_lm_report* report, double* records);
+/* Unknowns of the system the plan factors densely each iteration: 3 x (free
+ * nodes) in the dense form, 3 x (separator nodes) in the separator form (the
+ * nodes touched by a non-chain edge; the runs of consecutive nodes between
+ * them joined only by chain edges (i, i + 1) are eliminated as block
+ * tridiagonal systems).  The plan takes the separator form whenever some node
+ * lies in such a run.  No counterpart in the reference, whose DENSE_QR step
+ * (examples/pose_graph_2d.py:42-45) both forms reproduce exactly.            */
+int ct_pose2d_plan_system_size(const ct_pose2d_plan* plan, int32_t* unknowns);
 int ct_pose2d_plan_destroy(ct_pose2d_plan* plan);
 
 /* ------------------------------------------------------------------------ */
