@@ -553,12 +553,45 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=
     return out
 
 
+def profile_pose_graph(num_nodes: int, solves: int = 3) -> dict:
+    """Per-solve time of one SE2 pose graph of the serving mix's largest size (seeded; LM with the reference
+    example's default options, host-driven loop so every kernel launches), and the size of the system the plan
+    factors densely per iteration."""
+    import torch
+
+    import ceres_mi355x as tc
+    from ceres_mi355x.ba import lm_options
+    from ceres_mi355x.pose2d import PoseGraphDevice
+    from oracle import generators as gen
+
+    data = gen.make_pose_graph(num_nodes=num_nodes, seed=7)
+    dev = PoseGraphDevice(data.poses, data.edges, data.measurements)
+    opts = lm_options(tc.SolverOptions(), precision="fp64", loop="host")
+    out = {"workload": f"SE2 pose graph, {num_nodes} nodes, {len(data.edges)} edges (chain plus "
+                       f"{len(data.edges) - num_nodes + 1} random loop closures)",
+           "unknowns": 3 * (num_nodes - 1), "factored_unknowns": dev.system_size(opts)}
+    dev.run(opts)  # plan, first touch
+    times, iters = [], []
+    for _ in range(solves):
+        dev.set_poses(data.poses)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep, _ = dev.run(opts)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters.append(rep.num_records - 1)
+    out.update(ms_per_solve=1e3 * min(times), lm_iterations=iters[-1], final_cost=rep.final_cost)
+    dev.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16"])
+    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16", "pose2000"],
+                    help="pose2000: time and profile one 2000-node SE2 pose graph solve (no JSON line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only-value", action="store_true", help="time the value path only")
@@ -580,6 +613,9 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     name = args.workload
+    if name == "pose2000":
+        log(f"[bench] pose graph: {profile_pose_graph(2000)}")
+        return 0
     t0 = time.perf_counter()
     data = gen.ba_config(name)
     dev = make_device(data)
