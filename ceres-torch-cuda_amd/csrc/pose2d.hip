@@ -340,11 +340,33 @@ __global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
   double GLL[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, GLb[3] = {0, 0, 0};
   double Li[6] = {0, 0, 0, 0, 0, 0};
   bool ok = true;
+  // node k's blocks are loaded during step k - 1 (the pair index one step
+  // earlier still), so a step waits for no load of its own
+  double An[9], Bn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bn[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) An[i] = sp.nodeblk[9 * (size_t)s + i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, s) + i];
+  int qn = s + 1 <= t ? sp.chain_pair[s + 1] : -1;
   for (int k = s; k <= t; ++k) {
+    double A[9], B[9], bk[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      A[i] = An[i];
+      B[i] = Bn[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bk[i] = bn[i];
+    if (k + 1 <= t) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) An[i] = sp.nodeblk[9 * (size_t)(k + 1) + i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, k + 1) + i];
+      block_t(sp.pairblk, qn, Bn);
+      qn = k + 2 <= t ? sp.chain_pair[k + 2] : -1;
+    }
     double Ls[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (k > s) {
-      double B[9];
-      block_t(sp.pairblk, sp.chain_pair[k], B);
       // L_k,k-1 = B Li_prev^T: row i is Li_prev applied to row i of B
 #pragma unroll
       for (int i = 0; i < 3; ++i) lower_apply(Li, B + 3 * i, Ls + 3 * i);
@@ -354,7 +376,7 @@ __global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        M[3 * i + j] = sp.nodeblk[9 * (size_t)k + 3 * i + j] -
+        M[3 * i + j] = A[3 * i + j] -
                        (Ls[3 * i] * Ls[3 * j] + Ls[3 * i + 1] * Ls[3 * j + 1] + Ls[3 * i + 2] * Ls[3 * j + 2]);
     if (!chol3_inverse(M, Li)) ok = false;
     double* f = sp.fac + 15 * (size_t)k;
@@ -366,13 +388,12 @@ __global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
     double TL[9], tb[3];
     double HL[9];
     block_t(sp.pairblk, k == s && L >= 0 ? pl : -1, HL);
-    const int c = col_of(v, k);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
       for (int j = 0; j < 3; ++j)
         TL[3 * i + j] = HL[3 * i + j] - (Ls[3 * i] * WL[j] + Ls[3 * i + 1] * WL[3 + j] + Ls[3 * i + 2] * WL[6 + j]);
-      tb[i] = w.rhs[c + i] - (Ls[3 * i] * wb[0] + Ls[3 * i + 1] * wb[1] + Ls[3 * i + 2] * wb[2]);
+      tb[i] = bk[i] - (Ls[3 * i] * wb[0] + Ls[3 * i + 1] * wb[1] + Ls[3 * i + 2] * wb[2]);
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -486,17 +507,30 @@ __global__ __launch_bounds__(64) void k_pg_backsub(PGView v, PGWork w, PGSep sp)
   if (R >= 0)
     for (int i = 0; i < 3; ++i) xr[i] = sp.x_s[3 * R + i];
   double vp[3] = {0, 0, 0};
+  // as in k_pg_runs, node k's factor and right-hand side load during step k - 1
+  double fn[15], bn[3];
+#pragma unroll
+  for (int i = 0; i < 15; ++i) fn[i] = sp.fac[15 * (size_t)s + i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, s) + i];
   for (int k = s; k <= t; ++k) {
-    const double* f = sp.fac + 15 * (size_t)k;
-    double Li[6], Ls[9];
+    double Li[6], Ls[9], bk[3];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) Li[i] = f[i];
+    for (int i = 0; i < 6; ++i) Li[i] = fn[i];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Ls[i] = f[6 + i];
+    for (int i = 0; i < 9; ++i) Ls[i] = fn[6 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bk[i] = bn[i];
+    if (k + 1 <= t) {
+#pragma unroll
+      for (int i = 0; i < 15; ++i) fn[i] = sp.fac[15 * (size_t)(k + 1) + i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, k + 1) + i];
+    }
     const int c = col_of(v, k);
     double tvec[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) tvec[i] = w.rhs[c + i] - (Ls[3 * i] * vp[0] + Ls[3 * i + 1] * vp[1] + Ls[3 * i + 2] * vp[2]);
+    for (int i = 0; i < 3; ++i) tvec[i] = bk[i] - (Ls[3 * i] * vp[0] + Ls[3 * i + 1] * vp[1] + Ls[3 * i + 2] * vp[2]);
     if (k == s && L >= 0) {
       // - H(s, L) x_L, H(s, L) the transpose of the pair block H(L, s)
 #pragma unroll
@@ -514,23 +548,39 @@ __global__ __launch_bounds__(64) void k_pg_backsub(PGView v, PGWork w, PGSep sp)
 #pragma unroll
     for (int i = 0; i < 3; ++i) w.dx[c + i] = vp[i];
   }
-  // backward: x_k = Li_k^T (v_k - L_k+1,k^T x_k+1)
+  // backward: x_k = Li_k^T (v_k - L_k+1,k^T x_k+1); node k's Li and v load
+  // during step k + 1, and L_k+1,k is node k + 1's own sub-diagonal block
   double xn[3] = {0, 0, 0};
-  for (int k = t; k >= s; --k) {
-    const double* f = sp.fac + 15 * (size_t)k;
-    double Li[6];
+  double Lsn[9];  // L_k+1,k
+  double gn[6], vn[3];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) Li[i] = f[i];
-    const int c = col_of(v, k);
-    double tvec[3] = {w.dx[c], w.dx[c + 1], w.dx[c + 2]};
+  for (int i = 0; i < 6; ++i) gn[i] = sp.fac[15 * (size_t)t + i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) vn[i] = w.dx[col_of(v, t) + i];
+  for (int k = t; k >= s; --k) {
+    double Li[6], tvec[3], Lk[9];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Li[i] = gn[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tvec[i] = vn[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Lk[i] = sp.fac[15 * (size_t)k + 6 + i];  // L_k,k-1, used at step k - 1
+    if (k - 1 >= s) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) gn[i] = sp.fac[15 * (size_t)(k - 1) + i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) vn[i] = w.dx[col_of(v, k - 1) + i];
+    }
     if (k < t) {
-      const double* Lsn = sp.fac + 15 * (size_t)(k + 1) + 6;  // L_k+1,k
 #pragma unroll
       for (int i = 0; i < 3; ++i) tvec[i] -= Lsn[i] * xn[0] + Lsn[3 + i] * xn[1] + Lsn[6 + i] * xn[2];
     }
     lower_apply_t(Li, tvec, xn);
+    const int c = col_of(v, k);
 #pragma unroll
     for (int i = 0; i < 3; ++i) w.dx[c + i] = xn[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Lsn[i] = Lk[i];
   }
 }
 
