@@ -3,7 +3,11 @@
 // launches, and where its LMState and records live; the driver runs the loop
 // host-driven (one synchronisation per iteration) or device-driven (graph
 // replays of iterations_per_graph iterations; kernels return at once after
-// the LM has finished).
+// the LM has finished).  The device loop keeps one replay queued ahead of the
+// one it waits for: after each replay the state's done flag is copied to
+// pinned memory behind an event, and the host queues replay k only after the
+// flag of replay k - 2 has arrived (and is unset), so at most one replay runs
+// past the end of the solve.
 #pragma once
 
 #include <functional>
@@ -21,6 +25,8 @@ struct LMDriver {
   int max_iter = 0;
   int loop_mode = CT_LOOP_HOST;
   LMState* host_state = nullptr;  // pinned
+  int32_t* host_done = nullptr;   // pinned [2]: done flag after the replays of even / odd index
+  hipEvent_t done_event[2] = {nullptr, nullptr};
   LMState* dev_state = nullptr;
   double* dev_records = nullptr;
   std::function<int(hipStream_t, bool)> enqueue_iteration;  // (stream, evaluation only)

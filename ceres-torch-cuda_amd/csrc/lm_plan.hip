@@ -14,7 +14,11 @@ int lm_driver_setup(LMDriver& d, const ct_lm_options& o, hipStream_t stream, LMS
   if (per > o.max_num_iterations + 2) per = o.max_num_iterations + 2;
   d.graph_iterations = per;
   CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.host_state), sizeof(LMState), hipHostMallocDefault));
-  if (d.loop_mode == CT_LOOP_DEVICE) CT_HIP(hipStreamCreateWithFlags(&d.capture_stream, hipStreamNonBlocking));
+  if (d.loop_mode == CT_LOOP_DEVICE) {
+    CT_HIP(hipStreamCreateWithFlags(&d.capture_stream, hipStreamNonBlocking));
+    CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.host_done), 2 * sizeof(int32_t), hipHostMallocDefault));
+    for (hipEvent_t& e : d.done_event) CT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   return CT_OK;
 }
 
@@ -23,6 +27,12 @@ void lm_driver_release(LMDriver& d) {
   if (d.graph) (void)hipGraphDestroy(d.graph);
   if (d.capture_stream) (void)hipStreamDestroy(d.capture_stream);
   if (d.host_state) (void)hipHostFree(d.host_state);
+  if (d.host_done) (void)hipHostFree(d.host_done);
+  for (hipEvent_t& e : d.done_event) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  d.host_done = nullptr;
   d.graph_exec = nullptr;
   d.graph = nullptr;
   d.capture_stream = nullptr;
@@ -78,7 +88,17 @@ int lm_driver_run(LMDriver& d, ct_lm_report* report, double* records) {
     hipStream_t cs = d.capture_stream;
     CT_TRY(d.launch_init(cs));
     const int launches = (max_passes + d.graph_iterations - 1) / d.graph_iterations;
-    for (int k = 0; k < launches; ++k) CT_HIP(hipGraphLaunch(d.graph_exec, cs));
+    const int32_t* dev_done = &d.dev_state->done;
+    for (int k = 0; k < launches; ++k) {
+      if (k >= 2) {
+        // replay k - 2's flag (replay k - 1 keeps the device busy meanwhile)
+        CT_HIP(hipEventSynchronize(d.done_event[k & 1]));
+        if (d.host_done[k & 1]) break;
+      }
+      CT_HIP(hipGraphLaunch(d.graph_exec, cs));
+      CT_HIP(hipMemcpyAsync(&d.host_done[k & 1], dev_done, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
+      CT_HIP(hipEventRecord(d.done_event[k & 1], cs));
+    }
     CT_TRY(d.launch_finalize(cs));
     CT_HIP(hipEventRecord(ev_out, cs));
     CT_HIP(hipStreamWaitEvent(s, ev_out, 0));
