@@ -230,3 +230,22 @@ def test_iterative_schur_needs_the_points_as_the_eliminated_group():
         _check_elimination(problem2, _BAExtract(problem2), opts)
     # exact solvers take any order
     _check_elimination(problem2, _BAExtract(problem2), tc.SolverOptions(linear_solver_type=tc.LinearSolverType.DENSE_SCHUR))
+
+
+def test_pose2d_system_size_query_and_workspace_bounds():
+    """ct_pose2d_plan_system_size rejects null arguments without touching the device; the workspace
+    query bounds the separator tables by the node and pair counts (it grows with the pairs)."""
+    lib = nat.load()
+    n = ctypes.c_int32(-7)
+    assert lib.ct_pose2d_plan_system_size(None, ctypes.byref(n)) == nat.CT_ERR_ARGUMENT
+    assert lib.ct_pose2d_plan_system_size(None, None) == nat.CT_ERR_ARGUMENT
+    assert n.value == -7
+    opts = tc.solver.lm_options(tc.SolverOptions(max_num_iterations=10))
+    sizes = []
+    for pairs in (10, 100):
+        p = nat.ct_pose2d_problem()
+        p.num_nodes, p.num_edges, p.num_pairs, p.fixed_node = 50, 0, pairs, 0  # sizes only: no tables needed
+        nbytes = ctypes.c_size_t(0)
+        assert lib.ct_pose2d_workspace_bytes(ctypes.byref(p), ctypes.byref(opts), ctypes.byref(nbytes)) == 0
+        sizes.append(nbytes.value)
+    assert sizes[1] > sizes[0] > 0
