@@ -323,16 +323,22 @@ __device__ __forceinline__ void block_t(const double* __restrict__ pairblk, int 
     for (int j = 0; j < 3; ++j) B[3 * i + j] = q >= 0 ? pairblk[9 * (size_t)q + 3 * j + i] : 0.0;
 }
 
-// One thread per run s..t: block Cholesky along the chain,
+// One wavefront per run s..t: block Cholesky along the chain,
 //   L_kk L_kk^T = A_k - L_k,k-1 L_k,k-1^T,  L_k,k-1 = H(k, k-1) L_k-1,k-1^-T,
 // forward substitution of the couplings to the run's ends and of its
 // right-hand side, W = L^-1 [H(run, L) | H(run, R) | b], and the Schur terms
-// W^T W onto the ends.  The right coupling is non-zero only at t.
-__global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
+// W^T W onto the ends.  The right coupling is non-zero only at t.  The
+// recurrence is sequential along the chain: the wavefront stages kRunChunk
+// nodes' blocks in LDS (one load per lane, all in flight together) and lane 0
+// walks them.
+constexpr int kRunChunk = 64;
+
+__global__ __launch_bounds__(kRunChunk) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
+  __shared__ double sA[kRunChunk][9], sB[kRunChunk][9], sb[kRunChunk][3];
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  if (r >= sp.runs) return;
+  const int r = blockIdx.x;  // the grid has exactly one workgroup per run
+  const int lane = threadIdx.x;
   const int s = sp.run_nodes[4 * r], t = sp.run_nodes[4 * r + 1];
   const int L = sp.run_nodes[4 * r + 2], R = sp.run_nodes[4 * r + 3];
   const int pl = sp.run_pairs[2 * r], pr = sp.run_pairs[2 * r + 1];
@@ -340,76 +346,77 @@ __global__ __launch_bounds__(64) void k_pg_runs(PGView v, PGWork w, PGSep sp) {
   double GLL[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, GLb[3] = {0, 0, 0};
   double Li[6] = {0, 0, 0, 0, 0, 0};
   bool ok = true;
-  // node k's blocks are loaded during step k - 1 (the pair index one step
-  // earlier still), so a step waits for no load of its own
-  double An[9], Bn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bn[3];
+  for (int c0 = s; c0 <= t; c0 += kRunChunk) {
+    const int c1 = min(t, c0 + kRunChunk - 1);
+    {
+      const int k = c0 + lane;
+      if (k <= c1) {
+        double B[9];
+        block_t(sp.pairblk, k > s ? sp.chain_pair[k] : -1, B);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) An[i] = sp.nodeblk[9 * (size_t)s + i];
+        for (int i = 0; i < 9; ++i) {
+          sA[lane][i] = sp.nodeblk[9 * (size_t)k + i];
+          sB[lane][i] = B[i];
+        }
 #pragma unroll
-  for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, s) + i];
-  int qn = s + 1 <= t ? sp.chain_pair[s + 1] : -1;
-  for (int k = s; k <= t; ++k) {
-    double A[9], B[9], bk[3];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      A[i] = An[i];
-      B[i] = Bn[i];
+        for (int i = 0; i < 3; ++i) sb[lane][i] = w.rhs[col_of(v, k) + i];
+      }
     }
+    __syncthreads();
+    if (lane == 0) {
+      for (int k = c0; k <= c1; ++k) {
+        const int j = k - c0;
+        double Ls[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (k > s) {
+          // L_k,k-1 = B Li_prev^T: row i is Li_prev applied to row i of B
 #pragma unroll
-    for (int i = 0; i < 3; ++i) bk[i] = bn[i];
-    if (k + 1 <= t) {
+          for (int i = 0; i < 3; ++i) lower_apply(Li, &sB[j][3 * i], Ls + 3 * i);
+        }
+        double M[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) An[i] = sp.nodeblk[9 * (size_t)(k + 1) + i];
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, k + 1) + i];
-      block_t(sp.pairblk, qn, Bn);
-      qn = k + 2 <= t ? sp.chain_pair[k + 2] : -1;
+          for (int jj = 0; jj < 3; ++jj)
+            M[3 * i + jj] = sA[j][3 * i + jj] -
+                            (Ls[3 * i] * Ls[3 * jj] + Ls[3 * i + 1] * Ls[3 * jj + 1] + Ls[3 * i + 2] * Ls[3 * jj + 2]);
+        if (!chol3_inverse(M, Li)) ok = false;
+        double* f = sp.fac + 15 * (size_t)k;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) f[i] = Li[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[6 + i] = Ls[i];
+        // T = [H(k, L) | b_k] - L_k,k-1 [W_L | w_b]_(k-1), then W_k = Li T
+        double TL[9], tb[3];
+        double HL[9];
+        block_t(sp.pairblk, k == s && L >= 0 ? pl : -1, HL);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj)
+            TL[3 * i + jj] =
+                HL[3 * i + jj] - (Ls[3 * i] * WL[jj] + Ls[3 * i + 1] * WL[3 + jj] + Ls[3 * i + 2] * WL[6 + jj]);
+          tb[i] = sb[j][i] - (Ls[3 * i] * wb[0] + Ls[3 * i + 1] * wb[1] + Ls[3 * i + 2] * wb[2]);
+        }
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+          const double col[3] = {TL[jj], TL[3 + jj], TL[6 + jj]};
+          double out[3];
+          lower_apply(Li, col, out);
+          WL[jj] = out[0]; WL[3 + jj] = out[1]; WL[6 + jj] = out[2];
+        }
+        lower_apply(Li, tb, wb);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj)
+            GLL[3 * i + jj] += WL[i] * WL[jj] + WL[3 + i] * WL[3 + jj] + WL[6 + i] * WL[6 + jj];
+          GLb[i] += WL[i] * wb[0] + WL[3 + i] * wb[1] + WL[6 + i] * wb[2];
+        }
+      }
     }
-    double Ls[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (k > s) {
-      // L_k,k-1 = B Li_prev^T: row i is Li_prev applied to row i of B
-#pragma unroll
-      for (int i = 0; i < 3; ++i) lower_apply(Li, B + 3 * i, Ls + 3 * i);
-    }
-    double M[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        M[3 * i + j] = A[3 * i + j] -
-                       (Ls[3 * i] * Ls[3 * j] + Ls[3 * i + 1] * Ls[3 * j + 1] + Ls[3 * i + 2] * Ls[3 * j + 2]);
-    if (!chol3_inverse(M, Li)) ok = false;
-    double* f = sp.fac + 15 * (size_t)k;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) f[i] = Li[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) f[6 + i] = Ls[i];
-    // T = [H(k, L) | b_k] - L_k,k-1 [W_L | w_b]_(k-1), then W_k = Li T
-    double TL[9], tb[3];
-    double HL[9];
-    block_t(sp.pairblk, k == s && L >= 0 ? pl : -1, HL);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        TL[3 * i + j] = HL[3 * i + j] - (Ls[3 * i] * WL[j] + Ls[3 * i + 1] * WL[3 + j] + Ls[3 * i + 2] * WL[6 + j]);
-      tb[i] = bk[i] - (Ls[3 * i] * wb[0] + Ls[3 * i + 1] * wb[1] + Ls[3 * i + 2] * wb[2]);
-    }
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double col[3] = {TL[j], TL[3 + j], TL[6 + j]};
-      double out[3];
-      lower_apply(Li, col, out);
-      WL[j] = out[0]; WL[3 + j] = out[1]; WL[6 + j] = out[2];
-    }
-    lower_apply(Li, tb, wb);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-      for (int j = 0; j < 3; ++j) GLL[3 * i + j] += WL[i] * WL[j] + WL[3 + i] * WL[3 + j] + WL[6 + i] * WL[6 + j];
-      GLb[i] += WL[i] * wb[0] + WL[3 + i] * wb[1] + WL[6 + i] * wb[2];
-    }
+    __syncthreads();
   }
+  if (lane != 0) return;
   double* out = sp.contrib + 33 * (size_t)r;
   double WR[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (R >= 0) {
@@ -482,105 +489,117 @@ __global__ __launch_bounds__(kPgThreads) void k_pg_sep_assemble(PGView v, PGWork
     }
 }
 
-// dx of every free node: separator unknowns from x_s; each run (one thread)
-// by forward substitution of b - H(run, ends) x_ends, then backward.
-__global__ __launch_bounds__(64) void k_pg_backsub(PGView v, PGWork w, PGSep sp) {
+// dx of every free node.  Workgroups [0, runs): one wavefront per run, as in
+// k_pg_runs: forward substitution of b - H(run, ends) x_ends, then backward,
+// each over LDS-staged chunks walked by lane 0 (the forward values pass
+// through dx).  Workgroups from runs on copy the separator unknowns from x_s.
+__global__ __launch_bounds__(kRunChunk) void k_pg_backsub(PGView v, PGWork w, PGSep sp) {
+  __shared__ double sF[kRunChunk][15], sv[kRunChunk][3];
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
-  const int idx = blockIdx.x * 64 + threadIdx.x;
-  if (idx < sp.nsep) {
-    const int c = col_of(v, sp.sep_nodes[idx]);
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x >= sp.runs) {
+    const int idx = ((int)blockIdx.x - sp.runs) * kRunChunk + lane;
+    if (idx < sp.nsep) {
+      const int c = col_of(v, sp.sep_nodes[idx]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) w.dx[c + i] = sp.x_s[3 * idx + i];
+      for (int i = 0; i < 3; ++i) w.dx[c + i] = sp.x_s[3 * idx + i];
+    }
     return;
   }
-  const int r = idx - sp.nsep;
-  if (r >= sp.runs) return;
+  const int r = blockIdx.x;
   const int s = sp.run_nodes[4 * r], t = sp.run_nodes[4 * r + 1];
   const int L = sp.run_nodes[4 * r + 2], R = sp.run_nodes[4 * r + 3];
   const int pl = sp.run_pairs[2 * r], pr = sp.run_pairs[2 * r + 1];
-  // the ends' solution (read from x_s: other threads of this launch write
-  // the separator nodes' dx)
+  // the ends' solution (from x_s: other workgroups of this launch write the
+  // separator nodes' dx)
   double xl[3] = {0, 0, 0}, xr[3] = {0, 0, 0};
   if (L >= 0)
     for (int i = 0; i < 3; ++i) xl[i] = sp.x_s[3 * L + i];
   if (R >= 0)
     for (int i = 0; i < 3; ++i) xr[i] = sp.x_s[3 * R + i];
+  // forward: v_k = Li_k (b_k - couplings - L_k,k-1 v_k-1)
   double vp[3] = {0, 0, 0};
-  // as in k_pg_runs, node k's factor and right-hand side load during step k - 1
-  double fn[15], bn[3];
+  for (int c0 = s; c0 <= t; c0 += kRunChunk) {
+    const int c1 = min(t, c0 + kRunChunk - 1);
+    {
+      const int k = c0 + lane;
+      if (k <= c1) {
 #pragma unroll
-  for (int i = 0; i < 15; ++i) fn[i] = sp.fac[15 * (size_t)s + i];
+        for (int i = 0; i < 15; ++i) sF[lane][i] = sp.fac[15 * (size_t)k + i];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, s) + i];
-  for (int k = s; k <= t; ++k) {
-    double Li[6], Ls[9], bk[3];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) Li[i] = fn[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Ls[i] = fn[6 + i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) bk[i] = bn[i];
-    if (k + 1 <= t) {
-#pragma unroll
-      for (int i = 0; i < 15; ++i) fn[i] = sp.fac[15 * (size_t)(k + 1) + i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) bn[i] = w.rhs[col_of(v, k + 1) + i];
+        for (int i = 0; i < 3; ++i) sv[lane][i] = w.rhs[col_of(v, k) + i];
+      }
     }
-    const int c = col_of(v, k);
-    double tvec[3];
+    __syncthreads();
+    if (lane == 0) {
+      for (int k = c0; k <= c1; ++k) {
+        const int j = k - c0;
+        double Li[6];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) tvec[i] = bk[i] - (Ls[3 * i] * vp[0] + Ls[3 * i + 1] * vp[1] + Ls[3 * i + 2] * vp[2]);
-    if (k == s && L >= 0) {
-      // - H(s, L) x_L, H(s, L) the transpose of the pair block H(L, s)
+        for (int i = 0; i < 6; ++i) Li[i] = sF[j][i];
+        const double* Ls = &sF[j][6];
+        double tvec[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-        tvec[i] -= sp.pairblk[9 * (size_t)pl + i] * xl[0] + sp.pairblk[9 * (size_t)pl + 3 + i] * xl[1] +
-                   sp.pairblk[9 * (size_t)pl + 6 + i] * xl[2];
+        for (int i = 0; i < 3; ++i)
+          tvec[i] = sv[j][i] - (Ls[3 * i] * vp[0] + Ls[3 * i + 1] * vp[1] + Ls[3 * i + 2] * vp[2]);
+        if (k == s && L >= 0) {
+          // - H(s, L) x_L, H(s, L) the transpose of the pair block H(L, s)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            tvec[i] -= sp.pairblk[9 * (size_t)pl + i] * xl[0] + sp.pairblk[9 * (size_t)pl + 3 + i] * xl[1] +
+                       sp.pairblk[9 * (size_t)pl + 6 + i] * xl[2];
+        }
+        if (k == t && R >= 0) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            tvec[i] -= sp.pairblk[9 * (size_t)pr + 3 * i] * xr[0] + sp.pairblk[9 * (size_t)pr + 3 * i + 1] * xr[1] +
+                       sp.pairblk[9 * (size_t)pr + 3 * i + 2] * xr[2];
+        }
+        lower_apply(Li, tvec, vp);
+        const int c = col_of(v, k);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) w.dx[c + i] = vp[i];
+      }
     }
-    if (k == t && R >= 0) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        tvec[i] -= sp.pairblk[9 * (size_t)pr + 3 * i] * xr[0] + sp.pairblk[9 * (size_t)pr + 3 * i + 1] * xr[1] +
-                   sp.pairblk[9 * (size_t)pr + 3 * i + 2] * xr[2];
-    }
-    lower_apply(Li, tvec, vp);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) w.dx[c + i] = vp[i];
+    __syncthreads();
   }
-  // backward: x_k = Li_k^T (v_k - L_k+1,k^T x_k+1); node k's Li and v load
-  // during step k + 1, and L_k+1,k is node k + 1's own sub-diagonal block
+  // backward, chunks from the end: x_k = Li_k^T (v_k - L_k+1,k^T x_k+1)
   double xn[3] = {0, 0, 0};
-  double Lsn[9];  // L_k+1,k
-  double gn[6], vn[3];
+  double Lsn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // L_k+1,k
+  for (int e1 = t; e1 >= s; e1 -= kRunChunk) {
+    const int e0 = max(s, e1 - kRunChunk + 1);
+    {
+      const int k = e0 + lane;
+      if (k <= e1) {
 #pragma unroll
-  for (int i = 0; i < 6; ++i) gn[i] = sp.fac[15 * (size_t)t + i];
+        for (int i = 0; i < 15; ++i) sF[lane][i] = sp.fac[15 * (size_t)k + i];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) vn[i] = w.dx[col_of(v, t) + i];
-  for (int k = t; k >= s; --k) {
-    double Li[6], tvec[3], Lk[9];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) Li[i] = gn[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) tvec[i] = vn[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Lk[i] = sp.fac[15 * (size_t)k + 6 + i];  // L_k,k-1, used at step k - 1
-    if (k - 1 >= s) {
-#pragma unroll
-      for (int i = 0; i < 6; ++i) gn[i] = sp.fac[15 * (size_t)(k - 1) + i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) vn[i] = w.dx[col_of(v, k - 1) + i];
+        for (int i = 0; i < 3; ++i) sv[lane][i] = w.dx[col_of(v, k) + i];  // v_k, written by lane 0 above
+      }
     }
-    if (k < t) {
+    __syncthreads();
+    if (lane == 0) {
+      for (int k = e1; k >= e0; --k) {
+        const int j = k - e0;
+        double Li[6], tvec[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) tvec[i] -= Lsn[i] * xn[0] + Lsn[3 + i] * xn[1] + Lsn[6 + i] * xn[2];
+        for (int i = 0; i < 6; ++i) Li[i] = sF[j][i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tvec[i] = sv[j][i];
+        if (k < t) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) tvec[i] -= Lsn[i] * xn[0] + Lsn[3 + i] * xn[1] + Lsn[6 + i] * xn[2];
+        }
+        lower_apply_t(Li, tvec, xn);
+        const int c = col_of(v, k);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) w.dx[c + i] = xn[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Lsn[i] = sF[j][6 + i];
+      }
     }
-    lower_apply_t(Li, tvec, xn);
-    const int c = col_of(v, k);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) w.dx[c + i] = xn[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Lsn[i] = Lk[i];
+    __syncthreads();
   }
 }
 
@@ -958,7 +977,7 @@ int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
   CT_LAUNCH_CHECK();
   if (!evaluation_only && sep) {
     if (sp.runs > 0) {
-      hipLaunchKernelGGL(k_pg_runs, dim3((sp.runs + 63) / 64), dim3(64), 0, s, v, w, sp);
+      hipLaunchKernelGGL(k_pg_runs, dim3(sp.runs), dim3(kRunChunk), 0, s, v, w, sp);
       CT_LAUNCH_CHECK();
     }
     if (sp.nsep > 0) {
@@ -966,7 +985,9 @@ int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
       CT_LAUNCH_CHECK();
       CT_TRY(launch_rcs_cholesky_solve(w.H, sp.rhs_s, sp.x_s, 3 * sp.nsep, w.state, w.solve_tmp, s));
     }
-    hipLaunchKernelGGL(k_pg_backsub, dim3((sp.nsep + sp.runs + 63) / 64), dim3(64), 0, s, v, w, sp);
+    const int sep_groups = (sp.nsep + kRunChunk - 1) / kRunChunk;
+    if (sp.runs + sep_groups > 0)
+      hipLaunchKernelGGL(k_pg_backsub, dim3(sp.runs + sep_groups), dim3(kRunChunk), 0, s, v, w, sp);
     CT_LAUNCH_CHECK();
   }
   if (!evaluation_only) {
