@@ -192,6 +192,7 @@ struct ct_ba_plan {
   LMDriver driver;
   int num_items = 0;
   int num_groups = 0;       // point groups of the fused evaluation (0: two-kernel evaluation)
+  bool unobserved_points = false;  // some point has no observation (the fused evaluation then covers it)
 };
 
 namespace {
@@ -210,8 +211,8 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
   switch (group) {
     case CT_KERNEL_EVAL_OBS:
       if (pl->num_groups > 0)
-        return mixed ? launch_eval_fused<float>(v, L.bf, L.w, pl->num_groups, s)
-                     : launch_eval_fused<double>(v, L.bd, L.w, pl->num_groups, s);
+        return mixed ? launch_eval_fused<float>(v, L.bf, L.w, pl->num_groups, pl->unobserved_points, s)
+                     : launch_eval_fused<double>(v, L.bd, L.w, pl->num_groups, pl->unobserved_points, s);
       return mixed ? launch_eval_obs<float>(v, L.bf, L.w, s) : launch_eval_obs<double>(v, L.bd, L.w, s);
     case CT_KERNEL_EVAL_POINTS:
       if (pl->num_groups > 0) return CT_OK;  // the fused evaluation covered the points
@@ -413,6 +414,7 @@ int build_point_groups(ct_ba_plan* pl) {
   int used = 0;
   for (int q = 0; q < p.num_points; ++q) {
     const int k = po[q + 1] - po[q];
+    if (k == 0) pl->unobserved_points = true;
     if (k > kWave) return CT_OK;
     if (starts.empty() || used + k > kWave) {
       starts.push_back(q);

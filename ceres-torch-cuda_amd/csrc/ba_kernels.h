@@ -21,7 +21,8 @@ int launch_candidate(const BAView& v, const BABuffers<real>& b, const BAWork& w,
 // points, one lane per observation); replaces k_eval_obs + k_eval_points.
 constexpr int kFusedThreads = 256;
 template <typename real>
-int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, hipStream_t s);
+int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups,
+                      bool unobserved_points, hipStream_t s);
 // Fused step + candidate over the same point groups (replaces k_step_points +
 // k_candidate_obs); backsub = false takes dx_p from the refinement.
 template <typename real>

@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_points(BAView v, BABu
 // ---------------------------------------------------------------------------
 template <typename real>
 __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffers<real> b, BAWork w,
-                                                              int num_groups) {
+                                                              int num_groups, int unobserved_points) {
   constexpr int kWaves = kFusedThreads / kWave;
   __shared__ double red[kWaves];
   __shared__ double term[kWaves][9][kWave];   // per-observation V (6, assembly precision) and g (3) terms
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
     term[wave][8][lane] = (double)a2 * r[0] + (double)c2 * r[1];
   }
   __syncthreads();
-  if (grp < num_groups) {
+  if (unobserved_points && grp < num_groups) {
     // the group's points without observations have no lane of their own
     const int q0 = w.point_groups[grp], q1 = w.point_groups[grp + 1];
     for (int q = q0 + lane; q < q1; q += kWave)
@@ -1392,11 +1392,12 @@ int launch_eval_obs(const BAView& v, const BABuffers<real>& b, const BAWork& w, 
 }
 
 template <typename real>
-int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups, hipStream_t s) {
+int launch_eval_fused(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_groups,
+                      bool unobserved_points, hipStream_t s) {
   if (num_groups > 0) {
     const int waves = kFusedThreads / kWave;
     hipLaunchKernelGGL(k_eval_fused<real>, dim3((num_groups + waves - 1) / waves), dim3(kFusedThreads), 0, s, v, b, w,
-                       num_groups);
+                       num_groups, unobserved_points ? 1 : 0);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -1547,8 +1548,10 @@ template int launch_step_candidate<float>(const BAView&, const BABuffers<float>&
                                          hipStream_t);
 template int launch_step_candidate<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, bool,
                                           hipStream_t);
-template int launch_eval_fused<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
-template int launch_eval_fused<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
+template int launch_eval_fused<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, bool,
+                                       hipStream_t);
+template int launch_eval_fused<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, bool,
+                                        hipStream_t);
 template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 template int launch_dogleg_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
