@@ -13,16 +13,18 @@ Here the same names (plus ``dense_schur``) are served by the HIP library:
 ``ct_normal_equations_solve_f64`` forms ``A^T A`` with a tiled Gram kernel and
 solves it with the FP64 Cholesky of the reduced-camera-system solver.  The
 Schur-complement names return the same solution (eliminating unknowns of an
-SPD system does not change its solution).  The functions work with the
-reference's own registry too::
+SPD system does not change its solution).  The functions plug into the
+reference's own registry too; they then raise its exception class and return
+its result class::
 
     import ceres_torch.linear as ref_linear
     from ceres_mi355x.linear import register_rocm_backends
-    register_rocm_backends(register=ref_linear.register_optional_backend)
+    register_rocm_backends(ref_linear)
 """
 from __future__ import annotations
 
 import ctypes
+import sys
 from dataclasses import dataclass
 from typing import Any, Callable, Optional
 
@@ -144,17 +146,43 @@ def rocm_block_schur(A: torch.Tensor, b: torch.Tensor, *, damping: Optional[torc
     return _solve(A, b, damping, int(num_eliminate), "mi355x block schur (HIP Gram + FP64 Cholesky)")
 
 
-def register_rocm_backends(*, overwrite: bool = True,
-                           register: Optional[Callable[[str, OptionalBackend], None]] = None,
-                           get: Optional[Callable[[str], Optional[OptionalBackend]]] = None) -> tuple[str, ...]:
+def _adapt_to_registry(fn: OptionalBackend, registry: Any, name: str) -> OptionalBackend:
+    """Wrap a backend so that it speaks a foreign registry's types.
+
+    The reference's dispatcher (``linear.py:1311-1328``) falls back only on its
+    own ``OptionalBackendUnavailable`` and accepts a ``LinearSolverResult``
+    only when it is an instance of its own class; anything else it passes to
+    ``_summarize_solution`` as the solution tensor.  So a decline is re-raised
+    as the owner's exception and a result is rebuilt from the owner's
+    ``LinearSolverResult`` / ``LinearSolverSummary``.
+    """
+    owner_unavailable = registry.OptionalBackendUnavailable
+    owner_result = registry.LinearSolverResult
+    owner_summary = registry.LinearSolverSummary
+
+    def backend(A: torch.Tensor, b: torch.Tensor, **kwargs: Any):
+        try:
+            res = fn(A, b, **kwargs)
+        except OptionalBackendUnavailable as exc:
+            raise owner_unavailable(str(exc)) from exc
+        s = res.summary
+        return owner_result(res.x, owner_summary(s.residual_norm, s.num_iterations, s.success, s.message))
+
+    backend.__name__ = backend.__qualname__ = f"{fn.__name__}_for_{name}"
+    backend.__doc__ = fn.__doc__
+    return backend
+
+
+def register_rocm_backends(registry: Any = None, *, overwrite: bool = True) -> tuple[str, ...]:
     """Register the MI355X backends under the reference's plugin names (cuda_backends.py:158-207).
 
-    ``register`` / ``get`` default to this module's registry; pass the
-    reference's ``register_optional_backend`` / ``get_optional_backend`` to
-    plug into ``ceres_torch`` itself.  Returns the names registered.
+    ``registry`` is the module that owns the registry: this module by default,
+    or the reference's ``ceres_torch.linear``.  It must provide
+    ``register_optional_backend``, ``get_optional_backend``,
+    ``OptionalBackendUnavailable``, ``LinearSolverResult`` and
+    ``LinearSolverSummary``; for a foreign registry the backends raise its
+    exception class and return its result class.  Returns the names registered.
     """
-    register = register or register_optional_backend
-    get = get or get_optional_backend
     nat.load()
     backends = {
         "sparse_normal_cholesky": rocm_normal_equations,
@@ -163,16 +191,24 @@ def register_rocm_backends(*, overwrite: bool = True,
         "block_schur": rocm_block_schur,
         "dense_schur": rocm_block_schur,
     }
+    own = sys.modules[__name__]
+    if registry is None:
+        registry = own
+    for attr in ("register_optional_backend", "get_optional_backend", "OptionalBackendUnavailable",
+                 "LinearSolverResult", "LinearSolverSummary"):
+        if not hasattr(registry, attr):
+            raise TypeError(f"registry {registry!r} has no attribute {attr!r}")
     done = []
     for name, fn in backends.items():
-        if overwrite or get(name) is None:
-            register(name, fn)
+        if overwrite or registry.get_optional_backend(name) is None:
+            registry.register_optional_backend(name, fn if registry is own else _adapt_to_registry(fn, registry, name))
             done.append(name)
     return tuple(done)
 
 
-def unregister_rocm_backends(unregister: Optional[Callable[[str], None]] = None) -> None:
-    unregister = unregister or unregister_optional_backend
+def unregister_rocm_backends(registry: Any = None) -> None:
+    """Remove the names :func:`register_rocm_backends` registers from ``registry``."""
+    unregister = (registry or sys.modules[__name__]).unregister_optional_backend
     for name in ROCM_BACKEND_NAMES:
         unregister(name)
 

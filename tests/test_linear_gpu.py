@@ -73,6 +73,32 @@ def test_registry_contract_declines_host_tensors_and_dispatches():
     assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
 
 
+def test_backends_in_a_foreign_registry_solve_on_the_device():
+    # through a dispatcher with the reference's contract and its own classes:
+    # a device solve comes back as that registry's LinearSolverResult
+    import standin_registry as R
+
+    L.register_rocm_backends(R)
+    try:
+        rng = np.random.default_rng(7)
+        A = rng.normal(size=(60, 10))
+        b = rng.normal(size=60)
+        d = rng.uniform(0.1, 1.0, size=10)
+        for name in L.ROCM_BACKEND_NAMES:
+            res = R.try_optional_linear_backend(name, torch.tensor(A, device="cuda"), torch.tensor(b, device="cuda"),
+                                                damping=torch.tensor(d, device="cuda"), num_eliminate=4,
+                                                solver_type=None, block_sizes=None)
+            assert isinstance(res, R.LinearSolverResult) and isinstance(res.summary, R.LinearSolverSummary)
+            assert res.summary.success
+            ref = np.linalg.solve(A.T @ A + np.diag(d), A.T @ b)
+            np.testing.assert_allclose(res.x.cpu().numpy(), ref, rtol=1e-9, atol=1e-11)
+            assert res.summary.residual_norm > 0.0
+        # host tensors decline to None, so the reference falls back to its torch path
+        assert R.try_optional_linear_backend("block_schur", torch.tensor(A), torch.tensor(b), num_eliminate=4) is None
+    finally:
+        L.unregister_rocm_backends(R)
+
+
 # n <= 192: one-workgroup factorisation (tile boundaries at multiples of 16, the
 # right-hand-side row in a tile of its own at n = 176 and 192); above: tiled
 @pytest.mark.parametrize("n", [1, 9, 16, 17, 144, 175, 176, 191, 192, 193, 256, 257, 300, 320, 1000, 1800])

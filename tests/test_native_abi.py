@@ -187,6 +187,37 @@ def test_registry_declines_host_tensors_like_the_reference():
     assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
 
 
+def test_backends_in_a_foreign_registry_decline_with_its_exception():
+    # the reference's dispatcher catches only its own OptionalBackendUnavailable
+    # (linear.py:1311-1328): host tensors must come back as None through it
+    import torch
+
+    import standin_registry as R
+    from ceres_mi355x import linear as L
+
+    names = L.register_rocm_backends(R)
+    try:
+        assert set(names) == set(L.ROCM_BACKEND_NAMES)
+        A = torch.randn(20, 4, dtype=torch.float64)
+        b = torch.randn(20, dtype=torch.float64)
+        for name in L.ROCM_BACKEND_NAMES:
+            assert R.try_optional_linear_backend(name, A, b, damping=None, num_eliminate=1,
+                                                 solver_type=None, block_sizes=None) is None
+        with pytest.raises(R.OptionalBackendUnavailable):
+            R.get_optional_backend("block_schur")(A.to(torch.int64), b, num_eliminate=1)
+        # nothing registered in the package's own registry
+        assert all(L.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
+        # overwrite=False keeps the owner's existing registration
+        R.register_optional_backend("block_schur", lambda A, b, **kw: "theirs")
+        assert "block_schur" not in L.register_rocm_backends(R, overwrite=False)
+        assert R.get_optional_backend("block_schur")(A, b) == "theirs"
+    finally:
+        L.unregister_rocm_backends(R)
+    assert all(R.get_optional_backend(n) is None for n in L.ROCM_BACKEND_NAMES)
+    with pytest.raises(TypeError):
+        L.register_rocm_backends(object())
+
+
 def test_crs_matrix_round_trip_and_layout():
     # problem.py:15-41 of the reference: nonzeros row by row, rows = row starts
     import torch
