@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 8  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 9  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "dogleg",
            "step", "candidate", "lm_end"]
 REC = dict(
@@ -73,6 +73,8 @@ class ct_lm_options(ctypes.Structure):
         ("dogleg_type", ctypes.c_int32),
         ("preconditioner", ctypes.c_int32),
         ("max_solver_time_in_seconds", ctypes.c_double),
+        ("debug_invalid_steps", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
     ]
 
 

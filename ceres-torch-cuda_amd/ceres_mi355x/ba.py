@@ -10,6 +10,7 @@ observation pairs, then solved in FP64.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from dataclasses import dataclass
 
@@ -241,6 +242,28 @@ class BADevice:
             pass
 
 
+_debug_invalid_steps = 0
+
+
+@contextlib.contextmanager
+def debug_invalid_steps(count: int):
+    """Test hook: inside the block, the first ``count`` trust-region steps of
+    every bundle-adjustment solve are overwritten with NaN on the device.
+
+    This is the device form of the reference's own test, which patches the
+    linear solver to return NaN (``tests/test_problem_solver.py:637-664``): the
+    invalid-step branch of ``solver.py:168-190`` then runs.  Not for normal use.
+    """
+    global _debug_invalid_steps
+    if count < 0:
+        raise ValueError("count must be >= 0")
+    previous, _debug_invalid_steps = _debug_invalid_steps, int(count)
+    try:
+        yield
+    finally:
+        _debug_invalid_steps = previous
+
+
 def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: str = "dense_schur",
                iterations_per_graph: int = 8) -> nat.ct_lm_options:
     """Translate SolverOptions into the native option struct."""
@@ -272,6 +295,7 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     o.max_solver_time_in_seconds = float(getattr(options, "max_solver_time_in_seconds", 1e9))
     pc = getattr(options, "preconditioner_type", None)
     o.preconditioner = PRECONDITIONERS.get(pc.name, 0) if pc is not None else 0
+    o.debug_invalid_steps = _debug_invalid_steps
     return o
 
 

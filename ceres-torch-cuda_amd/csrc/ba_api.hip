@@ -48,6 +48,7 @@ int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o->dogleg_type == CT_TRADITIONAL_DOGLEG || o->dogleg_type == CT_SUBSPACE_DOGLEG, "unknown dogleg type");
   CT_CHECK_ARG(o->preconditioner >= CT_PRECONDITIONER_SCHUR_JACOBI && o->preconditioner <= CT_PRECONDITIONER_IDENTITY,
                "unknown preconditioner");
+  CT_CHECK_ARG(o->debug_invalid_steps >= 0, "debug_invalid_steps must be >= 0");
   return CT_OK;
 }
 
@@ -238,6 +239,7 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
       if (!dogleg_options(&pl->options)) return CT_OK;
       return mixed ? launch_dogleg_step<float>(v, L.bf, L.w, s) : launch_dogleg_step<double>(v, L.bd, L.w, s);
     case CT_KERNEL_STEP: {
+      if (pl->options.debug_invalid_steps > 0) CT_TRY(launch_poison_step(v, L.w, s));
       // the refinement and the dogleg leave the whole step in dxc / dxp
       const bool backsub = !refining(pl) && !dogleg_options(&pl->options);
       if (pl->num_groups > 0)

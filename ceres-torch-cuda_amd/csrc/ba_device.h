@@ -40,6 +40,8 @@ struct LMState {
   double dl_cgn, dl_cg;     // DOGLEG step = dl_cgn * (Gauss-Newton step) + dl_cg * g
   double max_time;          // seconds of device clock before NO_CONVERGENCE
   int32_t pending_time;     // the time test after an accepted step runs with the deferred checks
+  int32_t poison_left;      // test hook: steps still to be overwritten with NaN (debug_invalid_steps)
+  int32_t poison;           // this iteration's step is overwritten with NaN
   int32_t pad1;
   uint64_t t0;              // wall clock at start
   uint64_t t_end;           // wall clock at termination

@@ -35,6 +35,8 @@ int launch_backsub(const BAView& v, const BABuffers<real>& b, const BAWork& w, h
 template <typename real>
 int launch_dogleg_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 inline int dogleg_blocks(int num_points) { return num_points > 0 ? (num_points + kPointThreads - 1) / kPointThreads : 1; }
+// Test hook: NaN over the whole step when state->poison (ct_lm_options.debug_invalid_steps).
+int launch_poison_step(const BAView& v, const BAWork& w, hipStream_t s);
 template <typename real>
 int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,

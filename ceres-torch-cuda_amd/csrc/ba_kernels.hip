@@ -1292,6 +1292,18 @@ __global__ __launch_bounds__(kPointThreads) void k_dogleg_combine(BAView v, BABu
   for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < 3 * v.P; t += stride) w.dxp[t] = cgn * w.dxp[t] + cg * b.gp[t];
 }
 
+// Test hook (ct_lm_options.debug_invalid_steps): overwrite this iteration's
+// whole step with NaN, as a linear solver returning NaN would leave it; the
+// step kernels' finiteness test then rejects it.
+__global__ __launch_bounds__(kPointThreads) void k_poison_step(BAWork w, int n_cam, int n_pt) {
+  const LMState* st = w.state;
+  if (st->done || st->skip_step || !st->poison) return;
+  const double nan = __builtin_nan("");
+  const int stride = gridDim.x * kPointThreads;
+  for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < n_cam; t += stride) w.dxc[t] = nan;
+  for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < n_pt; t += stride) w.dxp[t] = nan;
+}
+
 // Plain evaluation (cost + corrected residuals) at given parameters.
 __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const double* cams, const double* pts,
                                                                   double* residuals, double* part) {
@@ -1490,6 +1502,14 @@ int launch_dogleg_step(const BAView& v, const BABuffers<real>& b, const BAWork& 
   const size_t work = std::max<size_t>(9 * (size_t)v.C, 3 * (size_t)v.P);
   const unsigned int grid = (unsigned int)std::min<size_t>(std::max<size_t>((work + kPointThreads - 1) / kPointThreads, 1), 4096);
   hipLaunchKernelGGL(k_dogleg_combine<real>, dim3(grid), dim3(kPointThreads), 0, s, v, b, w);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int launch_poison_step(const BAView& v, const BAWork& w, hipStream_t s) {
+  const size_t work = std::max<size_t>(std::max<size_t>(9 * (size_t)v.C, 3 * (size_t)v.P), 1);
+  const unsigned int grid = (unsigned int)std::min<size_t>((work + kPointThreads - 1) / kPointThreads, 1024);
+  hipLaunchKernelGGL(k_poison_step, dim3(grid), dim3(kPointThreads), 0, s, w, 9 * v.C, 3 * v.P);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -265,6 +265,10 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
     ::ct::set_error("DOGLEG is on the device path for bundle adjustment only");
     return CT_ERR_UNSUPPORTED;
   }
+  if (options->debug_invalid_steps != 0) {
+    ::ct::set_error("debug_invalid_steps is a bundle-adjustment test hook");
+    return CT_ERR_UNSUPPORTED;
+  }
   if (batch->num_problems == 0) return CT_OK;
   CT_CHECK_ARG(batch->offsets != nullptr && params != nullptr, "offsets / params are null");
   CT_CHECK_ARG(batch->num_samples == 0 || (batch->x != nullptr && batch->y != nullptr), "x / y are null");

@@ -231,6 +231,8 @@ __device__ __forceinline__ void lm_state_init(LMState* st, const ct_lm_options& 
   st->precond = o.preconditioner;
   st->max_time = o.max_solver_time_in_seconds;
   st->pending_time = 0;
+  st->poison_left = o.debug_invalid_steps > 0 ? o.debug_invalid_steps : 0;
+  st->poison = 0;
   st->dl_cgn = 1.0;
   st->dl_cg = 0.0;
   st->radius = o.initial_trust_region_radius;
@@ -322,6 +324,8 @@ __device__ __forceinline__ void lm_begin_update(LMState* st, double cost, double
   }
   st->skip_step = 0;
   if (st->strategy != CT_DOGLEG) st->num_linear_solves += 1;  // solver.py:129-133 counts no linear solve
+  st->poison = st->poison_left > 0;
+  if (st->poison) st->poison_left -= 1;
 }
 
 // End of the evaluation-only iteration 0.

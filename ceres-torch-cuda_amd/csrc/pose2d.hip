@@ -1025,6 +1025,10 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
     ::ct::set_error("DOGLEG is on the device path for bundle adjustment only");
     return CT_ERR_UNSUPPORTED;
   }
+  if (options->debug_invalid_steps != 0) {
+    ::ct::set_error("debug_invalid_steps is a bundle-adjustment test hook");
+    return CT_ERR_UNSUPPORTED;
+  }
   const size_t need = pg_carve(problem, options, nullptr).bytes;
   if (workspace_bytes < need) {
     set_error("workspace too small: need " + std::to_string(need) + " bytes");

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 8
+#define CT_ABI_VERSION 9
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -142,6 +142,12 @@ typedef struct ct_lm_options {
   int32_t preconditioner;               /* ct_preconditioner (ITERATIVE_SCHUR) */
   double max_solver_time_in_seconds;    /* device clock from the start of the solve (solver.py:342-345);
                                            SolverOptions' default is 1e9, and 0 stops after the first step */
+  int32_t debug_invalid_steps;          /* test hook (bundle adjustment; 0 in normal use): the first k
+                                           trust-region steps are overwritten with NaN on the device before
+                                           they are checked, so the invalid-step branch of solver.py:168-190
+                                           runs as after a linear solver returning NaN.  Curve fits and
+                                           pose graphs refuse a nonzero value. */
+  int32_t reserved0;
 } ct_lm_options;
 
 /* Termination codes match TerminationType (types.py:98-103). */

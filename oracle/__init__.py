@@ -16,6 +16,7 @@ trust-region semantics in numpy:
 
 How each function is pinned (golden vectors produced by running the reference
 in the build container, reference known-answer tests, closed forms, finite
-differences, brute force) is listed in ``tests/test_oracle.py``.  No oracle
-function is currently "parity unpinned".
+differences, brute force) is listed in the table at the top of
+``tests/test_oracle.py``; every public function used as a checker appears
+there with its pin.
 """

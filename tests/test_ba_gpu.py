@@ -370,3 +370,60 @@ def test_unobserved_blocks_and_single_observation_points_match_oracle(linear, lo
     unseen = np.bincount(data.obs_point, minlength=data.points.shape[0]) == 0
     np.testing.assert_array_equal(pts.cpu().numpy()[unseen], data.points[unseen])
     np.testing.assert_array_equal(cams.cpu().numpy()[5], data.cameras[5])
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_invalid_steps_end_in_failure_like_the_reference(loop):
+    """The reference's invalid-step test (tests/test_problem_solver.py:637-664)
+    on a bundle-adjustment problem: every step is overwritten with NaN on the
+    device; two in a row end the solve with FAILURE, the failing iteration is
+    not recorded, the radius goes 4 -> 2 and the parameters stay unchanged."""
+    print(f"invalid steps loop={loop}", flush=True)
+    from ceres_mi355x.ba import debug_invalid_steps
+
+    tc.set_device_loop(loop)
+    data = gen.ba_config("ba4")
+    problem, cams, pts = ba_problem(data)
+    cams0, pts0 = cams.clone(), pts.clone()
+    o = tc.SolverOptions(max_num_iterations=3, initial_trust_region_radius=4.0, max_num_consecutive_invalid_steps=2,
+                         gradient_tolerance=0.0, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR)
+    with debug_invalid_steps(100):
+        summary = tc.solve(o, problem)
+    assert summary.termination_type is tc.TerminationType.FAILURE
+    assert summary.message == "Too many invalid trust-region steps."
+    assert summary.num_unsuccessful_steps == 2
+    assert len(summary.iterations) == 2
+    assert summary.iterations[1].step_is_valid is False
+    assert summary.iterations[1].trust_region_radius == pytest.approx(2.0)
+    assert summary.final_cost == summary.initial_cost
+    assert torch.equal(cams, cams0) and torch.equal(pts, pts0)
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("linear", ["dense_schur", "iterative_schur"])
+def test_one_invalid_step_then_the_oracle_trajectory(loop, linear):
+    """Two NaN steps, then the solve continues: the whole trajectory, counters
+    included, matches the oracle whose first two linear solves return NaN."""
+    print(f"two invalid steps loop={loop} linear={linear}", flush=True)
+    from ceres_mi355x.ba import debug_invalid_steps
+    from test_oracle import PoisonedSolves
+
+    tc.set_device_loop(loop)
+    data = gen.ba_config("ba4")
+    problem, cams, pts = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=12, linear_solver_type=tc.LinearSolverType[linear.upper()],
+                         preconditioner_type=tc.PreconditionerType.SCHUR_JACOBI)
+    o.function_tolerance = o.gradient_tolerance = o.parameter_tolerance = 0.0
+    with debug_invalid_steps(2):
+        summary = tc.solve(o, problem)
+    oo = OracleOptions(max_num_iterations=12, function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+    if linear == "iterative_schur":
+        oo.preconditioner_type = "schur_jacobi"
+    ref, _ = trust_region_solve(PoisonedSolves(oba.BAModel(data, linear_solver=linear), 2), oo)
+    assert [it.step_is_valid for it in summary.iterations[:4]] == [True, False, False, True]
+    assert summary.num_unsuccessful_steps == ref.num_unsuccessful_steps
+    assert summary.num_successful_steps == ref.num_successful_steps
+    assert summary.num_residual_evaluations == ref.num_residual_evaluations
+    assert summary.num_linear_solves == ref.num_linear_solves
+    assert_trajectory_matches(summary, {"iterations": ref.iterations, "final_cost": ref.final_cost,
+                                        "initial_cost": ref.initial_cost})

@@ -15,7 +15,10 @@ formula:
 | lm.updated_trust_region_radius (DOGLEG) | known answers of each branch of solver.py:1051-1061 |
 | lm.traditional/subspace_dogleg_step     | brute force: the boundary crossing of the dogleg path on a fine grid; the model minimum over a polar grid of the 2-D subspace disc; normal equations for the interior step |
 | ba.BAModel.gauss_newton_step / jmul     | dense J^T J solve and dense J v on the same evaluation |
+| small_models.*.gauss_newton_step / jmul | least squares min |J dx + r| (numpy lstsq); J v vs central differences of the residual |
+| lm.refine_solution                      | reference KAT (tests/test_linear_schur.py:455-483); damped fixed point = (J^T J + D)^-1 J^T b |
 | lm.trust_region_solve (BA, curve, SE2)  | golden trajectories written by running the reference (tests/golden/) |
+| lm.trust_region_solve, invalid steps    | reference KAT (tests/test_problem_solver.py:637-664): NaN steps -> FAILURE, radius 4 -> 2 |
 | generators                              | sha256 of the inputs recorded with each golden trajectory |
 """
 from __future__ import annotations
@@ -27,8 +30,8 @@ import torch
 from oracle import ba as oba
 from oracle import generators as gen
 from oracle.lm import (LMRadius, OracleOptions, StepEvaluator, dense_linear_solve, orthonormal_basis,
-                       schur_solve_dense, subspace_dogleg_step, traditional_dogleg_step, trust_region_solve,
-                       updated_trust_region_radius)
+                       refine_solution, schur_solve_dense, subspace_dogleg_step, traditional_dogleg_step,
+                       trust_region_solve, updated_trust_region_radius)
 from oracle.losses import loss_rho, robustify
 from oracle.small_models import CurveModel, PoseGraphModel
 
@@ -265,6 +268,61 @@ def test_hello_world_known_answer():
     assert s.termination_type == "CONVERGENCE"
 
 
+class PoisonedSolves:
+    """Wraps an oracle model so that its first ``count`` linear solves return
+    NaN, as the reference's test patches ``solve_linear_system``
+    (tests/test_problem_solver.py:637-664)."""
+
+    def __init__(self, model, count: int):
+        self._model = model
+        self.left = count
+
+    def __getattr__(self, name):
+        return getattr(self._model, name)
+
+    def solve(self, ev, damping, opts):
+        step, iters = self._model.solve(ev, damping, opts)
+        if self.left > 0:
+            self.left -= 1
+            return np.full_like(step, np.nan), iters
+        return step, iters
+
+
+class _OneMinusX(_HelloWorld):
+    """residual 1 - x from x = 0 (tests/test_problem_solver.py:640-642)."""
+
+    def x0(self):
+        return np.array([0.0])
+
+    def evaluate(self, x, jacobian):
+        e, g = super().evaluate(x, jacobian)
+        e.r = np.array([1.0 - x[0]])
+        e.cost = 0.5 * float(e.r @ e.r)
+        return e, (e.J.T @ e.r if jacobian else None)
+
+
+def test_invalid_steps_known_answer():
+    # solver.py:168-190: every step NaN; two in a row end the solve with
+    # FAILURE; the failing iteration is not recorded; LM halves the radius
+    model = PoisonedSolves(_OneMinusX(), 100)
+    s, x = trust_region_solve(model, OracleOptions(max_num_iterations=3, initial_trust_region_radius=4.0,
+                                                   max_num_consecutive_invalid_steps=2, gradient_tolerance=0.0))
+    assert s.termination_type == "FAILURE"
+    assert s.message == "Too many invalid trust-region steps."
+    assert s.num_unsuccessful_steps == 2
+    assert len(s.iterations) == 2
+    assert s.iterations[1]["step_is_valid"] is False
+    assert s.iterations[1]["trust_region_radius"] == pytest.approx(2.0)
+    assert x[0] == 0.0 and s.final_cost == s.initial_cost == 0.5
+    # one NaN step, then the solve goes on from radius 2 (decrease factor 4 after the rejection)
+    s1, x1 = trust_region_solve(PoisonedSolves(_OneMinusX(), 1),
+                                OracleOptions(max_num_iterations=3, initial_trust_region_radius=4.0,
+                                              gradient_tolerance=0.0))
+    assert [it["step_is_valid"] for it in s1.iterations[:3]] == [True, False, True]
+    assert s1.iterations[1]["trust_region_radius"] == pytest.approx(2.0)
+    assert s1.iterations[2]["step_is_successful"] and s1.num_unsuccessful_steps == 1
+
+
 def test_tiny_bal_known_answer():
     # examples/simple_bundle_adjuster.py:142-176 with fixed cameras;
     # tests/test_examples.py:151-165 expects final_cost < 1e-12, points to 1e-5
@@ -498,6 +556,74 @@ def test_ba_gauss_newton_step_and_jmul_match_dense_algebra():
     np.testing.assert_allclose(model.gauss_newton_step(ev), np.linalg.solve(J.T @ J, -g), rtol=1e-8, atol=1e-12)
     v = np.random.default_rng(3).normal(size=J.shape[1])
     np.testing.assert_allclose(model.jmul(ev, v), J @ v, rtol=1e-12, atol=1e-12)
+
+
+def _small_model_cases():
+    batch = gen.make_curve_batch(num_problems=2, seed=5)
+    lo, hi = batch.offsets[0], batch.offsets[1]
+    curve = CurveModel(batch.x[lo:hi], batch.y[lo:hi], loss=None, m0=0.1, c0=0.2)
+    pg = gen.make_pose_graph(num_nodes=12, seed=6)
+    pose = PoseGraphModel(pg.poses, pg.edges, pg.measurements, loss=None)
+    return [curve, pose]
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_small_model_gauss_newton_step_and_jmul(which):
+    # gauss_newton_step is the minimiser of |J dx + r| (linear.py:570-571 solves
+    # the normal equations of it); numpy's lstsq reaches it by an SVD instead.
+    # jmul is J v for the Jacobian of the residual: with the trivial loss, the
+    # directional derivative of r along v, by central differences.
+    model = _small_model_cases()[which]
+    x0 = model.x0()
+    ev, _ = model.evaluate(x0, True)
+    gn = model.gauss_newton_step(ev)
+    ref = np.linalg.lstsq(ev.J, -ev.r, rcond=None)[0]
+    np.testing.assert_allclose(gn, ref, rtol=1e-7, atol=1e-10)
+    v = np.random.default_rng(11 + which).normal(size=ev.J.shape[1])
+    h = 1e-6
+    rp, _ = model.evaluate(model.apply(x0, h * v), False)
+    rm, _ = model.evaluate(model.apply(x0, -h * v), False)
+    fd = (rp.r - rm.r) / (2 * h)
+    np.testing.assert_allclose(model.jmul(ev, v), fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+
+
+def test_refine_solution_reference_known_answer():
+    # tests/test_linear_schur.py:455-483: a float32 least-squares solution of a
+    # nearly rank-deficient system, refined in float64 (linear.py:1331-1363),
+    # recovers the double solution and lowers the residual
+    A = np.array([[1.0, 1.0], [1.0, 1.0 + 1e-7], [1.0, 1.0 - 1e-7]])
+    true_x = np.array([2.0, -3.0])
+    b = A @ true_x
+    low = np.linalg.lstsq(A.astype(np.float32), b.astype(np.float32), rcond=None)[0].astype(np.float64)
+    refined = refine_solution(A, b, low, None, 5)
+    assert np.linalg.norm(b - A @ refined) < np.linalg.norm(b - A @ low)
+    np.testing.assert_allclose(refined, true_x, atol=1e-6, rtol=1e-6)
+    # no refinement steps: unchanged
+    assert refine_solution(A, b, low, None, 0) is low
+
+
+def test_refine_solution_damped_fixed_point():
+    # with damping the refinement works on [A; sqrt(D)] x = [b; 0]: from any
+    # start it lands on the damped normal-equation solution; a solution that
+    # already meets the 10 eps |b| test is returned as it is
+    rng = np.random.default_rng(12)
+    A = rng.normal(size=(30, 6))
+    b = rng.normal(size=30)
+    d = rng.uniform(0.5, 2.0, size=6)
+    exact = np.linalg.solve(A.T @ A + np.diag(d), A.T @ b)
+    out = refine_solution(A, b, np.zeros(6), d, 2)
+    np.testing.assert_allclose(out, exact, rtol=1e-12, atol=1e-13)
+    undamped = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert np.linalg.norm(out - undamped) > 1e-3  # the damping rows were not dropped
+    A2 = rng.normal(size=(6, 6))
+    x2 = rng.normal(size=6)
+    b2 = A2 @ x2
+    assert not np.any(b2 - A2 @ x2)  # the same product: the residual is exactly zero
+    np.testing.assert_array_equal(refine_solution(A2, b2, x2.copy(), None, 3), x2)
+    # dense_linear_solve refines the DENSE_SCHUR step only in the mixed option (linear.py:181-187)
+    o = OracleOptions(use_mixed_precision_solves=True, max_num_refinement_iterations=2)
+    x_mixed, _ = dense_linear_solve(A, b, d, 2, "dense_schur", o)
+    np.testing.assert_allclose(x_mixed, exact, rtol=1e-11, atol=1e-12)
 
 
 def _dogleg_opts(g):
