@@ -190,6 +190,8 @@ SIGNATURES = {
     "ct_dense_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "ct_normal_equations_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                                      _vp, _vp, ctypes.c_size_t, _vp]),
+    "ct_normal_equations_solve_f64_noalloc": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                             _vp, _vp, _vp, _vp]),
     "ct_block_schur_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp,
                                                 ctypes.c_int64, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_rcs_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),

@@ -5,7 +5,9 @@
 //   H = A^T A + diag(damping),  rhs = A^T b,  H x = rhs   (x = H^-1 A^T b)
 // H is formed by a tiled Gram kernel and solved with the FP64 Cholesky of the
 // reduced-camera-system solver.  Also the standalone factor / solve entry
-// points of that Cholesky.
+// points of that Cholesky, and a variant of the dense solve that allocates
+// nothing (its workspace is a static device array).
+#include <mutex>
 #include <vector>
 
 #include "ba_kernels.h"
@@ -158,6 +160,47 @@ int launch_normal_equations(const double* A, int64_t m, int n, int64_t lda, cons
   return launch_rcs_cholesky_solve(L.H, L.rhs, x, n, L.st, L.tmp, s);
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation dense solve: the workspace of ct_normal_equations_solve_f64
+// lives in this file-scope device array instead of a caller's buffer.  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is only a
+// stand-in for the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr int64_t kNoAllocMaxN = CT_NOALLOC_MAX_N;
+constexpr size_t kNoAllocScratchBytes = (size_t)4608 * 1024;  // >= the dense layout at n = 512 (4,502,624 bytes)
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up on the first call on that device).
+int scratch_address(void** out) {
+  int dev = -1;
+  CT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index out of range for the no-allocation scratch table");
+    return CT_ERR_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return CT_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CT_OK;
+}
+
+}  // namespace
+
 }  // namespace ct
 
 using namespace ct;
@@ -182,6 +225,29 @@ int ct_normal_equations_solve_f64(const double* A, int64_t m, int64_t n, int64_t
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   CT_TRY(launch_normal_equations(A, m, (int)n, lda, b, damping, x, workspace, s));
   return read_status(dense_layout((int)n, workspace).st, s);
+}
+
+int ct_normal_equations_solve_f64_noalloc(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
+                                          const double* damping, double* x, ct_stream_t stream) {
+  clear_error();
+  CT_TRY(check_dense_args(A, m, n, lda, b, x));
+  if (n > kNoAllocMaxN) {
+    set_error("n = " + std::to_string(n) + " exceeds the no-allocation limit of " + std::to_string(kNoAllocMaxN) +
+              " unknowns (use ct_normal_equations_solve_f64 with a workspace)");
+    return CT_ERR_ARGUMENT;
+  }
+  if (n == 0) return CT_OK;
+  if (dense_layout((int)n, nullptr).bytes > kNoAllocScratchBytes) {
+    set_error("no-allocation scratch too small for n = " + std::to_string(n));
+    return CT_ERR_ARGUMENT;
+  }
+  void* ws = nullptr;
+  CT_TRY(scratch_address(&ws));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the state word and the Cholesky's control words are cleared by fill
+  // kernels inside launch_normal_equations; H and rhs are written whole
+  CT_TRY(launch_normal_equations(A, m, (int)n, lda, b, damping, x, ws, s));
+  return read_status(dense_layout((int)n, ws).st, s);
 }
 
 int ct_block_schur_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,

@@ -399,6 +399,18 @@ int ct_dense_workspace_bytes(int64_t n, size_t* bytes);
 int ct_normal_equations_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
                                   const double* damping, double* x, void* workspace, size_t workspace_bytes,
                                   ct_stream_t stream);
+/* ct_normal_equations_solve_f64 without a workspace argument: no counterpart
+ * in the reference.  It is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the workspace is a static device array inside the library, one
+ * copy per GPU, held until the process ends (4.5 MiB), so the call itself
+ * allocates nothing.  n <= CT_NOALLOC_MAX_N; larger n returns CT_ERR_ARGUMENT.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams or threads) race and must be serialised by the caller.
+ * Synchronises the stream to report CT_ERR_NOT_SPD.                         */
+#define CT_NOALLOC_MAX_N 512
+int ct_normal_equations_solve_f64_noalloc(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
+                                          const double* damping, double* x, ct_stream_t stream);
 /* Schur complement solve with the first num_eliminate unknowns eliminated
  * (cuda_backends.py:232-264); the solution is that of the whole system.     */
 int ct_block_schur_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
