@@ -26,8 +26,10 @@ and the LM loop kept on the device (graph-captured iterations).  Beside it:
 
 The roofline entry covers the kernel with the largest share of an iteration,
 timed with HIP events on the stream the library launches on.  The CPU
-baseline times the float64 oracle (numpy, one thread) on two LM iterations of
-the same problem with the same linear solver.
+baseline times the float64 oracle (numpy) on one LM iteration of the whole
+problem with the same linear solver, BLAS on the host's cores (at most 16);
+beside it, the oracle's restatement of the reference's dense algebra on the
+16-camera workload.
 """
 from __future__ import annotations
 
@@ -315,14 +317,61 @@ def dense_cholesky_entry(dev, data, iters=5):
             "counters": "profiles/ba200_cholesky_mfma_counters.md"}
 
 
-def cpu_baseline(data, iterations=2, sample_points=20_000):
-    """Float64 oracle (numpy, ITERATIVE_SCHUR), one thread, on a bounded sample.
+def host_threads() -> int:
+    """Threads the CPU baselines use: the host cores this process may run on, at most 16 (one GPU's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n) or n), 16))
 
-    The sample is the same generator with the same cameras and observation
-    counts per point but ``sample_points`` points (about 1/5 of the
-    observations at the default workload).  Its LM iterations/s are scaled to
-    the full problem by the observation ratio (the per-iteration work is
-    dominated by the per-observation and per-observation-pair terms).
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return "unknown"
+
+
+def cpu_baseline(data, iterations=1):
+    """Float64 oracle (numpy, ITERATIVE_SCHUR) on the whole problem, on the host's cores.
+
+    One LM iteration of the full workload (the evaluation at the start, the
+    Schur-Jacobi PCG step and the candidate evaluation), as the GPU counts
+    iterations of back-to-back solves.  BLAS (the reduced camera system's
+    products and factorisations) runs on ``host_threads()`` threads; the
+    per-observation and per-point array arithmetic of numpy is single-threaded.
+    """
+    from threadpoolctl import threadpool_limits
+
+    from oracle.ba import BAModel
+    from oracle.lm import OracleOptions, trust_region_solve
+
+    threads = host_threads()
+    with threadpool_limits(limits=threads):
+        model = BAModel(data, linear_solver="iterative_schur")
+        opts = OracleOptions(max_num_iterations=iterations, function_tolerance=0.0, gradient_tolerance=0.0,
+                             parameter_tolerance=0.0, preconditioner_type="schur_jacobi")
+        t0 = time.perf_counter()
+        trust_region_solve(model, opts)
+        el = time.perf_counter() - t0
+    return dict(value=iterations / el, unit="LM iterations/s", cores=threads, kind="port",
+                sample=f"{iterations} LM iteration(s) of the float64 numpy oracle (block-sparse Schur, Schur-Jacobi "
+                       f"PCG) on the whole workload ({data.num_cameras} cameras / {data.num_points} points / "
+                       f"{data.num_observations} observations) in {el:.1f} s, evaluation at the start included; "
+                       f"BLAS on {threads} threads of {os.cpu_count()} ({_cpu_model()}), numpy array arithmetic "
+                       "single-threaded")
+
+
+def cpu_dense_faithful(name="ba16", iterations=1):
+    """The reference's own dense algebra, restated by the oracle, on the 16-camera workload.
+
+    Dense Jacobian (16,000 x 6,144), H = J^T J + D, LU of the eliminated point
+    block, LU of the Schur complement (linear.py:667-699), the same LM rules;
+    one LM iteration on the host's cores.
     """
     from threadpoolctl import threadpool_limits
 
@@ -330,25 +379,55 @@ def cpu_baseline(data, iterations=2, sample_points=20_000):
     from oracle.ba import BAModel
     from oracle.lm import OracleOptions, trust_region_solve
 
-    if data.num_points <= sample_points:
-        sample = data
-    else:
-        lo, hi = (int(v) for v in data.meta["obs_per_point"])
-        sample = gen.make_bundle_adjustment(num_cameras=data.num_cameras, num_points=sample_points,
-                                            obs_per_point=(lo, hi), seed=1)
-    with threadpool_limits(limits=1):
-        model = BAModel(sample, linear_solver="iterative_schur")
+    data = gen.ba_config(name)
+    threads = host_threads()
+    with threadpool_limits(limits=threads):
+        model = BAModel(data, linear_solver="dense_schur", dense_faithful=True)
         opts = OracleOptions(max_num_iterations=iterations, function_tolerance=0.0, gradient_tolerance=0.0,
-                             parameter_tolerance=0.0, preconditioner_type="schur_jacobi")
+                             parameter_tolerance=0.0)
         t0 = time.perf_counter()
         trust_region_solve(model, opts)
         el = time.perf_counter() - t0
-    scale = sample.num_observations / data.num_observations
-    return dict(value=iterations / el * scale, unit="LM iterations/s", cores=1, kind="port",
-                sample=f"{iterations} LM iterations of the float64 numpy oracle (block-sparse Schur, Schur-Jacobi "
-                       f"PCG) on {sample.num_cameras} cameras / {sample.num_points} points / "
-                       f"{sample.num_observations} observations of the same generator in {el:.1f} s, BLAS limited "
-                       f"to 1 thread; iterations/s scaled by the observation ratio {scale:.3f}")
+    return dict(value=iterations / el, unit="LM iterations/s", cores=threads, kind="port",
+                sample=f"{iterations} LM iteration(s) of the oracle's dense restatement of the reference's DENSE_SCHUR "
+                       f"algebra on {name} ({data.num_cameras} cameras, {data.num_points} points, "
+                       f"{data.num_observations} observations) in {el:.1f} s on {threads} BLAS threads "
+                       f"({_cpu_model()})")
+
+
+def similarity_aligned_rms(cams_a, pts_a, cams_b, pts_b) -> dict:
+    """RMS differences of two reconstructions after the 7-dof similarity that best maps a's points onto b's.
+
+    Bundle adjustment fixes its solution only up to rotation, translation and
+    scale, so two solves of the same problem may differ by such a transform.
+    The transform is the least-squares similarity of the point sets (Umeyama,
+    1991); point positions and camera centres c = -R(aa)^T t are compared
+    after it is applied to a.
+    """
+    import numpy as np
+    from scipy.spatial.transform import Rotation
+
+    A = np.asarray(pts_a, dtype=np.float64)
+    B = np.asarray(pts_b, dtype=np.float64)
+    ma, mb = A.mean(0), B.mean(0)
+    Ac, Bc = A - ma, B - mb
+    U, S, Vt = np.linalg.svd(Bc.T @ Ac / len(A))
+    d = np.sign(np.linalg.det(U @ Vt))
+    D = np.diag([1.0, 1.0, d])
+    R = U @ D @ Vt
+    scale = float(np.trace(np.diag(S) @ D) / (Ac * Ac).sum(1).mean())
+    t = mb - scale * R @ ma
+    pts_err = np.sqrt(np.mean(np.sum((scale * A @ R.T + t - B) ** 2, axis=1)))
+
+    def centres(cams):
+        cams = np.asarray(cams, dtype=np.float64)
+        rot = Rotation.from_rotvec(cams[:, :3]).as_matrix()
+        return -np.einsum("cji,cj->ci", rot, cams[:, 3:6])
+
+    ca, cb = centres(cams_a), centres(cams_b)
+    cam_err = np.sqrt(np.mean(np.sum((scale * ca @ R.T + t - cb) ** 2, axis=1)))
+    return {"points_rms": float(pts_err), "camera_centres_rms": float(cam_err), "scale": scale,
+            "points_rms_unaligned": float(np.sqrt(np.mean(np.sum((A - B) ** 2, axis=1))))}
 
 
 def mixed_vs_fp64(dev, data, *, max_iters):
@@ -370,14 +449,17 @@ def mixed_vs_fp64(dev, data, *, max_iters):
                                    max_iters=max_iters, target=target)
     camsm, ptsm = dev.cameras.clone(), dev.points.clone()
     diff = torch.cat([(camsm - cams64).reshape(-1), (ptsm - pts64).reshape(-1)])
+    aligned = similarity_aligned_rms(camsm.cpu().numpy(), ptsm.cpu().numpy(), cams64.cpu().numpy(),
+                                     pts64.cpu().numpy())
     return {
         "target": "cost <= (1+1e-6) x converged plain-FP64 final cost (all tolerances 0)",
         "fp64_s": hit64 and hit64[0], "fp64_iteration": hit64 and hit64[1],
         "mixed_s": hitm and hitm[0], "mixed_iteration": hitm and hitm[1],
         "fp64_final_cost": rep64.final_cost, "mixed_final_cost": repm.final_cost,
         "mixed_final_cost_rel_err": abs(repm.final_cost - rep64.final_cost) / rep64.final_cost,
-        "mixed_param_rms_err": float(torch.sqrt(torch.mean(diff * diff))),
-        "note": "parameters are compared without fixing the similarity gauge of bundle adjustment",
+        "mixed_param_rms_err_unaligned": float(torch.sqrt(torch.mean(diff * diff))),
+        "mixed_vs_fp64_after_similarity_alignment": aligned,
+        "note": "bundle adjustment fixes its solution up to a similarity; the aligned errors remove it",
     }
 
 
@@ -687,7 +769,10 @@ def main() -> int:
 
     cpu = None if args.no_cpu else cpu_baseline(data)
     if cpu:
-        log(f"[bench] cpu oracle: {cpu['value']:.4f} it/s")
+        log(f"[bench] cpu oracle: {cpu['value']:.4f} it/s ({cpu['sample']})")
+        if not (args.only_value or args.profile_only):
+            extra["cpu_dense_faithful_ba16"] = cpu_dense_faithful("ba16")
+            log(f"[bench] cpu dense-faithful ba16: {extra['cpu_dense_faithful_ba16']}")
 
     value = value_run["iters_per_s"]
     line = {

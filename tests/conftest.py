@@ -20,6 +20,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (ROCm GPU) and the built HIP library")
 
 
+def array_digest(*arrays) -> str:
+    """sha256 over the arrays' bytes, as the fixture scripts record it."""
+    import hashlib
+
+    import numpy as np
+
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
 def load_golden(name: str) -> dict:
     return json.loads((GOLDEN / f"{name}.json").read_text())
 

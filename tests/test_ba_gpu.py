@@ -279,6 +279,73 @@ def test_mixed_precision_reaches_fp64_final_cost():
     assert s32.termination_type == tc.TerminationType.CONVERGENCE
     assert s32.final_cost == pytest.approx(s64.final_cost, rel=1e-6)
     assert s32.precision == "mixed"
+    # against the FP64 oracle's own converged cost (tests/golden/make_oracle_fixtures.py)
+    g = load_golden("oracle_ba16_huber_converged")
+    assert g["input_sha256"] == _digest(data)
+    assert s32.final_cost == pytest.approx(g["summary"]["final_cost"], rel=1e-6)
+    assert s64.final_cost == pytest.approx(g["summary"]["final_cost"], rel=1e-9)
+
+
+def _digest(data):
+    from conftest import array_digest
+
+    return array_digest(data.cameras, data.points, data.observations, data.obs_camera, data.obs_point)
+
+
+@pytest.fixture(scope="module")
+def ba200_data():
+    return gen.ba_config("ba200")
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("linear", ["iterative_schur", "dense_schur"])
+def test_ba200_matches_oracle_trajectory(ba200_data, linear, loop):
+    """The 200-camera workload (100,000 points, ~650k observations) at its full size: the first 5 LM
+    iterations against the float64 oracle's (tests/golden/make_oracle_fixtures.py), with the same
+    bar as the reference-recorded trajectories: accept/reject sequence, costs to 1e-9, PCG counts
+    exactly, final cost to 1e-10; the final cameras to 1e-7."""
+    print(f"ba200 linear={linear} loop={loop}", flush=True)
+    tc.set_device_loop(loop)
+    g = load_golden(f"oracle_ba200_{linear}_5")
+    data = ba200_data
+    assert g["input_sha256"] == _digest(data)
+    problem, cams, pts = ba_problem(data)
+    summary = tc.solve(options_for(g), problem)
+    assert_trajectory_matches(summary, g["summary"])
+    assert summary.num_linear_solves == g["summary"]["num_linear_solves"]
+    np.testing.assert_allclose(cams.cpu().numpy(), np.array(g["final_cameras"]), rtol=1e-7, atol=1e-7)
+    np.testing.assert_allclose(pts[:200].cpu().numpy(), np.array(g["final_points_head"]), atol=1e-7)
+
+
+def test_ba200_mixed_precision_reaches_oracle_cost(ba200_data):
+    """The mixed form on the 200-camera workload, DENSE_SCHUR: after 8 iterations its cost is within
+    1e-6 of the FP64 oracle's (which has converged to 1e-13 within its first 5)."""
+    g = load_golden("oracle_ba200_dense_schur_5")
+    problem, _, _ = ba_problem(ba200_data)
+    o = tc.SolverOptions(max_num_iterations=8, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR,
+                         use_mixed_precision_solves=True, max_num_refinement_iterations=2,
+                         function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+    s = tc.solve(o, problem)
+    assert s.precision == "mixed"
+    assert s.final_cost == pytest.approx(g["summary"]["final_cost"], rel=1e-6)
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_largest_served_ba_request_matches_oracle(loop):
+    """The largest bundle adjustment of the served mix (20 cameras, 5,000 points, 2-6 observations per
+    point) against the oracle's trajectory."""
+    print(f"ba20x5000 loop={loop}", flush=True)
+    tc.set_device_loop(loop)
+    data = gen.make_bundle_adjustment(num_cameras=20, num_points=5000, obs_per_point=(2, 6), seed=41)
+    problem, cams, pts = ba_problem(data)
+    o = tc.SolverOptions(max_num_iterations=12, linear_solver_type=tc.LinearSolverType.DENSE_SCHUR)
+    o.function_tolerance = o.gradient_tolerance = o.parameter_tolerance = 0.0
+    summary = tc.solve(o, problem)
+    oo = OracleOptions(max_num_iterations=12, function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+    ref, (rc, rp) = trust_region_solve(oba.BAModel(data), oo)
+    assert_trajectory_matches(summary, {"iterations": ref.iterations, "final_cost": ref.final_cost,
+                                        "initial_cost": ref.initial_cost})
+    np.testing.assert_allclose(pts.cpu().numpy(), rp, atol=1e-6)
 
 
 def test_plan_rejects_camera_pair_blocks_out_of_row_major_order():
