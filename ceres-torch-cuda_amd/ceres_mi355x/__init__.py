@@ -34,10 +34,12 @@ from .losses import (
 from .manifolds import EuclideanManifold, Manifold, SubsetManifold
 from .problem import CRSMatrix, EvaluateOptions, EvaluationResult, ParameterBlock, Problem, ResidualBlock
 from .bal import BALProblem
+from .covariance import Covariance, CovarianceOptions, CovarianceSummary
 from .rotation import angle_axis_rotate_point, angle_axis_to_quaternion, unit_quaternion_rotate_point
 from .solver import set_device_loop, solve
 from .types import (
     CallbackReturnType,
+    CovarianceAlgorithmType,
     DenseLinearAlgebraLibraryType,
     DoglegType,
     IterationSummary,

@@ -187,6 +187,9 @@ SIGNATURES = {
     "ct_pose2d_plan_run": (ctypes.c_int, [_vp, ctypes.POINTER(ct_lm_report), _vp]),
     "ct_pose2d_plan_system_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "ct_pose2d_plan_destroy": (ctypes.c_int, [_vp]),
+    "ct_ba_covariance_workspace_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_ba_plan_covariance": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_double, _vp,
+                                             ctypes.POINTER(ctypes.c_int64), _vp, ctypes.c_size_t]),
     "ct_dense_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "ct_normal_equations_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                                      _vp, _vp, ctypes.c_size_t, _vp]),

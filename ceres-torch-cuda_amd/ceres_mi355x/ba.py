@@ -165,6 +165,17 @@ class BADevice:
         Returns (report, records[num_records, CT_REC_FIELDS] numpy).
         """
         lib = nat.load()
+        handle = self.plan(opts)
+        rep = nat.ct_lm_report()
+        nrec = (opts.max_num_iterations + 2) * nat.CT_REC_FIELDS
+        rec = (ctypes.c_double * nrec)()
+        nat.check(lib.ct_ba_plan_run(handle, ctypes.byref(rep), rec), "ct_ba_plan_run")
+        records = np.frombuffer(rec, dtype=np.float64).reshape(-1, nat.CT_REC_FIELDS)[: rep.num_records].copy()
+        return rep, records
+
+    def plan(self, opts: nat.ct_lm_options):
+        """The plan handle for these options (created on first use, kept until close())."""
+        lib = nat.load()
         key = tuple(getattr(opts, f) for f, _ in nat.ct_lm_options._fields_)
         plan = self._plans.get(key)
         prob = self.problem_struct()
@@ -180,13 +191,7 @@ class BADevice:
                       "ct_ba_plan_create")
             plan = (handle, ws, prob)
             self._plans[key] = plan
-        handle = plan[0]
-        rep = nat.ct_lm_report()
-        nrec = (opts.max_num_iterations + 2) * nat.CT_REC_FIELDS
-        rec = (ctypes.c_double * nrec)()
-        nat.check(lib.ct_ba_plan_run(handle, ctypes.byref(rep), rec), "ct_ba_plan_run")
-        records = np.frombuffer(rec, dtype=np.float64).reshape(-1, nat.CT_REC_FIELDS)[: rep.num_records].copy()
-        return rep, records
+        return plan[0]
 
     def evaluate(self, cameras=None, points=None):
         """(cost, residuals[N, 2] in caller observation order) at the given parameters."""

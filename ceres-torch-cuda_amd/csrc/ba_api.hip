@@ -644,6 +644,97 @@ int ct_ba_plan_profile(ct_ba_plan* pl, int iterations, double* kernel_ms, int32_
   return status;
 }
 
+int ct_ba_covariance_workspace_bytes(int32_t num_cameras, int64_t num_requests, size_t* bytes) {
+  clear_error();
+  CT_CHECK_ARG(bytes != nullptr && num_cameras >= 0 && num_requests >= 0, "bad argument");
+  CT_CHECK_ARG((int64_t)num_cameras * 9 <= kCovMaxN, "too many cameras for the dense covariance of S");
+  Carver cv(nullptr);
+  cv.take<double>(covariance_work_doubles(9 * num_cameras));
+  cv.take<int32_t>(4 * (size_t)num_requests);
+  cv.take<int32_t>(2);
+  *bytes = cv.offset + 256;
+  return CT_OK;
+}
+
+int ct_ba_plan_covariance(ct_ba_plan* pl, const int32_t* requests, int64_t num_requests, double pivot_rtol,
+                          double* out, int64_t* deficient_pivots, void* workspace, size_t workspace_bytes) {
+  clear_error();
+  CT_CHECK_ARG(pl != nullptr, "plan is null");
+  CT_CHECK_ARG(dogleg_options(&pl->options) && pl->options.precision == CT_PLAIN_FP64,
+               "covariance needs a plan created with the DOGLEG strategy and plain FP64 (undamped J^T J)");
+  const ct_ba_problem& p = pl->problem;
+  const int n = 9 * p.num_cameras;
+  CT_CHECK_ARG(n <= kCovMaxN, "too many cameras for the dense covariance of S");
+  CT_CHECK_ARG(pivot_rtol >= 0.0 && pivot_rtol < 1.0, "pivot_rtol must be in [0, 1)");
+  CT_CHECK_ARG(num_requests >= 0 && (num_requests == 0 || (requests != nullptr && out != nullptr)),
+               "requests / out are null");
+  CT_CHECK_ARG(num_requests <= INT32_MAX, "too many requests");
+  for (int64_t r = 0; r < num_requests; ++r) {
+    const int32_t* q = requests + 4 * r;
+    for (int side = 0; side < 2; ++side) {
+      const int kind = q[2 * side], idx = q[2 * side + 1];
+      CT_CHECK_ARG(kind == 0 || kind == 1, "request kind must be 0 (camera) or 1 (point)");
+      CT_CHECK_ARG(idx >= 0 && idx < (kind == 0 ? p.num_cameras : p.num_points), "request index out of range");
+    }
+  }
+  size_t need = 0;
+  CT_TRY(ct_ba_covariance_workspace_bytes(p.num_cameras, num_requests, &need));
+  CT_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "workspace too small (ct_ba_covariance_workspace_bytes)");
+  Carver cv(workspace);
+  double* work = cv.take<double>(covariance_work_doubles(n));
+  int32_t* dreq = cv.take<int32_t>(4 * (size_t)num_requests);
+  int32_t* counts = cv.take<int32_t>(2);
+  hipStream_t s = pl->stream;
+  Layout& L = pl->layout;
+  if (num_requests > 0)
+    CT_HIP(hipMemcpyAsync(dreq, requests, sizeof(int32_t) * 4 * (size_t)num_requests, hipMemcpyHostToDevice, s));
+  // the undamped system at the user's parameters: evaluation, point factors, S
+  CT_TRY(launch_lm_init(pl->init, L.w, pl->user_cams, pl->user_pts, s));
+  for (int g : {CT_KERNEL_EVAL_OBS, CT_KERNEL_EVAL_POINTS, CT_KERNEL_CAMERA_ITEMS, CT_KERNEL_CAMERA_REDUCE})
+    CT_TRY(enqueue_group(pl, g, s));
+  if (n > 0) CT_TRY(launch_rcs_cholesky_solve(L.w.S, nullptr, nullptr, n, L.w.state, L.w.solve_tmp, s));
+  CovLaunch c{};
+  c.L = L.w.S;
+  c.n = n;
+  c.JP = L.bd.JP;
+  c.Linv = L.bd.Linv;
+  c.obs_camera = p.obs_camera;
+  c.point_offsets = p.point_offsets;
+  c.camera_mask = p.camera_mask;
+  c.point_mask = p.point_mask;
+  c.num_points = p.num_points;
+  c.requests = dreq;
+  c.num_requests = num_requests;
+  c.pivot_rtol = pivot_rtol;
+  c.out = out;
+  c.work = work;
+  c.counts = counts;
+  c.st = L.w.state;
+  CT_TRY(launch_covariance(c, s));
+  int32_t flags[3] = {0, 0, 0};  // singular, deficient point pivots, deficient pivots of S
+  CT_HIP(hipMemcpyAsync(&flags[0], &L.w.state->step_invalid, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CT_HIP(hipMemcpyAsync(&flags[1], counts, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CT_HIP(hipStreamSynchronize(s));
+  if (!(flags[0] || flags[1] || flags[2])) {
+    if (deficient_pivots) *deficient_pivots = 0;
+    return CT_OK;
+  }
+  if (flags[1] == 0 && n > 0) {
+    // the points are fine and S is singular: count its null directions on S
+    // rebuilt from the assembly's partial blocks, by pivoted Cholesky
+    CT_TRY(enqueue_group(pl, CT_KERNEL_CAMERA_REDUCE, s));
+    CT_TRY(launch_covariance_rank(L.w.S, n, p.camera_mask, pivot_rtol, counts, s));
+    CT_HIP(hipMemcpyAsync(&flags[2], counts + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+  } else {
+    flags[2] = 0;  // S is built on singular point blocks: only the points' null directions are counted
+  }
+  const int64_t def = (int64_t)flags[1] + flags[2];
+  if (deficient_pivots) *deficient_pivots = def;
+  set_error("J^T J is singular: " + std::to_string(def) + " null direction(s) below the rank threshold");
+  return CT_ERR_NOT_SPD;
+}
+
 int ct_ba_evaluate(const ct_ba_problem* problem, const double* cameras, const double* points, double* cost_out_host,
                    double* residuals, void* workspace, size_t workspace_bytes, ct_stream_t stream) {
   clear_error();

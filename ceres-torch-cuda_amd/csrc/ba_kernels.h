@@ -45,6 +45,35 @@ int launch_evaluate_jacobian(const BAView& v, const double* cams, const double* 
                              const int64_t* cam_cols, const int64_t* pt_cols, int64_t ncols, int apply_loss,
                              double* jac, double* residuals, double* part, hipStream_t s);
 
+// Covariance blocks from the factored reduced camera system (cov_kernels.hip):
+// S^-1 = L^-T L^-1 formed densely in `work` (covariance_work_doubles(n)), then
+// one block of (J^T J)^-1 per request; state->step_invalid is set when H is
+// singular (a failed pivot of S or a non-finite point factor).
+constexpr int kCovMaxN = 8192;  // the column solves keep one column of L^-1 in LDS
+size_t covariance_work_doubles(int n);
+struct CovLaunch {
+  const double* L;               // [n][n] factor of S (lower triangle)
+  int n;
+  const double* JP;              // FP64 evaluation buffers
+  const double* Linv;
+  const int32_t* obs_camera;
+  const int32_t* point_offsets;
+  const uint8_t* camera_mask;    // may be null
+  const uint8_t* point_mask;
+  int num_points;
+  const int32_t* requests;       // [R][4] device
+  int64_t num_requests;
+  double pivot_rtol;
+  double* out;                   // [R][81]
+  double* work;                  // covariance_work_doubles(n)
+  int32_t* counts;               // [2] deficient point pivots, deficient pivots of S
+  LMState* st;
+};
+int launch_covariance(const CovLaunch& c, hipStream_t s);
+// Null directions of S (rebuilt unfactored) by pivoted Cholesky: counts[1].
+int launch_covariance_rank(double* S, int n, const uint8_t* camera_mask, double pivot_rtol, int32_t* counts,
+                           hipStream_t s);
+
 // LM state machine (lm_kernels.hip)
 struct LMInit {
   ct_lm_options opt;

@@ -387,6 +387,27 @@ int ct_pose2d_plan_run(ct_pose2d_plan* plan, ct_lm_report* report, double* recor
 int ct_pose2d_plan_system_size(const ct_pose2d_plan* plan, int32_t* unknowns);
 int ct_pose2d_plan_destroy(ct_pose2d_plan* plan);
 
+/* Covariance blocks of a bundle-adjustment problem at the user's parameters
+ * (replaces Covariance.compute / GetCovarianceBlockInTangentSpace,
+ * covariance.py:86-172, 279-296, for a full-rank Jacobian: the reference's
+ * SVD and QR forms both give (J^T J)^-1 then).  The plan must have been
+ * created with trust_region_strategy = CT_DOGLEG and CT_PLAIN_FP64, so that
+ * its system is the undamped J^T J (constant coordinates keep a unit
+ * diagonal and decouple).  S = U - W^T V^-1 W is factored in FP64, S^-1 =
+ * L^-T L^-1 formed densely (9C <= 8192), and each request
+ * {kind_a, index_a, kind_b, index_b} (kind 0 camera, 1 point; host array)
+ * writes the (9|3) x (9|3) block of (J^T J)^-1 over the two blocks' full
+ * coordinates, row-major, into out[r * 81 ...] (device).
+ * Rank test: a pivot of S's factor or of a point's 3x3 factor not above
+ * pivot_rtol times the largest pivot of its factor counts as a null
+ * direction (the reference applies 20 (m + n) eps to singular values,
+ * covariance.py:374-377); their number goes to *deficient_pivots (may be
+ * NULL) and the call returns CT_ERR_NOT_SPD, out left unwritten.
+ * Synchronises the plan's stream.                                           */
+int ct_ba_covariance_workspace_bytes(int32_t num_cameras, int64_t num_requests, size_t* bytes);
+int ct_ba_plan_covariance(ct_ba_plan* plan, const int32_t* requests, int64_t num_requests, double pivot_rtol,
+                          double* out, int64_t* deficient_pivots, void* workspace, size_t workspace_bytes);
+
 /* ------------------------------------------------------------------------ */
 /* Dense damped normal equations for a row-major A [m x n] (leading dimension
  * lda): x = (A^T A + diag(damping))^-1 A^T b, damping may be NULL.  Replaces

@@ -72,6 +72,36 @@ def run_pose(name: str, num_nodes: int, seed: int, max_iters: int, loop_fraction
                 final_poses_head=np.asarray(x)[:300].tolist())
 
 
+COVARIANCE_BLOCKS = [(("c", 2), ("c", 2)), (("c", 5), ("c", 11)), (("c", 15), ("c", 15)), (("p", 0), ("p", 0)),
+                     (("p", 17), ("p", 1999)), (("p", 500), ("c", 9)), (("c", 3), ("p", 1234)), (("p", 42), ("p", 43))]
+
+
+def run_covariance(name: str, config: str, constant_cameras=(0, 1)) -> dict:
+    """Covariance blocks of the dense SVD form (covariance.py:168-223) at the generator's start, gauge fixed
+    by constant cameras."""
+    from oracle.covariance import dense_covariance
+
+    d = gen.ba_config(config)
+    active = np.ones(d.num_cameras, bool)
+    active[list(constant_cameras)] = False
+    model = oba.BAModel(d, camera_active=active)
+    t0 = time.perf_counter()
+    ev, _ = model.evaluate(model.x0(), True)
+    J = model.dense_jacobian(ev)
+    cov, summary = dense_covariance(J)
+    ccol = -np.ones(d.num_cameras, np.int64)
+    ccol[active] = 3 * d.num_points + 9 * np.arange(int(active.sum()))
+
+    def cols(kind, i):
+        return np.arange(3 * i, 3 * i + 3) if kind == "p" else np.arange(ccol[i], ccol[i] + 9)
+
+    blocks = [dict(a=list(a), b=list(b), value=cov[np.ix_(cols(*a), cols(*b))].tolist()) for a, b in COVARIANCE_BLOCKS]
+    return dict(case=name, kind="ba_covariance", source="oracle", config=config,
+                constant_cameras=list(constant_cameras),
+                input_sha256=array_digest(d.cameras, d.points, d.observations, d.obs_camera, d.obs_point),
+                oracle_seconds=time.perf_counter() - t0, summary=summary, blocks=blocks)
+
+
 CASES = {
     # the 200-camera workload: first 5 LM iterations, both linear solvers
     "oracle_ba200_iterative_schur_5": lambda: run_ba("oracle_ba200_iterative_schur_5", "ba200", "iterative_schur", 5),
@@ -81,6 +111,8 @@ CASES = {
                                                   keep_iterations=40),
     # the largest pose graph of the served mix: 2000 SE2 nodes, 10 iterations
     "oracle_pose2000_10": lambda: run_pose("oracle_pose2000_10", 2000, 21, 10),
+    # covariance blocks of the 16-camera problem (dense SVD of its 16,000 x 6,126 Jacobian)
+    "oracle_ba16_covariance": lambda: run_covariance("oracle_ba16_covariance", "ba16"),
 }
 
 

@@ -16,6 +16,7 @@ formula:
 | lm.traditional/subspace_dogleg_step     | brute force: the boundary crossing of the dogleg path on a fine grid; the model minimum over a polar grid of the 2-D subspace disc; normal equations for the interior step |
 | ba.BAModel.gauss_newton_step / jmul     | dense J^T J solve and dense J v on the same evaluation |
 | small_models.*.gauss_newton_step / jmul | least squares min |J dx + r| (numpy lstsq); J v vs central differences of the residual |
+| covariance.dense_covariance             | reference KATs (tests/test_covariance_interpolation.py:7-223); (J^T J)^-1 for full-rank J |
 | lm.refine_solution                      | reference KAT (tests/test_linear_schur.py:455-483); damped fixed point = (J^T J + D)^-1 J^T b |
 | lm.trust_region_solve (BA, curve, SE2)  | golden trajectories written by running the reference (tests/golden/) |
 | lm.trust_region_solve, invalid steps    | reference KAT (tests/test_problem_solver.py:637-664): NaN steps -> FAILURE, radius 4 -> 2 |
@@ -699,3 +700,70 @@ def test_unobserved_blocks_stay_fixed_and_match_the_dense_algebra():
     np.testing.assert_allclose(pts, pts_d, atol=1e-9)
     np.testing.assert_array_equal(pts[unseen], d.points[unseen])
     np.testing.assert_array_equal(cams[5], d.cameras[5])
+
+
+# ---------------------------------------------------------------------------
+# covariance (src/ceres_torch/covariance.py), restated on dense Jacobians
+# ---------------------------------------------------------------------------
+
+
+def test_covariance_reference_known_answers():
+    from oracle.covariance import dense_covariance
+
+    # tests/test_covariance_interpolation.py:7-24: residual 2x - 1
+    cov, s = dense_covariance(np.array([[2.0]]))
+    np.testing.assert_allclose(cov, [[0.25]], atol=1e-9)
+    assert s["success"] and s["rank"] == 1 and s["nullity"] == 0 and s["reciprocal_condition_number"] == 1.0
+    # :27-44: NormalPrior diag(2, 4)
+    cov, _ = dense_covariance(np.diag([2.0, 4.0]))
+    np.testing.assert_allclose(cov, np.diag([0.25, 0.0625]), atol=1e-12)
+    # :62-81: SubsetManifold(3, [1]) leaves the tangent Jacobian diag(2, 3)
+    cov, _ = dense_covariance(np.array([[2.0, 0.0], [0.0, 3.0]]))
+    np.testing.assert_allclose(cov, np.diag([0.25, 1.0 / 9.0]), atol=1e-12)
+    # :84-105: x + y, rank 1 of 2
+    cov, s = dense_covariance(np.array([[1.0, 1.0]]))
+    assert not s["success"] and s["rank"] == 1 and s["nullity"] == 1 and "rank deficient" in s["message"]
+    cov, s = dense_covariance(np.array([[1.0, 1.0]]), null_space_rank=-1)
+    assert s["success"] and s["rank"] == 1
+    np.testing.assert_allclose(cov, np.full((2, 2), 0.25), atol=1e-9)
+    # :108-130: b - a - 1, the translation gauge
+    cov, s = dense_covariance(np.array([[-1.0, 1.0]]))
+    assert not s["success"] and s["rank"] == 1 and s["nullity"] == 1
+    cov, s = dense_covariance(np.array([[-1.0, 1.0]]), null_space_rank=-1)
+    np.testing.assert_allclose(cov @ np.ones(2), np.zeros(2), atol=1e-10)
+    assert np.array([-1.0, 1.0]) @ cov @ np.array([-1.0, 1.0]) == pytest.approx(1.0)
+    # :140-168: near singular, eigenvalue-ratio policy
+    A = np.diag([1.0, 1e-4])
+    cov, s = dense_covariance(A, min_reciprocal_condition_number=1e-6)
+    assert not s["success"] and s["rank"] == 2 and s["reciprocal_condition_number"] < 1e-6
+    cov, s = dense_covariance(A, min_reciprocal_condition_number=1e-6, null_space_rank=-1)
+    assert s["success"] and s["rank"] == 1 and s["nullity"] == 1
+    np.testing.assert_allclose(cov, np.diag([1.0, 0.0]), atol=1e-12)
+    # :171-183: Huber(1) on x - 10 at x = 0: the corrected Jacobian is sqrt(rho1) = sqrt(0.1)
+    _, _, Jc = robustify(("huber", 1.0), np.array([[-10.0]]), np.array([[[1.0]]]))
+    cov, _ = dense_covariance(Jc.reshape(1, 1))
+    np.testing.assert_allclose(cov, [[10.0]], rtol=1e-12)
+    # :186-198: SPARSE_QR equals DENSE_SVD for a full-rank problem
+    A = np.array([[2.0, 0.0], [1.0, 3.0], [0.5, -1.0]])
+    np.testing.assert_allclose(dense_covariance(A, algorithm="sparse_qr")[0], dense_covariance(A)[0], atol=1e-10)
+    # :201-223: SPARSE_QR ignores null_space_rank and fails on a rank-deficient J
+    A = np.array([[1.0, 1.0], [2.0, 2.0]])
+    assert dense_covariance(A, null_space_rank=-1)[1]["success"]
+    cov, s = dense_covariance(A, algorithm="sparse_qr", null_space_rank=-1)
+    assert cov is None and not s["success"] and "Sparse QR" in s["message"]
+
+
+def test_covariance_full_rank_is_the_inverse_normal_matrix():
+    # for a full-rank J both algorithms give (J^T J)^-1 (textbook); the QR
+    # form reaches it through R^-1 R^-T, the SVD form through V diag(s^-2) V^T
+    from oracle.covariance import dense_covariance
+
+    rng = np.random.default_rng(21)
+    J = rng.normal(size=(40, 7)) @ np.diag(np.logspace(0, 3, 7))
+    inv = np.linalg.inv(J.T @ J)
+    for alg in ("dense_svd", "sparse_qr"):
+        cov, s = dense_covariance(J, algorithm=alg)
+        np.testing.assert_allclose(cov, inv, rtol=1e-9, atol=1e-12 * np.abs(inv).max())
+        assert s["success"] and s["rank"] == 7
+        sv = np.linalg.svd(J, compute_uv=False)
+        assert s["reciprocal_condition_number"] == pytest.approx((sv.min() / sv.max()) ** 2)
