@@ -74,7 +74,7 @@ class ct_lm_options(ctypes.Structure):
         ("preconditioner", ctypes.c_int32),
         ("max_solver_time_in_seconds", ctypes.c_double),
         ("debug_invalid_steps", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("schur_operator", ctypes.c_int32),
     ]
 
 

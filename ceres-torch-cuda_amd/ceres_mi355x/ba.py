@@ -249,6 +249,17 @@ class BADevice:
 
 _debug_invalid_steps = 0
 
+# How ITERATIVE_SCHUR forms S p: "auto" (from the evaluation's records when the
+# plan allows it) or "explicit" (assembled S); see set_schur_operator.
+SCHUR_OPERATOR = {"mode": "auto"}
+
+
+def set_schur_operator(mode: str) -> None:
+    """'auto' (S p from the per-observation records, no off-diagonal blocks of S) or 'explicit'."""
+    if mode not in ("auto", "explicit"):
+        raise ValueError("mode must be 'auto' or 'explicit'")
+    SCHUR_OPERATOR["mode"] = mode
+
 
 @contextlib.contextmanager
 def debug_invalid_steps(count: int):
@@ -301,6 +312,7 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     pc = getattr(options, "preconditioner_type", None)
     o.preconditioner = PRECONDITIONERS.get(pc.name, 0) if pc is not None else 0
     o.debug_invalid_steps = _debug_invalid_steps
+    o.schur_operator = 0 if SCHUR_OPERATOR["mode"] == "auto" else 1
     return o
 
 

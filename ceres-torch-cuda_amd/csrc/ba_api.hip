@@ -49,6 +49,7 @@ int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o->preconditioner >= CT_PRECONDITIONER_SCHUR_JACOBI && o->preconditioner <= CT_PRECONDITIONER_IDENTITY,
                "unknown preconditioner");
   CT_CHECK_ARG(o->debug_invalid_steps >= 0, "debug_invalid_steps must be >= 0");
+  CT_CHECK_ARG(o->schur_operator == CT_SCHUR_AUTO || o->schur_operator == CT_SCHUR_EXPLICIT, "unknown schur_operator");
   return CT_OK;
 }
 
@@ -72,6 +73,13 @@ size_t max_items(const ct_ba_problem* p) {
          ((size_t)p->num_observations + W - 1) / W + 1;
 }
 
+// The implicit-S PCG needs plain FP64 records, the reduced system's vectors
+// in LDS and the fused point groups (checked at plan creation).
+bool implicit_possible(const ct_ba_problem* p, const ct_lm_options* o) {
+  return o->linear_solver == CT_ITERATIVE_SCHUR && !dogleg_options(o) && o->precision == CT_PLAIN_FP64 &&
+         o->schur_operator == CT_SCHUR_AUTO && p->num_cameras > 0 && p->num_cameras <= kImplicitMaxCameras;
+}
+
 struct Layout {
   BAWork w{};
   BlockItem* items = nullptr;
@@ -82,6 +90,7 @@ struct Layout {
   BABuffers<double> bd{};
   double* pcg_work = nullptr;
   size_t pcg_doubles = 0;
+  double* implicit_work = nullptr;  // ITERATIVE_SCHUR: the implicit-S PCG's buffers
   size_t bytes = 0;
 };
 
@@ -147,6 +156,9 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
     L.pcg_doubles = rcs_pcg_work_doubles((int)n);
     L.pcg_work = cv.take<double>(L.pcg_doubles);
     L.w.pcg_bar = base ? rcs_pcg_barrier(L.pcg_work, (int)n) : nullptr;
+    if (implicit_possible(p, o)) {
+      L.implicit_work = cv.take<double>(implicit_pcg_doubles((int)C, (int)N));
+    }
   }
   L.bytes = cv.offset + 256;
   return L;
@@ -194,6 +206,8 @@ struct ct_ba_plan {
   int num_items = 0;
   int num_groups = 0;       // point groups of the fused evaluation (0: two-kernel evaluation)
   bool unobserved_points = false;  // some point has no observation (the fused evaluation then covers it)
+  bool implicit = false;           // ITERATIVE_SCHUR applies S from the records (no off-diagonal blocks)
+  ImplicitPcg imp{};
 };
 
 namespace {
@@ -222,10 +236,12 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
       return mixed ? launch_camera_items<float>(v, L.bf, L.w, pl->num_items, s)
                    : launch_camera_items<double>(v, L.bd, L.w, pl->num_items, s);
     case CT_KERNEL_CAMERA_REDUCE:
-      return mixed ? launch_camera_reduce<float>(v, L.bf, L.w, s) : launch_camera_reduce<double>(v, L.bd, L.w, s);
+      return mixed ? launch_camera_reduce<float>(v, L.bf, L.w, pl->implicit, s)
+                   : launch_camera_reduce<double>(v, L.bd, L.w, pl->implicit, s);
     case CT_KERNEL_LM_BEGIN:
       return launch_lm_begin(pl->init, L.w, s);
     case CT_KERNEL_RCS_SOLVE:
+      if (pl->implicit) return launch_pcg_implicit(pl->imp, s);
       if (!uses_cholesky(&pl->options))
         return launch_rcs_pcg(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.pcg_work, L.pcg_doubles, s);
       return launch_rcs_cholesky_solve(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.w.solve_tmp, s);
@@ -430,7 +446,7 @@ int build_point_groups(ct_ba_plan* pl) {
   return CT_OK;
 }
 
-int build_items(ct_ba_plan* pl) {
+int build_items(ct_ba_plan* pl, bool diagonal_only) {
   const ct_ba_problem& p = pl->problem;
   const int B = p.num_blocks, C = p.num_cameras;
   std::vector<int32_t> bo(B + 1, 0), co(C + 1, 0), bc(2 * (size_t)B + 1, 0);
@@ -462,7 +478,7 @@ int build_items(ct_ba_plan* pl) {
       return CT_ERR_ARGUMENT;
     }
     if (ci == cj) split_range(items, blk, 1, co[ci], co[ci + 1]);
-    split_range(items, blk, 0, bo[blk], bo[blk + 1]);
+    if (ci == cj || !diagonal_only) split_range(items, blk, 0, bo[blk], bo[blk + 1]);
   }
   offsets[B] = (int32_t)items.size();
   if (items.size() > max_items(&p)) {
@@ -538,8 +554,32 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->init.num_cameras = problem->num_cameras;
   pl->init.n_cam_params = 9 * problem->num_cameras;
   pl->init.n_pt_params = 3 * problem->num_points;
-  CT_TRY(build_items(pl.get()));
   CT_TRY(build_point_groups(pl.get()));
+  // ITERATIVE_SCHUR applies S from the records when the plan has point groups
+  pl->implicit = implicit_possible(problem, options) && pl->num_groups > 0;
+  CT_TRY(build_items(pl.get(), pl->implicit));
+  if (pl->implicit) {
+    ImplicitPcg& a = pl->imp;
+    a = implicit_pcg_layout(pl->layout.implicit_work, problem->num_cameras, problem->num_observations);
+    a.obs_camera = problem->obs_camera;
+    a.obs_point = problem->obs_point;
+    a.point_offsets = problem->point_offsets;
+    a.camera_offsets = problem->camera_offsets;
+    a.point_groups = pl->layout.point_groups;
+    a.camera_obs = problem->camera_obs;
+    a.JP = pl->layout.bd.JP;
+    a.S = pl->layout.w.S;
+    a.Dc = pl->layout.w.Dc;
+    a.rhs = pl->layout.w.rhs;
+    a.x = pl->layout.w.dxc;
+    a.C = problem->num_cameras;
+    a.n = 9 * problem->num_cameras;
+    a.num_groups = pl->num_groups;
+    a.st = pl->layout.w.state;
+    pl->layout.w.pcg_bar = a.epoch;  // the LM begin kernel advances this solve epoch
+    CT_TRY(implicit_pcg_prepare(a, pl->stream));
+    CT_HIP(hipStreamSynchronize(pl->stream));
+  }
   if (pl->num_groups > 0) {
     const int fused_blocks = (pl->num_groups + kFusedThreads / kWave - 1) / (kFusedThreads / kWave);
     pl->init.num_cost_blocks = fused_blocks;

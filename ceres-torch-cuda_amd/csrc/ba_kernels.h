@@ -11,8 +11,10 @@ template <typename real>
 int launch_eval_points(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
 template <typename real>
 int launch_camera_items(const BAView& v, const BABuffers<real>& b, const BAWork& w, int num_items, hipStream_t s);
+// diagonal_only: write the C diagonal blocks of S (with rhs, g_c, D_c) and no others.
 template <typename real>
-int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool diagonal_only,
+                         hipStream_t s);
 template <typename real>
 int launch_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool fused_backsub, hipStream_t s);
 template <typename real>
@@ -107,6 +109,36 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
 size_t rcs_pcg_work_doubles(int n);
 unsigned int* rcs_pcg_barrier(double* work, int n);  // the PCG solve-epoch word inside its workspace
 int rcs_pcg_prepare(double* work, int n, hipStream_t s);  // zero the epoch and barrier flags (once per plan)
+// Implicit-S PCG (pcg_implicit.hip): S p from the JP records, block-Jacobi
+// factors of the explicitly assembled diagonal blocks of S.
+constexpr int kImplicitMaxCameras = 300;  // p, r and z (9C entries each) in LDS beside the y staging
+constexpr int kImpMaxGroups = 1024;
+struct ImplicitPcg {
+  const int32_t* obs_camera;     // [N]
+  const int32_t* obs_point;      // [N]
+  const int32_t* point_offsets;  // [P+1]
+  const int32_t* camera_offsets; // [C+1] slot ranges in camera-major order
+  const int32_t* point_groups;   // [num_groups+1] first point of each wavefront's group
+  const int32_t* camera_obs;     // [N] each camera's observations, camera by camera
+  const double* JP;              // [N][kJPStride]
+  const double* S;               // n x n storage; only the diagonal blocks are read
+  const double* Dc;              // [n] camera damping
+  const double* rhs;             // [n]
+  double* x;                     // [n] solution
+  double* y;                     // [N][9] Jc^T w per observation
+  double* Ap;                    // [n]
+  double* part;                  // [C] p_i . (S p)_i
+  double* fac;                   // [C][81] inverse of each preconditioner block
+  int32_t* kind;                 // [C] preconditioner kind (< 0: singular block)
+  unsigned int* epoch;           // solves so far (advanced by the LM begin kernel)
+  unsigned long long* flags;     // [kImpMaxGroups] grid-barrier slots
+  int C, n, num_groups;
+  LMState* st;
+};
+size_t implicit_pcg_doubles(int num_cameras, int num_observations);
+ImplicitPcg implicit_pcg_layout(double* base, int num_cameras, int num_observations);
+int implicit_pcg_prepare(const ImplicitPcg& a, hipStream_t s);  // zero the barrier words (once per plan)
+int launch_pcg_implicit(const ImplicitPcg& a, hipStream_t s);
 size_t rcs_cholesky_tmp_doubles(int n);
 // Clears the synchronisation words inside the solve scratch; once per plan, before the first solve.
 int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s);

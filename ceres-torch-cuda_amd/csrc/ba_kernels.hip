@@ -773,13 +773,16 @@ __device__ __forceinline__ void item_sums(const BABuffers<real>& b, const BAWork
 }
 
 template <typename real>
-__global__ __launch_bounds__(kReduceThreads) void k_camera_reduce(BAView v, BABuffers<real> b, BAWork w) {
+__global__ __launch_bounds__(kReduceThreads) void k_camera_reduce(BAView v, BABuffers<real> b, BAWork w,
+                                                                   int diagonal_only) {
   const LMState* st = w.state;
   if (st->done) return;
   const int C = v.C;
-  const long long ob = (long long)blockIdx.x * kReduceBlocks + threadIdx.x / 81;  // block of S, row-major
-  if (threadIdx.x >= 81 * kReduceBlocks || ob >= (long long)C * C) return;
-  const int R = (int)(ob / C), Q = (int)(ob - (long long)R * C);
+  // block of S: row-major over the C x C grid, or the C diagonal blocks only
+  const long long ob = (long long)blockIdx.x * kReduceBlocks + threadIdx.x / 81;
+  if (threadIdx.x >= 81 * kReduceBlocks || ob >= (diagonal_only ? (long long)C : (long long)C * C)) return;
+  const int R = diagonal_only ? (int)ob : (int)(ob / C);
+  const int Q = diagonal_only ? (int)ob : (int)(ob - (long long)R * C);
   const int e = threadIdx.x % 81;
   const int r = e / 9, c = e % 9;
   const int n = 9 * C;
@@ -1434,11 +1437,12 @@ int launch_camera_items(const BAView& v, const BABuffers<real>& b, const BAWork&
 }
 
 template <typename real>
-int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+int launch_camera_reduce(const BAView& v, const BABuffers<real>& b, const BAWork& w, bool diagonal_only,
+                         hipStream_t s) {
   if (v.B > 0) {
-    const long long blocks = (long long)v.C * v.C;
+    const long long blocks = diagonal_only ? (long long)v.C : (long long)v.C * v.C;
     hipLaunchKernelGGL(k_camera_reduce<real>, dim3((unsigned)((blocks + kReduceBlocks - 1) / kReduceBlocks)),
-                       dim3(kReduceThreads), 0, s, v, b, w);
+                       dim3(kReduceThreads), 0, s, v, b, w, diagonal_only ? 1 : 0);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;
@@ -1558,7 +1562,7 @@ int launch_evaluate_jacobian(const BAView& v, const double* cams, const double* 
   template int launch_eval_obs<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);        \
   template int launch_eval_points<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);     \
   template int launch_camera_items<real>(const BAView&, const BABuffers<real>&, const BAWork&, int, hipStream_t); \
-  template int launch_camera_reduce<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);   \
+  template int launch_camera_reduce<real>(const BAView&, const BABuffers<real>&, const BAWork&, bool, hipStream_t); \
   template int launch_step<real>(const BAView&, const BABuffers<real>&, const BAWork&, bool, hipStream_t);      \
   template int launch_candidate<real>(const BAView&, const BABuffers<real>&, const BAWork&, hipStream_t);
 CT_INSTANTIATE(float)

@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "ba_kernels.h"
+#include "pcg_device.h"
 
 namespace ct {
 
@@ -57,13 +58,6 @@ __host__ __device__ inline size_t factor_tiles(int n) {
 }
 
 // This wave's outstanding memory operations (stores included) are complete.
-__device__ __forceinline__ void own_stores_done() { __builtin_amdgcn_s_waitcnt(0); }
-
-__device__ __forceinline__ void lds_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 // v as held by `lane` (a uniform value: read through scalar registers)
 __device__ __forceinline__ double lane_value(double v, int lane) {
@@ -928,13 +922,11 @@ __global__ void k_inv_reduce(const double* __restrict__ part, double* __restrict
 // barrier).  The LM begin kernel advances the epoch before each solve; waits
 // are bounded and a timeout flags the step invalid.
 // ---------------------------------------------------------------------------
-constexpr int kBlk = 9;
 constexpr int kPcgWaves = kBlk;  // one wavefront per row of a camera block
 constexpr int kPcgThreads = kPcgWaves * kWave;
 constexpr int kPcgMaxOwn = 8;    // camera blocks per workgroup
 constexpr int kPcgMaxGroups = 1024;
 constexpr int kPcgMaxN = 12288;  // p is held in LDS
-constexpr double kEps = 2.220446049250313e-16;
 
 struct PcgWork {
   double* z;                  // [n]  preconditioned residual, read by every workgroup
@@ -942,97 +934,6 @@ struct PcgWork {
   unsigned int* epoch;        // solves so far (advanced by the LM begin kernel)
   unsigned long long* flags;  // [kPcgMaxGroups] last barrier passed by each workgroup
 };
-
-__device__ __forceinline__ bool grid_barrier(unsigned long long* flags, unsigned long long target, unsigned int G,
-                                             int* timed_out) {
-  own_stores_done();
-  __syncthreads();
-  // the release store writes L2 back (for the other XCDs) before the flag
-  if (threadIdx.x == 0) __hip_atomic_store(&flags[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  if (threadIdx.x < kWave) {
-    long spins = 0;
-    for (;;) {
-      bool ok = true;
-      for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
-        unsigned long long fv[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const unsigned int i = i0 + t * kWave;
-          fv[t] = i < G ? __hip_atomic_load(&flags[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) ok = ok && fv[t] >= target;
-      }
-      if (__all(ok)) break;
-      if (++spins > (1L << 24)) {
-        if (threadIdx.x == 0) *timed_out = 1;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-  return *timed_out == 0;
-}
-
-// Sum of part[0:G] in one fixed order (identical in every workgroup); valid
-// in every thread after the call.  A lane's loads are issued four at a time
-// before their adds.
-__device__ __forceinline__ double group_total(const double* part, unsigned int G, double* slot) {
-  if (threadIdx.x < kWave) {
-    double v = 0.0;
-    for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
-      double pv[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const unsigned int i = i0 + t * kWave;
-        pv[t] = i < G ? part[i] : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) v += pv[t];
-    }
-    v = wave_sum(v);
-    if (threadIdx.x == 0) *slot = v;
-  }
-  __syncthreads();
-  const double t = *slot;
-  __syncthreads();
-  return t;
-}
-
-// group_total of two partial arrays from one round of loads (each summed in
-// group_total's order).
-__device__ __forceinline__ void group_totals2(const double* a, const double* b, unsigned int G, double* slots,
-                                              double* ta, double* tb) {
-  if (threadIdx.x < kWave) {
-    double va = 0.0, vb = 0.0;
-    for (unsigned int i0 = threadIdx.x; i0 < G; i0 += 4 * kWave) {
-      double pa[4], pb[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const unsigned int i = i0 + t * kWave;
-        pa[t] = i < G ? a[i] : 0.0;
-        pb[t] = i < G ? b[i] : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        va += pa[t];
-        vb += pb[t];
-      }
-    }
-    va = wave_sum(va);
-    vb = wave_sum(vb);
-    if (threadIdx.x == 0) {
-      slots[0] = va;
-      slots[1] = vb;
-    }
-  }
-  __syncthreads();
-  *ta = slots[0];
-  *tb = slots[1];
-  __syncthreads();
-}
 
 // One row of S p: lane l sums S[row][c] p[c] over c = l, l + kWave, ... in
 // that order, with the loads of kRowBatch terms issued before their adds.
@@ -1088,106 +989,6 @@ __device__ __forceinline__ void update_direction(double* p, const double (&zv)[k
       const int c = c0 + t * kPcgThreads;
       if (c < n) p[c] = copy ? zv[t] : zv[t] + beta * p[c];
     }
-  }
-}
-
-// Cholesky of a 9x9 block in place (lower triangle), reciprocal pivots in dinv.
-__device__ __forceinline__ bool chol9(double* A, double* dinv) {
-  for (int j = 0; j < kBlk; ++j) {
-    double d = A[j * kBlk + j];
-    for (int k = 0; k < j; ++k) d -= A[j * kBlk + k] * A[j * kBlk + k];
-    if (!(d > 0.0) || !isfinite(d)) return false;
-    d = sqrt(d);
-    const double inv = 1.0 / d;
-    A[j * kBlk + j] = d;
-    dinv[j] = inv;
-    for (int i = j + 1; i < kBlk; ++i) {
-      double t = A[i * kBlk + j];
-      for (int k = 0; k < j; ++k) t -= A[i * kBlk + k] * A[j * kBlk + k];
-      A[i * kBlk + j] = t * inv;
-    }
-  }
-  return true;
-}
-
-// LU with partial pivoting of a 9x9 block in place (torch.linalg.solve's
-// factorisation); reciprocal pivots in dinv.  False if a pivot is zero.
-__device__ __forceinline__ bool lu9(double* A, int* piv, double* dinv) {
-  for (int j = 0; j < kBlk; ++j) {
-    int p = j;
-    double best = fabs(A[j * kBlk + j]);
-    for (int i = j + 1; i < kBlk; ++i) {
-      const double v = fabs(A[i * kBlk + j]);
-      if (v > best) {
-        best = v;
-        p = i;
-      }
-    }
-    piv[j] = p;
-    if (!(best > 0.0) || !isfinite(best)) return false;
-    if (p != j)
-      for (int k = 0; k < kBlk; ++k) {
-        const double t = A[j * kBlk + k];
-        A[j * kBlk + k] = A[p * kBlk + k];
-        A[p * kBlk + k] = t;
-      }
-    const double inv = 1.0 / A[j * kBlk + j];
-    dinv[j] = inv;
-    for (int i = j + 1; i < kBlk; ++i) {
-      const double l = A[i * kBlk + j] * inv;
-      A[i * kBlk + j] = l;
-      for (int k = j + 1; k < kBlk; ++k) A[i * kBlk + k] -= l * A[j * kBlk + k];
-    }
-  }
-  return true;
-}
-
-// z = M^-1 r for one block: Cholesky solve (kind 1, in registers), LU solve
-// (kind 0, rare: worked in place in z, which must not alias r), the inverse
-// diagonal (kind 2, JACOBI) or the identity (kind 3).
-__device__ __forceinline__ void block_solve(const double* A, const double* dinv, const int* piv, int kind, const double* r,
-                            double* z) {
-  if (kind == 2 || kind == 3) {
-#pragma unroll
-    for (int i = 0; i < kBlk; ++i) z[i] = kind == 2 ? dinv[i] * r[i] : r[i];
-    return;
-  }
-  if (kind == 1) {
-    double t[kBlk];
-#pragma unroll
-    for (int i = 0; i < kBlk; ++i) {
-      double s = r[i];
-#pragma unroll
-      for (int k = 0; k < i; ++k) s -= A[i * kBlk + k] * t[k];
-      t[i] = s * dinv[i];
-    }
-#pragma unroll
-    for (int i = kBlk - 1; i >= 0; --i) {
-      double s = t[i];
-#pragma unroll
-      for (int k = i + 1; k < kBlk; ++k) s -= A[k * kBlk + i] * t[k];
-      t[i] = s * dinv[i];
-    }
-#pragma unroll
-    for (int i = 0; i < kBlk; ++i) z[i] = t[i];
-    return;
-  }
-  for (int i = 0; i < kBlk; ++i) z[i] = r[i];
-  for (int j = 0; j < kBlk; ++j) {
-    const int p = piv[j];
-    const double t = z[j];
-    z[j] = z[p];
-    z[p] = t;
-  }
-  for (int i = 0; i < kBlk; ++i) {
-    double s = z[i];
-    for (int k = 0; k < i; ++k) s -= A[i * kBlk + k] * z[k];
-    z[i] = s;
-  }
-  for (int i = kBlk - 1; i >= 0; --i) {
-    double s = z[i];
-    for (int k = i + 1; k < kBlk; ++k) s -= A[i * kBlk + k] * z[k];
-    z[i] = s * dinv[i];
   }
 }
 

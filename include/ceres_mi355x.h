@@ -147,8 +147,15 @@ typedef struct ct_lm_options {
                                            they are checked, so the invalid-step branch of solver.py:168-190
                                            runs as after a linear solver returning NaN.  Curve fits and
                                            pose graphs refuse a nonzero value. */
-  int32_t reserved0;
+  int32_t schur_operator;               /* ct_schur_operator (ITERATIVE_SCHUR, plain FP64): CT_SCHUR_AUTO applies
+                                           S from the evaluation's records when the plan allows it (up to 600
+                                           cameras, no point seen by more than 64); CT_SCHUR_EXPLICIT always
+                                           assembles S.  Same step either way, to rounding. */
 } ct_lm_options;
+
+/* How ITERATIVE_SCHUR forms S p (no reference counterpart: the reference
+ * always builds S densely, linear.py:794-839). */
+typedef enum ct_schur_operator { CT_SCHUR_AUTO = 0, CT_SCHUR_EXPLICIT = 1 } ct_schur_operator;
 
 /* Termination codes match TerminationType (types.py:98-103). */
 typedef enum ct_termination {

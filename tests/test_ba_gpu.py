@@ -51,8 +51,11 @@ def options_for(golden, **kw):
 
 @pytest.fixture(autouse=True)
 def _loop_mode_reset():
+    from ceres_mi355x.ba import set_schur_operator
+
     yield
     tc.set_device_loop("device", 8)
+    set_schur_operator("auto")
 
 
 @pytest.mark.parametrize("loop", ["host", "device"])
@@ -62,6 +65,22 @@ def _loop_mode_reset():
 def test_ba_matches_reference_trajectory(case, loop):
     print(f"case={case} loop={loop}", flush=True)
     tc.set_device_loop(loop)
+    _check_reference_trajectory(case)
+
+
+@pytest.mark.parametrize("case", ["ba4_iterative_schur_zero_tol", "ba4_iterative_jacobi_zero_tol",
+                                  "ba4_iterative_identity_zero_tol"])
+def test_ba_explicit_schur_operator_matches_reference_trajectory(case):
+    """The explicitly assembled S (CT_SCHUR_EXPLICIT) on the ITERATIVE_SCHUR goldens; the default applies S
+    from the records."""
+    from ceres_mi355x.ba import set_schur_operator
+
+    print(f"case={case} explicit S", flush=True)
+    set_schur_operator("explicit")
+    _check_reference_trajectory(case)
+
+
+def _check_reference_trajectory(case):
     g = load_golden(case)
     data = gen.ba_config(g["config"])
     problem, cams, pts = ba_problem(data)
@@ -297,15 +316,21 @@ def ba200_data():
     return gen.ba_config("ba200")
 
 
-@pytest.mark.parametrize("loop", ["host", "device"])
-@pytest.mark.parametrize("linear", ["iterative_schur", "dense_schur"])
-def test_ba200_matches_oracle_trajectory(ba200_data, linear, loop):
+@pytest.mark.parametrize("loop,linear,schur", [("host", "iterative_schur", "auto"),
+                                               ("device", "iterative_schur", "auto"),
+                                               ("device", "iterative_schur", "explicit"),
+                                               ("host", "dense_schur", "auto"),
+                                               ("device", "dense_schur", "auto")])
+def test_ba200_matches_oracle_trajectory(ba200_data, linear, loop, schur):
     """The 200-camera workload (100,000 points, ~650k observations) at its full size: the first 5 LM
     iterations against the float64 oracle's (tests/golden/make_oracle_fixtures.py), with the same
     bar as the reference-recorded trajectories: accept/reject sequence, costs to 1e-9, PCG counts
     exactly, final cost to 1e-10; the final cameras to 1e-7."""
-    print(f"ba200 linear={linear} loop={loop}", flush=True)
+    from ceres_mi355x.ba import set_schur_operator
+
+    print(f"ba200 linear={linear} loop={loop} schur={schur}", flush=True)
     tc.set_device_loop(loop)
+    set_schur_operator(schur)
     g = load_golden(f"oracle_ba200_{linear}_5")
     data = ba200_data
     assert g["input_sha256"] == _digest(data)
