@@ -470,6 +470,13 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
     is captured in a HIP graph and replayed; time per batch is the mean over
     the replays, bracketed by device synchronisations.  Also reports the FP64
     form and a one-thread oracle sample.
+
+    Roofline entry: a sweep reads a problem's samples once (8 B per sample in
+    the mixed form's FP32 storage, 16 B in FP64); per problem there is one
+    sweep per evaluation at x (num_records) and one per candidate cost
+    (num_residual_evaluations), plus the mixed form's initial FP64 cost.  The
+    samples of a problem stay in cache between its sweeps, so the achieved
+    figure is the rate at which the sweeps consume samples, against HBM peak.
     """
     import numpy as np
     import torch
@@ -482,24 +489,28 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
 
     data = gen.make_curve_batch(num_problems=num_problems, seed=3)
     batch = CurveBatch(data.offsets, data.x, data.y, loss=tc.CauchyLoss(0.5))
+    sizes = np.diff(data.offsets)
     out = {"workload": f"{num_problems} curve fits y = exp(m x + c), N ~ U[50, 2000] samples "
                        f"({int(data.offsets[-1])} in total), Cauchy(0.5), {iterations} LM iterations max, "
                        "reference default tolerances, start m = c = 0"}
+    final = {}
     for precision in ("mixed", "fp64"):
         opts = lm_options(tc.SolverOptions(max_num_iterations=iterations), precision=precision)
         res = batch.solve(opts, records=False)  # tunes the launch shape on first use
-        (threads, groups), order = curve_schedule(batch, opts)  # the cached choice
+        final[precision] = res.reports["final_cost"].copy()
+        (threads, groups, waves), order = curve_schedule(batch, opts)  # the cached choice
         start = torch.zeros(num_problems, 2, dtype=torch.float64, device="cuda")
         params = start.clone()
         graph = torch.cuda.CUDAGraph()
         stream = torch.cuda.Stream()
+        shape = (threads, groups, waves)
         with torch.cuda.stream(stream):
-            batch.launch(opts, params, schedule=(threads, groups), order=order)
+            batch.launch(opts, params, schedule=shape, order=order)
             torch.cuda.synchronize()
             with torch.cuda.graph(graph, stream=stream):
                 params.zero_()
                 params.add_(start)
-                batch.launch(opts, params, schedule=(threads, groups), order=order)
+                batch.launch(opts, params, schedule=shape, order=order)
         graph.replay()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -509,13 +520,27 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
         sec = (time.perf_counter() - t0) / replays
         iters = int(np.sum(res.reports["num_records"]) - num_problems)
         conv = float(np.mean(res.reports["termination"] == 0))
+        sweeps = res.reports["num_records"].astype(np.int64) + res.reports["num_residual_evaluations"]
+        if precision == "mixed":
+            sweeps = sweeps + 1
+        sample_bytes = 8 if precision == "mixed" else 16
+        alg = float(np.sum(sweeps * sizes)) * sample_bytes
         out[precision] = {"seconds_per_batch": sec, "solves_per_s": num_problems / sec,
                           "lm_iterations_per_s": iters / sec, "lm_iterations": iters, "converged_fraction": conv,
-                          "schedule": {"threads": threads, "groups": groups, "order": order},
+                          "schedule": {"threads": threads, "groups": groups, "wave_problems": waves, "order": order},
                           "median_abs_err_m_vs_truth": float(np.median(np.abs(params[:, 0].cpu().numpy()
-                                                                              - data.true_m)))}
+                                                                              - data.true_m))),
+                          "roofline": {"bound": "hbm", "achieved": alg / sec / 1e9, "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": alg / sec / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                                       "algorithmic_bytes": alg, "sample_sweeps": float(np.sum(sweeps * sizes)),
+                                       "bytes_per_sample_sweep": sample_bytes}}
         log(f"[bench] curve batch {precision}: {sec * 1e3:.3f} ms/batch, {num_problems / sec:.0f} solves/s, "
-            f"schedule {threads}x{groups} {order}")
+            f"schedule {threads}x{groups} waves {waves} {order}")
+    # speed of the mixed form over the FP64 form, and how far its final costs (of the FP32-rounded samples,
+    # evaluated in FP64) are from the FP64 form's
+    out["mixed_vs_fp64"] = out["fp64"]["seconds_per_batch"] / out["mixed"]["seconds_per_batch"]
+    out["mixed_final_cost_max_rel_diff"] = float(np.max(np.abs(final["mixed"] - final["fp64"])
+                                                        / np.maximum(np.abs(final["fp64"]), 1e-300)))
     # one-thread oracle on the first problems of the batch
     from threadpoolctl import threadpool_limits
 

@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 9  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 10  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "dogleg",
            "step", "candidate", "lm_end"]
 REC = dict(
@@ -129,6 +129,8 @@ class ct_curve_batch(ctypes.Structure):
         ("y", _vp),
         ("order", _vp),
         ("loss", ct_loss),
+        ("x32", _vp),
+        ("y32", _vp),
     ]
 
 
@@ -150,7 +152,7 @@ class ct_pose2d_problem(ctypes.Structure):
 
 
 class ct_batch_schedule(ctypes.Structure):
-    _fields_ = [("threads", ctypes.c_int32), ("groups", ctypes.c_int32)]
+    _fields_ = [("threads", ctypes.c_int32), ("groups", ctypes.c_int32), ("wave_problems", ctypes.c_int32)]
 
 
 # numpy view of a device array of ct_lm_report

@@ -1,9 +1,11 @@
 """Launch-shape autotuner for the persistent batched solves.
 
 A batched launch is shaped by the workgroup size (threads per problem), the
-number of persistent workgroups pulling problems from the work queue, and the
+number of persistent workgroups pulling problems from the work queues, the
 order in which problems are queued (index order, or largest first, which
-balances uneven sizes).  ``curve_schedule`` times every candidate on the
+balances uneven sizes) and a size split: with the largest-first order, the
+problems of at most a threshold number of samples are solved by one wave each
+and only the larger ones by a whole workgroup.  ``curve_schedule`` times every candidate on the
 caller's batch (on a scratch copy of the start values) and keeps the fastest;
 results are cached per batch signature (problem count, sample count and size
 spread, precision, iteration limit) for the life of the process.
@@ -17,8 +19,10 @@ import numpy as np
 import torch
 
 THREADS = (64, 128, 256, 512)
-GROUPS_PER_CU = (2, 4, 8)
+GROUPS_PER_CU = (2, 4, 8, 16)
 ORDERS = ("index", "largest")
+SPLIT_SAMPLES = (128, 256, 512, 1024)  # problems of at most this many samples go one per wave
+MAX_THREADS_PER_CU = 2048              # 32 waves resident per compute unit
 
 _CACHE: dict = {}
 
@@ -35,9 +39,36 @@ def compute_units(device) -> int:
     return int(torch.cuda.get_device_properties(device).multi_processor_count)
 
 
-def candidates(device) -> list[tuple[tuple[int, int], str]]:
-    cus = compute_units(device)
-    return [((t, g * cus), o) for t in THREADS for g in GROUPS_PER_CU for o in ORDERS]
+def candidates(batch) -> list[tuple[tuple[int, int, int], str]]:
+    """((threads, groups, wave_limit), order) for every shape worth timing on this batch.
+
+    wave_limit > 0 (largest-first order only): problems of at most that many
+    samples are solved one per wave; 0: every problem by a whole workgroup.
+    """
+    cus = compute_units(batch.device)
+    out = []
+    for t in THREADS:
+        for g in GROUPS_PER_CU:
+            if t * g > MAX_THREADS_PER_CU:
+                continue
+            out += [((t, g * cus, 0), o) for o in ORDERS]
+            if t == 64:
+                continue  # a 64-thread workgroup is one wave already
+            for limit in SPLIT_SAMPLES:
+                if 0 < wave_problems(batch, limit) < batch.num_problems:
+                    out.append(((t, g * cus, limit), "largest"))
+    return out
+
+
+def wave_problems(batch, limit: int) -> int:
+    """Problems of at most ``limit`` samples: the trailing slots of the largest-first order."""
+    return int(np.sum(batch.sizes <= limit)) if limit > 0 else 0
+
+
+def launch_shape(batch, choice) -> tuple[tuple[int, int, int], str]:
+    """A tuned ((threads, groups, wave_limit), order) as the launch schedule of this batch."""
+    (threads, groups, limit), order = choice
+    return (threads, groups, wave_problems(batch, limit)), order
 
 
 def time_schedule(batch, options, schedule, order, repeats: int = 2) -> float:
@@ -54,22 +85,27 @@ def time_schedule(batch, options, schedule, order, repeats: int = 2) -> float:
     return start.elapsed_time(end) / repeats
 
 
-def curve_schedule(batch, options, *, retune: bool = False) -> tuple[tuple[int, int], str]:
-    """Fastest ((threads, groups), order) for this batch, timed once per signature."""
+def curve_schedule(batch, options, *, retune: bool = False) -> tuple[tuple[int, int, int], str]:
+    """Fastest ((threads, groups, wave_problems), order) for this batch.
+
+    Candidates are timed once per batch signature; the cache keeps the size
+    limit of the wave split, which is turned into this batch's problem count.
+    """
     key = _signature(batch, options)
-    if not retune and key in _CACHE:
-        return _CACHE[key][0]
-    best: Optional[tuple] = None
-    timings = []
-    for sched, order in candidates(batch.device):
-        ms = time_schedule(batch, options, sched, order)
-        timings.append((sched, order, ms))
-        if best is None or ms < best[1]:
-            best = ((sched, order), ms)
-    _CACHE[key] = (best[0], timings)
-    return best[0]
+    if retune or key not in _CACHE:
+        best: Optional[tuple] = None
+        timings = []
+        for choice in candidates(batch):
+            sched, order = launch_shape(batch, choice)
+            ms = time_schedule(batch, options, sched, order)
+            timings.append((choice, ms))
+            if best is None or ms < best[1]:
+                best = (choice, ms)
+        _CACHE[key] = (best[0], timings)
+    return launch_shape(batch, _CACHE[key][0])
 
 
 def tuning_table() -> dict:
-    """The cached decisions: signature -> (choice, [(schedule, order, ms), ...])."""
+    """The cached decisions: signature -> (choice, [(choice, ms), ...]), choice = ((threads, groups,
+    wave_limit), order)."""
     return dict(_CACHE)

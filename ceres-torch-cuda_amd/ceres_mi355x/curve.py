@@ -4,10 +4,12 @@ The reference solves one curve-fit ``Problem`` at a time: one autograd
 residual block per sample, a robust loss on each, the trust-region loop of
 ``solver.py:35-357`` and a dense QR step (``examples/robust_curve_fitting.py:80-97``,
 ``linear.py:359-371``).  Here many such problems of uneven size are solved in
-one persistent device launch (``ct_curve_solve``): each workgroup runs the
-whole LM loop of one problem at a time, taking problems from a work queue in a
-chosen order.  The launch shape (workgroup size, number of persistent
-workgroups, order) comes from ``ceres_mi355x.autotune``.
+one persistent device launch (``ct_curve_solve``): a team (a whole workgroup
+for the leading problems of the processing order, one wave for the trailing
+ones) runs the whole LM loop of one problem at a time, taking problems from
+work queues in a chosen order.  The launch shape (workgroup size, number of
+persistent workgroups, how many problems go one per wave, order) comes from
+``ceres_mi355x.autotune``.  The mixed form reads FP32 copies of the samples.
 """
 from __future__ import annotations
 
@@ -37,7 +39,8 @@ class CurveBatch:
     """Device-resident samples of many curve-fit problems.
 
     ``offsets`` [B+1] int64 (problem p owns samples offsets[p]..offsets[p+1]),
-    ``x`` / ``y`` [S] float64.  The batch is validated once on the host.
+    ``x`` / ``y`` [S] float64, kept with FP32 copies for the mixed form (which
+    solves the FP32-rounded samples).  The batch is validated once on the host.
     """
 
     def __init__(self, offsets, x, y, loss=None, device="cuda"):
@@ -46,6 +49,8 @@ class CurveBatch:
         self.offsets = torch.as_tensor(np.asarray(offsets, dtype=np.int64)).to(self.device)
         self.x = torch.as_tensor(np.asarray(x, dtype=np.float64)).to(self.device)
         self.y = torch.as_tensor(np.asarray(y, dtype=np.float64)).to(self.device)
+        self.x32 = self.x.to(torch.float32)
+        self.y32 = self.y.to(torch.float32)
         self.num_problems = int(self.offsets.numel()) - 1
         self.loss_native = native_loss(loss)
         self.sizes = np.diff(np.asarray(offsets, dtype=np.int64))
@@ -76,6 +81,8 @@ class CurveBatch:
         b.offsets = nat.ptr(self.offsets)
         b.x = nat.ptr(self.x)
         b.y = nat.ptr(self.y)
+        b.x32 = nat.ptr(self.x32)
+        b.y32 = nat.ptr(self.y32)
         b.order = nat.ptr(order)
         kind, a, bb, scale = self.loss_native
         b.loss = nat.ct_loss(LOSS_IDS[kind], float(a), float(bb), float(scale))
@@ -83,9 +90,13 @@ class CurveBatch:
 
     def launch(self, options: nat.ct_lm_options, params: torch.Tensor, *, schedule=None, order: str = "index",
                results: Optional[torch.Tensor] = None, records: Optional[torch.Tensor] = None) -> None:
-        """Enqueue one batched solve on the current stream (no synchronisation; graph-capturable)."""
-        threads, groups = schedule if schedule is not None else (256, 0)
-        sched = nat.ct_batch_schedule(int(threads), int(groups))
+        """Enqueue one batched solve on the current stream (no synchronisation; graph-capturable).
+
+        ``schedule`` is (threads, groups) or (threads, groups, wave_problems).
+        """
+        shape = tuple(schedule) if schedule is not None else (256, 0)
+        threads, groups, wave_problems = shape + (0,) * (3 - len(shape))
+        sched = nat.ct_batch_schedule(int(threads), int(groups), int(wave_problems))
         nat.check(nat.load().ct_curve_solve(ctypes.byref(self._struct(self.order(order))), ctypes.byref(options),
                                             ctypes.byref(sched), nat.ptr(params), nat.ptr(results),
                                             nat.ptr(records), nat.ptr(self._workspace), self._workspace.numel(),

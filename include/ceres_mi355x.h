@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 9
+#define CT_ABI_VERSION 10
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -324,6 +324,8 @@ typedef struct ct_curve_batch {
   const double* y;          /* [num_samples] device */
   const int32_t* order;     /* [num_problems] device processing order (a permutation), or NULL */
   ct_loss loss;
+  const float* x32;         /* [num_samples] device FP32 samples, read by the mixed form only (which then  */
+  const float* y32;         /* solves the FP32-rounded problem; x / y may be NULL for it)                  */
 } ct_curve_batch;
 
 /* Launch shape of a persistent batched solve (chosen by the autotuner of
@@ -331,15 +333,19 @@ typedef struct ct_curve_batch {
 typedef struct ct_batch_schedule {
   int32_t threads;          /* workgroup size: 64, 128, 256 or 512 */
   int32_t groups;           /* persistent workgroups; 0 = 8 per compute unit */
+  int32_t wave_problems;    /* the last wave_problems problems of the processing order are solved by one
+                               wave (64 lanes) each, the others by a whole workgroup; 0 = none */
 } ct_batch_schedule;
 
-/* Device workspace of ct_curve_solve (a work-queue counter). */
+/* Device workspace of ct_curve_solve (two work-queue counters). */
 int ct_curve_workspace_bytes(const ct_curve_batch* batch, size_t* bytes);
 
 /* Solve every problem of the batch: params [num_problems][2] (m, c) FP64
  * device, start values in, best values out; results [num_problems] device
  * (may be NULL); records [num_problems][max_iter+2][CT_REC_FIELDS] device (may
- * be NULL).  options->precision CT_MIXED accumulates J^T J in FP32.  Enqueues
+ * be NULL).  options->precision CT_MIXED evaluates residuals, Jacobians, J^T J
+ * and J^T r in FP32 from x32 / y32 and every cost (the candidate cost that
+ * accepts or rejects a step, the cost tests) in FP64.  Enqueues
  * two kernels on stream and returns: no host synchronisation, so a caller may
  * capture it in a graph.  Malformed sample ranges end that problem with
  * CT_REASON_INVALID_INPUT; ct_curve_validate checks them on the host.        */

@@ -35,9 +35,19 @@ def _golden_batch():
     return g, data, batch, opts
 
 
-@pytest.mark.parametrize("schedule,order", [((64, 0), "index"), ((256, 0), "largest"), ((512, 64), "index")])
-def test_curve_batch_matches_reference_trajectories(schedule, order):
+def _waves(batch, limit):
+    """Problems of at most ``limit`` samples: solved one per wave at the end of the largest-first order."""
+    return int(np.sum(batch.sizes <= limit))
+
+
+@pytest.mark.parametrize("schedule,order,wave_limit", [((64, 0), "index", 0), ((256, 0), "largest", 0),
+                                                       ((512, 64), "index", 0), ((256, 0), "largest", 1000),
+                                                       ((128, 8), "largest", 10 ** 6)])
+def test_curve_batch_matches_reference_trajectories(schedule, order, wave_limit):
     g, data, batch, opts = _golden_batch()
+    if wave_limit:
+        schedule = schedule + (_waves(batch, wave_limit),)
+        assert 0 < schedule[2] <= batch.num_problems
     res = batch.solve(opts, schedule=schedule, order=order, records=True)
     params = res.params.cpu().numpy()
     for case in g["problems"]:
@@ -71,19 +81,56 @@ def test_curve_results_do_not_depend_on_queue_order():
     assert torch.equal(a.params, b.params)  # same workgroup size: bit-identical per problem
     c = batch.solve(opts, schedule=(512, 0), order="index")
     np.testing.assert_allclose(c.reports["final_cost"], a.reports["final_cost"], rtol=1e-12)
+    # a problem solved by one wave gives the same bits whatever the workgroup size and count
+    w = batch.num_problems
+    d = batch.solve(opts, schedule=(128, 16, w), order="largest")
+    e = batch.solve(opts, schedule=(512, 4, w), order="index")
+    assert torch.equal(d.params, e.params)
+    np.testing.assert_allclose(d.reports["final_cost"], a.reports["final_cost"], rtol=1e-12)
 
 
 def test_curve_mixed_precision_reaches_fp64_cost():
+    """The mixed form (FP32 samples, FP32 evaluation, FP64 costs) against the FP64 form and the FP64 oracle."""
+    from oracle.lm import OracleOptions, trust_region_solve
+    from oracle.small_models import CurveModel
+
     data = gen.make_curve_batch(num_problems=256, seed=7)
     batch = CurveBatch(data.offsets, data.x, data.y, loss=tc.CauchyLoss(0.5))
+    x_r = data.x.astype(np.float32).astype(np.float64)
+    y_r = data.y.astype(np.float32).astype(np.float64)
+    rounded = CurveBatch(data.offsets, x_r, y_r, loss=tc.CauchyLoss(0.5))
     base = tc.SolverOptions(max_num_iterations=200, function_tolerance=1e-13, gradient_tolerance=0.0,
                             parameter_tolerance=0.0)
-    r64 = batch.solve(lm_options(base, precision="fp64"), schedule=(256, 0), order="largest")
-    r32 = batch.solve(lm_options(base, precision="mixed"), schedule=(256, 0), order="largest")
-    np.testing.assert_allclose(r32.reports["final_cost"], r64.reports["final_cost"], rtol=1e-6)
+    r32 = batch.solve(lm_options(base, precision="mixed"), schedule=(256, 0, 128), order="largest")
+    r64 = rounded.solve(lm_options(base, precision="fp64"), schedule=(256, 0), order="largest")
+    assert np.all(r32.reports["termination"] == 0)
+    # the same FP32-rounded problem: the FP64 costs of both forms agree far below the FP32 evaluation error
+    np.testing.assert_allclose(r32.reports["final_cost"], r64.reports["final_cost"], rtol=1e-9)
+    np.testing.assert_allclose(r32.params.cpu().numpy(), r64.params.cpu().numpy(), rtol=1e-5, atol=1e-7)
+    # against the FP64 oracle on the original samples (the rounding of the samples moves the optimum slightly)
+    opts = OracleOptions(max_num_iterations=200, function_tolerance=1e-13, gradient_tolerance=0.0,
+                         parameter_tolerance=0.0)
+    for i in range(0, 256, 51):
+        lo, hi = data.offsets[i], data.offsets[i + 1]
+        ref, _ = trust_region_solve(CurveModel(data.x[lo:hi], data.y[lo:hi]), opts)
+        assert abs(r32.reports["final_cost"][i] - ref.final_cost) <= 1e-6 * ref.final_cost, i
     # the fit recovers the generating parameters
-    err_m = np.abs(r64.params[:, 0].cpu().numpy() - data.true_m)
+    err_m = np.abs(r32.params[:, 0].cpu().numpy() - data.true_m)
     assert np.median(err_m) < 0.02
+
+
+def test_curve_mixed_form_requires_fp32_samples():
+    g, data, batch, _ = _golden_batch()
+    b = batch._struct(None)
+    b.x32 = None
+    opts = lm_options(tc.SolverOptions(max_num_iterations=5), precision="mixed")
+    params = torch.zeros(batch.num_problems, 2, dtype=torch.float64, device="cuda")
+    sched = nat.ct_batch_schedule(64, 0, 0)
+    rc = nat.load().ct_curve_solve(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(sched), nat.ptr(params), None,
+                                   None, nat.ptr(batch._workspace), batch._workspace.numel(),
+                                   nat.current_stream_handle())
+    assert rc == nat.CT_ERR_ARGUMENT
+    assert b"x32" in nat.load().ct_last_error()
 
 
 def test_curve_graph_capture_replays_match_eager():
