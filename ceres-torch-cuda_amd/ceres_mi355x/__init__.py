@@ -14,7 +14,9 @@ from .costs import (
     DynamicAutoDiffCostFunction,
     ExpCurveResidual,
     Pose2DRelativeError,
+    QuaternionReprojectionError,
     SnavelyReprojectionError,
+    quaternion_reprojection,
     relative_pose,
     snavely_reprojection,
 )
@@ -31,11 +33,21 @@ from .losses import (
     TrivialLoss,
     TukeyLoss,
 )
-from .manifolds import EuclideanManifold, Manifold, SubsetManifold
+from .manifolds import EigenQuaternionManifold, EuclideanManifold, Manifold, QuaternionManifold, SubsetManifold
 from .problem import CRSMatrix, EvaluateOptions, EvaluationResult, ParameterBlock, Problem, ResidualBlock
 from .bal import BALProblem
 from .covariance import Covariance, CovarianceOptions, CovarianceSummary
-from .rotation import angle_axis_rotate_point, angle_axis_to_quaternion, unit_quaternion_rotate_point
+from .rotation import (
+    angle_axis_rotate_point,
+    angle_axis_to_quaternion,
+    convert_ceres_to_eigen_quaternion,
+    convert_eigen_to_ceres_quaternion,
+    normalize_quaternion,
+    quaternion_conjugate,
+    quaternion_product,
+    quaternion_rotate_point,
+    unit_quaternion_rotate_point,
+)
 from .solver import set_device_loop, solve
 from .types import (
     CallbackReturnType,

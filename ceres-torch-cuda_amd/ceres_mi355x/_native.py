@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 10  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 12  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "dogleg",
            "step", "candidate", "lm_end"]
 REC = dict(
@@ -75,6 +75,7 @@ class ct_lm_options(ctypes.Structure):
         ("max_solver_time_in_seconds", ctypes.c_double),
         ("debug_invalid_steps", ctypes.c_int32),
         ("schur_operator", ctypes.c_int32),
+        ("debug_fill_lds", ctypes.c_int32),
     ]
 
 
@@ -116,6 +117,8 @@ class ct_ba_problem(ctypes.Structure):
         ("camera_mask", _vp),
         ("point_mask", _vp),
         ("loss", ct_loss),
+        ("camera_model", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

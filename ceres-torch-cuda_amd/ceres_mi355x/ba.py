@@ -98,14 +98,30 @@ def native_loss_struct(native: tuple) -> nat.ct_loss:
     return nat.ct_loss(LOSS_IDS[kind], float(a), float(b), float(scale))
 
 
+# ct_camera_model and the number of stored values per camera
+CAMERA_MODELS = {"angle_axis": (0, 9), "quaternion": (1, 10), "eigen_quaternion": (2, 10)}
+
+
 class BADevice:
-    """Device-resident BA problem: tables, observations, masks and parameters."""
+    """Device-resident BA problem: tables, observations, masks and parameters.
+
+    ``cameras`` [C, 9] (angle-axis) or [C, 10] (``camera_model`` "quaternion":
+    q in w, x, y, z order, or "eigen_quaternion": x, y, z, w; then t, f, k1,
+    k2); ``camera_mask`` [C, 9] is over the tangent coordinates.
+    """
 
     def __init__(self, cameras: torch.Tensor, points: torch.Tensor, obs_camera, obs_point, observations, *,
-                 loss_native=("trivial", 0.0, 0.0, 1.0), camera_mask=None, point_mask=None, device="cuda"):
+                 loss_native=("trivial", 0.0, 0.0, 1.0), camera_mask=None, point_mask=None, device="cuda",
+                 camera_model: str = "angle_axis"):
         nat.load()
         self.device = torch.device(device)
+        if camera_model not in CAMERA_MODELS:
+            raise ValueError(f"camera_model must be one of {sorted(CAMERA_MODELS)}")
+        self.camera_model = camera_model
+        stride = CAMERA_MODELS[camera_model][1]
         C, P = int(cameras.shape[0]), int(points.shape[0])
+        if cameras.numel() != stride * C:
+            raise ValueError(f"a {camera_model} camera has {stride} values")
         self.tables = build_tables(obs_camera, obs_point, C, P)
         t = self.tables
         obs = np.asarray(observations, dtype=np.float64).reshape(-1, 2)[t.order]
@@ -153,6 +169,7 @@ class BADevice:
         p.camera_mask = nat.ptr(self.camera_mask)
         p.point_mask = nat.ptr(self.point_mask)
         p.loss = native_loss_struct(self.loss_native)
+        p.camera_model = CAMERA_MODELS[self.camera_model][0]
         return p
 
     def set_parameters(self, cameras: torch.Tensor, points: torch.Tensor) -> None:
@@ -248,6 +265,7 @@ class BADevice:
 
 
 _debug_invalid_steps = 0
+_debug_fill_lds = 0
 
 # How ITERATIVE_SCHUR forms S p: "auto" (from the evaluation's records when the
 # plan allows it) or "explicit" (assembled S); see set_schur_operator.
@@ -278,6 +296,19 @@ def debug_invalid_steps(count: int):
         yield
     finally:
         _debug_invalid_steps = previous
+
+
+@contextlib.contextmanager
+def debug_fill_lds():
+    """Test hook: inside the block, every bundle-adjustment solve fills each compute unit's LDS
+    with NaN right before each reduced-camera solve, so a solver kernel that reads LDS it has not
+    written in its own launch fails its parity test.  Not for normal use."""
+    global _debug_fill_lds
+    previous, _debug_fill_lds = _debug_fill_lds, 1
+    try:
+        yield
+    finally:
+        _debug_fill_lds = previous
 
 
 def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: str = "dense_schur",
@@ -312,6 +343,7 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     pc = getattr(options, "preconditioner_type", None)
     o.preconditioner = PRECONDITIONERS.get(pc.name, 0) if pc is not None else 0
     o.debug_invalid_steps = _debug_invalid_steps
+    o.debug_fill_lds = _debug_fill_lds
     o.schur_operator = 0 if SCHUR_OPERATOR["mode"] == "auto" else 1
     return o
 

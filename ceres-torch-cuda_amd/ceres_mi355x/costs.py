@@ -19,7 +19,7 @@ from typing import Callable, Optional, Sequence
 
 import torch
 
-from .rotation import angle_axis_rotate_point
+from .rotation import angle_axis_rotate_point, convert_eigen_to_ceres_quaternion, quaternion_rotate_point
 
 
 def as_residual_tensor(value, like: Optional[torch.Tensor] = None) -> torch.Tensor:
@@ -88,6 +88,7 @@ class SnavelyReprojectionError(CostFunction):
     parameter_block_sizes = [9, 3]
     num_residuals = 2
     device_kind = "ba"
+    camera_model = "angle_axis"
 
     def __init__(self, observed_x: float, observed_y: float) -> None:
         self.observed_x = float(observed_x)
@@ -96,6 +97,56 @@ class SnavelyReprojectionError(CostFunction):
     def residuals(self, camera, point):
         obs = torch.tensor([self.observed_x, self.observed_y], dtype=camera.dtype, device=camera.device)
         return snavely_reprojection(camera, point) - obs
+
+
+def quaternion_reprojection(rotation: torch.Tensor, camera: torch.Tensor, point: torch.Tensor,
+                            order: str = "ceres") -> torch.Tensor:
+    """Predicted pixel of ``point`` in a camera whose rotation is a quaternion.
+
+    The BAL projection of simple_bundle_adjuster.py:73-79 with the rotation
+    quaternion_rotate_point(q, X) (rotation.py:217-218) in place of the
+    angle-axis one; ``camera`` is (t(3), f, k1, k2), ``order`` the storage
+    order of ``rotation`` ("ceres": w, x, y, z; "eigen": x, y, z, w).
+    """
+    q = rotation.reshape(4)
+    if order == "eigen":
+        q = convert_eigen_to_ceres_quaternion(q)
+    elif order != "ceres":
+        raise ValueError("order must be 'ceres' or 'eigen'")
+    p = quaternion_rotate_point(q, point) + camera[:3]
+    xp = -p[0] / p[2]
+    yp = -p[1] / p[2]
+    r2 = xp * xp + yp * yp
+    distortion = 1.0 + r2 * (camera[4] + camera[5] * r2)
+    return camera[3] * distortion * torch.stack([xp, yp])
+
+
+class QuaternionReprojectionError(CostFunction):
+    """Reprojection residual of one observation; blocks [rotation(4), camera(6): t, f, k1, k2, point(3)].
+
+    The rotation block carries a QuaternionManifold (``order="ceres"``) or an
+    EigenQuaternionManifold (``order="eigen"``); on the device the two camera
+    blocks form one quaternion camera of 10 values and 9 tangent coordinates.
+    """
+
+    parameter_block_sizes = [4, 6, 3]
+    num_residuals = 2
+    device_kind = "ba"
+
+    def __init__(self, observed_x: float, observed_y: float, order: str = "ceres") -> None:
+        if order not in ("ceres", "eigen"):
+            raise ValueError("order must be 'ceres' or 'eigen'")
+        self.observed_x = float(observed_x)
+        self.observed_y = float(observed_y)
+        self.order = order
+
+    @property
+    def camera_model(self) -> str:
+        return "quaternion" if self.order == "ceres" else "eigen_quaternion"
+
+    def residuals(self, rotation, camera, point):
+        obs = torch.tensor([self.observed_x, self.observed_y], dtype=camera.dtype, device=camera.device)
+        return quaternion_reprojection(rotation, camera, point, self.order) - obs
 
 
 class ExpCurveResidual(CostFunction):

@@ -151,6 +151,8 @@ class Covariance:
             raise NotImplementedError("the pseudo-inverse covariance (null_space_rank != 0) is not on the MI355X "
                                       "device path")
         ex = _BAExtract(problem)  # raises NotImplementedError for other problems
+        if ex.camera_model != "angle_axis":
+            raise NotImplementedError("the covariance of quaternion cameras is not on the MI355X device path")
         s = self.summary
         s.num_requested_blocks = len(pairs)
         s.num_computed_blocks = len(self._blocks)

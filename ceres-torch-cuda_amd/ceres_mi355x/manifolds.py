@@ -1,15 +1,22 @@
-"""Manifolds on the device path (``src/ceres_torch/manifolds.py:17-124``).
+"""Manifolds on the device path (``src/ceres_torch/manifolds.py:17-171``).
 
 Euclidean Plus is ``x + delta`` (``manifolds.py:69-91``).  A SubsetManifold
 holds chosen coordinates constant (``manifolds.py:94-124``); on the device it
 becomes a per-parameter mask that zeroes those Jacobian columns, which leaves
 their step at exactly zero - the same update the reduced tangent space gives.
+QuaternionManifold / EigenQuaternionManifold (``manifolds.py:127-171``) move a
+4-value rotation along its 3-dimensional tangent; on the device they are the
+rotation of the quaternion camera models (the HIP kernels apply the same Plus
+and the Jacobian ``J_ambient @ plus_jacobian``).
 """
 from __future__ import annotations
 
 from typing import Sequence
 
 import torch
+
+from .rotation import (convert_ceres_to_eigen_quaternion, convert_eigen_to_ceres_quaternion, normalize_quaternion,
+                       quaternion_conjugate, quaternion_product)
 
 
 class Manifold:
@@ -28,9 +35,15 @@ class Manifold:
     def Plus(self, x, delta):
         return self.plus(x, delta)
 
+    def Minus(self, y, x):
+        return self.minus(y, x)
+
+    def PlusJacobian(self, x):
+        return self.plus_jacobian(x)
+
     def free_mask(self) -> list[bool]:
         """Ambient coordinates that move under Plus (device form)."""
-        raise NotImplementedError(f"{type(self).__name__} has no device form")
+        raise NotImplementedError(f"{type(self).__name__} has no device form as a per-coordinate mask")
 
 
 class EuclideanManifold(Manifold):
@@ -78,3 +91,51 @@ class SubsetManifold(Manifold):
 
     def free_mask(self) -> list[bool]:
         return [i not in self.constant_parameters for i in range(self.size)]
+
+
+class QuaternionManifold(Manifold):
+    """Unit quaternions (w, x, y, z): Plus(x, d) = normalize(q_d * normalize(x)) with
+    q_d = (cos|d|, sin|d| d / |d|) (manifolds.py:127-158)."""
+
+    ambient_size = 4
+    tangent_size = 3
+    device_camera_model = "quaternion"
+
+    def plus(self, x, delta):
+        x = normalize_quaternion(x.reshape(4))
+        delta = delta.reshape(3)
+        nd = torch.linalg.norm(delta)
+        eps = torch.finfo(delta.dtype).eps
+        scale = torch.sin(nd) / nd if float(nd) > eps else 1.0 - nd * nd / 6.0
+        q_delta = torch.cat([torch.cos(nd).reshape(1), scale * delta])
+        return normalize_quaternion(quaternion_product(q_delta, x))
+
+    def minus(self, y, x):
+        q = quaternion_product(normalize_quaternion(y.reshape(4)), quaternion_conjugate(normalize_quaternion(x.reshape(4))))
+        u = q[1:]
+        un = torch.linalg.norm(u)
+        if float(un) > torch.finfo(q.dtype).eps:
+            return torch.atan2(un, q[0]) / un * u
+        return u
+
+    def plus_jacobian(self, x):
+        w, a, b, c = normalize_quaternion(x.reshape(4))
+        return torch.stack([torch.stack([-a, -b, -c]), torch.stack([w, c, -b]),
+                            torch.stack([-c, w, a]), torch.stack([b, -a, w])])
+
+
+class EigenQuaternionManifold(QuaternionManifold):
+    """QuaternionManifold for quaternions stored (x, y, z, w) (manifolds.py:161-171)."""
+
+    device_camera_model = "eigen_quaternion"
+
+    def plus(self, x, delta):
+        return convert_ceres_to_eigen_quaternion(super().plus(convert_eigen_to_ceres_quaternion(x.reshape(4)), delta))
+
+    def minus(self, y, x):
+        return super().minus(convert_eigen_to_ceres_quaternion(y.reshape(4)),
+                             convert_eigen_to_ceres_quaternion(x.reshape(4)))
+
+    def plus_jacobian(self, x):
+        J = super().plus_jacobian(convert_eigen_to_ceres_quaternion(x.reshape(4)))
+        return torch.cat([J[1:], J[:1]], dim=0)

@@ -97,9 +97,47 @@ def _effective_solver(t: LinearSolverType) -> LinearSolverType:
 # ---------------------------------------------------------------------------
 
 
+class _QuaternionCamera:
+    """The two parameter blocks of a quaternion camera (QuaternionReprojectionError): the rotation
+    (4 values, QuaternionManifold or EigenQuaternionManifold) and t, f, k1, k2 (6 values).  To the
+    device they are one camera of 10 stored values and 9 tangent coordinates (rotation first)."""
+
+    def __init__(self, rotation, rest, order: str):
+        from .manifolds import EigenQuaternionManifold, QuaternionManifold
+
+        want = EigenQuaternionManifold if order == "eigen" else QuaternionManifold
+        m = rotation.manifold
+        if not isinstance(m, want) or (order == "ceres" and isinstance(m, EigenQuaternionManifold)):
+            raise NotImplementedError(f"the rotation block of a QuaternionReprojectionError(order={order!r}) needs "
+                                      f"a {want.__name__} on the MI355X device path")
+        if rest.size != 6:
+            raise NotImplementedError("the camera block of a QuaternionReprojectionError has 6 values")
+        self.rotation, self.rest = rotation, rest
+
+    @property
+    def blocks(self):
+        return (self.rotation, self.rest)
+
+    @property
+    def tangent_size(self) -> int:
+        return self.rotation.tangent_size + self.rest.tangent_size
+
+    def free_mask(self) -> list[bool]:
+        """Tangent coordinates that move: the rotation's three, then the rest block's six."""
+        return [not self.rotation.constant] * 3 + list(self.rest.free_mask())
+
+    def values(self) -> torch.Tensor:
+        return torch.cat([self.rotation.tensor.detach().reshape(-1), self.rest.tensor.detach().reshape(-1)])
+
+    def write(self, v: torch.Tensor) -> None:
+        self.rotation.tensor.reshape(-1).copy_(v[:4].to(self.rotation.tensor.device))
+        self.rest.tensor.reshape(-1).copy_(v[4:].to(self.rest.tensor.device))
+
+
 class _BAExtract:
     def __init__(self, problem: Problem):
         self.problem = problem
+        self.camera_model = "angle_axis"
         if problem.bulk_ba and problem._residual_blocks:
             raise NotImplementedError("mixing bulk and per-block bundle adjustment residuals is not supported")
         if problem.bulk_ba:
@@ -117,6 +155,10 @@ class _BAExtract:
             rbs = problem._residual_blocks
             if not rbs or any(rb.cost_function.device_kind != "ba" for rb in rbs):
                 raise NotImplementedError("not a bundle-adjustment problem")
+            models = {getattr(rb.cost_function, "camera_model", "angle_axis") for rb in rbs}
+            if len(models) != 1:
+                raise NotImplementedError("the device path needs one camera model for all residual blocks")
+            self.camera_model = models.pop()
             losses = {native_loss(rb.loss_function) for rb in rbs}
             if len(losses) != 1:
                 raise NotImplementedError("the device path needs one loss function for all residual blocks")
@@ -124,33 +166,58 @@ class _BAExtract:
             cam_index, pt_index = {}, {}
             self.cam_blocks, self.pt_blocks = [], []
             oc, op, ob = [], [], []
+            quaternion = self.camera_model != "angle_axis"
+            rotation_of, rest_of = {}, {}
             for rb in rbs:
-                cb, pb = rb.parameter_blocks
-                if cb not in cam_index:
-                    cam_index[cb] = len(self.cam_blocks)
-                    self.cam_blocks.append(cb)
+                if quaternion:
+                    qb, cb, pb = rb.parameter_blocks
+                    if rotation_of.setdefault(id(cb), qb) is not qb or rest_of.setdefault(id(qb), cb) is not cb:
+                        raise NotImplementedError("each rotation block must pair with one camera block on the "
+                                                  "MI355X device path")
+                    key = id(qb)
+                    if key not in cam_index:
+                        cam_index[key] = len(self.cam_blocks)
+                        self.cam_blocks.append(_QuaternionCamera(qb, cb, rb.cost_function.order))
+                else:
+                    cb, pb = rb.parameter_blocks
+                    key = cb
+                    if cb not in cam_index:
+                        cam_index[cb] = len(self.cam_blocks)
+                        self.cam_blocks.append(cb)
                 if pb not in pt_index:
                     pt_index[pb] = len(self.pt_blocks)
                     self.pt_blocks.append(pb)
-                oc.append(cam_index[cb])
+                oc.append(cam_index[key])
                 op.append(pt_index[pb])
                 ob.append((rb.cost_function.observed_x, rb.cost_function.observed_y))
-            if set(cam_index) & set(pt_index):
+            cam_ids = {id(b) for c in self.cam_blocks for b in (c.blocks if quaternion else (c,))}
+            if cam_ids & {id(b) for b in pt_index}:
                 raise NotImplementedError("a parameter block is used both as camera and point")
             self.obs_camera = np.asarray(oc)
             self.obs_point = np.asarray(op)
             self.observations = np.asarray(ob, dtype=np.float64)
             self.cam_base = self.pt_base = None
-        for b in self.cam_blocks + self.pt_blocks:
+        for b in self.parameter_blocks():
             if b.lower_bound is not None or b.upper_bound is not None:
                 raise NotImplementedError("parameter bounds are not on the MI355X device path")
             if b.tensor.dtype != torch.float64:
                 raise NotImplementedError("bundle-adjustment parameters must be float64 tensors")
 
+    def parameter_blocks(self) -> list:
+        """Every parameter block of the cameras, then the points."""
+        cams = [b for c in self.cam_blocks for b in (c.blocks if isinstance(c, _QuaternionCamera) else (c,))]
+        return cams + list(self.pt_blocks)
+
     def stacked(self):
         if self.cam_base is not None:
             return self.cam_base.detach(), self.pt_base.detach()
-        cams = torch.stack([b.tensor.detach().reshape(-1) for b in self.cam_blocks]) if self.cam_blocks else torch.zeros(0, 9, dtype=torch.float64)
+        stride = 9 if self.camera_model == "angle_axis" else 10
+        if not self.cam_blocks:
+            cams = torch.zeros(0, stride, dtype=torch.float64)
+        elif stride == 10:
+            cams = torch.stack([c.values() for c in self.cam_blocks])
+        else:
+            cams = torch.stack([b.tensor.detach().reshape(-1) for b in self.cam_blocks])
         pts = torch.stack([b.tensor.detach().reshape(-1) for b in self.pt_blocks]) if self.pt_blocks else torch.zeros(0, 3, dtype=torch.float64)
         return cams, pts
 
@@ -168,7 +235,10 @@ class _BAExtract:
             cams_h = cams.to("cpu")
             pts_h = pts.to("cpu")
             for i, b in enumerate(self.cam_blocks):
-                b.tensor.reshape(-1).copy_(cams_h[i].to(b.tensor.device))
+                if isinstance(b, _QuaternionCamera):
+                    b.write(cams_h[i])
+                else:
+                    b.tensor.reshape(-1).copy_(cams_h[i].to(b.tensor.device))
             for i, b in enumerate(self.pt_blocks):
                 b.tensor.reshape(-1).copy_(pts_h[i].to(b.tensor.device))
 
@@ -179,14 +249,14 @@ def _device_for(problem: Problem) -> BADevice:
     cams, pts = ex.stacked()
     cm, pm = ex.masks()
     # constant blocks and manifolds may change between solves: the masks are part of the key
-    key = (len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss,
+    key = (len(ex.cam_blocks), len(ex.pt_blocks), len(ex.obs_camera), ex.loss, ex.camera_model,
            None if cm is None else cm.tobytes(), None if pm is None else pm.tobytes())
     if cached is not None and cached[0] == key:
         dev = cached[1]
         dev.set_parameters(cams, pts)
         return dev, ex
     dev = BADevice(cams, pts, ex.obs_camera, ex.obs_point, ex.observations, loss_native=ex.loss,
-                   camera_mask=cm, point_mask=pm)
+                   camera_mask=cm, point_mask=pm, camera_model=ex.camera_model)
     problem._device_cache = (key, dev)
     return dev, ex
 
@@ -347,14 +417,21 @@ def _column_maps(problem: Problem, ex: "_BAExtract", blocks):
     for b in selected:
         if b.tangent_size == 0:
             continue
-        mask = b.free_mask()
+        # a quaternion rotation: its three tangent coordinates (no per-value mask)
+        mask = [True] * 3 if getattr(b.manifold, "device_camera_model", None) else b.free_mask()
         cols = np.full(len(mask), -1, dtype=np.int64)
         free = np.nonzero(mask)[0]
         cols[free] = offset + np.arange(free.size)
         column[id(b)] = cols
         offset += b.tangent_size
-    cam = np.stack([column.get(id(b), np.full(9, -1, np.int64)) for b in ex.cam_blocks]) if ex.cam_blocks else \
-        np.zeros((0, 9), np.int64)
+
+    def cam_cols(c):
+        if isinstance(c, _QuaternionCamera):
+            return np.concatenate([column.get(id(c.rotation), np.full(3, -1, np.int64)),
+                                   column.get(id(c.rest), np.full(6, -1, np.int64))])
+        return column.get(id(c), np.full(9, -1, np.int64))
+
+    cam = np.stack([cam_cols(c) for c in ex.cam_blocks]) if ex.cam_blocks else np.zeros((0, 9), np.int64)
     pt = np.stack([column.get(id(b), np.full(3, -1, np.int64)) for b in ex.pt_blocks]) if ex.pt_blocks else \
         np.zeros((0, 3), np.int64)
     return cam, pt, offset

@@ -32,8 +32,12 @@ int validate_problem(const ct_ba_problem* p) {
   }
   CT_CHECK_ARG(p->loss.type >= CT_LOSS_TRIVIAL && p->loss.type <= CT_LOSS_TUKEY, "unknown loss type");
   CT_CHECK_ARG((int64_t)p->num_cameras * 9 <= (int64_t)1 << 20, "too many cameras for the dense reduced system");
+  CT_CHECK_ARG(p->camera_model >= CT_CAMERA_ANGLE_AXIS && p->camera_model <= CT_CAMERA_EIGEN_QUATERNION,
+               "unknown camera model");
   return CT_OK;
 }
+
+int camera_stride(const ct_ba_problem* p) { return p->camera_model == CT_CAMERA_ANGLE_AXIS ? 9 : 10; }
 
 int validate_options(const ct_lm_options* o) {
   CT_CHECK_ARG(o != nullptr, "options is null");
@@ -50,6 +54,7 @@ int validate_options(const ct_lm_options* o) {
                "unknown preconditioner");
   CT_CHECK_ARG(o->debug_invalid_steps >= 0, "debug_invalid_steps must be >= 0");
   CT_CHECK_ARG(o->schur_operator == CT_SCHUR_AUTO || o->schur_operator == CT_SCHUR_EXPLICIT, "unknown schur_operator");
+  CT_CHECK_ARG(o->debug_fill_lds == 0 || o->debug_fill_lds == 1, "debug_fill_lds must be 0 or 1");
   return CT_OK;
 }
 
@@ -99,7 +104,7 @@ Layout carve(const ct_ba_problem* p, const ct_lm_options* o, void* base) {
   Carver cv(base);
   const size_t C = p->num_cameras, P = p->num_points, N = p->num_observations;
   const size_t n = 9 * C;
-  L.w.xcam = cv.take<double>(2 * 9 * C);
+  L.w.xcam = cv.take<double>(2 * (size_t)camera_stride(p) * C);
   L.w.xpt = cv.take<double>(2 * 3 * P);
   L.w.S = cv.take<double>(n * n);
   L.w.rhs = cv.take<double>(n);
@@ -184,6 +189,8 @@ BAView make_view(const ct_ba_problem* p) {
   v.point_mask = p->point_mask;
   v.loss = p->loss;
   if (v.loss.scale == 0.0) v.loss.scale = 1.0;
+  v.cam_model = p->camera_model;
+  v.cam_stride = camera_stride(p);
   return v;
 }
 
@@ -241,6 +248,7 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
     case CT_KERNEL_LM_BEGIN:
       return launch_lm_begin(pl->init, L.w, s);
     case CT_KERNEL_RCS_SOLVE:
+      if (pl->options.debug_fill_lds) CT_TRY(launch_fill_lds(__builtin_nan(""), s));
       if (pl->implicit) return launch_pcg_implicit(pl->imp, s);
       if (!uses_cholesky(&pl->options))
         return launch_rcs_pcg(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.pcg_work, L.pcg_doubles, s);
@@ -552,7 +560,8 @@ int ct_ba_plan_create(const ct_ba_problem* problem, const ct_lm_options* options
   pl->init.num_eval_blocks = eval_blocks(problem);
   pl->init.num_step_blocks = step_blocks(problem);
   pl->init.num_cameras = problem->num_cameras;
-  pl->init.n_cam_params = 9 * problem->num_cameras;
+  pl->init.n_cam_params = camera_stride(problem) * problem->num_cameras;
+  pl->init.split_blocks = problem->camera_model != CT_CAMERA_ANGLE_AXIS;
   pl->init.n_pt_params = 3 * problem->num_points;
   CT_TRY(build_point_groups(pl.get()));
   // ITERATIVE_SCHUR applies S from the records when the plan has point groups

@@ -42,7 +42,7 @@ struct LMState {
   int32_t pending_time;     // the time test after an accepted step runs with the deferred checks
   int32_t poison_left;      // test hook: steps still to be overwritten with NaN (debug_invalid_steps)
   int32_t poison;           // this iteration's step is overwritten with NaN
-  int32_t pad1;
+  int32_t split_blocks;     // SCHUR_JACOBI blocks of a camera split into rotation (3) and the rest (6)
   uint64_t t0;              // wall clock at start
   uint64_t t_end;           // wall clock at termination
 };
@@ -51,6 +51,8 @@ struct LMState {
 struct BAView {
   int32_t C, P, N, B;
   int64_t T;
+  int32_t cam_model;   // ct_camera_model
+  int32_t cam_stride;  // stored values per camera: 9 or 10
   const int32_t* __restrict__ obs_camera;
   const int32_t* __restrict__ obs_point;
   const int32_t* __restrict__ point_offsets;

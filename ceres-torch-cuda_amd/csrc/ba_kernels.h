@@ -86,6 +86,7 @@ struct LMInit {
   int num_cameras;
   int n_cam_params;
   int n_pt_params;
+  int split_blocks;      // LMState::split_blocks
 };
 int launch_lm_init(const LMInit& init, const BAWork& w, const double* user_cams, const double* user_pts,
                    hipStream_t s);
@@ -108,7 +109,9 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
                    size_t work_doubles, hipStream_t s);
 size_t rcs_pcg_work_doubles(int n);
 unsigned int* rcs_pcg_barrier(double* work, int n);  // the PCG solve-epoch word inside its workspace
-int rcs_pcg_prepare(double* work, int n, hipStream_t s);  // zero the epoch and barrier flags (once per plan)
+int rcs_pcg_prepare(double* work, int n, hipStream_t s);
+// Test hook: every compute unit's LDS filled with `value` (ct_lm_options::debug_fill_lds).
+int launch_fill_lds(double value, hipStream_t s);  // zero the epoch and barrier flags (once per plan)
 // Implicit-S PCG (pcg_implicit.hip): S p from the JP records, block-Jacobi
 // factors of the explicitly assembled diagonal blocks of S.
 constexpr int kImplicitMaxCameras = 300;  // p, r and z (9C entries each) in LDS beside the y staging

@@ -16,6 +16,17 @@ namespace ct {
 // (rotation.py:104-110, 211-222); its derivative uses Rodrigues' form, which
 // is the same function.
 // ---------------------------------------------------------------------------
+// p + 2w(u x p) + u x (2 u x p) for a unit quaternion (w, u) (rotation.py:211-214).
+__device__ __forceinline__ void unit_quat_rotate(double w, double u0, double u1, double u2, const double* X,
+                                                 double* out) {
+  const double v0 = 2.0 * (u1 * X[2] - u2 * X[1]);
+  const double v1 = 2.0 * (u2 * X[0] - u0 * X[2]);
+  const double v2 = 2.0 * (u0 * X[1] - u1 * X[0]);
+  out[0] = X[0] + w * v0 + (u1 * v2 - u2 * v1);
+  out[1] = X[1] + w * v1 + (u2 * v0 - u0 * v2);
+  out[2] = X[2] + w * v2 + (u0 * v1 - u1 * v0);
+}
+
 __device__ __forceinline__ void aa_rotate(const double* aa, const double* X, double* out) {
   const double t2 = aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2];
   const double theta = sqrt(t2);
@@ -26,12 +37,7 @@ __device__ __forceinline__ void aa_rotate(const double* aa, const double* X, dou
   double w = c, u0 = k * aa[0], u1 = k * aa[1], u2 = k * aa[2];
   const double nrm = fmax(sqrt(w * w + u0 * u0 + u1 * u1 + u2 * u2), 2.2250738585072014e-308);
   w /= nrm; u0 /= nrm; u1 /= nrm; u2 /= nrm;
-  const double v0 = 2.0 * (u1 * X[2] - u2 * X[1]);
-  const double v1 = 2.0 * (u2 * X[0] - u0 * X[2]);
-  const double v2 = 2.0 * (u0 * X[1] - u1 * X[0]);
-  out[0] = X[0] + w * v0 + (u1 * v2 - u2 * v1);
-  out[1] = X[1] + w * v1 + (u2 * v0 - u0 * v2);
-  out[2] = X[2] + w * v2 + (u0 * v1 - u1 * v0);
+  unit_quat_rotate(w, u0, u1, u2, X, out);
 }
 
 __device__ __forceinline__ void snavely_residual(const double* cam, const double* X, const double* obs,
@@ -134,6 +140,199 @@ __device__ __forceinline__ void snavely_jacobian(const double* cam, const double
   Jc[0][8] = f * xp * r2 * r2; Jc[1][8] = f * yp * r2 * r2;
 }
 
+// ---------------------------------------------------------------------------
+// Quaternion cameras [q(4), t(3), f, k1, k2], held here in Ceres order
+// (w, x, y, z; the Eigen order x, y, z, w is permuted on load and store).  The
+// rotation is quaternion_rotate_point(q, X) = unit_quaternion_rotate_point(
+// q / |q|, X) (rotation.py:68-69, 211-218).  The camera's tangent is 9-wide:
+// the 3-vector of QuaternionManifold.plus (manifolds.py:127-137), then t, f,
+// k1, k2.  The reference's tangent Jacobian is J_ambient(q) plus_jacobian(q)
+// (problem.py:505-514); along the manifold the rotated point R X moves by
+// d(R X)/d delta = -2 [R X]_x (q_delta = (cos|d|, sin|d| d/|d|) rotates by
+// 2|d|), and the normalisation in quaternion_rotate_point divides by |q|.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_quat_camera(const double* __restrict__ src, int model, double (&cam)[10]) {
+  if (model == CT_CAMERA_EIGEN_QUATERNION) {
+    cam[0] = src[3]; cam[1] = src[0]; cam[2] = src[1]; cam[3] = src[2];
+  } else {
+    cam[0] = src[0]; cam[1] = src[1]; cam[2] = src[2]; cam[3] = src[3];
+  }
+#pragma unroll
+  for (int j = 4; j < 10; ++j) cam[j] = src[j];
+}
+
+__device__ __forceinline__ void store_quat_camera(const double (&cam)[10], int model, double* __restrict__ dst) {
+  if (model == CT_CAMERA_EIGEN_QUATERNION) {
+    dst[3] = cam[0]; dst[0] = cam[1]; dst[1] = cam[2]; dst[2] = cam[3];
+  } else {
+    dst[0] = cam[0]; dst[1] = cam[1]; dst[2] = cam[2]; dst[3] = cam[3];
+  }
+#pragma unroll
+  for (int j = 4; j < 10; ++j) dst[j] = cam[j];
+}
+
+// R(q / |q|) X; returns 1 / |q| (|q| clamped at the smallest normal, as normalize_quaternion).
+__device__ __forceinline__ double quat_rotate(const double* q, const double* X, double* out) {
+  const double inv = 1.0 / fmax(sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]),
+                                2.2250738585072014e-308);
+  unit_quat_rotate(q[0] * inv, q[1] * inv, q[2] * inv, q[3] * inv, X, out);
+  return inv;
+}
+
+__device__ __forceinline__ void quat_camera_residual(const double (&cam)[10], const double* X, const double* obs,
+                                                     double* r) {
+  double p[3];
+  quat_rotate(cam, X, p);
+  p[0] += cam[4]; p[1] += cam[5]; p[2] += cam[6];
+  const double xp = -p[0] / p[2];
+  const double yp = -p[1] / p[2];
+  const double r2 = xp * xp + yp * yp;
+  const double d = 1.0 + r2 * (cam[8] + cam[9] * r2);
+  const double fd = cam[7] * d;
+  r[0] = fd * xp - obs[0];
+  r[1] = fd * yp - obs[1];
+}
+
+__device__ __forceinline__ void quat_camera_jacobian(const double (&cam)[10], const double* X, const double* obs,
+                                                     double (&r)[2], double (&Jc)[2][9], double (&Jp)[2][3]) {
+  double pr[3];
+  const double inv = quat_rotate(cam, X, pr);
+  const double p0 = pr[0] + cam[4], p1 = pr[1] + cam[5], p2 = pr[2] + cam[6];
+  const double f = cam[7], k1 = cam[8], k2 = cam[9];
+  const double iz = 1.0 / p2;
+  const double xp = -p0 * iz;
+  const double yp = -p1 * iz;
+  const double r2 = xp * xp + yp * yp;
+  const double d = 1.0 + r2 * (k1 + k2 * r2);
+  const double fd = f * d;
+  r[0] = fd * xp - obs[0];
+  r[1] = fd * yp - obs[1];
+  const double dd = k1 + 2.0 * k2 * r2;
+  const double u_x = f * (d + 2.0 * xp * xp * dd);
+  const double u_y = f * 2.0 * xp * yp * dd;
+  const double v_y = f * (d + 2.0 * yp * yp * dd);
+  double G[2][3];
+  G[0][0] = -u_x * iz; G[0][1] = -u_y * iz; G[0][2] = -(u_x * xp + u_y * yp) * iz;
+  G[1][0] = -u_y * iz; G[1][1] = -v_y * iz; G[1][2] = -(u_y * xp + v_y * yp) * iz;
+  // rotation columns: -2 [R X]_x / |q|
+  const double s = -2.0 * inv;
+  const double D[3][3] = {{0.0, -s * pr[2], s * pr[1]}, {s * pr[2], 0.0, -s * pr[0]}, {-s * pr[1], s * pr[0], 0.0}};
+  // d(R X)/dX = I + 2w [u]_x + 2 (u u^T - |u|^2 I), the derivative of the rotate formula at the unit quaternion
+  const double w = cam[0] * inv, u0 = cam[1] * inv, u1 = cam[2] * inv, u2 = cam[3] * inv;
+  const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+  const double M[3][3] = {{1.0 + 2.0 * (u0 * u0 - uu), 2.0 * (u0 * u1 - w * u2), 2.0 * (u0 * u2 + w * u1)},
+                          {2.0 * (u1 * u0 + w * u2), 1.0 + 2.0 * (u1 * u1 - uu), 2.0 * (u1 * u2 - w * u0)},
+                          {2.0 * (u2 * u0 - w * u1), 2.0 * (u2 * u1 + w * u0), 1.0 + 2.0 * (u2 * u2 - uu)}};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Jc[k][j] = G[k][0] * D[0][j] + G[k][1] * D[1][j] + G[k][2] * D[2][j];
+      Jc[k][3 + j] = G[k][j];
+      Jp[k][j] = G[k][0] * M[0][j] + G[k][1] * M[1][j] + G[k][2] * M[2][j];
+    }
+  }
+  Jc[0][6] = d * xp; Jc[1][6] = d * yp;
+  Jc[0][7] = f * xp * r2; Jc[1][7] = f * yp * r2;
+  Jc[0][8] = f * xp * r2 * r2; Jc[1][8] = f * yp * r2 * r2;
+}
+
+// QuaternionManifold.plus (manifolds.py:130-137) on the rotation and Euclidean
+// Plus on t, f, k1, k2: cam (Ceres order) <- Plus(cam, d[0..8]).  A constant
+// rotation block (rotate false) keeps its stored values: the reference's
+// apply_delta (problem.py:462-474) only moves blocks with a tangent space.
+__device__ __forceinline__ void quat_camera_plus(double (&cam)[10], const double* d, bool rotate) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) cam[4 + j] += d[3 + j];
+  if (!rotate) return;
+  const double inv = 1.0 / fmax(sqrt(cam[0] * cam[0] + cam[1] * cam[1] + cam[2] * cam[2] + cam[3] * cam[3]),
+                                2.2250738585072014e-308);
+  const double x0 = cam[0] * inv, x1 = cam[1] * inv, x2 = cam[2] * inv, x3 = cam[3] * inv;
+  const double nd = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double sn, cs;
+  sincos(nd, &sn, &cs);
+  const double scale = nd > 2.220446049250313e-16 ? sn / nd : 1.0 - nd * nd / 6.0;
+  const double z0 = cs, z1 = scale * d[0], z2 = scale * d[1], z3 = scale * d[2];
+  // quaternion_product(q_delta, x) (rotation.py:72-83)
+  const double y0 = z0 * x0 - z1 * x1 - z2 * x2 - z3 * x3;
+  const double y1 = z0 * x1 + z1 * x0 + z2 * x3 - z3 * x2;
+  const double y2 = z0 * x2 - z1 * x3 + z2 * x0 + z3 * x1;
+  const double y3 = z0 * x3 + z1 * x2 - z2 * x1 + z3 * x0;
+  const double iy = 1.0 / fmax(sqrt(y0 * y0 + y1 * y1 + y2 * y2 + y3 * y3), 2.2250738585072014e-308);
+  cam[0] = y0 * iy; cam[1] = y1 * iy; cam[2] = y2 * iy; cam[3] = y3 * iy;
+}
+
+// The rotation of quaternion camera ci moves unless its block is constant.
+__device__ __forceinline__ bool rotation_free(const BAView& v, int ci) {
+  return v.camera_mask == nullptr || v.camera_mask[9 * (size_t)ci] != 0;
+}
+
+// The problem's camera model: residual (and Jacobian) of one observation of
+// camera ci, whose values start at cams + cam_stride * ci.
+__device__ __forceinline__ void camera_jacobian(const BAView& v, const double* __restrict__ cams, int ci,
+                                                const double* X, const double* obs, double (&r)[2],
+                                                double (&Jc)[2][9], double (&Jp)[2][3]) {
+  if (v.cam_model == CT_CAMERA_ANGLE_AXIS) {
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * (size_t)ci + j];
+    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+  } else {
+    double cam[10];
+    load_quat_camera(cams + 10 * (size_t)ci, v.cam_model, cam);
+    quat_camera_jacobian(cam, X, obs, r, Jc, Jp);
+  }
+}
+
+__device__ __forceinline__ void camera_residual(const BAView& v, const double* __restrict__ cams, int ci,
+                                                const double* X, const double* obs, double* r) {
+  if (v.cam_model == CT_CAMERA_ANGLE_AXIS) {
+    snavely_residual(cams + 9 * (size_t)ci, X, obs, r);
+  } else {
+    double cam[10];
+    load_quat_camera(cams + 10 * (size_t)ci, v.cam_model, cam);
+    quat_camera_residual(cam, X, obs, r);
+  }
+}
+
+// Residual of camera ci after its step d[0..8] (Plus applied on the fly, as the
+// camera step stores it).
+__device__ __forceinline__ void camera_residual_after_step(const BAView& v, const double* __restrict__ cams, int ci,
+                                                           const double* d, const double* X, const double* obs,
+                                                           double* r) {
+  if (v.cam_model == CT_CAMERA_ANGLE_AXIS) {
+    double cam[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * (size_t)ci + j] + d[j];
+    snavely_residual(cam, X, obs, r);
+  } else {
+    double cam[10];
+    load_quat_camera(cams + 10 * (size_t)ci, v.cam_model, cam);
+    quat_camera_plus(cam, d, rotation_free(v, ci));
+    quat_camera_residual(cam, X, obs, r);
+  }
+}
+
+// Step of quaternion camera ci: ncams <- Plus(cams, dxc); adds |dx|^2 and |x'|^2
+// (all ten stored values) and flags a non-finite step.
+__device__ __forceinline__ void step_quat_camera(const BAView& v, const double* __restrict__ cams,
+                                                 double* __restrict__ ncams, const double* __restrict__ dxc, int ci,
+                                                 double& dx2, double& x2, double& bad) {
+  double d[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    d[j] = dxc[9 * (size_t)ci + j];
+    dx2 += d[j] * d[j];
+    if (!isfinite(d[j])) bad = 1.0;
+  }
+  double cam[10];
+  load_quat_camera(cams + 10 * (size_t)ci, v.cam_model, cam);
+  quat_camera_plus(cam, d, rotation_free(v, ci));
+  store_quat_camera(cam, v.cam_model, ncams + 10 * (size_t)ci);
+#pragma unroll
+  for (int j = 0; j < 10; ++j) x2 += cam[j] * cam[j];
+}
+
 // Ceres corrector on a 2-residual block (losses.py:158-193).  Returns the
 // block cost 1/2 rho0 and rescales r, Jc, Jp in place.
 __device__ __forceinline__ double robustify2(const ct_loss& L, double (&r)[2], double (&Jc)[2][9],
@@ -200,20 +399,17 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_eval_obs(BAView v, BABuffe
   const LMState* st = w.state;
   if (st->done) return;
   const int cur = st->cur;
-  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * v.cam_stride;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
   const int o = blockIdx.x * kPointThreads + threadIdx.x;
   double cost = 0.0;
   if (o < v.N) {
     const int ci = v.obs_camera[o];
     const int p = v.obs_point[o];
-    double cam[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
     const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
     double r[2], Jc[2][9], Jp[2][3];
-    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    camera_jacobian(v, cams, ci, X, obs, r, Jc, Jp);
     if (v.camera_mask) {
 #pragma unroll
       for (int j = 0; j < 9; ++j) {
@@ -383,7 +579,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
   const LMState* st = w.state;
   if (st->done) return;
   const int cur = st->cur;
-  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * v.cam_stride;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int grp = blockIdx.x * kWaves + wave;
@@ -403,13 +599,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_eval_fused(BAView v, BABuffer
     const int po0 = v.point_offsets[p];
     head = lane - (o - po0);
     count = v.point_offsets[p + 1] - po0;
-    double cam[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
     const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
     double Jp[2][3];
-    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    camera_jacobian(v, cams, ci, X, obs, r, Jc, Jp);
     if (v.camera_mask) {
 #pragma unroll
       for (int j = 0; j < 9; ++j) {
@@ -972,7 +1165,8 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_update(BAView v, BA
 }
 
 // ---------------------------------------------------------------------------
-// K5a: apply the step, x' = x + dx (Euclidean Plus, manifolds.py:69-91), with
+// K5a: apply the step, x' = Plus(x, dx) (Euclidean Plus, manifolds.py:69-91;
+// QuaternionManifold.plus on quaternion camera rotations), with
 // the point back-substitution fused when the solve did not already do it.
 // Block partials: |dx|^2, |x'|^2 over all parameters, non-finite count.
 // ---------------------------------------------------------------------------
@@ -982,19 +1176,23 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_step_points(BAView v, BABu
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int cur = st->cur;
-  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * v.cam_stride;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
-  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * v.cam_stride;
   double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
   const int t = blockIdx.x * kPointThreads + threadIdx.x;
   double dx2 = 0.0, x2 = 0.0, bad = 0.0;
-  if (t < 9 * v.C) {
-    const double d = w.dxc[t];
-    const double nx = cams[t] + d;
-    ncams[t] = nx;
-    dx2 += d * d;
-    x2 += nx * nx;
-    if (!isfinite(d)) bad = 1.0;
+  if (v.cam_model == CT_CAMERA_ANGLE_AXIS) {
+    if (t < 9 * v.C) {
+      const double d = w.dxc[t];
+      const double nx = cams[t] + d;
+      ncams[t] = nx;
+      dx2 += d * d;
+      x2 += nx * nx;
+      if (!isfinite(d)) bad = 1.0;
+    }
+  } else if (t < v.C) {
+    step_quat_camera(v, cams, ncams, w.dxc, t, dx2, x2, bad);
   }
   if (t < v.P) {
     double d0, d1, d2;
@@ -1037,7 +1235,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_candidate_obs(BAView v, BA
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int cur = st->cur;
-  const double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  const double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * v.cam_stride;
   const double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
   const int o = blockIdx.x * kPointThreads + threadIdx.x;
   double cost = 0.0, md = 0.0;
@@ -1057,13 +1255,10 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_candidate_obs(BAView v, BA
     }
     const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
     md = -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
-    double cam[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) cam[j] = ncams[9 * ci + j];
     const double X[3] = {npts[3 * p], npts[3 * p + 1], npts[3 * p + 2]};
     const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
     double rr[2];
-    snavely_residual(cam, X, obs, rr);
+    camera_residual(v, ncams, ci, X, obs, rr);
     cost = robust_cost2(v.loss, rr);
   }
   const double a0 = block_sum<kPointThreads>(cost, red);
@@ -1097,21 +1292,26 @@ __global__ __launch_bounds__(kFusedThreads) void k_step_candidate(BAView v, BABu
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int cur = st->cur;
-  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * 9;
+  const double* __restrict__ cams = w.xcam + (size_t)cur * v.C * v.cam_stride;
   const double* __restrict__ pts = w.xpt + (size_t)cur * v.P * 3;
-  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * 9;
+  double* __restrict__ ncams = w.xcam + (size_t)(1 - cur) * v.C * v.cam_stride;
   double* __restrict__ npts = w.xpt + (size_t)(1 - cur) * v.P * 3;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int grp = blockIdx.x * kWaves + wave;
   double dx2 = 0.0, x2 = 0.0, bad = 0.0, cost = 0.0, md = 0.0;
-  // cameras: x'_c = x_c + dx_c
-  for (int t = blockIdx.x * kFusedThreads + threadIdx.x; t < 9 * v.C; t += gridDim.x * kFusedThreads) {
-    const double dd = w.dxc[t];
-    const double nx = cams[t] + dd;
-    ncams[t] = nx;
-    dx2 += dd * dd;
-    x2 += nx * nx;
-    if (!isfinite(dd)) bad = 1.0;
+  // cameras: x'_c = Plus(x_c, dx_c)
+  if (v.cam_model == CT_CAMERA_ANGLE_AXIS) {
+    for (int t = blockIdx.x * kFusedThreads + threadIdx.x; t < 9 * v.C; t += gridDim.x * kFusedThreads) {
+      const double dd = w.dxc[t];
+      const double nx = cams[t] + dd;
+      ncams[t] = nx;
+      dx2 += dd * dd;
+      x2 += nx * nx;
+      if (!isfinite(dd)) bad = 1.0;
+    }
+  } else {
+    for (int t = blockIdx.x * kFusedThreads + threadIdx.x; t < v.C; t += gridDim.x * kFusedThreads)
+      step_quat_camera(v, cams, ncams, w.dxc, t, dx2, x2, bad);
   }
   int o = 0, p = 0, head = 0, count = 0, ci = 0;
   bool active = false;
@@ -1183,20 +1383,18 @@ __global__ __launch_bounds__(kFusedThreads) void k_step_candidate(BAView v, BABu
     const double d0 = pnew[wave][0][head], d1 = pnew[wave][1][head], d2 = pnew[wave][2][head];
     double u0 = (double)Jq[0] * d0 + (double)Jq[1] * d1 + (double)Jq[2] * d2;
     double u1 = (double)Jq[3] * d0 + (double)Jq[4] * d1 + (double)Jq[5] * d2;
-    double cam[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       const double dd = w.dxc[9 * ci + j];
       u0 += (double)J[j] * dd;
       u1 += (double)J[9 + j] * dd;
-      cam[j] = cams[9 * ci + j] + dd;
     }
     const double r0 = b.r[2 * (size_t)o], r1 = b.r[2 * (size_t)o + 1];
     md = -(r0 * u0 + r1 * u1) - 0.5 * (u0 * u0 + u1 * u1);
     const double X[3] = {pnew[wave][3][head], pnew[wave][4][head], pnew[wave][5][head]};
     const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
     double rr[2];
-    snavely_residual(cam, X, obs, rr);
+    camera_residual_after_step(v, cams, ci, w.dxc + 9 * (size_t)ci, X, obs, rr);
     cost = robust_cost2(v.loss, rr);
   }
   const double a0 = block_sum<kFusedThreads>(dx2, red);
@@ -1307,6 +1505,17 @@ __global__ __launch_bounds__(kPointThreads) void k_poison_step(BAWork w, int n_c
   for (int t = blockIdx.x * kPointThreads + threadIdx.x; t < n_pt; t += stride) w.dxp[t] = nan;
 }
 
+// Test hook: fill this workgroup's whole LDS allocation with v (volatile
+// stores, so nothing elides them).  Launched with the largest LDS a workgroup
+// may take, one workgroup holds a compute unit's LDS; two per compute unit
+// cover every one of them.
+__global__ __launch_bounds__(256) void k_fill_lds(double v, int count) {
+  extern __shared__ double lds_fill[];
+  volatile double* b = lds_fill;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) b[i] = v;
+  __syncthreads();
+}
+
 // Plain evaluation (cost + corrected residuals) at given parameters.
 __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const double* cams, const double* pts,
                                                                   double* residuals, double* part) {
@@ -1319,7 +1528,7 @@ __global__ __launch_bounds__(kPointThreads) void k_evaluate_only(BAView v, const
       const int ci = v.obs_camera[o];
       const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
       double r[2];
-      snavely_residual(cams + 9 * ci, X, obs, r);
+      camera_residual(v, cams, ci, X, obs, r);
       const double s = r[0] * r[0] + r[1] * r[1];
       double rho1, rho2;
       const double rho0 = loss_rho(v.loss, s, &rho1, &rho2);
@@ -1356,13 +1565,10 @@ __global__ __launch_bounds__(kPointThreads) void k_evaluate_jacobian(BAView v, c
   double cost = 0.0;
   if (o < v.N) {
     const int ci = v.obs_camera[o], p = v.obs_point[o];
-    double cam[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) cam[j] = cams[9 * ci + j];
     const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     const double obs[2] = {v.obs_xy[2 * o], v.obs_xy[2 * o + 1]};
     double r[2], Jc[2][9], Jp[2][3];
-    snavely_jacobian(cam, X, obs, r, Jc, Jp);
+    camera_jacobian(v, cams, ci, X, obs, r, Jc, Jp);
     if (apply_loss) {
       cost = robustify2(v.loss, r, Jc, Jp);
     } else {
@@ -1396,6 +1602,19 @@ __global__ __launch_bounds__(kPointThreads) void k_evaluate_jacobian(BAView v, c
 namespace {
 inline int blocks_for(int n) { return (n + kPointThreads - 1) / kPointThreads; }
 }  // namespace
+
+int launch_fill_lds(double value, hipStream_t s) {
+  int dev = 0, cus = 0, lds = 0;
+  CT_HIP(hipGetDevice(&dev));
+  CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  CT_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (lds <= 0 || cus <= 0) return CT_OK;
+  CT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_lds), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             lds));
+  hipLaunchKernelGGL(k_fill_lds, dim3(2 * cus), dim3(256), (size_t)lds, s, value, lds / (int)sizeof(double));
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
 
 template <typename real>
 int launch_eval_obs(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {

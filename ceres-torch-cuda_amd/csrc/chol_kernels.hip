@@ -1023,11 +1023,15 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg(const double* __restrict__ 
     timed_out = 0;
     singular = 0;
   }
-  // diagonal blocks of the owned rows, r = rhs, x = 0
+  // diagonal blocks of the owned rows, r = rhs, x = 0; with split blocks
+  // (quaternion cameras: the rotation and the other six values are separate
+  // parameter blocks) the rotation / rest coupling is left out of the block
+  const bool split = st->split_blocks != 0;
   for (int i = tid; i < nown * kBlk * kBlk; i += kPcgThreads) {
     const int b = i / (kBlk * kBlk), e = i % (kBlk * kBlk);
     const int row = kBlk * (g + b * G) + e / kBlk, col = kBlk * (g + b * G) + e % kBlk;
-    Lf[b][e] = S[(size_t)row * n + col];
+    const bool cross = split && ((e / kBlk < 3) != (e % kBlk < 3));
+    Lf[b][e] = cross ? 0.0 : S[(size_t)row * n + col];
     Sd[b][e] = Lf[b][e];
   }
   for (int i = tid; i < nown * kBlk; i += kPcgThreads) {

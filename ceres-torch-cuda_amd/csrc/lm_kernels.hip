@@ -69,7 +69,10 @@ __global__ void k_lm_init(LMInit init, BAWork w, const double* __restrict__ ucam
     w.xpt[i] = v;
     w.xpt[init.n_pt_params + i] = v;
   }
-  if (t == 0) lm_state_init(w.state, init.opt);
+  if (t == 0) {
+    lm_state_init(w.state, init.opt);
+    w.state->split_blocks = init.split_blocks;
+  }
 }
 
 __global__ __launch_bounds__(kLMThreads) void k_lm_begin(LMInit init, BAWork w) {

@@ -233,6 +233,7 @@ __device__ __forceinline__ void lm_state_init(LMState* st, const ct_lm_options& 
   st->pending_time = 0;
   st->poison_left = o.debug_invalid_steps > 0 ? o.debug_invalid_steps : 0;
   st->poison = 0;
+  st->split_blocks = 0;
   st->dl_cgn = 1.0;
   st->dl_cg = 0.0;
   st->radius = o.initial_trust_region_radius;

@@ -45,9 +45,11 @@ __device__ __forceinline__ bool skip_implicit(const LMState* st) { return st->do
 // The inverse of the 9x9 block at Sb (row stride n) by LU with partial
 // pivoting (the factorisation of lu9, rolled loops: a rare path), worked in
 // the LDS arrays W (factors) and M (inverse, row-major).  False if singular.
-__device__ __noinline__ bool lu_inverse(const double* Sb, int n, double* W, double* M, double* dv, int* piv) {
+__device__ __noinline__ bool lu_inverse(const double* Sb, int n, bool split, double* W, double* M, double* dv,
+                                        int* piv) {
 #pragma unroll 1
-  for (int e = 0; e < 81; ++e) W[e] = Sb[(size_t)(e / kBlk) * n + e % kBlk];
+  for (int e = 0; e < 81; ++e)
+    W[e] = split && ((e / kBlk < 3) != (e % kBlk < 3)) ? 0.0 : Sb[(size_t)(e / kBlk) * n + e % kBlk];
 #pragma unroll 1
   for (int j = 0; j < kBlk; ++j) {
     int p = j;
@@ -126,9 +128,13 @@ __global__ __launch_bounds__(kSetupThreads) void k_pcg_implicit_setup(ImplicitPc
   const bool live = threadIdx.x < kSetupBlocks * kBlk && i < a.C;
   for (int t = blockIdx.x * kSetupThreads + threadIdx.x; t < n; t += gridDim.x * kSetupThreads) a.x[t] = 0.0;
   if (live) {
-    // row c of S_ii
+    // row c of S_ii; with split blocks (quaternion cameras: the rotation and
+    // the other six values are separate parameter blocks) the rotation / rest
+    // coupling is left out of the block
+    const bool split = st->split_blocks != 0;
 #pragma unroll
-    for (int k = 0; k < kBlk; ++k) W[lb][c * kBlk + k] = a.S[(size_t)(9 * i + c) * n + 9 * i + k];
+    for (int k = 0; k < kBlk; ++k)
+      W[lb][c * kBlk + k] = split && ((c < 3) != (k < 3)) ? 0.0 : a.S[(size_t)(9 * i + c) * n + 9 * i + k];
     if (c == 0) ok[lb] = 1;
   }
   __syncthreads();
@@ -182,7 +188,9 @@ __global__ __launch_bounds__(kSetupThreads) void k_pcg_implicit_setup(ImplicitPc
       kind = 1;
     } else if (c == 0) {
       // not SPD (rare): the inverse from an LU factorisation with partial pivoting, one thread
-      kind = lu_inverse(a.S + (size_t)(9 * i) * n + 9 * i, n, W[lb], M[lb], dv[lb], piv[lb]) ? 0 : -1;
+      kind = lu_inverse(a.S + (size_t)(9 * i) * n + 9 * i, n, st->split_blocks != 0, W[lb], M[lb], dv[lb], piv[lb])
+                 ? 0
+                 : -1;
     }
   }
   __syncthreads();
@@ -375,7 +383,13 @@ __global__ __launch_bounds__(kImpThreads) void k_pcg_implicit(ImplicitPcg a, uns
         if (fabs(rz_old) <= kEps) break;
         beta = rz_new / rz_old;
       }
-      for (int i = tid; i < n; i += kImpThreads) p[i] = z[i] + beta * p[i];
+      // the first direction is z itself: p holds whatever the LDS last held
+      // (0 * NaN would poison it), so it is not read then
+      if (it < 0) {
+        for (int i = tid; i < n; i += kImpThreads) p[i] = z[i];
+      } else {
+        for (int i = tid; i < n; i += kImpThreads) p[i] = z[i] + beta * p[i];
+      }
       __syncthreads();
       rz_old = rz_new;
     }

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 10
+#define CT_ABI_VERSION 12
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -151,6 +151,10 @@ typedef struct ct_lm_options {
                                            S from the evaluation's records when the plan allows it (up to 600
                                            cameras, no point seen by more than 64); CT_SCHUR_EXPLICIT always
                                            assembles S.  Same step either way, to rounding. */
+  int32_t debug_fill_lds;               /* test hook (bundle adjustment; 0 in normal use): 1 fills the LDS of
+                                           every compute unit with NaN right before each reduced-camera solve,
+                                           so a solver kernel that reads LDS it has not written in its own
+                                           launch fails its parity test.  Curve fits and pose graphs ignore it. */
 } ct_lm_options;
 
 /* How ITERATIVE_SCHUR forms S p (no reference counterpart: the reference
@@ -217,6 +221,26 @@ typedef struct ct_lm_report {
  * grouped by point.  The camera-pair tables come from ct_ba_* helpers in the
  * Python layer (built once per problem structure).                          */
 /* ------------------------------------------------------------------------ */
+/* Camera parameterisations of a bundle-adjustment problem.  Every camera has a
+ * 9-wide tangent space [rotation(3), translation(3), f, k1, k2] (the order of
+ * camera_mask, camera columns and the reduced camera system); the stored
+ * values are:
+ *   CT_CAMERA_ANGLE_AXIS       [angle-axis(3), t(3), f, k1, k2]: 9 values, Euclidean
+ *                              Plus (examples/simple_bundle_adjuster.py:73-79);
+ *   CT_CAMERA_QUATERNION       [q(4) in w, x, y, z order, t(3), f, k1, k2]: 10 values; the
+ *                              rotation is quaternion_rotate_point(q, X) (rotation.py:68-69,
+ *                              211-218), q moves by QuaternionManifold.plus (manifolds.py:
+ *                              127-158) and t, f, k1, k2 by Euclidean Plus: in the reference,
+ *                              q and the other six values are two parameter blocks, so the
+ *                              SCHUR_JACOBI preconditioner has a 3x3 and a 6x6 block per camera;
+ *   CT_CAMERA_EIGEN_QUATERNION as CT_CAMERA_QUATERNION with q stored in x, y, z, w order
+ *                              (EigenQuaternionManifold, manifolds.py:161-171).          */
+typedef enum ct_camera_model {
+  CT_CAMERA_ANGLE_AXIS = 0,
+  CT_CAMERA_QUATERNION = 1,
+  CT_CAMERA_EIGEN_QUATERNION = 2
+} ct_camera_model;
+
 typedef struct ct_ba_problem {
   int32_t num_cameras;
   int32_t num_points;
@@ -232,9 +256,11 @@ typedef struct ct_ba_problem {
   const int32_t* block_cameras;    /* [B*2] (i, j), i <= j, row-major: (0,0), (0,1), ..., (1,1), ... */
   const int32_t* block_offsets;    /* [B+1] term ranges */
   const int32_t* block_terms;      /* [T*2] (obs a of camera i, obs b of camera j) */
-  const uint8_t* camera_mask;      /* [C*9] 1 = free, 0 = held constant */
+  const uint8_t* camera_mask;      /* [C*9] 1 = free, 0 = held constant (tangent coordinates) */
   const uint8_t* point_mask;       /* [P*3] */
   ct_loss loss;
+  int32_t camera_model;            /* ct_camera_model: 9 (angle-axis) or 10 stored values per camera */
+  int32_t reserved;
 } ct_ba_problem;
 
 /* Device workspace needed by ct_ba_plan_create for this problem/options. */
@@ -243,7 +269,8 @@ int ct_ba_workspace_bytes(const ct_ba_problem* problem, const ct_lm_options* opt
 typedef struct ct_ba_plan ct_ba_plan; /* opaque, host memory only */
 
 /* Bind a problem, options, parameter buffers and workspace to a plan.
- * cameras [C*9] and points [P*3] (device, float64) hold the initial estimate
+ * cameras [C*9] (C*10 for the quaternion camera models) and points [P*3]
+ * (device, float64) hold the initial estimate
  * and receive the final (best) state, as Problem parameter tensors do in the
  * reference (problem.py:457-460, solver.py:672).  Replaces the per-iteration
  * Python loop of solver.py:77-345 + Problem.evaluate (problem.py:377-413) +

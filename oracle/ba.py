@@ -16,7 +16,15 @@ computes densely for a Snavely/BAL bundle-adjustment problem:
 * explicit-S PCG with the SCHUR_JACOBI preconditioner: ``iterative_schur_solve``
   (``linear.py:702-776``), ``_build_matrix_preconditioner``
   (``linear.py:1156-1182``) and ``_preconditioned_conjugate_gradient``
-  (``linear.py:1216-1256``).
+  (``linear.py:1216-1256``), one diagonal block per parameter block
+  (``solver.py:153``, ``linear.py:1269-1289``);
+* quaternion cameras [q(4), t(3), f, k1, k2]: the rotation is
+  ``quaternion_rotate_point`` (``rotation.py:217-218``), q moves by
+  ``QuaternionManifold.plus`` (``manifolds.py:130-137``; Eigen order
+  ``manifolds.py:161-171``) and its tangent Jacobian is
+  ``J_ambient @ plus_jacobian`` (``problem.py:505-514``), here in closed form;
+  the rotation and the other six values are separate parameter blocks, so the
+  preconditioner has a 3x3 and a 6x6 block per camera.
 """
 from __future__ import annotations
 
@@ -152,6 +160,136 @@ def snavely_residual_jacobian(cams: np.ndarray, pts: np.ndarray, obs: np.ndarray
 
 
 # ----------------------------------------------------------------------------
+# quaternion cameras
+# ----------------------------------------------------------------------------
+
+
+def normalize_quaternion(q: np.ndarray) -> np.ndarray:
+    """rotation.py:68-69."""
+    return q / np.maximum(np.sqrt(np.sum(q * q, axis=-1, keepdims=True)), _TINY)
+
+
+def quaternion_product(z: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """rotation.py:72-83."""
+    z0, z1, z2, z3 = np.moveaxis(z, -1, 0)
+    w0, w1, w2, w3 = np.moveaxis(w, -1, 0)
+    return np.stack([z0 * w0 - z1 * w1 - z2 * w2 - z3 * w3,
+                     z0 * w1 + z1 * w0 + z2 * w3 - z3 * w2,
+                     z0 * w2 - z1 * w3 + z2 * w0 + z3 * w1,
+                     z0 * w3 + z1 * w2 - z2 * w1 + z3 * w0], axis=-1)
+
+
+def quaternion_rotate_point(q: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """rotation.py:211-218: unit_quaternion_rotate_point(q / |q|, x)."""
+    qn = normalize_quaternion(q)
+    u = qn[..., 1:]
+    uv = 2.0 * np.cross(u, x)
+    return x + qn[..., :1] * uv + np.cross(u, uv)
+
+
+def quaternion_plus(q: np.ndarray, delta: np.ndarray) -> np.ndarray:
+    """QuaternionManifold.plus (manifolds.py:130-137), row-wise: normalize(q_delta * normalize(q))."""
+    x = normalize_quaternion(q)
+    nd = np.sqrt(np.sum(delta * delta, axis=-1, keepdims=True))
+    big = nd > _EPS
+    scale = np.where(big, np.sin(nd) / np.where(big, nd, 1.0), 1.0 - nd * nd / 6.0)
+    q_delta = np.concatenate([np.cos(nd), scale * delta], axis=-1)
+    return normalize_quaternion(quaternion_product(q_delta, x))
+
+
+def quaternion_plus_jacobian(q: np.ndarray) -> np.ndarray:
+    """QuaternionManifold.plus_jacobian (manifolds.py:148-158): [4, 3] at normalize(q)."""
+    w, a, b, c = normalize_quaternion(q)
+    return np.array([[-a, -b, -c], [w, c, -b], [-c, w, a], [b, -a, w]])
+
+
+def ceres_order(cams: np.ndarray, camera_model: str) -> np.ndarray:
+    """Quaternion cameras with q in (w, x, y, z) order."""
+    if camera_model == "eigen_quaternion":
+        return np.concatenate([cams[..., 3:4], cams[..., 0:3], cams[..., 4:]], axis=-1)
+    return cams
+
+
+def storage_order(cams: np.ndarray, camera_model: str) -> np.ndarray:
+    """Inverse of ceres_order."""
+    if camera_model == "eigen_quaternion":
+        return np.concatenate([cams[..., 1:4], cams[..., 0:1], cams[..., 4:]], axis=-1)
+    return cams
+
+
+def quaternion_project(cams: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """The Snavely projection with the rotation quaternion_rotate_point(q, X); cams in Ceres order."""
+    p = quaternion_rotate_point(cams[..., 0:4], pts) + cams[..., 4:7]
+    xp = -p[..., 0] / p[..., 2]
+    yp = -p[..., 1] / p[..., 2]
+    r2 = xp * xp + yp * yp
+    d = 1.0 + r2 * (cams[..., 8] + cams[..., 9] * r2)
+    return cams[..., 7:8] * d[..., None] * np.stack([xp, yp], axis=-1)
+
+
+def quaternion_residual_jacobian(cams: np.ndarray, pts: np.ndarray, obs: np.ndarray):
+    """Residual (N,2), tangent camera Jacobian (N,2,9), point Jacobian (N,2,3); cams in Ceres order.
+
+    Rotation columns: d(R X)/d delta = -2 [R X]_x / |q| (q_delta rotates by 2|delta|; the
+    normalisation of quaternion_rotate_point divides by |q|).  Point columns: the derivative of
+    p + 2w(u x p) + u x (2u x p) in p at the unit quaternion, I + 2w[u]_x + 2(u u^T - |u|^2 I).
+    """
+    qn = normalize_quaternion(cams[:, 0:4])
+    inv = 1.0 / np.maximum(np.sqrt(np.sum(cams[:, 0:4] ** 2, axis=-1)), _TINY)
+    pr = quaternion_rotate_point(cams[:, 0:4], pts)
+    p = pr + cams[:, 4:7]
+    f, k1, k2 = cams[:, 7], cams[:, 8], cams[:, 9]
+    xp = -p[:, 0] / p[:, 2]
+    yp = -p[:, 1] / p[:, 2]
+    r2 = xp * xp + yp * yp
+    d = 1.0 + r2 * (k1 + k2 * r2)
+    res = np.stack([f * d * xp, f * d * yp], axis=-1) - obs
+    inv_z = 1.0 / p[:, 2]
+    dxy_dp = np.zeros((p.shape[0], 2, 3))
+    dxy_dp[:, 0, 0] = -inv_z
+    dxy_dp[:, 0, 2] = -xp * inv_z
+    dxy_dp[:, 1, 1] = -inv_z
+    dxy_dp[:, 1, 2] = -yp * inv_z
+    dd = k1 + 2.0 * k2 * r2
+    duv_dxy = np.empty((p.shape[0], 2, 2))
+    duv_dxy[:, 0, 0] = f * (d + 2.0 * xp * xp * dd)
+    duv_dxy[:, 0, 1] = f * 2.0 * xp * yp * dd
+    duv_dxy[:, 1, 0] = duv_dxy[:, 0, 1]
+    duv_dxy[:, 1, 1] = f * (d + 2.0 * yp * yp * dd)
+    duv_dp = duv_dxy @ dxy_dp
+    w, u = qn[:, 0], qn[:, 1:]
+    eye = np.eye(3)
+    M = (eye + 2.0 * w[:, None, None] * _skew(u) + 2.0 * (u[:, :, None] * u[:, None, :]
+                                                          - np.sum(u * u, axis=-1)[:, None, None] * eye))
+    Jc = np.empty((p.shape[0], 2, 9))
+    Jc[:, :, 0:3] = duv_dp @ (-2.0 * inv[:, None, None] * _skew(pr))
+    Jc[:, :, 3:6] = duv_dp
+    Jc[:, 0, 6] = d * xp
+    Jc[:, 1, 6] = d * yp
+    Jc[:, 0, 7] = f * xp * r2
+    Jc[:, 1, 7] = f * yp * r2
+    Jc[:, 0, 8] = f * xp * r2 * r2
+    Jc[:, 1, 8] = f * yp * r2 * r2
+    Jp = duv_dp @ M
+    return res, Jc, Jp
+
+
+def camera_plus(cams: np.ndarray, dc: np.ndarray, camera_model: str, active=None) -> np.ndarray:
+    """Plus on every camera: Euclidean for angle-axis cameras; QuaternionManifold.plus on the
+    rotation and Euclidean on t, f, k1, k2 for quaternion cameras.  ``active`` (per camera, default
+    all): apply_delta (problem.py:462-474) leaves constant blocks untouched, so a constant
+    quaternion keeps its stored (possibly non-unit) values."""
+    if camera_model == "angle_axis":
+        return cams + dc
+    c = ceres_order(cams, camera_model)
+    q = quaternion_plus(c[:, 0:4], dc[:, 0:3])
+    if active is not None:
+        q = np.where(np.asarray(active, bool)[:, None], q, c[:, 0:4])
+    out = np.concatenate([q, c[:, 4:] + dc[:, 3:]], axis=1)
+    return storage_order(out, camera_model)
+
+
+# ----------------------------------------------------------------------------
 # problem structure
 # ----------------------------------------------------------------------------
 
@@ -216,17 +354,21 @@ class BAEvaluation:
     Jp: np.ndarray | None  # [N, 2, 3] corrected (zero for constant points)
 
 
-def ba_evaluate(struct: BAStructure, cams, pts, obs, loss, jacobian: bool) -> BAEvaluation:
+def ba_evaluate(struct: BAStructure, cams, pts, obs, loss, jacobian: bool,
+                camera_model: str = "angle_axis") -> BAEvaluation:
     c = cams[struct.obs_camera]
     p = pts[struct.obs_point]
+    quaternion = camera_model != "angle_axis"
+    if quaternion:
+        c = ceres_order(c, camera_model)
     if jacobian:
-        res, Jc, Jp = snavely_residual_jacobian(c, p, obs)
+        res, Jc, Jp = quaternion_residual_jacobian(c, p, obs) if quaternion else snavely_residual_jacobian(c, p, obs)
         Jc = Jc * struct.camera_active[struct.obs_camera][:, None, None]
         Jp = Jp * struct.point_active[struct.obs_point][:, None, None]
         J = np.concatenate([Jc, Jp], axis=2)
         cost_b, r, J = robustify(loss, res, J)
         return BAEvaluation(float(np.sum(cost_b)), r, J[:, :, :9], J[:, :, 9:])
-    res = project(c, p) - obs
+    res = (quaternion_project(c, p) if quaternion else project(c, p)) - obs
     cost_b, r, _ = robustify(loss, res, None)
     return BAEvaluation(float(np.sum(cost_b)), r, None, None)
 
@@ -327,6 +469,7 @@ def schur_back_substitute(struct: BAStructure, Vinv, W, ga, dxc: np.ndarray) -> 
 
 
 def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0, preconditioner="schur_jacobi"):
+    """``block``: one size for every diagonal block, or the list of block sizes."""
     """_preconditioned_conjugate_gradient (linear.py:1216-1256) on the explicit Schur complement.
 
     ``preconditioner`` follows build_schur_complement_preconditioner (linear.py:462-537):
@@ -335,7 +478,7 @@ def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0, preconditioner
     "identity" - none (linear.py:480-481, 54-55).
     """
     n = rhs.shape[0]
-    nb = n // block
+    sizes = [block] * (n // block) if np.isscalar(block) else list(block)
     if preconditioner == "identity":
         def apply(r):
             return r
@@ -345,30 +488,30 @@ def block_jacobi_pcg(S, rhs, block, *, tol, max_iter, min_iter=0, preconditioner
         def apply(r):
             return dinv * r
     elif preconditioner == "schur_jacobi":
-        apply = _block_apply(S, nb, block)
+        apply = _block_apply(S, sizes)
     else:
         raise ValueError(preconditioner)
     return _pcg(S, rhs, apply, tol=tol, max_iter=max_iter, min_iter=min_iter)
 
 
-def _block_apply(S, nb, block):
-    blocks = S.reshape(nb, block, nb, block)[np.arange(nb), :, np.arange(nb), :]
+def _block_apply(S, sizes):
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     factors = []
-    for i in range(nb):
+    for i in range(len(sizes)):
+        blk = S[offsets[i]:offsets[i + 1], offsets[i]:offsets[i + 1]]
         try:
-            factors.append((np.linalg.cholesky(blocks[i]), True))
+            factors.append((np.linalg.cholesky(blk), True))
         except np.linalg.LinAlgError:
-            factors.append((blocks[i], False))
+            factors.append((blk, False))
 
     def apply(r):
         z = np.empty_like(r)
         for i, (f, chol) in enumerate(factors):
-            ri = r[i * block : (i + 1) * block]
+            sl = slice(offsets[i], offsets[i + 1])
             if chol:
-                y = np.linalg.solve(f, ri)
-                z[i * block : (i + 1) * block] = np.linalg.solve(f.T, y)
+                z[sl] = np.linalg.solve(f.T, np.linalg.solve(f, r[sl]))
             else:
-                z[i * block : (i + 1) * block] = np.linalg.solve(f, ri)
+                z[sl] = np.linalg.solve(f, r[sl])
         return z
 
     return apply
@@ -431,6 +574,7 @@ class BAModel:
         )
         self.obs = np.asarray(data.observations, dtype=np.float64)
         self.loss = data.loss
+        self.camera_model = getattr(data, "camera_model", "angle_axis")
         self.linear_solver = linear_solver
         self.dense_faithful = dense_faithful
 
@@ -458,12 +602,12 @@ class BAModel:
     def apply(self, x, step):
         cams, pts = x
         dp, dc = split_columns(self.struct, step)
-        return (cams + dc, pts + dp)
+        return (camera_plus(cams, dc, self.camera_model, self.struct.camera_active), pts + dp)
 
     # -- evaluation ----------------------------------------------------------
     def evaluate(self, x, jacobian: bool):
         cams, pts = x
-        ev = ba_evaluate(self.struct, cams, pts, self.obs, self.loss, jacobian)
+        ev = ba_evaluate(self.struct, cams, pts, self.obs, self.loss, jacobian, self.camera_model)
         if not jacobian:
             return ev, None
         gp, gc = gradient_blocks(self.struct, ev)
@@ -573,7 +717,8 @@ class BAModel:
         rhs_a = rhs[act]
         iters = 1
         if self.linear_solver == "iterative_schur":
-            xb, iters, _ = block_jacobi_pcg(S_a, rhs_a, 9, tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
+            xb, iters, _ = block_jacobi_pcg(S_a, rhs_a, self._camera_block_sizes() * int(st.camera_active.sum()),
+                                            tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
                                             preconditioner=opts.preconditioner_type,
                                             min_iter=opts.min_linear_solver_iterations)
         else:
@@ -584,6 +729,11 @@ class BAModel:
         dxp = schur_back_substitute(st, Vinv, W, ga, dxc)
         return column_vector(st, dxp, dxc), iters
 
+    def _camera_block_sizes(self) -> list:
+        """Parameter-block tangent sizes of one camera: the rotation and the rest are separate
+        blocks for quaternion cameras."""
+        return [9] if self.camera_model == "angle_axis" else [3, 6]
+
     def _block_sizes(self):
         st = self.struct
-        return [3] * int(st.point_active.sum()) + [9] * int(st.camera_active.sum())
+        return [3] * int(st.point_active.sum()) + self._camera_block_sizes() * int(st.camera_active.sum())

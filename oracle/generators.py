@@ -38,7 +38,7 @@ class BAProblemData:
     point); this is also the residual-block insertion order.
     """
 
-    cameras: np.ndarray          # [C, 9] float64, initial estimate
+    cameras: np.ndarray          # [C, 9] float64 (angle-axis) or [C, 10] (quaternion models), initial estimate
     points: np.ndarray           # [P, 3] float64, initial estimate
     obs_camera: np.ndarray       # [N] int32
     obs_point: np.ndarray        # [N] int32 (non-decreasing)
@@ -48,6 +48,7 @@ class BAProblemData:
     outlier_mask: np.ndarray | None = None
     loss: tuple = ("trivial",)   # ("trivial",) | ("huber", a) | ("cauchy", a) ...
     meta: dict = field(default_factory=dict)
+    camera_model: str = "angle_axis"  # | "quaternion" (q = w, x, y, z) | "eigen_quaternion" (x, y, z, w)
 
     @property
     def num_cameras(self) -> int:
@@ -154,6 +155,32 @@ def make_bundle_adjustment(
         loss=loss,
         meta=dict(seed=seed, obs_per_point=obs_per_point, outlier_fraction=outlier_fraction),
     )
+
+
+def with_quaternion_cameras(data: BAProblemData, order: str = "ceres", scale=1.0) -> BAProblemData:
+    """The same problem with each camera's rotation as a quaternion [q(4), t(3), f, k1, k2].
+
+    q = angle_axis_to_quaternion(aa) (rotation.py:104-110) times ``scale`` (a
+    scalar or one factor per camera): quaternion_rotate_point normalises, so
+    the projections are unchanged while a non-unit start exercises the 1/|q|
+    of the rotation Jacobian.  ``order`` "eigen" stores q as (x, y, z, w).
+    """
+    from .ba import angle_axis_to_quaternion
+
+    def convert(cams):
+        if cams is None:
+            return None
+        q = angle_axis_to_quaternion(cams[:, 0:3]) * np.reshape(np.asarray(scale, dtype=np.float64), (-1, 1))
+        if order == "eigen":
+            q = q[:, [1, 2, 3, 0]]
+        return np.concatenate([q, cams[:, 3:9]], axis=1)
+
+    out = BAProblemData(cameras=convert(data.cameras), points=data.points.copy(), obs_camera=data.obs_camera,
+                        obs_point=data.obs_point, observations=data.observations,
+                        true_cameras=convert(data.true_cameras), true_points=data.true_points,
+                        outlier_mask=data.outlier_mask, loss=data.loss, meta=dict(data.meta),
+                        camera_model="quaternion" if order == "ceres" else "eigen_quaternion")
+    return out
 
 
 def ba_config(name: str, seed: int | None = None) -> BAProblemData:

@@ -466,6 +466,20 @@ def refine_solution(A, b, x, damping, iterations: int) -> np.ndarray:
     return x
 
 
+def retained_block_sizes(block_sizes, num_eliminate: int, num_columns: int) -> list:
+    """linear.py:1269-1289: the parameter blocks left after eliminating the first columns."""
+    sizes = list(block_sizes) if block_sizes else [1] * num_columns
+    out, offset = [], 0
+    for size in sizes:
+        nxt = offset + size
+        if nxt > num_eliminate:
+            out.append(nxt - num_eliminate if offset < num_eliminate else size)
+        offset = nxt
+    if not out:
+        raise ValueError("num_eliminate leaves no retained Schur columns")
+    return out
+
+
 def dense_linear_solve(J, b, damping, num_eliminate, solver, opts, block_sizes=None):
     """The reference's pure-torch branches of ``solve_linear_system``."""
     if J.shape[1] == 0:
@@ -484,7 +498,8 @@ def dense_linear_solve(J, b, damping, num_eliminate, solver, opts, block_sizes=N
         Haa_inv_ga = _solve_square(H[:e, :e], rhs[:e])
         S = H[e:, e:] - H[e:, :e] @ Haa_inv_Hab
         rhs_s = rhs[e:] - H[e:, :e] @ Haa_inv_ga
-        xb, iters, _ = block_jacobi_pcg(S, rhs_s, 9, tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
+        blocks = retained_block_sizes(block_sizes, e, J.shape[1]) if block_sizes is not None else 9
+        xb, iters, _ = block_jacobi_pcg(S, rhs_s, blocks, tol=opts.eta, max_iter=opts.max_linear_solver_iterations,
                                         preconditioner=opts.preconditioner_type,
                                         min_iter=opts.min_linear_solver_iterations)
         return np.concatenate([Haa_inv_ga - Haa_inv_Hab @ xb, xb]), iters

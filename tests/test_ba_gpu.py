@@ -80,6 +80,26 @@ def test_ba_explicit_schur_operator_matches_reference_trajectory(case):
     _check_reference_trajectory(case)
 
 
+@pytest.mark.parametrize("loop", ["host", "device"])
+@pytest.mark.parametrize("case,schur", [("ba4_iterative_schur_zero_tol", "auto"),
+                                        ("ba4_iterative_schur_zero_tol", "explicit"),
+                                        ("ba4_iterative_jacobi_zero_tol", "auto"),
+                                        ("ba4_dense_schur_zero_tol", "auto"),
+                                        ("ba6_huber_dense_schur", "auto")])
+def test_solvers_read_no_lds_they_did_not_write(case, schur, loop):
+    """With every compute unit's LDS filled with NaN right before each reduced-camera solve (the
+    debug_fill_lds hook), the reference trajectories still hold: no PCG or Cholesky kernel reads
+    LDS it has not written in its own launch.  (The implicit PCG once formed its first direction as
+    z + 0 * p from an unwritten p, which left NaN behind another kernel's LDS as NaN.)"""
+    from ceres_mi355x.ba import debug_fill_lds, set_schur_operator
+
+    print(f"case={case} schur={schur} loop={loop} LDS filled with NaN", flush=True)
+    tc.set_device_loop(loop)
+    set_schur_operator(schur)
+    with debug_fill_lds():
+        _check_reference_trajectory(case)
+
+
 def _check_reference_trajectory(case):
     g = load_golden(case)
     data = gen.ba_config(g["config"])

@@ -21,6 +21,10 @@ formula:
 | lm.trust_region_solve (BA, curve, SE2)  | golden trajectories written by running the reference (tests/golden/) |
 | lm.trust_region_solve, invalid steps    | reference KAT (tests/test_problem_solver.py:637-664): NaN steps -> FAILURE, radius 4 -> 2 |
 | generators                              | sha256 of the inputs recorded with each golden trajectory |
+| ba.quaternion_plus / camera_plus        | reference KAT (tests/test_manifolds_rotation.py:6-13); scipy Rotation composition (rotvec 2 delta on the left); Eigen order = Ceres order permuted (tests/test_quaternion.py) |
+| ba.quaternion_project                   | equals the angle-axis projection of the same rotation at non-unit q (tests/test_quaternion.py) |
+| ba.quaternion_residual_jacobian         | torch autograd of the residual times plus_jacobian, the reference's procedure (problem.py:505-514), at non-unit q; central differences along Plus at unit q (tests/test_quaternion.py) |
+| lm.retained_block_sizes, split blocks   | known answers of linear.py:1269-1289; block-sparse 3 + 6 SCHUR_JACOBI trajectory = dense restatement with the reference's block rule (tests/test_quaternion.py) |
 """
 from __future__ import annotations
 
