@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 12  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 13  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "dogleg",
            "step", "candidate", "lm_end"]
 REC = dict(
@@ -36,6 +36,25 @@ REC = dict(
     step_is_successful=9,
     linear_solver_iterations=10,
     timestamp=11,
+)
+
+
+# line-search iteration records (CT_LS_REC_*)
+CT_LS_REC_FIELDS = 13
+LS_REC = dict(
+    iteration=0,
+    cost=1,
+    gradient_norm=2,
+    gradient_max_norm=3,
+    step_size=4,
+    line_search_iterations=5,
+    line_search_function_evaluations=6,
+    line_search_gradient_evaluations=7,
+    line_search_direction_restarts=8,
+    step_norm=9,
+    cost_change=10,
+    step_is_successful=11,
+    timestamp=12,
 )
 
 
@@ -92,6 +111,48 @@ class ct_lm_report(ctypes.Structure):
         ("initial_cost", ctypes.c_double),
         ("final_cost", ctypes.c_double),
         ("device_seconds", ctypes.c_double),
+    ]
+
+
+class ct_line_search_options(ctypes.Structure):
+    _fields_ = [
+        ("max_num_iterations", ctypes.c_int32),
+        ("direction", ctypes.c_int32),
+        ("line_search_type", ctypes.c_int32),
+        ("nonlinear_cg_type", ctypes.c_int32),
+        ("max_lbfgs_rank", ctypes.c_int32),
+        ("use_approximate_eigenvalue_bfgs_scaling", ctypes.c_int32),
+        ("interpolation", ctypes.c_int32),
+        ("max_num_step_size_iterations", ctypes.c_int32),
+        ("max_num_direction_restarts", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("min_line_search_step_size", ctypes.c_double),
+        ("sufficient_function_decrease", ctypes.c_double),
+        ("max_step_contraction", ctypes.c_double),
+        ("min_step_contraction", ctypes.c_double),
+        ("sufficient_curvature_decrease", ctypes.c_double),
+        ("function_tolerance", ctypes.c_double),
+        ("gradient_tolerance", ctypes.c_double),
+        ("parameter_tolerance", ctypes.c_double),
+        ("max_solver_time_in_seconds", ctypes.c_double),
+    ]
+
+
+class ct_line_search_report(ctypes.Structure):
+    _fields_ = [
+        ("termination", ctypes.c_int32),
+        ("reason", ctypes.c_int32),
+        ("num_records", ctypes.c_int32),
+        ("num_successful_steps", ctypes.c_int32),
+        ("num_unsuccessful_steps", ctypes.c_int32),
+        ("num_jacobian_evaluations", ctypes.c_int32),
+        ("num_line_search_steps", ctypes.c_int32),
+        ("num_line_search_function_evaluations", ctypes.c_int32),
+        ("num_line_search_gradient_evaluations", ctypes.c_int32),
+        ("num_line_search_direction_restarts", ctypes.c_int32),
+        ("initial_cost", ctypes.c_double),
+        ("final_cost", ctypes.c_double),
+        ("seconds", ctypes.c_double),
     ]
 
 
@@ -195,6 +256,12 @@ SIGNATURES = {
     "ct_ba_covariance_workspace_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "ct_ba_plan_covariance": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_double, _vp,
                                              ctypes.POINTER(ctypes.c_int64), _vp, ctypes.c_size_t]),
+    "ct_ba_line_search_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem),
+                                                         ctypes.POINTER(ct_line_search_options),
+                                                         ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_ba_line_search_solve": (ctypes.c_int, [ctypes.POINTER(ct_ba_problem), ctypes.POINTER(ct_line_search_options),
+                                               _vp, _vp, _vp, ctypes.c_size_t, _vp,
+                                               ctypes.POINTER(ct_line_search_report), _vp]),
     "ct_dense_workspace_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "ct_normal_equations_solve_f64": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp,
                                                      _vp, _vp, ctypes.c_size_t, _vp]),

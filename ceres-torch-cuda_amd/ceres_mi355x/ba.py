@@ -251,6 +251,26 @@ class BADevice:
                   "ct_ba_evaluate_jacobian")
         return cost.value, res, jac
 
+    def line_search(self, opts: nat.ct_line_search_options):
+        """The LINE_SEARCH minimizer in place on self.cameras/self.points (ct_ba_line_search_solve).
+
+        Returns (report, records[num_records, CT_LS_REC_FIELDS] numpy).
+        """
+        lib = nat.load()
+        prob = self.problem_struct()
+        nbytes = ctypes.c_size_t(0)
+        nat.check(lib.ct_ba_line_search_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(nbytes)),
+                  "ct_ba_line_search_workspace_bytes")
+        ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
+        rep = nat.ct_line_search_report()
+        rec = np.zeros((opts.max_num_iterations + 1, nat.CT_LS_REC_FIELDS), dtype=np.float64)
+        nat.check(lib.ct_ba_line_search_solve(ctypes.byref(prob), ctypes.byref(opts), nat.ptr(self.cameras),
+                                              nat.ptr(self.points), nat.ptr(ws), ws.numel(),
+                                              nat.current_stream_handle(), ctypes.byref(rep),
+                                              rec.ctypes.data_as(ctypes.c_void_p)),
+                  "ct_ba_line_search_solve")
+        return rep, rec[: int(rep.num_records)].copy()
+
     def close(self) -> None:
         lib = nat.load()
         for handle, _ws, _p in self._plans.values():
@@ -296,6 +316,36 @@ def debug_invalid_steps(count: int):
         yield
     finally:
         _debug_invalid_steps = previous
+
+
+LS_DIRECTIONS = {"STEEPEST_DESCENT": 0, "NONLINEAR_CONJUGATE_GRADIENT": 1, "LBFGS": 2, "BFGS": 3}
+LS_TYPES = {"ARMIJO": 0, "WOLFE": 1}
+NCG_TYPES = {"FLETCHER_REEVES": 0, "POLAK_RIBIERE": 1, "HESTENES_STIEFEL": 2}
+LS_INTERPOLATIONS = {"BISECTION": 0, "QUADRATIC": 1, "CUBIC": 2}
+
+
+def line_search_options(options) -> nat.ct_line_search_options:
+    """SolverOptions -> ct_line_search_options (the fields solver.py:360-607 and line_search.py read)."""
+    o = nat.ct_line_search_options()
+    o.max_num_iterations = int(options.max_num_iterations)
+    o.direction = LS_DIRECTIONS[options.line_search_direction_type.name]
+    o.line_search_type = LS_TYPES[options.line_search_type.name]
+    o.nonlinear_cg_type = NCG_TYPES[options.nonlinear_conjugate_gradient_type.name]
+    o.max_lbfgs_rank = int(options.max_lbfgs_rank)
+    o.use_approximate_eigenvalue_bfgs_scaling = int(bool(options.use_approximate_eigenvalue_bfgs_scaling))
+    o.interpolation = LS_INTERPOLATIONS[options.line_search_interpolation_type.name]
+    o.max_num_step_size_iterations = int(options.max_num_line_search_step_size_iterations)
+    o.max_num_direction_restarts = int(options.max_num_line_search_direction_restarts)
+    o.min_line_search_step_size = float(options.min_line_search_step_size)
+    o.sufficient_function_decrease = float(options.line_search_sufficient_function_decrease)
+    o.max_step_contraction = float(options.max_line_search_step_contraction)
+    o.min_step_contraction = float(options.min_line_search_step_contraction)
+    o.sufficient_curvature_decrease = float(options.line_search_sufficient_curvature_decrease)
+    o.function_tolerance = float(options.function_tolerance)
+    o.gradient_tolerance = float(options.gradient_tolerance)
+    o.parameter_tolerance = float(options.parameter_tolerance)
+    o.max_solver_time_in_seconds = float(getattr(options, "max_solver_time_in_seconds", 1e9))
+    return o
 
 
 @contextlib.contextmanager

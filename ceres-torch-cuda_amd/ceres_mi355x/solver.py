@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .ba import PRECONDITIONERS, BADevice, lm_options
+from .ba import PRECONDITIONERS, BADevice, line_search_options, lm_options
 from .losses import native_loss
 from .problem import EvaluationResult, Problem
 from .types import (
@@ -41,6 +41,8 @@ _MESSAGES = {
     7: "No active parameters.",
     8: "Invalid problem input.",
     9: "Maximum solver time reached.",
+    10: "Projected gradient tolerance reached.",
+    11: "Line search failed to find a decreasing step.",
 }
 
 # Device-path settings with no counterpart in SolverOptions.
@@ -57,8 +59,6 @@ def set_device_loop(mode: str, iterations_per_graph: Optional[int] = None) -> No
 
 
 def _check_supported(options: SolverOptions) -> None:
-    if options.minimizer_type is MinimizerType.LINE_SEARCH:
-        raise NotImplementedError("the MI355X device path implements the TRUST_REGION minimizer only")
     if options.use_nonmonotonic_steps:
         raise NotImplementedError("non-monotonic steps are not on the MI355X device path")
     if options.use_inner_iterations:
@@ -296,8 +296,12 @@ def solve(options: SolverOptions, problem: Problem) -> SolverSummary:
         kinds.add("ba")
     if options.trust_region_strategy_type is TrustRegionStrategyType.DOGLEG and kinds != {"ba"}:
         raise NotImplementedError("DOGLEG is on the MI355X device path for bundle adjustment only")
+    if options.minimizer_type is MinimizerType.LINE_SEARCH and kinds != {"ba"}:
+        raise NotImplementedError("the LINE_SEARCH minimizer is on the MI355X device path for bundle adjustment only")
     if not torch.cuda.is_available():
         raise nat.NativeLibraryError("the MI355X solver needs a ROCm GPU (torch.cuda.is_available() is False)")
+    if kinds == {"ba"} and options.minimizer_type is MinimizerType.LINE_SEARCH:
+        return _solve_ba_line_search(options, problem, start)
     if kinds == {"ba"}:
         return _solve_ba(options, problem, start)
     if kinds == {"curve"}:
@@ -316,7 +320,27 @@ def solve(options: SolverOptions, problem: Problem) -> SolverSummary:
 
 def summary_from_report(options: SolverOptions, problem: Problem, rep, records, start: float,
                         precision: str) -> SolverSummary:
-    s = SolverSummary(
+    s = _summary_base(options, problem)
+    s.termination_type = _TERMINATION[int(rep.termination)]
+    s.message = _MESSAGES.get(int(rep.reason), "Solver failed.")
+    s.initial_cost = float(rep.initial_cost)
+    s.final_cost = float(rep.final_cost)
+    s.num_successful_steps = int(rep.num_successful_steps)
+    s.num_unsuccessful_steps = int(rep.num_unsuccessful_steps)
+    s.num_residual_evaluations = int(rep.num_residual_evaluations)
+    s.num_jacobian_evaluations = int(rep.num_jacobian_evaluations)
+    s.num_linear_solves = int(rep.num_linear_solves)
+    s.iterations = _records_to_iterations(records, options.eta)
+    s.device_time_in_seconds = float(rep.device_seconds)
+    s.minimizer_time_in_seconds = float(rep.device_seconds)
+    s.precision = precision
+    s.total_time_in_seconds = time.perf_counter() - start
+    return s
+
+
+def _summary_base(options: SolverOptions, problem: Problem) -> SolverSummary:
+    """The option and problem fields of a summary (solver.py _new_summary)."""
+    return SolverSummary(
         minimizer_type=options.minimizer_type,
         num_parameter_blocks=problem.num_parameter_blocks(),
         num_parameters=problem.num_parameters(),
@@ -336,21 +360,6 @@ def summary_from_report(options: SolverOptions, problem: Problem, rep, records, 
         dense_linear_algebra_library_type=options.dense_linear_algebra_library_type,
         sparse_linear_algebra_library_type=options.sparse_linear_algebra_library_type,
     )
-    s.termination_type = _TERMINATION[int(rep.termination)]
-    s.message = _MESSAGES.get(int(rep.reason), "Solver failed.")
-    s.initial_cost = float(rep.initial_cost)
-    s.final_cost = float(rep.final_cost)
-    s.num_successful_steps = int(rep.num_successful_steps)
-    s.num_unsuccessful_steps = int(rep.num_unsuccessful_steps)
-    s.num_residual_evaluations = int(rep.num_residual_evaluations)
-    s.num_jacobian_evaluations = int(rep.num_jacobian_evaluations)
-    s.num_linear_solves = int(rep.num_linear_solves)
-    s.iterations = _records_to_iterations(records, options.eta)
-    s.device_time_in_seconds = float(rep.device_seconds)
-    s.minimizer_time_in_seconds = float(rep.device_seconds)
-    s.precision = precision
-    s.total_time_in_seconds = time.perf_counter() - start
-    return s
 
 
 def _eliminated_blocks(problem: Problem) -> set:
@@ -399,6 +408,62 @@ def _solve_ba(options: SolverOptions, problem: Problem, start: float) -> SolverS
     s.postprocessor_time_in_seconds = time.perf_counter() - post
     s.total_time_in_seconds = time.perf_counter() - start
     return s
+
+
+def _solve_ba_line_search(options: SolverOptions, problem: Problem, start: float) -> SolverSummary:
+    """The LINE_SEARCH minimizer (solver.py:360-607) on the device: ct_ba_line_search_solve."""
+    dev, ex = _device_for(problem)
+    pre = time.perf_counter()
+    rep, records = dev.line_search(line_search_options(options))
+    torch.cuda.synchronize()
+    s = _summary_base(options, problem)
+    s.termination_type = _TERMINATION[int(rep.termination)]
+    s.message = _MESSAGES.get(int(rep.reason), "Solver failed.")
+    s.initial_cost = float(rep.initial_cost)
+    s.final_cost = float(rep.final_cost)
+    s.num_successful_steps = int(rep.num_successful_steps)
+    s.num_unsuccessful_steps = int(rep.num_unsuccessful_steps)
+    s.num_jacobian_evaluations = int(rep.num_jacobian_evaluations)
+    s.num_line_search_steps = int(rep.num_line_search_steps)
+    s.num_line_search_function_evaluations = int(rep.num_line_search_function_evaluations)
+    s.num_line_search_gradient_evaluations = int(rep.num_line_search_gradient_evaluations)
+    s.num_line_search_direction_restarts = int(rep.num_line_search_direction_restarts)
+    s.iterations = _line_search_records_to_iterations(records)
+    s.minimizer_time_in_seconds = float(rep.seconds)
+    s.device_time_in_seconds = float(rep.seconds)
+    s.precision = "fp64"
+    s.preprocessor_time_in_seconds = pre - start
+    post = time.perf_counter()
+    ex.write_back(dev.cameras, dev.points)
+    s.postprocessor_time_in_seconds = time.perf_counter() - post
+    s.total_time_in_seconds = time.perf_counter() - start
+    return s
+
+
+def _line_search_records_to_iterations(records: np.ndarray) -> list[IterationSummary]:
+    its = []
+    prev_t = 0.0
+    R = nat.LS_REC
+    for row in records:
+        t = float(row[R["timestamp"]])
+        its.append(IterationSummary(
+            iteration=int(row[R["iteration"]]),
+            cost=float(row[R["cost"]]),
+            gradient_norm=float(row[R["gradient_norm"]]),
+            gradient_max_norm=float(row[R["gradient_max_norm"]]),
+            step_size=float(row[R["step_size"]]),
+            line_search_iterations=int(row[R["line_search_iterations"]]),
+            line_search_function_evaluations=int(row[R["line_search_function_evaluations"]]),
+            line_search_gradient_evaluations=int(row[R["line_search_gradient_evaluations"]]),
+            line_search_direction_restarts=int(row[R["line_search_direction_restarts"]]),
+            step_norm=float(row[R["step_norm"]]),
+            cost_change=float(row[R["cost_change"]]),
+            step_is_successful=bool(row[R["step_is_successful"]]),
+            iteration_time_in_seconds=max(0.0, t - prev_t),
+            cumulative_time_in_seconds=t,
+        ))
+        prev_t = t
+    return its
 
 
 # a dense Jacobian above this size is refused (the reference would build it on the host)

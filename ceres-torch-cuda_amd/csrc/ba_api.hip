@@ -196,6 +196,11 @@ BAView make_view(const ct_ba_problem* p) {
 
 }  // namespace
 
+// Helpers for the library's other bundle-adjustment entry points.
+int ba_validate_problem(const ct_ba_problem* p) { return validate_problem(p); }
+
+BAView ba_make_view(const ct_ba_problem* p) { return make_view(p); }
+
 }  // namespace ct
 
 using namespace ct;
@@ -334,14 +339,12 @@ int validate_point_indices(const ct_ba_problem& p) {
   return CT_OK;
 }
 
-int validate_indices(const ct_ba_problem& p, const std::vector<int32_t>& camera_offsets) {
-  CT_TRY(validate_point_indices(p));
-  const int64_t C = p.num_cameras, N = p.num_observations, T = p.num_terms;
+int validate_camera_csr(const ct_ba_problem& p, const std::vector<int32_t>& camera_offsets) {
+  const int64_t C = p.num_cameras, N = p.num_observations;
   if (N == 0) return CT_OK;
-  std::vector<int32_t> oc, cobs, terms;
+  std::vector<int32_t> oc, cobs;
   CT_TRY(copy_back(oc, p.obs_camera, N));
   CT_TRY(copy_back(cobs, p.camera_obs, N));
-  CT_TRY(copy_back(terms, p.block_terms, 2 * T));
   CT_CHECK_ARG(camera_offsets[0] == 0 && camera_offsets[C] == N,
                "camera_offsets must start at 0 and end at num_observations");
   for (int64_t c = 0; c < C; ++c)
@@ -350,6 +353,16 @@ int validate_indices(const ct_ba_problem& p, const std::vector<int32_t>& camera_
   for (int64_t c = 0; c < C; ++c)
     for (int32_t t = camera_offsets[c]; t < camera_offsets[c + 1]; ++t)
       CT_CHECK_ARG(oc[cobs[t]] == c, "camera_obs must list each camera's observations");
+  return CT_OK;
+}
+
+int validate_indices(const ct_ba_problem& p, const std::vector<int32_t>& camera_offsets) {
+  CT_TRY(validate_point_indices(p));
+  CT_TRY(validate_camera_csr(p, camera_offsets));
+  const int64_t N = p.num_observations, T = p.num_terms;
+  if (N == 0) return CT_OK;
+  std::vector<int32_t> terms;
+  CT_TRY(copy_back(terms, p.block_terms, 2 * T));
   for (int64_t t = 0; t < 2 * T; ++t) CT_CHECK_ARG(terms[t] >= 0 && terms[t] < N, "block_terms out of range");
   return CT_OK;
 }
@@ -511,8 +524,20 @@ int build_items(ct_ba_plan* pl, bool diagonal_only) {
   return CT_OK;
 }
 
-
 }  // namespace
+
+namespace ct {
+
+// The point and camera CSR tables of a problem (no camera-pair tables), checked on the host.
+int ba_check_tables(const ct_ba_problem& p) {
+  CT_TRY(validate_point_indices(p));
+  if (p.num_observations == 0) return CT_OK;
+  std::vector<int32_t> co;
+  CT_TRY(copy_back(co, p.camera_offsets, (size_t)p.num_cameras + 1));
+  return validate_camera_csr(p, co);
+}
+
+}  // namespace ct
 
 extern "C" {
 

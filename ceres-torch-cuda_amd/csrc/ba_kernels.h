@@ -110,6 +110,11 @@ int launch_rcs_pcg(const double* S, const double* rhs, double* x, int n, LMState
 size_t rcs_pcg_work_doubles(int n);
 unsigned int* rcs_pcg_barrier(double* work, int n);  // the PCG solve-epoch word inside its workspace
 int rcs_pcg_prepare(double* work, int n, hipStream_t s);
+// Shared by the bundle-adjustment entry points (ba_api.hip).
+int ba_validate_problem(const ct_ba_problem* p);
+int ba_check_tables(const ct_ba_problem& p);  // point and camera CSR tables
+BAView ba_make_view(const ct_ba_problem* p);
+
 // Test hook: every compute unit's LDS filled with `value` (ct_lm_options::debug_fill_lds).
 int launch_fill_lds(double value, hipStream_t s);  // zero the epoch and barrier flags (once per plan)
 // Implicit-S PCG (pcg_implicit.hip): S p from the JP records, block-Jacobi

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 12
+#define CT_ABI_VERSION 13
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -179,7 +179,9 @@ typedef enum ct_reason {
   CT_REASON_INVALID_STEPS = 6,    /* "Too many invalid trust-region steps." */
   CT_REASON_NO_PARAMETERS = 7,    /* "No active parameters." */
   CT_REASON_INVALID_INPUT = 8,    /* batched problems: malformed sample range */
-  CT_REASON_MAX_TIME = 9          /* "Maximum solver time reached." */
+  CT_REASON_MAX_TIME = 9,         /* "Maximum solver time reached." */
+  CT_REASON_PROJECTED_GRADIENT = 10,  /* line search: "Projected gradient tolerance reached." */
+  CT_REASON_LINE_SEARCH_FAILED = 11   /* line search: "Line search failed to find a decreasing step." */
 } ct_reason;
 
 /* Per-iteration record layout (doubles), mirroring IterationSummary
@@ -334,6 +336,109 @@ int ct_ba_evaluate_jacobian(const ct_ba_problem* problem, const double* cameras,
                             const int32_t* row_of_obs, const int64_t* camera_columns, const int64_t* point_columns,
                             int64_t num_columns, int32_t apply_loss, double* jacobian, double* residuals,
                             double* cost_out_host, void* workspace, size_t workspace_bytes, ct_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* The LINE_SEARCH minimizer on bundle adjustment: _line_search_solve
+ * (solver.py:360-607) with its search directions (solver.py:1078-1162) and
+ * step-size interpolation (line_search.py:9-81).  Every evaluation (cost and
+ * gradient J^T r), the LBFGS two-loop recursion, the BFGS update and Plus run
+ * on the device; the loop runs on the host and reads a few scalars back per
+ * trial step.  No bounds, callbacks or non-monotonic steps (the Python layer
+ * refuses them).                                                            */
+/* ------------------------------------------------------------------------ */
+typedef enum ct_line_search_direction {
+  CT_LS_STEEPEST_DESCENT = 0,
+  CT_LS_NONLINEAR_CONJUGATE_GRADIENT = 1,
+  CT_LS_LBFGS = 2,
+  CT_LS_BFGS = 3
+} ct_line_search_direction;
+
+typedef enum ct_line_search_type { CT_LS_ARMIJO = 0, CT_LS_WOLFE = 1 } ct_line_search_type;
+
+typedef enum ct_nonlinear_cg_type {
+  CT_NCG_FLETCHER_REEVES = 0,
+  CT_NCG_POLAK_RIBIERE = 1,
+  CT_NCG_HESTENES_STIEFEL = 2
+} ct_nonlinear_cg_type;
+
+typedef enum ct_line_search_interpolation {
+  CT_LS_BISECTION = 0,
+  CT_LS_QUADRATIC = 1,
+  CT_LS_CUBIC = 2
+} ct_line_search_interpolation;
+
+/* BFGS keeps a dense n x n inverse Hessian (n = 9C + 3P tangent coordinates);
+ * the device path takes it up to this n. */
+#define CT_LS_MAX_BFGS_N 4096
+
+/* SolverOptions' fields of the line search (types.py:164-198). */
+typedef struct ct_line_search_options {
+  int32_t max_num_iterations;
+  int32_t direction;                     /* ct_line_search_direction */
+  int32_t line_search_type;              /* ct_line_search_type */
+  int32_t nonlinear_cg_type;             /* ct_nonlinear_cg_type */
+  int32_t max_lbfgs_rank;
+  int32_t use_approximate_eigenvalue_bfgs_scaling;
+  int32_t interpolation;                 /* ct_line_search_interpolation */
+  int32_t max_num_step_size_iterations;  /* max_num_line_search_step_size_iterations */
+  int32_t max_num_direction_restarts;    /* max_num_line_search_direction_restarts */
+  int32_t reserved;
+  double min_line_search_step_size;
+  double sufficient_function_decrease;   /* line_search_sufficient_function_decrease */
+  double max_step_contraction;           /* max_line_search_step_contraction */
+  double min_step_contraction;           /* min_line_search_step_contraction */
+  double sufficient_curvature_decrease;  /* line_search_sufficient_curvature_decrease */
+  double function_tolerance;
+  double gradient_tolerance;
+  double parameter_tolerance;
+  double max_solver_time_in_seconds;
+} ct_line_search_options;
+
+typedef struct ct_line_search_report {
+  int32_t termination;                   /* ct_termination */
+  int32_t reason;                        /* ct_reason */
+  int32_t num_records;                   /* iteration records written */
+  int32_t num_successful_steps;
+  int32_t num_unsuccessful_steps;
+  int32_t num_jacobian_evaluations;
+  int32_t num_line_search_steps;
+  int32_t num_line_search_function_evaluations;
+  int32_t num_line_search_gradient_evaluations;
+  int32_t num_line_search_direction_restarts;
+  double initial_cost;
+  double final_cost;
+  double seconds;                        /* host wall time of the solve */
+} ct_line_search_report;
+
+/* One record per iteration (IterationSummary of the line search, solver.py:387-395, 499-509). */
+enum {
+  CT_LS_REC_ITERATION = 0,
+  CT_LS_REC_COST = 1,
+  CT_LS_REC_GRADIENT_NORM = 2,
+  CT_LS_REC_GRADIENT_MAX_NORM = 3,
+  CT_LS_REC_STEP_SIZE = 4,
+  CT_LS_REC_LINE_SEARCH_ITERATIONS = 5,
+  CT_LS_REC_FUNCTION_EVALUATIONS = 6,
+  CT_LS_REC_GRADIENT_EVALUATIONS = 7,
+  CT_LS_REC_DIRECTION_RESTARTS = 8,
+  CT_LS_REC_STEP_NORM = 9,
+  CT_LS_REC_COST_CHANGE = 10,
+  CT_LS_REC_STEP_IS_SUCCESSFUL = 11,
+  CT_LS_REC_TIMESTAMP = 12,              /* seconds since the start of the solve */
+  CT_LS_REC_FIELDS = 13
+};
+
+/* Device workspace of ct_ba_line_search_solve for this problem and options. */
+int ct_ba_line_search_workspace_bytes(const ct_ba_problem* problem, const ct_line_search_options* options,
+                                      size_t* bytes);
+
+/* Solve from cameras / points (device, float64, the problem's camera model)
+ * and leave the last accepted state in them (solver.py:552, 560).  report and
+ * records (host; records holds (max_num_iterations + 1) * CT_LS_REC_FIELDS
+ * doubles, may be NULL) receive the summary.  Synchronous on stream.       */
+int ct_ba_line_search_solve(const ct_ba_problem* problem, const ct_line_search_options* options, double* cameras,
+                            double* points, void* workspace, size_t workspace_bytes, ct_stream_t stream,
+                            ct_line_search_report* report, double* records);
 
 /* ------------------------------------------------------------------------ */
 /* Batched curve fits: y = exp(m x + c), one residual block per sample with a

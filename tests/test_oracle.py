@@ -24,6 +24,7 @@ formula:
 | ba.quaternion_plus / camera_plus        | reference KAT (tests/test_manifolds_rotation.py:6-13); scipy Rotation composition (rotvec 2 delta on the left); Eigen order = Ceres order permuted (tests/test_quaternion.py) |
 | ba.quaternion_project                   | equals the angle-axis projection of the same rotation at non-unit q (tests/test_quaternion.py) |
 | ba.quaternion_residual_jacobian         | torch autograd of the residual times plus_jacobian, the reference's procedure (problem.py:505-514), at non-unit q; central differences along Plus at unit q (tests/test_quaternion.py) |
+| line_search.* (the LINE_SEARCH loop)    | interpolation steps exact on quadratics and cubics; LBFGS two-loop = dense BFGS updates; NCG betas = linear CG's under exact line searches; reference KATs: Rosenbrock to (1, 1) for LBFGS, BFGS, PR-CG and the three interpolations (tests/test_line_search.py) |
 | lm.retained_block_sizes, split blocks   | known answers of linear.py:1269-1289; block-sparse 3 + 6 SCHUR_JACOBI trajectory = dense restatement with the reference's block rule (tests/test_quaternion.py) |
 """
 from __future__ import annotations
