@@ -176,8 +176,10 @@ def profile_kernels(dev, data, *, precision, linear, iters):
     return {k: (ms[i] / max(cnt[i], 1), cnt[i]) for i, k in enumerate(nat.KERNELS)}, mean_lin
 
 
-def num_items(data, max_work=224):
-    """Camera-block work items (the library splits each block's terms / observations into <= max_work pieces)."""
+def num_items(data, max_work=224, diagonal_only=False):
+    """Camera-block work items (the library splits each block's terms / observations into <= max_work pieces);
+    with the implicit PCG only the diagonal blocks are assembled.  Returns (items, observation items, terms read,
+    blocks)."""
     import numpy as np
 
     from ceres_mi355x.ba import build_tables
@@ -186,9 +188,18 @@ def num_items(data, max_work=224):
     terms = np.diff(t.block_offsets)
     cam_obs = np.diff(t.camera_offsets)
     diag = t.block_cameras[:, 0] == t.block_cameras[:, 1]
+    if diagonal_only:
+        terms = terms[diag]
     items = int(np.sum(-(-terms // max_work)))
     items1 = int(np.sum(-(-cam_obs[t.block_cameras[diag, 0]] // max_work)))
-    return items + items1, items1, int(t.block_terms.shape[0]), int(t.block_cameras.shape[0])
+    return items + items1, items1, int(np.sum(terms)), int(t.block_cameras.shape[0])
+
+
+def implicit_pcg(data, precision: str, linear: str) -> bool:
+    """Whether the library applies S from the records (its ITERATIVE_SCHUR default): plain FP64, at most 300
+    cameras, and point groups (no point seen by more than 64 cameras)."""
+    return (linear == "iterative_schur" and precision == "fp64" and 0 < data.num_cameras <= 300
+            and fused_evaluation(data))
 
 
 def fused_evaluation(data) -> bool:
@@ -198,11 +209,12 @@ def fused_evaluation(data) -> bool:
     return int(np.max(np.bincount(np.asarray(data.obs_point)))) <= 64 if len(data.obs_point) else False
 
 
-def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0, items=None):
+def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0, items=None, implicit=False):
     """Compulsory HBM bytes (or FP64 flops) of one launch of a kernel group.
 
     Every input is read once and every output written once; gathered re-reads
     are not counted.  s = bytes per assembly scalar (4 mixed, 8 FP64).
+    ``implicit``: the PCG applies S from the evaluation's records (k_pcg_implicit).
     """
     C, P, N = data.num_cameras, data.num_points, data.num_observations
     s = 4 if precision == "mixed" else 8
@@ -228,8 +240,16 @@ def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0
     if kernel == "camera_reduce":
         n_items, n_items1, T, B = items
         reads = n_items * 81 * s + n_items1 * 27 * 8 + 8 * (B + 1) + 16 * n_items
-        writes = n * n * 8 + 3 * n * 8 + 16 * C
+        # the implicit PCG needs S's diagonal blocks only
+        writes = (C * 81 * 8 if implicit else n * n * 8) + 3 * n * 8 + 16 * C
         return "hbm", reads + writes
+    if kernel == "rcs_solve" and implicit:
+        # setup: the C diagonal 9x9 blocks of S read, their inverses written, x cleared, rhs read;
+        # each CG iteration: phase 1 reads every observation's JP record (24 doubles) and indices and
+        # writes its y (9 doubles); phase 2 reads y, camera_obs, the damping and writes S p
+        setup = C * 81 * 8 * 2 + 4 * C + 2 * n * 8
+        per_iteration = N * (24 * 8 + 4 + 4 + 9 * 8 + 4 + 9 * 8) + 4 * (P + 1) + n * 8 * 2 + 4 * (C + 1)
+        return "hbm", setup + linear_iterations * per_iteration
     if kernel == "rcs_solve":
         if linear_iterations > 0:  # PCG: S read once per iteration, plus its diagonal blocks once
             return "hbm", linear_iterations * (n * n * 8 + 8 * n * 8) + n * 9 * 8 + n * 8
@@ -252,7 +272,7 @@ def algorithmic_work(kernel: str, data, precision: str, *, linear_iterations=0.0
 
 
 KERNEL_SYMBOL = {"eval_obs": "k_eval_fused", "eval_points": "k_eval_points", "camera_items": "k_camera_items",
-                 "camera_reduce": "k_camera_reduce", "rcs_solve": "k_pcg", "step": "k_step_candidate",
+                 "camera_reduce": "k_camera_reduce", "rcs_solve": "k_pcg_implicit", "step": "k_step_candidate",
                  "candidate": "k_candidate_obs"}
 COUNTER_SUMMARY = Path(__file__).resolve().parent / "profiles" / "ba200_fp64_pcg_hbm_counters.md"
 
@@ -278,7 +298,8 @@ def roofline_entry(kernels, data, precision, mean_lin, items, linear="iterative_
     """Roofline of the kernel group with the largest time per launch (lm_begin/lm_end/refine excluded)."""
     cands = {k: v for k, v in kernels.items() if k not in ("lm_begin", "lm_end", "refine") and v[1] > 0}
     dominant = max(cands, key=lambda k: cands[k][0])
-    bound, work = algorithmic_work(dominant, data, precision, linear_iterations=mean_lin, items=items)
+    bound, work = algorithmic_work(dominant, data, precision, linear_iterations=mean_lin, items=items,
+                                   implicit=implicit_pcg(data, precision, linear))
     sec = kernels[dominant][0] * 1e-3
     if bound == "hbm":
         ach = work / sec / 1e9
@@ -764,8 +785,9 @@ def main() -> int:
     kernels = None
     per_kernel = None
     if not args.no_profile:
-        items = num_items(data)
         prec, lin_solver = args.profile_precision, args.profile_linear
+        implicit = implicit_pcg(data, prec, lin_solver)
+        items = num_items(data, diagonal_only=implicit)
         kernels, mean_lin = profile_kernels(dev, data, precision=prec, linear=lin_solver, iters=min(args.steps, 10))
         if lin_solver != "iterative_schur":
             mean_lin = 0.0
@@ -774,7 +796,7 @@ def main() -> int:
         roofline = roofline_entry(kernels, data, prec, mean_lin, items, linear=lin_solver, workload=name)
         per_kernel = {}
         for k, (ms, cnt) in kernels.items():
-            bound, work = algorithmic_work(k, data, prec, linear_iterations=mean_lin, items=items)
+            bound, work = algorithmic_work(k, data, prec, linear_iterations=mean_lin, items=items, implicit=implicit)
             if bound == "hbm" and cnt:
                 per_kernel[k] = {"ms": ms, "GB/s": work / (ms * 1e-3) / 1e9, "algorithmic_bytes": work}
         if not (args.only_value or args.profile_only):
