@@ -61,9 +61,17 @@ def check_resources(stderr: str, src: str) -> list[str]:
     return problems
 
 
+# Per-file flags.  The batched bundle-adjustment kernel runs a whole LM solve
+# in one loop; machine-level loop-invariant code motion would hoist every
+# phase's FP64 constants and addresses to the top of that loop, where they
+# stay live across all phases and push the kernel into spills.
+FILE_FLAGS = {"ba_batch.hip": ["-mllvm", "-disable-machine-licm"]}
+
+
 def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ_DIR / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *FILE_FLAGS.get(src.name, []), "-Rpass-analysis=kernel-resource-usage", "-c", str(src),
+           "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -20,7 +20,7 @@ CT_ERR_WORKSPACE = -4
 CT_ERR_UNSUPPORTED = -5
 
 CT_REC_FIELDS = 12
-ABI_VERSION = 13  # CT_ABI_VERSION of include/ceres_mi355x.h
+ABI_VERSION = 14  # CT_ABI_VERSION of include/ceres_mi355x.h
 KERNELS = ["eval_obs", "eval_points", "camera_items", "camera_reduce", "lm_begin", "rcs_solve", "refine", "dogleg",
            "step", "candidate", "lm_end"]
 REC = dict(
@@ -215,6 +215,44 @@ class ct_pose2d_problem(ctypes.Structure):
     ]
 
 
+class ct_ba_batch_problem(ctypes.Structure):
+    _fields_ = [
+        ("num_cameras", ctypes.c_int32),
+        ("num_points", ctypes.c_int32),
+        ("num_observations", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("camera_base", ctypes.c_int64),
+        ("point_base", ctypes.c_int64),
+        ("obs_base", ctypes.c_int64),
+        ("point_offsets_base", ctypes.c_int64),
+        ("camera_offsets_base", ctypes.c_int64),
+        ("block_offsets_base", ctypes.c_int64),
+        ("term_base", ctypes.c_int64),
+        ("work_base", ctypes.c_int64),
+        ("loss", ct_loss),
+    ]
+
+
+class ct_ba_batch(ctypes.Structure):
+    _fields_ = [
+        ("num_problems", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("problems", _vp),
+        ("obs_camera", _vp),
+        ("obs_point", _vp),
+        ("observations", _vp),
+        ("point_offsets", _vp),
+        ("camera_offsets", _vp),
+        ("camera_obs", _vp),
+        ("block_offsets", _vp),
+        ("block_terms", _vp),
+        ("order", _vp),
+    ]
+
+
+CT_BA_BATCH_MAX_CAMERAS = 21
+
+
 class ct_batch_schedule(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("groups", ctypes.c_int32), ("wave_problems", ctypes.c_int32)]
 
@@ -246,6 +284,10 @@ SIGNATURES = {
     "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
                                       ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_curve_validate": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_int32)]),
+    "ct_ba_batch_problem_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_ba_batch_solve": (ctypes.c_int, [ctypes.POINTER(ct_ba_batch), ctypes.POINTER(ct_lm_options), _vp, _vp, _vp,
+                                         _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_pose2d_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_pose2d_problem), ctypes.POINTER(ct_lm_options),
                                                  ctypes.POINTER(ctypes.c_size_t)]),
     "ct_pose2d_plan_create": (ctypes.c_int, [ctypes.POINTER(ct_pose2d_problem), ctypes.POINTER(ct_lm_options), _vp,

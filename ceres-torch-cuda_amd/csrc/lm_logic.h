@@ -19,6 +19,21 @@ __device__ __forceinline__ double lm_now(const LMState* st) {
   return (double)(wall_clock64() - st->t0) * 1e-8;
 }
 
+__device__ __forceinline__ double damping_of(double colsq, double min_diag, double max_diag, double radius,
+                                             int jacobi_scaling) {
+  const double dg = jacobi_scaling ? colsq : 1.0;
+  return fmin(fmax(dg, min_diag), max_diag) / fmax(radius, 2.220446049250313e-16);
+}
+
+// Diagonal added to the normal equations of one column: the LM damping
+// (linear.py:539-551), or for DOGLEG none (the Gauss-Newton system J^T J),
+// except a unit diagonal on the columns of constant parameters, which keeps
+// their (zero) rows of the system nonsingular without touching any other.
+__device__ __forceinline__ double column_damping(const LMState* st, double colsq, bool free_column) {
+  if (st->strategy == CT_DOGLEG) return free_column ? 0.0 : 1.0;
+  return damping_of(colsq, st->min_diag, st->max_diag, st->radius, st->jacobi_scaling);
+}
+
 __device__ __forceinline__ void lm_terminate(LMState* st, int type, int reason) {
   st->done = 1;
   st->termination = type;

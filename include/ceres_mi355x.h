@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct ihipStream_t* ct_stream_t; /* == hipStream_t */
 
-#define CT_ABI_VERSION 13
+#define CT_ABI_VERSION 14
 
 typedef enum ct_status {
   CT_OK = 0,
@@ -489,6 +489,67 @@ int ct_curve_solve(const ct_curve_batch* batch, const ct_lm_options* options, co
  * arrays with a plain device-to-host copy, which does not wait for work on a
  * non-blocking stream: the caller's uploads of them must be complete.       */
 int ct_curve_validate(const ct_curve_batch* batch, int32_t* first_bad_problem);
+
+/* ------------------------------------------------------------------------ */
+/* Batched small bundle adjustments: many independent problems (angle-axis
+ * Snavely cameras, every parameter free, at most CT_BA_BATCH_MAX_CAMERAS
+ * cameras each) solved in one persistent launch, one workgroup per problem
+ * running the whole LM loop of solver.py:35-357 with the DENSE_SCHUR step of
+ * schur_solve_dense (linear.py:667-699) in FP64.  The reference solves one
+ * Problem per solve() call (solver.py:28); batching has no counterpart there.
+ * Every index table holds problem-local values (cameras 0..C-1, points
+ * 0..P-1, observations 0..N-1), laid out as ct_ba_problem's tables of that
+ * problem alone; the per-problem record says where its slice of each
+ * concatenated array starts.                                               */
+/* ------------------------------------------------------------------------ */
+#define CT_BA_BATCH_MAX_CAMERAS 21   /* 9C <= 189: the reduced camera system is factored by one workgroup */
+
+typedef struct ct_ba_batch_problem {
+  int32_t num_cameras;             /* C, 1..CT_BA_BATCH_MAX_CAMERAS */
+  int32_t num_points;              /* P */
+  int32_t num_observations;        /* N */
+  int32_t reserved;
+  int64_t camera_base;             /* first camera in cameras [.][9] */
+  int64_t point_base;              /* first point in points [.][3] */
+  int64_t obs_base;                /* first observation in obs_camera, obs_point, observations, camera_obs */
+  int64_t point_offsets_base;      /* first of the problem's P+1 entries of point_offsets */
+  int64_t camera_offsets_base;     /* first of its C+1 entries of camera_offsets */
+  int64_t block_offsets_base;      /* first of its C(C+1)/2+1 entries of block_offsets */
+  int64_t term_base;               /* first of its observation pairs in block_terms */
+  int64_t work_base;               /* byte offset of its workspace slice (256-byte aligned, >= 256) */
+  ct_loss loss;
+} ct_ba_batch_problem;
+
+typedef struct ct_ba_batch {
+  int32_t num_problems;
+  int32_t reserved;
+  const ct_ba_batch_problem* problems;  /* [num_problems] device */
+  const int32_t* obs_camera;       /* camera of each observation, grouped by point */
+  const int32_t* obs_point;        /* point of each observation, non-decreasing within a problem */
+  const double* observations;      /* [*][2] observed pixels */
+  const int32_t* point_offsets;    /* observation range of each point */
+  const int32_t* camera_offsets;   /* CSR of observations by camera */
+  const int32_t* camera_obs;
+  const int32_t* block_offsets;    /* term ranges of the camera pairs (i, j), i <= j, row-major */
+  const int32_t* block_terms;      /* [*][2] (observation a of camera i, observation b of camera j) */
+  const int32_t* order;            /* [num_problems] processing order (a permutation), or NULL */
+} ct_ba_batch;
+
+/* Bytes of one problem's workspace slice. */
+int ct_ba_batch_problem_bytes(int32_t num_cameras, int32_t num_points, int32_t num_observations, size_t* bytes);
+
+/* Solve every problem of the batch.  cameras [*][9] and points [*][3] (device
+ * FP64) hold the start values and receive each problem's best state;
+ * results [num_problems] and records [num_problems][max_iter+2][CT_REC_FIELDS]
+ * (device, may be NULL).  options: LEVENBERG_MARQUARDT with DENSE_SCHUR in
+ * plain FP64 (other strategies, solvers and precisions are refused).  The
+ * workspace's first 256 bytes hold the work-queue counter; the problems'
+ * slices follow at their work_base.  Enqueues two kernels and returns (no
+ * host synchronisation: a caller may capture it in a graph).  A problem whose
+ * sizes are out of range ends with CT_REASON_INVALID_INPUT.                 */
+int ct_ba_batch_solve(const ct_ba_batch* batch, const ct_lm_options* options, double* cameras, double* points,
+                      ct_lm_report* results, double* records, void* workspace, size_t workspace_bytes,
+                      ct_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* SE2 pose graphs (examples/pose_graph_2d.py:6-46): poses (x, y, theta),
