@@ -637,9 +637,10 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "by_kind": by_kind}
 
     def warm(server):
-        # compile paths, caching allocator
+        # compile paths, caching allocator: the served paths and the plain per-problem paths
         for r in (reqs[int(np.argmax(kinds == 0))], reqs[int(np.argmax(kinds == 1))], reqs[int(np.argmax(kinds == 2))]):
             server.solve_alone(r)
+            server.solve_plain(r)
 
     server = SolveServer(tc.SolverOptions(), num_slots=slots)
     warm(server)
@@ -648,7 +649,7 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=
     sample = reqs[:baseline_sample]
     t1 = time.perf_counter()
     for r in sample:
-        server.solve_alone(r)
+        server.solve_plain(r)
     alone_s = (time.perf_counter() - t1) / len(sample)
     server.close()
     saturated = {}
@@ -662,7 +663,8 @@ def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=
     out = {
         "workload": f"{num_requests} requests: 60% curve fits (N ~ U[50,2000], Cauchy(0.5)), 30% bundle adjustments "
                     "(2-20 cameras, 100-5000 points, 2-6 obs/point), 10% SE2 pose graphs (100-2000 nodes); reference "
-                    "default options; stream slots + continuous curve batching",
+                    "default options; continuous batching of curve fits and of bundle adjustments (one persistent launch "
+                    "per batch), pose graphs on stream slots",
         "light_load": {"offered_rate_per_s": rate, "slots": slots, **light},
         "saturated": saturated,
         "solves_per_s": saturated[best]["solves_per_s"],

@@ -221,6 +221,7 @@ class ct_ba_batch_problem(ctypes.Structure):
         ("num_points", ctypes.c_int32),
         ("num_observations", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("num_terms", ctypes.c_int64),
         ("camera_base", ctypes.c_int64),
         ("point_base", ctypes.c_int64),
         ("obs_base", ctypes.c_int64),
@@ -284,8 +285,9 @@ SIGNATURES = {
     "ct_curve_solve": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ct_lm_options),
                                       ctypes.POINTER(ct_batch_schedule), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_curve_validate": (ctypes.c_int, [ctypes.POINTER(ct_curve_batch), ctypes.POINTER(ctypes.c_int32)]),
-    "ct_ba_batch_problem_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+    "ct_ba_batch_problem_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                                                  ctypes.POINTER(ctypes.c_size_t)]),
+    "ct_ba_batch_tables": (ctypes.c_int, [ctypes.c_int32] + [_vp] * 14),
     "ct_ba_batch_solve": (ctypes.c_int, [ctypes.POINTER(ct_ba_batch), ctypes.POINTER(ct_lm_options), _vp, _vp, _vp,
                                          _vp, _vp, ctypes.c_size_t, _vp]),
     "ct_pose2d_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(ct_pose2d_problem), ctypes.POINTER(ct_lm_options),

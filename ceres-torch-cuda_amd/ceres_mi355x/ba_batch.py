@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .ba import build_tables, native_loss_struct
+from .ba import native_loss_struct
 
 MAX_CAMERAS = nat.CT_BA_BATCH_MAX_CAMERAS
 
@@ -35,12 +35,12 @@ class BAProblemInput:
     loss_native: tuple = ("trivial", 0.0, 0.0, 1.0)
 
 
-def problem_bytes(num_cameras: int, num_points: int, num_observations: int) -> int:
+def problem_bytes(num_cameras: int, num_points: int, num_observations: int, num_terms: int) -> int:
     """Device workspace of one problem in a batch (exact, from the library)."""
     lib = nat.load()
     out = ctypes.c_size_t(0)
     nat.check(lib.ct_ba_batch_problem_bytes(int(num_cameras), int(num_points), int(num_observations),
-                                            ctypes.byref(out)), "ct_ba_batch_problem_bytes")
+                                            int(num_terms), ctypes.byref(out)), "ct_ba_batch_problem_bytes")
     return int(out.value)
 
 
@@ -49,7 +49,50 @@ def device_bytes(num_cameras: int, num_points: int, num_observations: int, num_t
     C, P, N = int(num_cameras), int(num_points), int(num_observations)
     B = C * (C + 1) // 2
     tables = 4 * (3 * N + (P + 1) + (C + 1) + (B + 1) + 2 * int(num_terms))
-    return problem_bytes(C, P, N) + tables + 16 * N + 8 * (9 * C + 3 * P) + 128
+    return problem_bytes(C, P, N, num_terms) + tables + 16 * N + 8 * (9 * C + 3 * P) + 128
+
+
+def _np_ptr(a: np.ndarray) -> int:
+    return a.ctypes.data if a.size else None
+
+
+def build_batch_tables(cams_per, pts_per, obs_camera: list, obs_point: list) -> dict:
+    """Every problem's tables (those of ``ba.build_tables``, problem-local, concatenated in problem order) from
+    the library's native builder (``ct_ba_batch_tables``: counting sorts, one pass per problem)."""
+    lib = nat.load()
+    C = np.ascontiguousarray(cams_per, dtype=np.int32)
+    P = np.ascontiguousarray(pts_per, dtype=np.int32)
+    N = np.array([np.asarray(o).size for o in obs_camera], dtype=np.int32)
+    K = int(C.size)
+    oc = np.ascontiguousarray(np.concatenate([np.asarray(o).reshape(-1) for o in obs_camera]) if K else [],
+                              dtype=np.int32)
+    op = np.ascontiguousarray(np.concatenate([np.asarray(o).reshape(-1) for o in obs_point]) if K else [],
+                              dtype=np.int32)
+    T = np.zeros(max(K, 1), dtype=np.int64)
+    args = [K, _np_ptr(C), _np_ptr(P), _np_ptr(N), _np_ptr(oc), _np_ptr(op), T.ctypes.data]
+    nat.check(lib.ct_ba_batch_tables(*args, *([None] * 8)), "ct_ba_batch_tables")
+    Ntot = int(N.sum())
+    B = C.astype(np.int64) * (C + 1) // 2
+    out = dict(order=np.empty(Ntot, np.int32), obs_camera=np.empty(Ntot, np.int32), obs_point=np.empty(Ntot, np.int32),
+               point_offsets=np.empty(int(P.sum()) + K, np.int32), camera_offsets=np.empty(int(C.sum()) + K, np.int32),
+               camera_obs=np.empty(Ntot, np.int32), block_offsets=np.empty(int(B.sum()) + K, np.int32),
+               block_terms=np.empty(2 * int(T[:K].sum()), np.int32))
+    names = ["order", "obs_camera", "obs_point", "point_offsets", "camera_offsets", "camera_obs", "block_offsets",
+             "block_terms"]
+    # every output needs a valid host address, also an empty one
+    bufs = [out[k] if out[k].size else np.zeros(1, np.int32) for k in names]
+    nat.check(lib.ct_ba_batch_tables(*args, *[b.ctypes.data for b in bufs]), "ct_ba_batch_tables")
+    T = T[:K]
+    out["num_terms"] = T
+    out["obase"] = np.cumsum(N, dtype=np.int64) - N
+    out["cbase"] = np.cumsum(C, dtype=np.int64) - C
+    out["pbase"] = np.cumsum(P, dtype=np.int64) - P
+    out["tbase"] = np.cumsum(T) - T
+    ar = np.arange(K, dtype=np.int64)
+    out["poff_base"] = out["pbase"] + ar
+    out["coff_base"] = out["cbase"] + ar
+    out["boff_base"] = np.cumsum(B) - B + ar
+    return out
 
 
 class BABatch:
@@ -64,72 +107,58 @@ class BABatch:
         nat.load()
         self.device = torch.device(device)
         self.num_problems = len(problems)
-        metas, cams, pts, ocam, opt, obs, poff, coff, cobs, boff, terms = ([] for _ in range(11))
-        base = dict(camera=0, point=0, obs=0, point_offsets=0, camera_offsets=0, block_offsets=0, term=0)
-        work = 256
-        self.sizes = []
-        for pr in problems:
-            cam = np.asarray(pr.cameras, dtype=np.float64).reshape(-1, 9)
-            pt = np.asarray(pr.points, dtype=np.float64).reshape(-1, 3)
-            C, P = cam.shape[0], pt.shape[0]
-            if not 1 <= C <= MAX_CAMERAS:
-                raise ValueError(f"a batched bundle adjustment has 1..{MAX_CAMERAS} cameras, got {C}")
-            oc = np.asarray(pr.obs_camera, dtype=np.int64).reshape(-1)
-            op = np.asarray(pr.obs_point, dtype=np.int64).reshape(-1)
+        cams = [np.asarray(pr.cameras, dtype=np.float64).reshape(-1, 9) for pr in problems]
+        pts = [np.asarray(pr.points, dtype=np.float64).reshape(-1, 3) for pr in problems]
+        C = np.array([c.shape[0] for c in cams], dtype=np.int64)
+        P = np.array([p.shape[0] for p in pts], dtype=np.int64)
+        if C.size and (C.min() < 1 or C.max() > MAX_CAMERAS):
+            raise ValueError(f"a batched bundle adjustment has 1..{MAX_CAMERAS} cameras, got {int(C.max())}")
+        ocs = [np.asarray(pr.obs_camera, dtype=np.int64).reshape(-1) for pr in problems]
+        ops = [np.asarray(pr.obs_point, dtype=np.int64).reshape(-1) for pr in problems]
+        for oc, op in zip(ocs, ops):
             if oc.shape != op.shape:
                 raise ValueError("obs_camera and obs_point differ in length")
-            if oc.size and (oc.min() < 0 or oc.max() >= C or op.min() < 0 or op.max() >= P):
-                raise ValueError("observation index out of range")
-            t = build_tables(oc, op, C, P)
-            N = int(oc.size)
-            xy = np.asarray(pr.observations, dtype=np.float64).reshape(-1, 2)[t.order]
-            T = int(t.block_terms.shape[0])
+        tb = build_batch_tables(C, P, ocs, ops)  # checks every index
+        N = np.array([oc.size for oc in ocs], dtype=np.int64)
+        T = tb["num_terms"]
+        xy = (np.concatenate([np.asarray(pr.observations, dtype=np.float64).reshape(-1, 2) for pr in problems])
+              if problems else np.zeros((0, 2)))
+        gorder = tb["order"] + np.repeat(tb["obase"], N).astype(np.int32)
+        if not np.array_equal(gorder, np.arange(gorder.size, dtype=np.int32)):  # usually grouped by point already
+            xy = xy[gorder]
+        metas = []
+        work = 256
+        self.sizes = []
+        for k, pr in enumerate(problems):
             m = nat.ct_ba_batch_problem()
-            m.num_cameras, m.num_points, m.num_observations = C, P, N
-            m.camera_base, m.point_base, m.obs_base = base["camera"], base["point"], base["obs"]
-            m.point_offsets_base, m.camera_offsets_base = base["point_offsets"], base["camera_offsets"]
-            m.block_offsets_base, m.term_base = base["block_offsets"], base["term"]
+            m.num_cameras, m.num_points, m.num_observations, m.num_terms = int(C[k]), int(P[k]), int(N[k]), int(T[k])
+            m.camera_base, m.point_base, m.obs_base = int(tb["cbase"][k]), int(tb["pbase"][k]), int(tb["obase"][k])
+            m.point_offsets_base, m.camera_offsets_base = int(tb["poff_base"][k]), int(tb["coff_base"][k])
+            m.block_offsets_base, m.term_base = int(tb["boff_base"][k]), int(tb["tbase"][k])
             m.work_base = work
             m.loss = native_loss_struct(pr.loss_native)
             metas.append(m)
-            cams.append(cam)
-            pts.append(pt)
-            ocam.append(t.obs_camera)
-            opt.append(t.obs_point)
-            obs.append(xy)
-            poff.append(t.point_offsets)
-            coff.append(t.camera_offsets)
-            cobs.append(t.camera_obs)
-            boff.append(t.block_offsets)
-            terms.append(t.block_terms.reshape(-1))
-            base["camera"] += C
-            base["point"] += P
-            base["obs"] += N
-            base["point_offsets"] += P + 1
-            base["camera_offsets"] += C + 1
-            base["block_offsets"] += C * (C + 1) // 2 + 1
-            base["term"] += T
-            work += problem_bytes(C, P, N)
-            self.sizes.append((C, P, N, T))
+            work += problem_bytes(int(C[k]), int(P[k]), int(N[k]), int(T[k]))
+            self.sizes.append((int(C[k]), int(P[k]), int(N[k]), int(T[k])))
         self.workspace_bytes = work
         dev = self.device
 
-        def cat(parts, dtype):
-            a = np.concatenate(parts).astype(dtype) if parts else np.zeros(0, dtype)
+        def up(a, dtype):
+            a = np.asarray(a, dtype=dtype).reshape(-1)
             if a.size == 0:
                 a = np.zeros(1, dtype)  # a valid device address for empty tables
             return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
-        self.cameras = cat([c.reshape(-1) for c in cams], np.float64)
-        self.points = cat([p.reshape(-1) for p in pts], np.float64)
-        self.obs_camera = cat(ocam, np.int32)
-        self.obs_point = cat(opt, np.int32)
-        self.observations = cat([o.reshape(-1) for o in obs], np.float64)
-        self.point_offsets = cat(poff, np.int32)
-        self.camera_offsets = cat(coff, np.int32)
-        self.camera_obs = cat(cobs, np.int32)
-        self.block_offsets = cat(boff, np.int32)
-        self.block_terms = cat(terms, np.int32)
+        self.cameras = up(np.concatenate([c.reshape(-1) for c in cams]) if cams else [], np.float64)
+        self.points = up(np.concatenate([p.reshape(-1) for p in pts]) if pts else [], np.float64)
+        self.obs_camera = up(tb["obs_camera"], np.int32)
+        self.obs_point = up(tb["obs_point"], np.int32)
+        self.observations = up(xy, np.float64)
+        self.point_offsets = up(tb["point_offsets"], np.int32)
+        self.camera_offsets = up(tb["camera_offsets"], np.int32)
+        self.camera_obs = up(tb["camera_obs"], np.int32)
+        self.block_offsets = up(tb["block_offsets"], np.int32)
+        self.block_terms = up(tb["block_terms"], np.int32)
         arr = (nat.ct_ba_batch_problem * max(1, self.num_problems))(*metas)
         raw = np.frombuffer(bytes(arr), dtype=np.uint8)
         self.problem_table = torch.from_numpy(raw.copy()).to(dev)

@@ -6,24 +6,32 @@ Requests of three kinds arrive at any time:
 * ``BARequest``: a bundle adjustment (cameras, points, observations);
 * ``PoseRequest``: an SE2 pose graph.
 
-Admission is by device-memory budget: a request is admitted when its
-estimated working set fits beside the requests already on the GPU, in arrival
-order.  Bundle adjustments and pose graphs run on ``num_slots`` worker
-threads, each with its own HIP stream, so their kernels overlap on the
-device (the library calls release the GIL).  Curve fits are batched
-continuously: one worker launches a persistent batched solve of every curve
-request that is waiting when the previous batch finishes, so requests that
-arrive while a batch runs join the next one.  Each request's result is the
-same as solving it alone (per-problem arithmetic does not depend on the other
-problems of a batch with a fixed launch shape; plans are per request).
+Admission is by device-memory budget: a request is admitted when the device
+bytes it needs (its inputs, tables and the library's exact workspace for it,
+from ``ct_ba_batch_problem_bytes`` / ``ct_pose2d_workspace_bytes``) fit beside
+the requests already on the GPU, in arrival order.
+
+Curve fits and bundle adjustments of at most 21 cameras are batched
+continuously: a worker launches a persistent batched solve of the admitted
+requests of its kind that are waiting when its previous batch finishes (up to
+a batch limit), so requests that arrive while a batch runs join the next one.
+Bundle adjustments have two such workers, so one batch's host tables are
+built while another runs on the device, and all batches overlap on the
+device.  Pose graphs (and larger bundle
+adjustments) run on ``num_slots`` worker threads, each with its own HIP stream
+and per-request plan, so their kernels overlap too (the library calls release
+the GIL).  Each request's result is the same as solving it alone through the
+same path: a batched problem's arithmetic does not depend on the other
+problems of its batch, and plans are per request.
 
 Slots run the host-driven LM loop by default (one synchronisation of the
 slot's own stream per iteration): a graph capture in one slot thread does not
-survive device-memory allocations made meanwhile by other slot threads, so
+survive device-memory allocations made meanwhile by other threads, so
 concurrent slots capture nothing.  Their kernels still overlap on the device.
 """
 from __future__ import annotations
 
+import ctypes
 import threading
 import time
 from collections import deque
@@ -36,6 +44,7 @@ import torch
 
 from . import _native as nat
 from .ba import BADevice, lm_options
+from .ba_batch import MAX_CAMERAS, BABatch, BAProblemInput, device_bytes as ba_device_bytes
 from .curve import CurveBatch
 from .losses import native_loss
 from .pose2d import PoseGraphDevice
@@ -95,49 +104,87 @@ class _Ticket:
     admitted: float = 0.0
 
 
-def estimate_bytes(req) -> int:
-    """Device working set of a request, generously rounded: inputs, Jacobians, the dense normal matrix, its
-    factor's inverse and scratch."""
+def _ba_terms(obs_camera, obs_point) -> int:
+    """Observation pairs of a common point with cam(a) <= cam(b): the camera-pair assembly's terms."""
+    oc = np.asarray(obs_camera, dtype=np.int64).reshape(-1)
+    op = np.asarray(obs_point, dtype=np.int64).reshape(-1)
+    if oc.size == 0:
+        return 0
+    k = np.bincount(op)
+    same = np.unique(op * (int(oc.max()) + 1) + oc, return_counts=True)[1]  # observations per (point, camera)
+    return int((np.sum(k.astype(np.int64) ** 2) + np.sum(same.astype(np.int64) ** 2)) // 2)
+
+
+def _pose_pairs(edges) -> int:
+    e = np.sort(np.asarray(edges, dtype=np.int64).reshape(-1, 2), axis=1)
+    return int(np.unique(e, axis=0).shape[0]) if e.size else 0
+
+
+def request_bytes(req, options: Optional[SolverOptions] = None) -> int:
+    """Device bytes a request holds while it is on the GPU: inputs, tables, parameters, results and the
+    library's exact workspace for its solve path."""
     if isinstance(req, CurveRequest):
-        return 64 * int(np.asarray(req.x).size) + 4096
+        return 24 * int(np.asarray(req.x).size) + 128
     if isinstance(req, BARequest):
         C, P, N = len(req.cameras), len(req.points), len(req.obs_camera)
+        T = _ba_terms(req.obs_camera, req.obs_point)
+        if C <= MAX_CAMERAS:
+            return ba_device_bytes(C, P, N, T)
+        # per-request plan: the plan's own workspace query needs the device tables; bound it from its layout
         n = 9 * C
-        pairs = int(np.sum(np.bincount(np.asarray(req.obs_point), minlength=P) ** 2))
-        return 400 * N + 200 * P + 32 * n * n + 16 * pairs + (1 << 20)
+        return 400 * N + 200 * P + 32 * n * n + 16 * T + (1 << 20)
     if isinstance(req, PoseRequest):
-        n = 3 * len(req.poses)
-        return 200 * len(req.edges) + 100 * len(req.poses) + 32 * n * n + (1 << 20)
+        lib = nat.load()
+        prob = nat.ct_pose2d_problem()
+        prob.num_nodes, prob.num_edges = len(req.poses), len(req.edges)
+        prob.num_pairs, prob.fixed_node = _pose_pairs(req.edges), int(req.fixed_node)
+        opts = lm_options(options or SolverOptions())
+        out = ctypes.c_size_t(0)
+        nat.check(lib.ct_pose2d_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out)),
+                  "ct_pose2d_workspace_bytes")
+        E, N = len(req.edges), len(req.poses)
+        tables = 4 * (2 * E + (N + 1) + 2 * E + 3 * prob.num_pairs + 1 + E)
+        return int(out.value) + tables + 24 * E + 24 * N
     raise TypeError(f"unknown request type {type(req).__name__}")
+
+
+estimate_bytes = request_bytes  # the name earlier callers use
 
 
 class SolveServer:
     """Solve requests concurrently on the GPU; ``submit`` returns a Future of ``Result``."""
 
     def __init__(self, options: Optional[SolverOptions] = None, *, hbm_budget_bytes: int = 16 << 30,
-                 num_slots: int = 4, max_curve_batch: int = 4096, curve_schedule=(64, 0), loop: str = "host"):
+                 num_slots: int = 4, max_curve_batch: int = 4096, max_ba_batch: int = 64, ba_workers: int = 2,
+                 curve_schedule=(64, 0), loop: str = "host"):
         nat.load()
         self.options = options or SolverOptions()
         self.options.validate()
         self.budget = int(hbm_budget_bytes)
         self.in_use = 0
         self.max_curve_batch = int(max_curve_batch)
+        self.max_ba_batch = int(max_ba_batch)
         self.curve_schedule = curve_schedule
         self.loop = loop
         self._lock = threading.Condition()
         self._pending: deque[_Ticket] = deque()       # arrival order, not yet admitted
         self._solo: deque[_Ticket] = deque()          # admitted BA / pose requests
         self._curves: deque[_Ticket] = deque()        # admitted curve requests
+        self._bas: deque[_Ticket] = deque()           # admitted bundle adjustments of at most MAX_CAMERAS cameras
         self._closed = False
         self._threads = [threading.Thread(target=self._slot_worker, name=f"slot{i}", daemon=True)
                          for i in range(num_slots)]
         self._threads.append(threading.Thread(target=self._curve_worker, name="curves", daemon=True))
+        # two batch workers per kind that batches large numbers: while one batch runs on the device the next
+        # one's host tables are built
+        self._threads += [threading.Thread(target=self._ba_worker, name=f"bundle-adjustments{i}", daemon=True)
+                          for i in range(max(1, int(ba_workers)))]
         for t in self._threads:
             t.start()
 
     # -- submission and admission ------------------------------------------
     def submit(self, request) -> Future:
-        nbytes = estimate_bytes(request)
+        nbytes = request_bytes(request, self.options)
         if nbytes > self.budget:
             raise ValueError(f"request needs ~{nbytes} bytes, more than the whole budget ({self.budget})")
         fut: Future = Future()
@@ -154,8 +201,15 @@ class SolveServer:
             t = self._pending.popleft()
             t.admitted = time.perf_counter()
             self.in_use += t.nbytes
-            (self._curves if isinstance(t.request, CurveRequest) else self._solo).append(t)
+            self._queue_for(t.request).append(t)
         self._lock.notify_all()
+
+    def _queue_for(self, req) -> deque:
+        if isinstance(req, CurveRequest):
+            return self._curves
+        if isinstance(req, BARequest) and len(req.cameras) <= MAX_CAMERAS:
+            return self._bas
+        return self._solo
 
     def _release(self, nbytes: int) -> None:
         with self._lock:
@@ -223,6 +277,25 @@ class SolveServer:
             finally:
                 self._release(sum(t.nbytes for t in tickets))
 
+    def _ba_worker(self) -> None:
+        stream = torch.cuda.Stream()
+        while True:
+            tickets = self._next(self._bas, self.max_ba_batch)
+            if not tickets:
+                return
+            try:
+                with torch.cuda.stream(stream):
+                    results = self._solve_bas([t.request for t in tickets])
+                done = time.perf_counter()
+                for t, res in zip(tickets, results):
+                    res.arrival, res.admitted, res.finished = t.arrival, t.admitted, done
+                    t.future.set_result(res)
+            except Exception as exc:  # noqa: BLE001
+                for t in tickets:
+                    t.future.set_exception(exc)
+            finally:
+                self._release(sum(t.nbytes for t in tickets))
+
     # -- solves ----------------------------------------------------------------
     def _lm(self, precision: str = "fp64"):
         return lm_options(self.options, precision=precision, loop=self.loop, linear="dense_schur")
@@ -273,10 +346,42 @@ class SolveServer:
                                 int(rep["termination"]), int(rep["num_records"]) - 1)
         return out  # type: ignore[return-value]
 
-    def solve_alone(self, req) -> Result:
-        """The same request solved by itself (no batching, no concurrency), for comparison."""
+    def _solve_bas(self, reqs: list) -> list[Result]:
+        """One persistent launch for every bundle adjustment of the batch (ct_ba_batch_solve)."""
+        batch = BABatch([BAProblemInput(r.cameras, r.points, r.obs_camera, r.obs_point, r.observations,
+                                        native_loss(r.loss)) for r in reqs])
+        reps, _, cams, pts = batch.solve(self._lm())
+        out = []
+        for k in range(len(reqs)):
+            c, p = batch.split(cams, pts, k)
+            rep = reps[k]
+            out.append(Result("ba", np.concatenate([c.reshape(-1), p.reshape(-1)]), float(rep["initial_cost"]),
+                              float(rep["final_cost"]), int(rep["termination"]), int(rep["num_records"]) - 1))
+        return out
+
+    def _solve_path(self, req) -> Result:
+        if isinstance(req, CurveRequest):
+            return self._solve_curves([req])[0]
+        if isinstance(req, BARequest) and len(req.cameras) <= MAX_CAMERAS:
+            return self._solve_bas([req])[0]
+        return self._solve_one(req)
+
+    def solve_plain(self, req) -> Result:
+        """The request solved by itself through the per-problem path (one plan per bundle adjustment or pose
+        graph, one launch per curve fit), as a caller without the server would: the plain per-problem GPU
+        baseline."""
         t0 = time.perf_counter()
         res = self._solve_curves([req])[0] if isinstance(req, CurveRequest) else self._solve_one(req)
+        torch.cuda.synchronize()
+        res.arrival = res.admitted = t0
+        res.finished = time.perf_counter()
+        return res
+
+    def solve_alone(self, req) -> Result:
+        """The same request solved by itself through the path the server takes for it (no other request in its
+        batch, no concurrency), for comparison."""
+        t0 = time.perf_counter()
+        res = self._solve_path(req)
         torch.cuda.synchronize()
         res.arrival = res.admitted = t0
         res.finished = time.perf_counter()

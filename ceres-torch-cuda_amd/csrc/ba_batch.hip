@@ -9,11 +9,13 @@
 //       step changes only the damping);
 //   E2  per point: V_p = sum Jp^T Jp + D_p, g_p, the 3x3 factor L, q = L^-1 g_p,
 //       P_o = Jp_o L^-T and e_o = P_o q - r_o per observation;
-//   E3  per camera column: column norm, damping D_c, g_c and the reduced
-//       right-hand side sum Jc^T e;
-//   A   per 3-row strip of a camera-pair block (i <= j) of the reduced camera
-//       system S = U + D_c - sum Jc_a^T (P_a P_b^T) Jc_b over the block's
-//       observation pairs in table order;
+//   A1  per work item (at most kItemWork observation pairs of one camera-pair
+//       block i <= j, or kItemWork observations of one camera) and 3-row
+//       strip: the item's partial of S = U + D_c - sum Jc_a^T (P_a P_b^T) Jc_b
+//       and, for observation items, of the column norms, g_c and sum Jc^T e;
+//   A2  per element of S and per camera column: the item partials summed in
+//       item order, the damping D_c, S written, rhs, g_c;
+//   (the gradient norms of E2 and A2 go to the LM begin update)
 //   C   S factored with the right-hand side as an extra row by the
 //       one-workgroup Cholesky (16x16 tiles on v_mfma_f64_16x16x4f64, the
 //       packed factor in LDS), then the backward substitution;
@@ -23,6 +25,13 @@
 // lm_logic.h on it.  Every sum has a fixed order (per-thread strided sums,
 // then wave trees and waves in index order), so a problem's result does not
 // depend on the rest of the batch or on which workgroup solved it.
+#include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
 #include "ba_kernels.h"
 #include "ba_camera.h"
 #include "chol_small.h"
@@ -50,17 +59,30 @@ struct BatchWork {
   double* S;     // [n][n] reduced camera system (lower triangle used)
   double* rhs;   // [n]
   double* dxc;   // [n]
-  double* Dc;    // [n]
+  double* itemS;    // [items][81] partial camera-pair blocks
+  double* itemV;    // [observation items][27] column norms, g_c and sum Jc^T e of the item's columns
 };
 
-__host__ __device__ inline size_t batch_layout(int C, int P, int N, double* base, BatchWork* w) {
+constexpr int kItemWork = 32;  // observation pairs or observations per work item
+constexpr int kMaxBlocks = CT_BA_BATCH_MAX_CAMERAS * (CT_BA_BATCH_MAX_CAMERAS + 1) / 2;
+
+// Work items of a problem with B camera-pair blocks, T pairs and N observations (an upper bound:
+// each block's or camera's last item may be partial).
+__host__ __device__ inline size_t max_items(int C, int64_t T, int N) {
+  const size_t B = (size_t)C * (C + 1) / 2;
+  return (size_t)(T / kItemWork) + B + (size_t)(N / kItemWork) + (size_t)C + 1;
+}
+
+__host__ __device__ inline size_t batch_layout(int C, int P, int N, int64_t T, double* base, BatchWork* w) {
   const size_t n = 9 * (size_t)C;
+  const size_t items = max_items(C, T, N);
+  const size_t obs_items = (size_t)(N / kItemWork) + (size_t)C + 1;
   size_t off = 0;
   double** slots[] = {&w->xc, &w->xp, &w->r, &w->Jp, &w->JP, &w->e, &w->Linv, &w->q, &w->dxp, &w->S, &w->rhs,
-                      &w->dxc, &w->Dc};
+                      &w->dxc, &w->itemS, &w->itemV};
   const size_t counts[] = {2 * n, 6 * (size_t)P, 2 * (size_t)N, 6 * (size_t)N, 24 * (size_t)N, 2 * (size_t)N,
-                           6 * (size_t)P, 3 * (size_t)P, 3 * (size_t)P, n * n, n, n, n};
-  for (int k = 0; k < 13; ++k) {
+                           6 * (size_t)P, 3 * (size_t)P, 3 * (size_t)P, n * n, n, n, 81 * items, 27 * obs_items};
+  for (int k = 0; k < 14; ++k) {
     off = (off + 31) & ~size_t(31);
     *slots[k] = base ? base + off : nullptr;
     off += counts[k];
@@ -265,73 +287,86 @@ __device__ void eliminate_points(const Problem& pb, const BatchWork& w, const LM
   }
 }
 
-// E3: per camera column k = 9 i + c over camera i's observations in table
-// order: column norm -> damping, g_c and the reduced right-hand side sum Jc^T e.
-__device__ void camera_columns(const Problem& pb, const BatchWork& w, const LMState* st, double& g2,
-                               double& gmax) {
-  for (int k = fresh_tid(); k < pb.n; k += kBT) {
-    const int i = k / 9, c = k - 9 * i;
-    double colsq = 0.0, gc = 0.0, rh = 0.0;
-    for (int t = pb.coff[i]; t < pb.coff[i + 1]; ++t) {
-      const int o = pb.cobs[t];
-      const double j0 = w.JP[24 * (size_t)o + c], j1 = w.JP[24 * (size_t)o + 9 + c];
-      colsq += j0 * j0 + j1 * j1;
-      gc += j0 * w.r[2 * (size_t)o] + j1 * w.r[2 * (size_t)o + 1];
-      rh += j0 * w.e[2 * (size_t)o] + j1 * w.e[2 * (size_t)o + 1];
-    }
-    w.Dc[k] = column_damping(st, colsq, true);
-    w.rhs[k] = rh;
-    g2 += gc * gc;
-    gmax = fmax(gmax, fabs(gc));
-  }
+// Item index of a problem (LDS, built once per problem): the first item of
+// each camera-pair block, then the first observation item of each camera.
+struct ItemIndex {
+  int blk0[kMaxBlocks + 1];               // pair items of block b: [blk0[b], blk0[b + 1])
+  int cam0[CT_BA_BATCH_MAX_CAMERAS + 1];  // observation items of camera i: [cam0[i], cam0[i + 1])
+};
+
+__device__ void build_item_index(const Problem& pb, ItemIndex& ix) {
+  if (threadIdx.x != 0) return;
+  const int B = pb.C * (pb.C + 1) / 2;
+  ix.blk0[0] = 0;
+  for (int b = 0; b < B; ++b) ix.blk0[b + 1] = ix.blk0[b] + (pb.boff[b + 1] - pb.boff[b] + kItemWork - 1) / kItemWork;
+  ix.cam0[0] = ix.blk0[B];
+  for (int i = 0; i < pb.C; ++i)
+    ix.cam0[i + 1] = ix.cam0[i] + (pb.coff[i + 1] - pb.coff[i] + kItemWork - 1) / kItemWork;
 }
 
-// A: S by 3-row strips of the camera-pair blocks.  A block off the diagonal
-// is stored transposed below the diagonal (the Cholesky reads the lower
-// triangle); a diagonal block is stored whole.
-__device__ void assemble_rcs(const Problem& pb, const BatchWork& w) {
-  const int C = pb.C, n = pb.n;
-  const int units = 3 * (C * (C + 1) / 2);
-  for (int u = fresh_tid(); u < units; u += kBT) {
-    const int b = u / 3, s = u - 3 * b;
-    int i, j;
-    block_pair(b, C, i, j);
+// Last k in [0, count) with a[k] <= v (a non-decreasing, a[0] <= v).
+__device__ __forceinline__ int last_le(const int* a, int count, int v) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A1: one unit per (item, 3-row strip s).  A pair item sums
+// -Jc_a^T (P_a P_b^T) Jc_b over its pairs; an observation item of camera i
+// sums U = Jc^T Jc and, for the strip's three columns, the column norm,
+// g_c = Jc^T r and Jc^T e.  The strip's rows of Jc_a are read by offset (a
+// register array indexed by the runtime strip would live in scratch memory).
+__device__ void camera_items(const Problem& pb, const BatchWork& w, const ItemIndex& ix) {
+  const int B = pb.C * (pb.C + 1) / 2;
+  const int I0 = ix.blk0[B], I = ix.cam0[pb.C];
+  for (int u = fresh_tid(); u < 3 * I; u += kBT) {
+    const int q = u / 3, s = u - 3 * q;
     double acc[3][9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int c = 0; c < 9; ++c) acc[r][c] = 0.0;
-    // the strip's rows of Jc_a are read by offset (a register array indexed by
-    // the runtime strip would live in scratch memory)
-    for (int t = pb.boff[b]; t < pb.boff[b + 1]; ++t) {
-      const int oa = pb.terms[2 * (size_t)t], ob = pb.terms[2 * (size_t)t + 1];
-      const double* ra = w.JP + 24 * (size_t)oa;
-      double rb[24], pa[6], a0[3], a1[3];
-      load_record(w.JP + 24 * (size_t)ob, rb);
+    if (q < I0) {
+      const int b = last_le(ix.blk0, B, q);
+      const int t0 = pb.boff[b] + (q - ix.blk0[b]) * kItemWork;
+      const int t1 = min(t0 + kItemWork, pb.boff[b + 1]);
+      for (int t = t0; t < t1; ++t) {
+        const int oa = pb.terms[2 * (size_t)t], ob = pb.terms[2 * (size_t)t + 1];
+        const double* ra = w.JP + 24 * (size_t)oa;
+        double rb[24], pa[6], a0[3], a1[3];
+        load_record(w.JP + 24 * (size_t)ob, rb);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) pa[k] = ra[18 + k];
+        for (int k = 0; k < 6; ++k) pa[k] = ra[18 + k];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        a0[r] = ra[3 * s + r];
-        a1[r] = ra[9 + 3 * s + r];
+        for (int r = 0; r < 3; ++r) {
+          a0[r] = ra[3 * s + r];
+          a1[r] = ra[9 + 3 * s + r];
+        }
+        // M = P_a P_b^T (2x2), T = M Jc_b (2x9)
+        const double m00 = pa[0] * rb[18] + pa[1] * rb[19] + pa[2] * rb[20];
+        const double m01 = pa[0] * rb[21] + pa[1] * rb[22] + pa[2] * rb[23];
+        const double m10 = pa[3] * rb[18] + pa[4] * rb[19] + pa[5] * rb[20];
+        const double m11 = pa[3] * rb[21] + pa[4] * rb[22] + pa[5] * rb[23];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+          const double t0v = m00 * rb[c] + m01 * rb[9 + c];
+          const double t1v = m10 * rb[c] + m11 * rb[9 + c];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) acc[r][c] -= a0[r] * t0v + a1[r] * t1v;
+        }
       }
-      // M = P_a P_b^T (2x2), T = M Jc_b (2x9)
-      const double m00 = pa[0] * rb[18] + pa[1] * rb[19] + pa[2] * rb[20];
-      const double m01 = pa[0] * rb[21] + pa[1] * rb[22] + pa[2] * rb[23];
-      const double m10 = pa[3] * rb[18] + pa[4] * rb[19] + pa[5] * rb[20];
-      const double m11 = pa[3] * rb[21] + pa[4] * rb[22] + pa[5] * rb[23];
-#pragma unroll
-      for (int c = 0; c < 9; ++c) {
-        const double t0 = m00 * rb[c] + m01 * rb[9 + c];
-        const double t1 = m10 * rb[c] + m11 * rb[9 + c];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) acc[r][c] -= a0[r] * t0 + a1[r] * t1;
-      }
-    }
-    if (i == j) {
-      // U_i = sum Jc^T Jc over camera i's observations, plus the damping
-      for (int t = pb.coff[i]; t < pb.coff[i + 1]; ++t) {
-        const double* ro = w.JP + 24 * (size_t)pb.cobs[t];
+    } else {
+      const int i = last_le(ix.cam0, pb.C, q);
+      const int t0 = pb.coff[i] + (q - ix.cam0[i]) * kItemWork;
+      const int t1 = min(t0 + kItemWork, pb.coff[i + 1]);
+      double colsq[3] = {0.0, 0.0, 0.0}, gc[3] = {0.0, 0.0, 0.0}, rh[3] = {0.0, 0.0, 0.0};
+      for (int t = t0; t < t1; ++t) {
+        const int o = pb.cobs[t];
+        const double* ro = w.JP + 24 * (size_t)o;
         double jc[18], a0[3], a1[3];
 #pragma unroll
         for (int k = 0; k < 18; ++k) jc[k] = ro[k];
@@ -340,27 +375,74 @@ __device__ void assemble_rcs(const Problem& pb, const BatchWork& w) {
           a0[r] = ro[3 * s + r];
           a1[r] = ro[9 + 3 * s + r];
         }
+        const double r0 = w.r[2 * (size_t)o], r1 = w.r[2 * (size_t)o + 1];
+        const double e0 = w.e[2 * (size_t)o], e1 = w.e[2 * (size_t)o + 1];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) acc[r][c] += a0[r] * jc[c] + a1[r] * jc[9 + c];
+          colsq[r] += a0[r] * a0[r] + a1[r] * a1[r];
+          gc[r] += a0[r] * r0 + a1[r] * r1;
+          rh[r] += a0[r] * e0 + a1[r] * e1;
+        }
       }
+      double* v = w.itemV + 27 * (size_t)(q - I0);
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const double d = w.Dc[9 * i + 3 * s + r];
-#pragma unroll
-        for (int c = 0; c < 9; ++c)
-          if (c == 3 * s + r) acc[r][c] += d;
+        v[3 * s + r] = colsq[r];
+        v[9 + 3 * s + r] = gc[r];
+        v[18 + 3 * s + r] = rh[r];
       }
+    }
+    double* dst = w.itemS + 81 * (size_t)q + 27 * s;
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) w.S[(size_t)(9 * i + 3 * s + r) * n + 9 * j + c] = acc[r][c];
+      for (int c = 0; c < 9; ++c) dst[9 * r + c] = acc[r][c];
+  }
+}
+
+// A2: one unit per element of S's camera-pair blocks (row-major within the
+// block) and one per camera column, summing the item partials in item order.
+// A block off the diagonal is stored transposed below the diagonal (the
+// Cholesky reads the lower triangle); a diagonal block whole, with the
+// damping of its columns (linear.py:539-551) from the observation items'
+// column norms, summed by the diagonal element in the same order as by its
+// column unit.
+__device__ void reduce_rcs(const Problem& pb, const BatchWork& w, const ItemIndex& ix, const LMState* st,
+                           double& g2, double& gmax) {
+  const int B = pb.C * (pb.C + 1) / 2, n = pb.n;
+  const int I0 = ix.blk0[B];
+  for (int u = fresh_tid(); u < 81 * B + n; u += kBT) {
+    if (u < 81 * B) {
+      const int b = u / 81, e = u - 81 * b;
+      const int r = e / 9, c = e - 9 * r;
+      int i, j;
+      block_pair(b, pb.C, i, j);
+      double sum = 0.0;
+      for (int q = ix.blk0[b]; q < ix.blk0[b + 1]; ++q) sum += w.itemS[81 * (size_t)q + e];
+      if (i == j) {
+        double colsq = 0.0;
+        for (int q = ix.cam0[i]; q < ix.cam0[i + 1]; ++q) {
+          sum += w.itemS[81 * (size_t)q + e];
+          colsq += w.itemV[27 * (size_t)(q - I0) + c];
+        }
+        if (r == c) sum += column_damping(st, colsq, true);
+        w.S[(size_t)(9 * i + r) * n + 9 * i + c] = sum;
+      } else {
+        w.S[(size_t)(9 * j + c) * n + 9 * i + r] = sum;
+      }
     } else {
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 9; ++c) w.S[(size_t)(9 * j + c) * n + 9 * i + 3 * s + r] = acc[r][c];
+      const int k = u - 81 * B, i = k / 9, c = k - 9 * i;
+      double gc = 0.0, rh = 0.0;
+      for (int q = ix.cam0[i]; q < ix.cam0[i + 1]; ++q) {
+        const double* v = w.itemV + 27 * (size_t)(q - I0);
+        gc += v[9 + c];
+        rh += v[18 + c];
+      }
+      w.rhs[k] = rh;
+      g2 += gc * gc;
+      gmax = fmax(gmax, fabs(gc));
     }
   }
 }
@@ -444,6 +526,7 @@ __device__ void candidate(const Problem& pb, const BatchWork& w, int cur, double
 
 struct BatchShared {
   SmallCholShared chol;
+  ItemIndex items;
   LMState st;
   double red[kBWaves][5];
   double red3[kBWaves][3];
@@ -465,13 +548,13 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
   pb.P = pr.num_points;
   pb.N = pr.num_observations;
   pb.n = 9 * pb.C;
-  if (pb.C < 1 || pb.C > CT_BA_BATCH_MAX_CAMERAS || pb.P < 0 || pb.N < 0 || (pr.work_base & 255) != 0 ||
-      pr.work_base < 256) {
+  if (pb.C < 1 || pb.C > CT_BA_BATCH_MAX_CAMERAS || pb.P < 0 || pb.N < 0 || pr.num_terms < 0 ||
+      (pr.work_base & 255) != 0 || pr.work_base < 256) {
     report_invalid(a, pi);
     return;
   }
   BatchWork w;
-  const size_t slice = batch_layout(pb.C, pb.P, pb.N, nullptr, &w);
+  const size_t slice = batch_layout(pb.C, pb.P, pb.N, pr.num_terms, nullptr, &w);
   if ((size_t)pr.work_base + slice > a.work_bytes) {
     report_invalid(a, pi);
     return;
@@ -485,7 +568,7 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
   pb.boff = a.b.block_offsets + pr.block_offsets_base;
   pb.terms = a.b.block_terms + 2 * pr.term_base;
   pb.loss = pr.loss;
-  batch_layout(pb.C, pb.P, pb.N, reinterpret_cast<double*>(a.work + pr.work_base), &w);
+  batch_layout(pb.C, pb.P, pb.N, pr.num_terms, reinterpret_cast<double*>(a.work + pr.work_base), &w);
   double* ucam = a.cameras + 9 * pr.camera_base;
   double* upt = a.points + 3 * pr.point_base;
   double* rec = a.records ? a.records + (size_t)pi * (a.opt.max_num_iterations + 2) * CT_REC_FIELDS : nullptr;
@@ -503,6 +586,7 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
     w.xp[3 * pb.P + i] = v;
   }
   if (tid == 0) lm_state_init(st, a.opt);
+  build_item_index(pb, sh.items);
   wg_sync();
 
   // thread 0 keeps the evaluation at x for the passes that reuse it
@@ -520,7 +604,9 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
       wg_sync();
       eliminate_points(pb, w, st, part[1], gmax);
       wg_sync();
-      camera_columns(pb, w, st, part[1], gmax);
+      camera_items(pb, w, sh.items);
+      wg_sync();
+      reduce_rcs(pb, w, sh.items, st, part[1], gmax);
       block_sums<3>(part, sh.red3);
       const double gm = block_max<kBT>(gmax, &sh.red[0][0]);
       if (tid == 0) {
@@ -541,8 +627,6 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
       continue;
     }
     reuse = false;
-    assemble_rcs(pb, w);
-    wg_sync();
     small_factor<9 * CT_BA_BATCH_MAX_CAMERAS>(w.S, w.rhs, pb.n, sh.chol);
     double* y = sh.chol.Lp + packed_index(pb.n, 0);  // row n: y = L^-1 rhs
     small_backward(sh.chol.Lp, sh.chol.dinv, y, pb.n);
@@ -610,14 +694,14 @@ __global__ void k_ba_batch_reset(unsigned int* q) {
 using namespace ct;
 
 extern "C" int ct_ba_batch_problem_bytes(int32_t num_cameras, int32_t num_points, int32_t num_observations,
-                                         size_t* bytes) {
+                                         int64_t num_terms, size_t* bytes) {
   clear_error();
   CT_CHECK_ARG(bytes != nullptr, "bytes is null");
   CT_CHECK_ARG(num_cameras >= 1 && num_cameras <= CT_BA_BATCH_MAX_CAMERAS,
                "a batched bundle adjustment has 1..CT_BA_BATCH_MAX_CAMERAS cameras");
-  CT_CHECK_ARG(num_points >= 0 && num_observations >= 0, "negative problem size");
+  CT_CHECK_ARG(num_points >= 0 && num_observations >= 0 && num_terms >= 0, "negative problem size");
   BatchWork w;
-  *bytes = batch_layout(num_cameras, num_points, num_observations, nullptr, &w);
+  *bytes = batch_layout(num_cameras, num_points, num_observations, num_terms, nullptr, &w);
   return CT_OK;
 }
 
@@ -663,5 +747,178 @@ extern "C" int ct_ba_batch_solve(const ct_ba_batch* batch, const ct_lm_options* 
   const int groups = batch->num_problems < cus ? batch->num_problems : cus;
   hipLaunchKernelGGL(k_ba_batch, dim3(groups), dim3(kBT), 0, s, a);
   CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host tables of a batch (the per-problem tables of ceres_mi355x.ba.build_tables,
+// in one native pass per problem: counting sorts, O(N + T)).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TableOut {
+  int32_t* order;
+  int32_t* cam;
+  int32_t* pt;
+  int32_t* point_offsets;
+  int32_t* camera_offsets;
+  int32_t* camera_obs;
+  int32_t* block_offsets;
+  int32_t* block_terms;
+};
+
+// One problem: observations grouped by (point, camera) with ties in input
+// order, the CSR tables, and the observation pairs of each camera-pair block
+// in ascending (a, b).  Returns the number of pairs; with out == nullptr it
+// only counts.
+int64_t problem_tables(int C, int P, int N, const int32_t* oc, const int32_t* op, const TableOut* out,
+                       std::vector<int32_t>& scratch) {
+  scratch.assign((size_t)3 * N + (size_t)P + (size_t)C + 2, 0);
+  int32_t* by_cam = scratch.data();       // [N]
+  int32_t* perm = by_cam + N;             // [N]
+  int32_t* cams = perm + N;               // [N] camera of each sorted observation
+  int32_t* cnt = cams + N;                // [max(P, C) + 2]
+  // stable counting sort by camera, then by point: sorted by (point, camera), ties by input order
+  std::fill(cnt, cnt + C + 1, 0);
+  for (int o = 0; o < N; ++o) ++cnt[oc[o] + 1];
+  for (int c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+  for (int o = 0; o < N; ++o) by_cam[cnt[oc[o]]++] = o;
+  std::vector<int32_t> poff((size_t)P + 1, 0);
+  for (int o = 0; o < N; ++o) ++poff[op[o] + 1];
+  for (int p = 0; p < P; ++p) poff[p + 1] += poff[p];
+  {
+    std::vector<int32_t> next(poff.begin(), poff.end() - 1);
+    for (int k = 0; k < N; ++k) {
+      const int o = by_cam[k];
+      perm[next[op[o]]++] = o;
+    }
+  }
+  for (int k = 0; k < N; ++k) cams[k] = oc[perm[k]];
+  // block of pair (i, j), i <= j: row-major upper triangle
+  auto block_of = [C](int i, int j) { return i * C - i * (i - 1) / 2 + (j - i); };
+  const int B = C * (C + 1) / 2;
+  std::vector<int64_t> bcount((size_t)B + 1, 0);
+  int64_t T = 0;
+  for (int p = 0; p < P; ++p)
+    for (int a = poff[p]; a < poff[p + 1]; ++a)
+      for (int b = poff[p]; b < poff[p + 1]; ++b)
+        if (cams[a] <= cams[b]) {
+          ++bcount[block_of(cams[a], cams[b]) + 1];
+          ++T;
+        }
+  if (out == nullptr) return T;
+  for (int k = 0; k < N; ++k) {
+    out->order[k] = perm[k];
+    out->cam[k] = cams[k];
+    out->pt[k] = op[perm[k]];
+  }
+  for (int p = 0; p <= P; ++p) out->point_offsets[p] = poff[p];
+  // camera CSR of the sorted observations (stable: ascending observation)
+  std::vector<int32_t> coff((size_t)C + 1, 0);
+  for (int k = 0; k < N; ++k) ++coff[cams[k] + 1];
+  for (int c = 0; c < C; ++c) coff[c + 1] += coff[c];
+  for (int c = 0; c <= C; ++c) out->camera_offsets[c] = coff[c];
+  {
+    std::vector<int32_t> next(coff.begin(), coff.end() - 1);
+    for (int k = 0; k < N; ++k) out->camera_obs[next[cams[k]]++] = k;
+  }
+  for (int b = 0; b < B; ++b) bcount[b + 1] += bcount[b];
+  for (int b = 0; b <= B; ++b) out->block_offsets[b] = (int32_t)bcount[b];
+  std::vector<int64_t> next(bcount.begin(), bcount.end() - 1);
+  for (int p = 0; p < P; ++p)
+    for (int a = poff[p]; a < poff[p + 1]; ++a)
+      for (int b = poff[p]; b < poff[p + 1]; ++b)
+        if (cams[a] <= cams[b]) {
+          const int64_t at = next[block_of(cams[a], cams[b])]++;
+          out->block_terms[2 * at] = a;
+          out->block_terms[2 * at + 1] = b;
+        }
+  return T;
+}
+
+}  // namespace
+
+extern "C" int ct_ba_batch_tables(int32_t num_problems, const int32_t* num_cameras, const int32_t* num_points,
+                                  const int32_t* num_observations, const int32_t* obs_camera,
+                                  const int32_t* obs_point, int64_t* num_terms, int32_t* order,
+                                  int32_t* sorted_obs_camera, int32_t* sorted_obs_point, int32_t* point_offsets,
+                                  int32_t* camera_offsets, int32_t* camera_obs, int32_t* block_offsets,
+                                  int32_t* block_terms) {
+  clear_error();
+  CT_CHECK_ARG(num_problems >= 0, "negative batch size");
+  CT_CHECK_ARG(num_problems == 0 || (num_cameras && num_points && num_observations && num_terms),
+               "null size array");
+  const bool build = order != nullptr;
+  if (build) {
+    CT_CHECK_ARG(sorted_obs_camera && sorted_obs_point && point_offsets && camera_offsets && camera_obs &&
+                     block_offsets && block_terms,
+                 "null table array");
+  }
+  const int K = num_problems;
+  // each problem's slice of every array
+  std::vector<int64_t> ob(K + 1, 0), pb(K + 1, 0), cb(K + 1, 0), bb(K + 1, 0), tb(K + 1, 0);
+  for (int k = 0; k < K; ++k) {
+    const int C = num_cameras[k], P = num_points[k], N = num_observations[k];
+    CT_CHECK_ARG(C >= 1 && C <= CT_BA_BATCH_MAX_CAMERAS, "a batched bundle adjustment has 1..21 cameras");
+    CT_CHECK_ARG(P >= 0 && N >= 0, "negative problem size");
+    ob[k + 1] = ob[k] + N;
+    pb[k + 1] = pb[k] + P + 1;
+    cb[k + 1] = cb[k] + C + 1;
+    bb[k + 1] = bb[k] + C * (C + 1) / 2 + 1;
+    if (build) {
+      CT_CHECK_ARG(num_terms[k] >= 0, "negative num_terms");
+      tb[k + 1] = tb[k] + num_terms[k];
+    }
+  }
+  CT_CHECK_ARG(ob[K] == 0 || (obs_camera && obs_point), "null observation arrays");
+  // problems are independent: worker threads take them from a shared counter
+  std::atomic<int> next{0};
+  std::atomic<int> bad{-1};  // first problem with an error
+  std::string bad_msg;
+  std::mutex bad_lock;
+  auto work = [&]() {
+    std::vector<int32_t> scratch;
+    for (int k = next.fetch_add(1); k < K; k = next.fetch_add(1)) {
+      const int C = num_cameras[k], P = num_points[k], N = num_observations[k];
+      const int32_t* oc = obs_camera + ob[k];
+      const int32_t* op = obs_point + ob[k];
+      bool ok = true;
+      for (int o = 0; o < N && ok; ++o) {
+        if (oc[o] < 0 || oc[o] >= C || op[o] < 0 || op[o] >= P) {
+          std::lock_guard<std::mutex> g(bad_lock);
+          if (bad.load() < 0 || k < bad.load()) {
+            bad.store(k);
+            bad_msg = "problem " + std::to_string(k) + ": observation " + std::to_string(o) + " out of range";
+          }
+          ok = false;
+        }
+      }
+      if (!ok) continue;
+      if (build) {
+        const TableOut out{order + ob[k],          sorted_obs_camera + ob[k], sorted_obs_point + ob[k],
+                           point_offsets + pb[k],  camera_offsets + cb[k],    camera_obs + ob[k],
+                           block_offsets + bb[k],  block_terms + 2 * tb[k]};
+        if (problem_tables(C, P, N, oc, op, &out, scratch) != num_terms[k]) {
+          std::lock_guard<std::mutex> g(bad_lock);
+          if (bad.load() < 0 || k < bad.load()) {
+            bad.store(k);
+            bad_msg = "problem " + std::to_string(k) + ": num_terms differs from the counting call";
+          }
+        }
+      } else {
+        num_terms[k] = problem_tables(C, P, N, oc, op, nullptr, scratch);
+      }
+    }
+  };
+  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const int nthreads = (int)std::min<int64_t>(hw, std::max(1, K));
+  std::vector<std::thread> pool;
+  for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+  work();
+  for (std::thread& th : pool) th.join();
+  if (bad.load() >= 0) {
+    set_error(bad_msg);
+    return CT_ERR_ARGUMENT;
+  }
   return CT_OK;
 }

@@ -1008,7 +1008,10 @@ extern "C" {
 
 int ct_pose2d_workspace_bytes(const ct_pose2d_problem* problem, const ct_lm_options* options, size_t* bytes) {
   clear_error();
-  CT_TRY(validate_pg(problem));
+  // sizes only: a caller may ask before its tables exist on the device
+  CT_CHECK_ARG(problem != nullptr, "problem is null");
+  CT_CHECK_ARG(problem->num_nodes >= 1 && problem->num_edges >= 0 && problem->num_pairs >= 0, "bad problem size");
+  CT_CHECK_ARG((int64_t)problem->num_nodes * 3 <= 24576, "too many nodes for the dense normal equations");
   CT_CHECK_ARG(options != nullptr && bytes != nullptr, "null argument");
   *bytes = pg_carve(problem, options, nullptr).bytes;
   return CT_OK;

@@ -509,6 +509,7 @@ typedef struct ct_ba_batch_problem {
   int32_t num_points;              /* P */
   int32_t num_observations;        /* N */
   int32_t reserved;
+  int64_t num_terms;               /* T: its observation pairs in block_terms */
   int64_t camera_base;             /* first camera in cameras [.][9] */
   int64_t point_base;              /* first point in points [.][3] */
   int64_t obs_base;                /* first observation in obs_camera, obs_point, observations, camera_obs */
@@ -535,8 +536,25 @@ typedef struct ct_ba_batch {
   const int32_t* order;            /* [num_problems] processing order (a permutation), or NULL */
 } ct_ba_batch;
 
+/* Host tables of a batch (host memory, synchronous).  Inputs: per problem C, P,
+ * N and the problem-local obs_camera / obs_point, concatenated in problem
+ * order.  Called with order == NULL it only fills num_terms [num_problems];
+ * called again with every output (sizes: order, sorted_obs_camera,
+ * sorted_obs_point, camera_obs [sum N]; point_offsets [sum (P+1)];
+ * camera_offsets [sum (C+1)]; block_offsets [sum (C(C+1)/2+1)]; block_terms
+ * [2 sum T]) it writes each problem's tables as ct_ba_problem's (observations
+ * grouped by point, cameras ascending within a point, ties in input order;
+ * order[k] = the input observation at sorted position k; each block's pairs
+ * in ascending (a, b)), in problem order.  No reference counterpart.       */
+int ct_ba_batch_tables(int32_t num_problems, const int32_t* num_cameras, const int32_t* num_points,
+                       const int32_t* num_observations, const int32_t* obs_camera, const int32_t* obs_point,
+                       int64_t* num_terms, int32_t* order, int32_t* sorted_obs_camera, int32_t* sorted_obs_point,
+                       int32_t* point_offsets, int32_t* camera_offsets, int32_t* camera_obs, int32_t* block_offsets,
+                       int32_t* block_terms);
+
 /* Bytes of one problem's workspace slice. */
-int ct_ba_batch_problem_bytes(int32_t num_cameras, int32_t num_points, int32_t num_observations, size_t* bytes);
+int ct_ba_batch_problem_bytes(int32_t num_cameras, int32_t num_points, int32_t num_observations, int64_t num_terms,
+                              size_t* bytes);
 
 /* Solve every problem of the batch.  cameras [*][9] and points [*][3] (device
  * FP64) hold the start values and receive each problem's best state;
@@ -576,6 +594,8 @@ typedef struct ct_pose2d_problem {
 
 typedef struct ct_pose2d_plan ct_pose2d_plan;
 
+/* Device workspace of a plan; reads only the sizes (num_nodes, num_edges,
+ * num_pairs, fixed_node), so a caller may ask before uploading the tables. */
 int ct_pose2d_workspace_bytes(const ct_pose2d_problem* problem, const ct_lm_options* options, size_t* bytes);
 /* poses [N][3] FP64 device: start values in; ct_pose2d_plan_run leaves the
  * best values there.  Validates every table index on the host, after

@@ -5,8 +5,10 @@ import numpy as np
 import pytest
 
 import ceres_mi355x as tc
-from ceres_mi355x.serving import BARequest, CurveRequest, PoseRequest, SolveServer, estimate_bytes
+from ceres_mi355x.serving import BARequest, CurveRequest, PoseRequest, SolveServer, request_bytes
 from oracle import generators as gen
+from oracle.ba import BAModel
+from oracle.lm import OracleOptions, trust_region_solve
 
 pytestmark = pytest.mark.gpu
 
@@ -17,7 +19,7 @@ def _mix(seed=21):
     for i in range(24):
         cb = gen.make_curve_batch(num_problems=1, seed=int(rng.integers(1 << 30)))
         reqs.append(CurveRequest(cb.x, cb.y, loss=tc.CauchyLoss(0.5)))
-    for C, P in ((3, 150), (8, 600), (12, 900), (20, 400)):
+    for C, P in ((3, 150), (8, 600), (12, 900), (20, 400), (24, 300)):
         d = gen.make_bundle_adjustment(num_cameras=C, num_points=P, obs_per_point=(2, min(6, C)),
                                        seed=int(rng.integers(1 << 30)))
         reqs.append(BARequest(d.cameras, d.points, d.obs_camera, d.obs_point, d.observations))
@@ -46,7 +48,7 @@ def test_results_in_the_mix_equal_results_alone():
 
 def test_admission_keeps_within_budget():
     reqs = _mix(seed=5)[:10]
-    budget = max(estimate_bytes(r) for r in reqs)  # at most one large request at a time
+    budget = max(request_bytes(r) for r in reqs)  # at most one large request at a time
     with SolveServer(tc.SolverOptions(max_num_iterations=10), hbm_budget_bytes=budget, num_slots=4) as server:
         futures = [server.submit(r) for r in reqs]
         results = [f.result(timeout=120) for f in futures]
@@ -54,3 +56,25 @@ def test_admission_keeps_within_budget():
     assert all(r.finished >= r.admitted >= r.arrival for r in results)
     with pytest.raises(ValueError):
         SolveServer(hbm_budget_bytes=1).submit(reqs[0])
+
+
+def test_served_bundle_adjustments_match_the_oracle():
+    """Batched BA requests in a served mix: each final cost equals the float64 oracle's (1e-10 relative),
+    including a 24-camera request, which takes a per-request plan on a slot."""
+    rng = np.random.default_rng(9)
+    datas, reqs = [], []
+    for C, P in ((2, 120), (6, 500), (15, 1500), (21, 800), (24, 300)):
+        d = gen.make_bundle_adjustment(num_cameras=C, num_points=P, obs_per_point=(2, min(6, C)),
+                                       seed=int(rng.integers(1 << 30)))
+        datas.append(d)
+        reqs.append(BARequest(d.cameras, d.points, d.obs_camera, d.obs_point, d.observations))
+    for i in range(6):
+        cb = gen.make_curve_batch(num_problems=1, seed=100 + i)
+        reqs.append(CurveRequest(cb.x, cb.y, loss=tc.CauchyLoss(0.5)))
+    with SolveServer(tc.SolverOptions(), num_slots=2) as server:
+        results = [f.result(timeout=120) for f in [server.submit(r) for r in reqs]]
+    for d, res in zip(datas, results):
+        ref, _ = trust_region_solve(BAModel(d), OracleOptions())
+        assert res.kind == "ba"
+        assert res.final_cost == pytest.approx(ref.final_cost, rel=1e-10), d.num_cameras
+        assert res.iterations == len(ref.iterations) - 1, d.num_cameras
