@@ -95,6 +95,7 @@ class ct_lm_options(ctypes.Structure):
         ("debug_invalid_steps", ctypes.c_int32),
         ("schur_operator", ctypes.c_int32),
         ("debug_fill_lds", ctypes.c_int32),
+        ("solver_workgroups", ctypes.c_int32),
     ]
 
 

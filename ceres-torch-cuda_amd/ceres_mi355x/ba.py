@@ -362,7 +362,7 @@ def debug_fill_lds():
 
 
 def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: str = "dense_schur",
-               iterations_per_graph: int = 8) -> nat.ct_lm_options:
+               iterations_per_graph: int = 8, solver_workgroups: int = 0) -> nat.ct_lm_options:
     """Translate SolverOptions into the native option struct."""
     o = nat.ct_lm_options()
     o.max_num_iterations = int(options.max_num_iterations)
@@ -395,6 +395,7 @@ def lm_options(options, *, precision: str = "fp64", loop: str = "host", linear: 
     o.debug_invalid_steps = _debug_invalid_steps
     o.debug_fill_lds = _debug_fill_lds
     o.schur_operator = 0 if SCHUR_OPERATOR["mode"] == "auto" else 1
+    o.solver_workgroups = int(solver_workgroups)
     return o
 
 

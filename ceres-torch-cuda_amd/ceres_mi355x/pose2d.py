@@ -58,11 +58,22 @@ class PoseGraphDevice:
         self.num_edges = int(edges.shape[0])
         tables = build_pose_tables(edges, self.num_nodes)
         dev = self.device
-        self.edges = torch.from_numpy(np.ascontiguousarray(edges, dtype=np.int32)).to(dev)
-        self.measurements = torch.as_tensor(np.asarray(measurements, dtype=np.float64).reshape(-1, 3)).to(dev)
-        self.tables = {k: torch.from_numpy(v).to(dev) for k, v in tables.items()}
+        # one upload for the integer tables and one for the values (views of the two device arrays)
+        ints = [("edges", np.ascontiguousarray(edges, dtype=np.int32).reshape(-1))]
+        ints += [(k, v.reshape(-1)) for k, v in tables.items()]
+        flat = np.concatenate([a for _, a in ints]) if ints else np.zeros(0, np.int32)
+        dints = torch.from_numpy(np.ascontiguousarray(flat, dtype=np.int32)).to(dev)
+        views, at = {}, 0
+        for k, a in ints:
+            views[k] = dints[at:at + a.size]
+            at += a.size
+        self.edges = views.pop("edges").view(-1, 2)
+        self.tables = {k: views[k] for k in tables}
+        meas = np.asarray(measurements, dtype=np.float64).reshape(-1)
+        vals = torch.from_numpy(np.concatenate([meas, poses.detach().cpu().numpy().reshape(-1)])).to(dev)
+        self.measurements = vals[:meas.size].view(-1, 3)
+        self.poses = vals[meas.size:].view(-1, 3)
         self.num_pairs = int(tables["pair_nodes"].shape[0])
-        self.poses = poses.to(dev).contiguous().clone()
         self.loss_native = loss_native
         self.fixed_node = int(fixed_node)
         self._plans: dict = {}

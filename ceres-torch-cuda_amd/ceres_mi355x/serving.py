@@ -104,20 +104,28 @@ class _Ticket:
     admitted: float = 0.0
 
 
-def _ba_terms(obs_camera, obs_point) -> int:
-    """Observation pairs of a common point with cam(a) <= cam(b): the camera-pair assembly's terms."""
-    oc = np.asarray(obs_camera, dtype=np.int64).reshape(-1)
-    op = np.asarray(obs_point, dtype=np.int64).reshape(-1)
-    if oc.size == 0:
+def _ba_terms(num_cameras: int, num_points: int, obs_camera, obs_point) -> int:
+    """Observation pairs of a common point with cam(a) <= cam(b) (the camera-pair assembly's terms), counted by
+    the library's native table builder; requests with more cameras than a batch holds are bounded instead."""
+    oc = np.ascontiguousarray(obs_camera, dtype=np.int32).reshape(-1)
+    op = np.ascontiguousarray(obs_point, dtype=np.int32).reshape(-1)
+    if num_cameras > MAX_CAMERAS:
+        k = np.bincount(op, minlength=num_points).astype(np.int64)
+        return int(np.sum(k * k))
+    lib = nat.load()
+    sizes = np.array([num_cameras, num_points, oc.size], dtype=np.int32)
+    T = np.zeros(1, dtype=np.int64)
+    nat.check(lib.ct_ba_batch_tables(1, sizes[0:].ctypes.data, sizes[1:].ctypes.data, sizes[2:].ctypes.data,
+                                     oc.ctypes.data if oc.size else None, op.ctypes.data if op.size else None,
+                                     T.ctypes.data, *([None] * 8)), "ct_ba_batch_tables")
+    return int(T[0])
+
+
+def _pose_pairs(edges, num_nodes: int) -> int:
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    if e.size == 0:
         return 0
-    k = np.bincount(op)
-    same = np.unique(op * (int(oc.max()) + 1) + oc, return_counts=True)[1]  # observations per (point, camera)
-    return int((np.sum(k.astype(np.int64) ** 2) + np.sum(same.astype(np.int64) ** 2)) // 2)
-
-
-def _pose_pairs(edges) -> int:
-    e = np.sort(np.asarray(edges, dtype=np.int64).reshape(-1, 2), axis=1)
-    return int(np.unique(e, axis=0).shape[0]) if e.size else 0
+    return int(np.unique(np.minimum(e[:, 0], e[:, 1]) * num_nodes + np.maximum(e[:, 0], e[:, 1])).size)
 
 
 def request_bytes(req, options: Optional[SolverOptions] = None) -> int:
@@ -127,7 +135,7 @@ def request_bytes(req, options: Optional[SolverOptions] = None) -> int:
         return 24 * int(np.asarray(req.x).size) + 128
     if isinstance(req, BARequest):
         C, P, N = len(req.cameras), len(req.points), len(req.obs_camera)
-        T = _ba_terms(req.obs_camera, req.obs_point)
+        T = _ba_terms(C, P, req.obs_camera, req.obs_point)
         if C <= MAX_CAMERAS:
             return ba_device_bytes(C, P, N, T)
         # per-request plan: the plan's own workspace query needs the device tables; bound it from its layout
@@ -137,7 +145,7 @@ def request_bytes(req, options: Optional[SolverOptions] = None) -> int:
         lib = nat.load()
         prob = nat.ct_pose2d_problem()
         prob.num_nodes, prob.num_edges = len(req.poses), len(req.edges)
-        prob.num_pairs, prob.fixed_node = _pose_pairs(req.edges), int(req.fixed_node)
+        prob.num_pairs, prob.fixed_node = _pose_pairs(req.edges, len(req.poses)), int(req.fixed_node)
         opts = lm_options(options or SolverOptions())
         out = ctypes.c_size_t(0)
         nat.check(lib.ct_pose2d_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out)),
@@ -164,6 +172,7 @@ class SolveServer:
         self.in_use = 0
         self.max_curve_batch = int(max_curve_batch)
         self.max_ba_batch = int(max_ba_batch)
+        self.slot_workgroups = 64  # persistent workgroups of a slot's reduced-system factorisation
         self.curve_schedule = curve_schedule
         self.loop = loop
         self._lock = threading.Condition()
@@ -248,7 +257,7 @@ class SolveServer:
             t = tickets[0]
             try:
                 with torch.cuda.stream(stream):
-                    res = self._solve_one(t.request)
+                    res = self._solve_one(t.request, self.slot_workgroups)
                     torch.cuda.current_stream().synchronize()
                 res.arrival, res.admitted, res.finished = t.arrival, t.admitted, time.perf_counter()
                 t.future.set_result(res)
@@ -297,10 +306,11 @@ class SolveServer:
                 self._release(sum(t.nbytes for t in tickets))
 
     # -- solves ----------------------------------------------------------------
-    def _lm(self, precision: str = "fp64"):
-        return lm_options(self.options, precision=precision, loop=self.loop, linear="dense_schur")
+    def _lm(self, precision: str = "fp64", solver_workgroups: int = 0):
+        return lm_options(self.options, precision=precision, loop=self.loop, linear="dense_schur",
+                          solver_workgroups=solver_workgroups)
 
-    def _solve_one(self, req) -> Result:
+    def _solve_one(self, req, solver_workgroups: int = 0) -> Result:
         if isinstance(req, BARequest):
             dev = BADevice(torch.as_tensor(req.cameras, dtype=torch.float64),
                            torch.as_tensor(req.points, dtype=torch.float64), req.obs_camera, req.obs_point,
@@ -315,7 +325,9 @@ class SolveServer:
             dev = PoseGraphDevice(req.poses, req.edges, req.measurements, loss_native=native_loss(req.loss),
                                   fixed_node=req.fixed_node)
             try:
-                rep, _ = dev.run(self._lm())
+                # the separator factorisation's column chain keeps few workgroups busy: a share of the chip
+                # per solve lets the slots' solves run side by side
+                rep, _ = dev.run(self._lm(solver_workgroups=solver_workgroups))
                 params = dev.poses.cpu().numpy()
             finally:
                 dev.close()
@@ -364,7 +376,7 @@ class SolveServer:
             return self._solve_curves([req])[0]
         if isinstance(req, BARequest) and len(req.cameras) <= MAX_CAMERAS:
             return self._solve_bas([req])[0]
-        return self._solve_one(req)
+        return self._solve_one(req, self.slot_workgroups)
 
     def solve_plain(self, req) -> Result:
         """The request solved by itself through the per-problem path (one plan per bundle adjustment or pose

@@ -257,7 +257,8 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
       if (pl->implicit) return launch_pcg_implicit(pl->imp, s);
       if (!uses_cholesky(&pl->options))
         return launch_rcs_pcg(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.pcg_work, L.pcg_doubles, s);
-      return launch_rcs_cholesky_solve(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.w.solve_tmp, s);
+      return launch_rcs_cholesky_solve(L.w.S, L.w.rhs, L.w.dxc, n, L.w.state, L.w.solve_tmp, s,
+                                       pl->options.solver_workgroups);
     case CT_KERNEL_REFINE:
       if (!refining(pl)) return CT_OK;
       CT_TRY(launch_backsub<float>(v, L.bf, L.w, s));

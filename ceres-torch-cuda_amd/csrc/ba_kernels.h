@@ -99,8 +99,9 @@ int launch_lm_finalize(const LMInit& init, const BAWork& w, double* user_cams, d
 //   (rhs == nullptr: factor only, keeping what later triangular solves need);
 //   skip when state->done or state->skip_step; flag state->step_invalid on a
 //   non-positive or non-finite pivot.
+// max_groups: persistent workgroups of the tiled factorisation (0: up to two per compute unit).
 int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* state, double* tmp,
-                              hipStream_t s);
+                              hipStream_t s, int max_groups = 0);
 // Triangular solves only, with the factor left in S by the Cholesky.
 int launch_rcs_tri_solve(const double* L, const double* rhs, double* x, int n, LMState* state, double* tmp,
                          hipStream_t s);

@@ -955,7 +955,7 @@ int rcs_cholesky_prepare(double* tmp, int n, hipStream_t s) {
 }
 
 int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LMState* st, double* tmp,
-                              hipStream_t s) {
+                              hipStream_t s, int max_groups) {
   if (n <= 0) return CT_OK;
   if (n <= kSmallMax) {
     hipLaunchKernelGGL(k_chol_small, dim3(1), dim3(kSmallThreads), 0, s, S, rhs, x, tmp, n, st);
@@ -965,8 +965,9 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
   int cus = 0;
   CT_TRY(device_cus(&cus));
   const size_t tiles = tri_blocks(n) * tri_blocks(n);  // factor and inverse tiles
-  const int G = (int)std::min<size_t>(tiles, 2 * (size_t)cus);  // two workgroups fit a CU (LDS)
-  hipLaunchKernelGGL(k_chol_tiles, dim3(G), dim3(kFThreads), 0, s, S, n, factor_work(tmp, n), st);
+  size_t G = std::min<size_t>(tiles, 2 * (size_t)cus);  // two workgroups fit a CU (LDS)
+  if (max_groups > 0) G = std::min<size_t>(G, (size_t)max_groups);  // tickets need no co-residency
+  hipLaunchKernelGGL(k_chol_tiles, dim3((unsigned)G), dim3(kFThreads), 0, s, S, n, factor_work(tmp, n), st);
   CT_LAUNCH_CHECK();
   if (rhs == nullptr) return CT_OK;  // factorisation only
   return launch_inverse_apply(rhs, x, n, st, tmp, s);

@@ -1,7 +1,41 @@
 // Host-side LM loop driver (see lm_plan.h).
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "lm_plan.h"
 
 namespace ct {
+
+namespace {
+
+// Pinned host buffers of released drivers, kept for the next plan: a server
+// creates and destroys a plan per request, and pinned allocation and release
+// are slow calls that serialise with other threads' work.
+std::mutex g_pinned_lock;
+std::map<size_t, std::vector<void*>> g_pinned_free;
+
+int pinned_take(size_t bytes, void** out) {
+  {
+    std::lock_guard<std::mutex> g(g_pinned_lock);
+    std::vector<void*>& v = g_pinned_free[bytes];
+    if (!v.empty()) {
+      *out = v.back();
+      v.pop_back();
+      return CT_OK;
+    }
+  }
+  CT_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return CT_OK;
+}
+
+void pinned_give(void* p, size_t bytes) {
+  if (p == nullptr) return;
+  std::lock_guard<std::mutex> g(g_pinned_lock);
+  g_pinned_free[bytes].push_back(p);
+}
+
+}  // namespace
 
 int lm_driver_setup(LMDriver& d, const ct_lm_options& o, hipStream_t stream, LMState* dev_state,
                     double* dev_records) {
@@ -13,10 +47,10 @@ int lm_driver_setup(LMDriver& d, const ct_lm_options& o, hipStream_t stream, LMS
   int per = o.iterations_per_graph > 0 ? o.iterations_per_graph : 8;
   if (per > o.max_num_iterations + 2) per = o.max_num_iterations + 2;
   d.graph_iterations = per;
-  CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.host_state), sizeof(LMState), hipHostMallocDefault));
+  CT_TRY(pinned_take(sizeof(LMState), reinterpret_cast<void**>(&d.host_state)));
   if (d.loop_mode == CT_LOOP_DEVICE) {
     CT_HIP(hipStreamCreateWithFlags(&d.capture_stream, hipStreamNonBlocking));
-    CT_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.host_done), 2 * sizeof(int32_t), hipHostMallocDefault));
+    CT_TRY(pinned_take(2 * sizeof(int32_t), reinterpret_cast<void**>(&d.host_done)));
     for (hipEvent_t& e : d.done_event) CT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   return CT_OK;
@@ -26,8 +60,8 @@ void lm_driver_release(LMDriver& d) {
   if (d.graph_exec) (void)hipGraphExecDestroy(d.graph_exec);
   if (d.graph) (void)hipGraphDestroy(d.graph);
   if (d.capture_stream) (void)hipStreamDestroy(d.capture_stream);
-  if (d.host_state) (void)hipHostFree(d.host_state);
-  if (d.host_done) (void)hipHostFree(d.host_done);
+  pinned_give(d.host_state, sizeof(LMState));
+  pinned_give(d.host_done, 2 * sizeof(int32_t));
   for (hipEvent_t& e : d.done_event) {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;

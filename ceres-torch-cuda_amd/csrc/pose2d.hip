@@ -781,7 +781,7 @@ PGLayout pg_carve(const ct_pose2d_problem* p, const ct_lm_options* o, void* base
 // separator nodes, the runs between them, and the off-diagonal separator
 // blocks.  *use is false when no node lies in a run (the dense form then).
 int build_separator(const ct_pose2d_problem& p, const std::vector<int32_t>& ed, const std::vector<int32_t>& pn,
-                    PGLayout& L, bool* use, int* nsep_out) {
+                    PGLayout& L, bool* use, int* nsep_out, hipStream_t s) {
   *use = false;
   const int N = p.num_nodes, E = p.num_edges, P = p.num_pairs, fixed = p.fixed_node;
   std::vector<char> is_sep(N, 0);
@@ -846,9 +846,11 @@ int build_separator(const ct_pose2d_problem& p, const std::vector<int32_t>& ed, 
   const std::pair<int32_t*, const std::vector<int32_t>*> uploads[] = {
       {L.run_nodes, &run_nodes}, {L.run_pairs, &run_pairs}, {L.chain_pair, &chain_pair},
       {L.sep_nodes, &sep_nodes}, {L.sep_runs, &sep_runs},   {L.entries, &entries}};
+  // stream-ordered copies (a legacy-stream copy would wait on, and disturb a graph capture in, other streams)
   for (const auto& u : uploads)
     if (!u.second->empty())
-      CT_HIP(hipMemcpy(u.first, u.second->data(), sizeof(int32_t) * u.second->size(), hipMemcpyHostToDevice));
+      CT_HIP(hipMemcpyAsync(u.first, u.second->data(), sizeof(int32_t) * u.second->size(), hipMemcpyHostToDevice, s));
+  CT_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
   PGSep& sp = L.sep;
   sp.runs = (int32_t)(run_nodes.size() / 4);
   sp.nsep = nsep;
@@ -878,24 +880,25 @@ int validate_pg(const ct_pose2d_problem* p) {
 }
 
 template <typename T>
-int copy_back(std::vector<T>& h, const T* d, size_t count) {
+int copy_back(std::vector<T>& h, const T* d, size_t count, hipStream_t s) {
   h.resize(count);
-  if (count) CT_HIP(hipMemcpy(h.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost));
+  if (count) CT_HIP(hipMemcpyAsync(h.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost, s));
   return CT_OK;
 }
 
 // every index the kernels follow, checked once on the host; ed and pn keep
 // the host copies of the edges and pair nodes
-int validate_pg_tables(const ct_pose2d_problem& p, std::vector<int32_t>& ed, std::vector<int32_t>& pn) {
+int validate_pg_tables(const ct_pose2d_problem& p, std::vector<int32_t>& ed, std::vector<int32_t>& pn, hipStream_t s) {
   const int N = p.num_nodes, E = p.num_edges, P = p.num_pairs;
   if (E == 0) return CT_OK;
   std::vector<int32_t> no, ne, po, pe;
-  CT_TRY(copy_back(ed, p.edges, 2 * (size_t)E));
-  CT_TRY(copy_back(no, p.node_offsets, (size_t)N + 1));
-  CT_TRY(copy_back(ne, p.node_edges, 2 * (size_t)E));
-  CT_TRY(copy_back(pn, p.pair_nodes, 2 * (size_t)P));
-  CT_TRY(copy_back(po, p.pair_offsets, (size_t)P + 1));
-  CT_TRY(copy_back(pe, p.pair_edges, (size_t)E));
+  CT_TRY(copy_back(ed, p.edges, 2 * (size_t)E, s));
+  CT_TRY(copy_back(no, p.node_offsets, (size_t)N + 1, s));
+  CT_TRY(copy_back(ne, p.node_edges, 2 * (size_t)E, s));
+  CT_TRY(copy_back(pn, p.pair_nodes, 2 * (size_t)P, s));
+  CT_TRY(copy_back(po, p.pair_offsets, (size_t)P + 1, s));
+  CT_TRY(copy_back(pe, p.pair_edges, (size_t)E, s));
+  CT_HIP(hipStreamSynchronize(s));
   for (int e = 0; e < E; ++e)
     CT_CHECK_ARG(ed[2 * e] >= 0 && ed[2 * e] < N && ed[2 * e + 1] >= 0 && ed[2 * e + 1] < N &&
                      ed[2 * e] != ed[2 * e + 1],
@@ -983,7 +986,8 @@ int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
     if (sp.nsep > 0) {
       hipLaunchKernelGGL(k_pg_sep_assemble, dim3(blocks_of(sp.nsep + sp.nent)), dim3(kPgThreads), 0, s, v, w, sp);
       CT_LAUNCH_CHECK();
-      CT_TRY(launch_rcs_cholesky_solve(w.H, sp.rhs_s, sp.x_s, 3 * sp.nsep, w.state, w.solve_tmp, s));
+      CT_TRY(launch_rcs_cholesky_solve(w.H, sp.rhs_s, sp.x_s, 3 * sp.nsep, w.state, w.solve_tmp, s,
+                                        pl->options.solver_workgroups));
     }
     const int sep_groups = (sp.nsep + kRunChunk - 1) / kRunChunk;
     if (sp.runs + sep_groups > 0)
@@ -991,7 +995,8 @@ int pg_enqueue(ct_pose2d_plan* pl, hipStream_t s, bool evaluation_only) {
     CT_LAUNCH_CHECK();
   }
   if (!evaluation_only) {
-    if (!sep && v.n > 0) CT_TRY(launch_rcs_cholesky_solve(w.H, w.rhs, w.dx, v.n, w.state, w.solve_tmp, s));
+    if (!sep && v.n > 0)
+      CT_TRY(launch_rcs_cholesky_solve(w.H, w.rhs, w.dx, v.n, w.state, w.solve_tmp, s, pl->options.solver_workgroups));
     hipLaunchKernelGGL(k_pg_step, dim3(blocks_of(v.N)), dim3(kPgThreads), 0, s, v, w);
     CT_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pg_cand, dim3(blocks_of(v.E)), dim3(kPgThreads), 0, s, v, w);
@@ -1037,11 +1042,9 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
     set_error("workspace too small: need " + std::to_string(need) + " bytes");
     return CT_ERR_WORKSPACE;
   }
-  // the tables are read back with synchronous copies, which do not wait for a
-  // non-blocking stream: the caller's uploads on `stream` finish first
-  CT_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  // the tables are read back on the caller's stream, behind its uploads
   std::vector<int32_t> ed, pn;
-  CT_TRY(validate_pg_tables(*problem, ed, pn));
+  CT_TRY(validate_pg_tables(*problem, ed, pn, reinterpret_cast<hipStream_t>(stream)));
   auto pl = std::make_unique<ct_pose2d_plan>();
   pl->problem = *problem;
   pl->options = *options;
@@ -1065,7 +1068,7 @@ int ct_pose2d_plan_create(const ct_pose2d_problem* problem, const ct_lm_options*
   pl->user_poses = poses;
   pl->stream = reinterpret_cast<hipStream_t>(stream);
   int nsep = 0;
-  CT_TRY(build_separator(*problem, ed, pn, pl->layout, &pl->separator, &nsep));
+  CT_TRY(build_separator(*problem, ed, pn, pl->layout, &pl->separator, &nsep, pl->stream));
   pl->solve_n = pl->separator ? 3 * nsep : v.n;
   if (pl->solve_n > 0) {
     CT_TRY(rcs_cholesky_prepare(pl->layout.w.solve_tmp, pl->solve_n, pl->stream));

@@ -155,6 +155,11 @@ typedef struct ct_lm_options {
                                            every compute unit with NaN right before each reduced-camera solve,
                                            so a solver kernel that reads LDS it has not written in its own
                                            launch fails its parity test.  Curve fits and pose graphs ignore it. */
+  int32_t solver_workgroups;            /* persistent workgroups of a reduced-system factorisation larger than
+                                           192 unknowns (bundle adjustment, pose graphs); 0 = up to two per
+                                           compute unit.  A server running several solves at once gives each a
+                                           share of the chip (the factorisation's column chain keeps few
+                                           workgroups busy at a time).  No reference counterpart. */
 } ct_lm_options;
 
 /* How ITERATIVE_SCHUR forms S p (no reference counterpart: the reference
