@@ -583,7 +583,87 @@ def bench_curve_batch(num_problems=4096, iterations=30, replays=5, cpu_problems=
     return out
 
 
-def bench_serving(num_requests=300, rate=40.0, seed=4, slots=4, saturated_slots=(4, 8), baseline_sample=60):
+def bench_ba_batch(num_problems=101, seed=4, repeats=3, plain_sample=30):
+    """Batched small bundle adjustments (the serving mix's share: 2-20 cameras, 100-5000 points, 2-6
+    observations per point, reference default options, DENSE_SCHUR) in one persistent launch
+    (ct_ba_batch_solve), timed with HIP events on the launching stream; the host table build
+    (ct_ba_batch_tables + uploads) is reported beside it.  The plain per-problem GPU baseline solves the first
+    `plain_sample` problems one after another through per-problem plans (host-driven loop, as a caller without
+    batching would)."""
+    import numpy as np
+    import torch
+
+    import ceres_mi355x as tc
+    from ceres_mi355x.ba import BADevice, lm_options
+    from ceres_mi355x.ba_batch import BABatch, BAProblemInput
+    from oracle import generators as gen
+
+    rng = np.random.default_rng(seed)
+    datas = []
+    for _ in range(num_problems):
+        C = int(rng.integers(2, 21))
+        P = int(rng.integers(100, 5001))
+        datas.append(gen.make_bundle_adjustment(num_cameras=C, num_points=P, obs_per_point=(2, min(6, C)),
+                                                seed=int(rng.integers(1 << 30))))
+    inputs = [BAProblemInput(d.cameras, d.points, d.obs_camera, d.obs_point, d.observations) for d in datas]
+    opts = lm_options(tc.SolverOptions())
+    BABatch(inputs[:2]).solve(opts)  # load the code object
+    builds = []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch = BABatch(inputs)
+        torch.cuda.synchronize()
+        builds.append(time.perf_counter() - t0)
+    s = torch.cuda.current_stream()
+    results = torch.zeros(num_problems * 64, dtype=torch.uint8, device="cuda")
+    times = []
+    for _ in range(repeats):
+        cams, pts = batch.cameras.clone(), batch.points.clone()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        batch.launch(opts, cams, pts, results)
+        e1.record(s)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    reps, _, _, _ = batch.solve(opts)
+    iters = int(sum(int(r["num_records"]) - 1 for r in reps))
+    dev_s = np.array([float(r["device_seconds"]) for r in reps])
+    launch_s = min(times)
+    plain = 0.0
+    for d in datas[:plain_sample]:
+        dev = BADevice(torch.as_tensor(d.cameras, dtype=torch.float64), torch.as_tensor(d.points, dtype=torch.float64),
+                       d.obs_camera, d.obs_point, d.observations)
+        dev.run(lm_options(tc.SolverOptions(), loop="host"))  # plan creation and first solve untimed
+        dev.set_parameters(torch.as_tensor(d.cameras, dtype=torch.float64),
+                           torch.as_tensor(d.points, dtype=torch.float64))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.run(lm_options(tc.SolverOptions(), loop="host"))
+        torch.cuda.synchronize()
+        plain += time.perf_counter() - t0
+        dev.close()
+    plain_rate = plain_sample / plain
+    out = {
+        "workload": f"{num_problems} bundle adjustments (2-20 cameras, 100-5000 points, 2-6 obs/point; "
+                    f"{sum(d.num_observations for d in datas)} observations in total), reference default options, "
+                    "DENSE_SCHUR, FP64, one persistent launch",
+        "launch_ms": launch_s * 1e3,
+        "solves_per_s": num_problems / launch_s,
+        "lm_iterations": iters,
+        "lm_iterations_per_s": iters / launch_s,
+        "host_table_build_ms": min(builds) * 1e3,
+        "per_problem_device_ms": {"median": float(np.median(dev_s) * 1e3), "max": float(dev_s.max() * 1e3)},
+        "plain_per_problem_plans_solves_per_s": plain_rate,
+        "plain_sample": f"first {plain_sample} problems, per-problem plans (host loop), plan creation untimed",
+        "batched_vs_plain": (num_problems / launch_s) / plain_rate,
+    }
+    log(f"[bench] batched BA: {out['solves_per_s']:.0f} solves/s in one launch ({out['launch_ms']:.1f} ms, "
+        f"host tables {out['host_table_build_ms']:.1f} ms); per-problem plans {plain_rate:.0f} solves/s")
+    return out
+
+
+def bench_serving(num_requests=300, rate=40.0, seed=4, slots=8, saturated_slots=(8, 16), baseline_sample=60):
     """Served request stream of 60% curve fits (as the batched workload, Cauchy(0.5)), 30% small bundle
     adjustments (2-20 cameras, 100-5000 points, 2-6 observations per point) and 10% SE2 pose graphs (100-2000
     nodes); reference default solver options.
@@ -720,8 +800,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16", "pose2000"],
-                    help="pose2000: time and profile one 2000-node SE2 pose graph solve (no JSON line)")
+    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16", "pose2000", "ba_batch"],
+                    help="pose2000: time and profile one 2000-node SE2 pose graph solve; ba_batch: the serving mix's "
+                         "101 bundle adjustments in one batched launch (no JSON line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only-value", action="store_true", help="time the value path only")
@@ -745,6 +826,9 @@ def main() -> int:
     name = args.workload
     if name == "pose2000":
         log(f"[bench] pose graph: {profile_pose_graph(2000)}")
+        return 0
+    if name == "ba_batch":
+        log(f"[bench] batched bundle adjustment: {bench_ba_batch()}")
         return 0
     t0 = time.perf_counter()
     data = gen.ba_config(name)
@@ -781,6 +865,7 @@ def main() -> int:
         log(f"[bench] ba16_huber time-to-tol {extra['time_to_tol_ba16_huber']}")
         sdev.close()
         extra["curve_batch"] = bench_curve_batch()
+        extra["ba_batch"] = bench_ba_batch()
         extra["serving"] = bench_serving()
 
     roofline = None
