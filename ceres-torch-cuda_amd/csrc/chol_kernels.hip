@@ -57,36 +57,6 @@ __host__ __device__ inline size_t factor_tiles(int n) {
   return nt * (nt + 1) / 2;
 }
 
-// Pivot step JJ of the one-wave factorisation of a 16-column panel held in
-// registers: lane l owns row l of the panel (the panel's first 16 rows are
-// its diagonal block).  Steps are expanded at compile time so that a[] is
-// only ever indexed by constants.  The pivot and the column below it reach
-// every lane through scalar registers (v_readlane), so a step needs no LDS
-// round trip.  The pivot's reciprocal goes to pinv[JJ] and a bad pivot sets
-// *pbad (LDS stores inside the step keep those results from being carried to
-// the end).
-template <int JJ>
-__device__ __forceinline__ void panel_step(double (&a)[kPW], double* pinv, int* pbad, int l) {
-  const double piv = lane_value(a[JJ], JJ);
-  const double inv = pivot_rsqrt(piv);
-  if (l == JJ) {
-    pinv[JJ] = inv;
-    if (!(piv > 0.0) || !isfinite(piv)) *pbad = 1;
-  }
-  // entries right of a lane's own diagonal are never read: they are scaled
-  // and updated like the others (they stay finite) instead of being masked
-  a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
-  const double f = l > JJ ? a[JJ] : 0.0;
-#pragma unroll
-  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
-}
-
-template <int... J>
-__device__ __forceinline__ void panel_steps(double (&a)[kPW], double* pinv, int* pbad, int l,
-                                            std::integer_sequence<int, J...>) {
-  (panel_step<J>(a, pinv, pbad, l), ...);
-}
-
 // x = S^-1 rhs (rhs == nullptr: factor only); the factor overwrites the lower
 // triangle of A and the pivots' reciprocals go to tmp[n:2n].
 __global__ __launch_bounds__(kSmallThreads) void k_chol_small(double* __restrict__ A, const double* __restrict__ rhs,
@@ -475,7 +445,11 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
 #pragma unroll
           for (int q = 0; q < kPW; ++q) a[q] = row < kTB ? D[row][base + q] : 0.0;
           lds_wave_sync();
-          panel_steps(a, pinv + base, &s_bad, l, std::make_integer_sequence<int, kPW>{});
+          double pin = 0.0;
+          int pbad = 0;
+          panel_steps_ahead(a, pin, pbad, l, kPW, std::make_integer_sequence<int, kPW>{});
+          if (l < kPW) pinv[base + l] = pin;
+          if (pbad && l == 0) s_bad = 1;
           if (row < kTB) {
 #pragma unroll
             for (int q = 0; q < kPW; ++q) D[row][base + q] = a[q];

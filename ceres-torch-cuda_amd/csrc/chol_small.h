@@ -63,28 +63,43 @@ constexpr int kBelowPerWave = kWave - kPW;  // panel rows below the diagonal blo
 
 __device__ __forceinline__ int packed_index(int i, int j) { return i * (i + 1) / 2 + j; }
 
-// Pivot step JJ of a panel whose first `valid` columns are real (the others
-// are padding: their pivot reciprocal is taken as 0, so they change nothing).
-// `writer`: this wave records the pivots' reciprocals and a bad pivot.
+// One 16-column panel held in registers, factored in 16 pivot steps: lane l
+// owns row l of the panel (its first 16 rows are the diagonal block, which
+// every factoring wave holds identically).  Pivots and columns reach every
+// lane through scalar registers (v_readlane), so a step needs no LDS round
+// trip.  A wave issues in order, so each step updates the next column first
+// and takes the next pivot's reciprocal square root before the remaining
+// column updates: that root's latency overlaps them instead of starting the
+// next step.  The first `valid` columns are real; padding columns get the
+// reciprocal 0 and change nothing.  On return lane JJ < 16 holds step JJ's
+// reciprocal in `pin`, and `bad` is set if a real pivot was not positive and
+// finite.  Steps are expanded at compile time, so a[] is only indexed by
+// constants.
 template <int JJ>
-__device__ __forceinline__ void small_panel_step(double (&a)[kPW], double* pinv, int* pbad, int l, int valid,
-                                                 bool writer) {
-  const double piv = lane_value(a[JJ], JJ);
-  const double inv = JJ < valid ? pivot_rsqrt(piv) : 0.0;
-  if (writer && l == JJ && JJ < valid) {
-    pinv[JJ] = inv;
-    if (!(piv > 0.0) || !isfinite(piv)) *pbad = 1;
-  }
+__device__ __forceinline__ void panel_step_ahead(double (&a)[kPW], double& piv, double& inv, double& pin, int& bad,
+                                                 int l, int valid) {
+  // piv, inv: this step's pivot and reciprocal square root, taken during the previous step
+  if (JJ < valid && (!(piv > 0.0) || !isfinite(piv))) bad = 1;
+  if (l == JJ) pin = inv;
   a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
   const double f = l > JJ ? a[JJ] : 0.0;
+  if constexpr (JJ + 1 < kPW) {
+    a[JJ + 1] = fma(-f, lane_value(f, JJ + 1), a[JJ + 1]);
+    const double next = lane_value(a[JJ + 1], JJ + 1);
+    const double next_inv = JJ + 1 < valid ? pivot_rsqrt(next) : 0.0;
 #pragma unroll
-  for (int m = JJ + 1; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
+    for (int m = JJ + 2; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
+    piv = next;
+    inv = next_inv;
+  }
 }
 
 template <int... J>
-__device__ __forceinline__ void small_panel_steps(double (&a)[kPW], double* pinv, int* pbad, int l, int valid,
-                                                  bool writer, std::integer_sequence<int, J...>) {
-  (small_panel_step<J>(a, pinv, pbad, l, valid, writer), ...);
+__device__ __forceinline__ void panel_steps_ahead(double (&a)[kPW], double& pin, int& bad, int l, int valid,
+                                                  std::integer_sequence<int, J...>) {
+  double piv = lane_value(a[0], 0);
+  double inv = valid > 0 ? pivot_rsqrt(piv) : 0.0;
+  (panel_step_ahead<J>(a, piv, inv, pin, bad, l, valid), ...);
 }
 
 // Lower tile t (row-major enumeration of the tiles with bj <= bi) -> (bi, bj).
@@ -286,7 +301,13 @@ __device__ __forceinline__ void small_factor(const double* __restrict__ A, const
     }
     __syncthreads();  // every factoring wave has read the raw diagonal block
     if (w < fwaves) {
-      small_panel_steps(a, sh.dinv + c0, &sh.bad, lane, valid, w == 0, std::make_integer_sequence<int, kPW>{});
+      double pin = 0.0;
+      int pbad = 0;
+      panel_steps_ahead(a, pin, pbad, lane, valid, std::make_integer_sequence<int, kPW>{});
+      if (w == 0) {
+        if (lane < valid) sh.dinv[c0 + lane] = pin;
+        if (pbad && lane == 0) sh.bad = 1;
+      }
       if (row <= n && (lane >= kPW || w == 0)) {
 #pragma unroll
         for (int q = 0; q < kPW; ++q) {
