@@ -663,6 +663,85 @@ def bench_ba_batch(num_problems=101, seed=4, repeats=3, plain_sample=30):
     return out
 
 
+def bench_block_schur(rows=409600, eliminate=512, remain=128, repeats=3):
+    """The reference's dense Schur benchmark shape through the registered ``block_schur`` backend.
+
+    Reference: ``benchmarking.py:131-172`` (``schur_benchmark``): A [409600 x 640]
+    Gaussian, x_true Gaussian, b = A x_true, ``solve_linear_system(A, b,
+    DENSE_SCHUR, num_eliminate=512)`` and the residual norm |A x - b|, one
+    warmup and the median of three runs; published 0.5114 s on CUDA
+    (``docs/PERFORMANCE_SUMMARY.md:61,73``).  Same shape and distribution here,
+    drawn on the device from a seeded generator.  The solve forms H = A^T A and
+    A^T b from row chunks of A on the matrix cores (k_gram_partials), reduces
+    the chunks in a fixed order and factors H (n = 640) with the tiled FP64
+    Cholesky; FP64 throughout.  The Gram flops are m n (n + 1) + 2 m n (lower
+    triangle of H and A^T b); the solve-only time bounds the Gram kernel's rate
+    from below (the kernel alone is in the rocprofv3 summary).
+    """
+    import ctypes
+
+    import torch
+
+    from ceres_mi355x import _native as nat
+    from ceres_mi355x import linear as L
+
+    cols = eliminate + remain
+    gen_ = torch.Generator(device="cuda").manual_seed(11)
+    A = torch.randn((rows, cols), generator=gen_, dtype=torch.float64, device="cuda")
+    x_true = torch.randn((cols,), generator=gen_, dtype=torch.float64, device="cuda")
+    b = A @ x_true
+    L.register_rocm_backends()
+
+    def run():
+        res = L.call_optional_backend("block_schur", A, b, num_eliminate=eliminate)
+        return res, torch.linalg.norm(A @ res.x - b)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    timed(run)  # warmup
+    full = sorted(timed(run)[0] for _ in range(repeats))
+    # solve alone on the device (HIP events on the stream the library launches on)
+    lib = nat.load()
+    nbytes = ctypes.c_size_t(0)
+    nat.check(lib.ct_dense_workspace_bytes(cols, ctypes.byref(nbytes)), "ct_dense_workspace_bytes")
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    x = torch.empty(cols, dtype=torch.float64, device="cuda")
+    solve_ms = []
+    for _ in range(repeats + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = lib.ct_block_schur_solve_f64(nat.ptr(A), rows, cols, cols, nat.ptr(b), eliminate, None, nat.ptr(x),
+                                          nat.ptr(ws), ws.numel(), nat.current_stream_handle())
+        e1.record()
+        nat.check(rc, "ct_block_schur_solve_f64")
+        torch.cuda.synchronize()
+        solve_ms.append(e0.elapsed_time(e1))
+    solve_ms = sorted(solve_ms[1:])
+    res, resid = run()
+    flops = rows * cols * (cols + 1) + 2.0 * rows * cols
+    med_solve = solve_ms[len(solve_ms) // 2]
+    ach = flops / (med_solve * 1e-3) / 1e12
+    out = {
+        "workload": f"schur/dense/{rows}x{cols}/elim{eliminate} (A Gaussian, b = A x_true, FP64)",
+        "median_s": full[len(full) // 2], "min_s": full[0], "max_s": full[-1], "repeats": repeats,
+        "timed": "registered block_schur backend + residual norm |A x - b|, as the reference's run()",
+        "reference_cuda_median_s": 0.5113517,
+        "speedup_vs_reference": 0.5113517 / full[len(full) // 2],
+        "residual_norm": float(resid), "solution_max_abs_error": float((res.x - x_true).abs().max()),
+        "solve_only_ms": med_solve, "gram_flops": flops,
+        "gram_rate_lower_bound": {"achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach / FP64_PEAK_TFLOPS},
+    }
+    del A, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_serving(num_requests=300, rate=40.0, seed=4, slots=8, saturated_slots=(8, 16), baseline_sample=60):
     """Served request stream of 60% curve fits (as the batched workload, Cauchy(0.5)), 30% small bundle
     adjustments (2-20 cameras, 100-5000 points, 2-6 observations per point) and 10% SE2 pose graphs (100-2000
@@ -800,9 +879,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16", "pose2000", "ba_batch"],
+    ap.add_argument("--workload", default="ba200", choices=["ba200", "ba16", "pose2000", "ba_batch", "block_schur"],
                     help="pose2000: time and profile one 2000-node SE2 pose graph solve; ba_batch: the serving mix's "
-                         "101 bundle adjustments in one batched launch (no JSON line)")
+                         "101 bundle adjustments in one batched launch; block_schur: the dense Schur solve of a "
+                         "409600 x 640 system (no JSON line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only-value", action="store_true", help="time the value path only")
@@ -829,6 +909,9 @@ def main() -> int:
         return 0
     if name == "ba_batch":
         log(f"[bench] batched bundle adjustment: {bench_ba_batch()}")
+        return 0
+    if name == "block_schur":
+        log(f"[bench] dense block Schur: {bench_block_schur()}")
         return 0
     t0 = time.perf_counter()
     data = gen.ba_config(name)
@@ -867,6 +950,8 @@ def main() -> int:
         extra["curve_batch"] = bench_curve_batch()
         extra["ba_batch"] = bench_ba_batch()
         extra["serving"] = bench_serving()
+        extra["block_schur"] = bench_block_schur()
+        log(f"[bench] dense block Schur: {extra['block_schur']}")
 
     roofline = None
     kernels = None

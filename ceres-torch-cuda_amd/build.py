@@ -37,11 +37,11 @@ COMMON = [
 # scratch.  A register-allocator spill copy placed inside divergent control
 # flow only saves the lanes active there, so a value that is live in other
 # lanes comes back as garbage when restored under a wider exec mask; kernels
-# are kept small enough that the allocator never spills.  The one kernel that
-# issues MFMA may hold its matrix-core accumulators in AGPRs: it is allowed
-# exactly that many (4 blocks of v_mfma_f64_16x16x4f64 results, 4 doubles
-# each = 32 registers) and no more, so any spill into AGPRs still fails.
-MFMA_ACCUMULATOR_AGPRS = {"k_chol_tiles": 32}
+# are kept small enough that the allocator never spills.  The kernels that
+# accumulate on the matrix cores may hold their accumulators in AGPRs: each is
+# allowed exactly that many (4 blocks of v_mfma_f64_16x16x4f64 results, 4
+# doubles each = 32 registers) and no more, so any spill into AGPRs still fails.
+MFMA_ACCUMULATOR_AGPRS = {"k_chol_tiles": 32, "k_gram_partials": 32}
 RESOURCE_RE = re.compile(r"remark: .*?(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|VGPRs Spill|SGPRs Spill): (\S+)")
 
 

@@ -3,87 +3,213 @@
 // sparse_cholesky / sparse_schur / block_schur (cuda_backends.py:158-307,
 // ceres_torch_cuda.cpp:3-16) for a dense row-major A [m x n]:
 //   H = A^T A + diag(damping),  rhs = A^T b,  H x = rhs   (x = H^-1 A^T b)
-// H is formed by a tiled Gram kernel and solved with the FP64 Cholesky of the
+// H and A^T b are formed from row chunks of A on the matrix cores (partial
+// tiles, then a fixed-order reduce) and solved with the FP64 Cholesky of the
 // reduced-camera-system solver.  Also the standalone factor / solve entry
 // points of that Cholesky, and a variant of the dense solve that allocates
 // nothing (its workspace is a static device array).
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
 #include "ba_kernels.h"
+#include "chol_small.h"
 
 namespace ct {
 
 namespace {
 
-constexpr int kGT = 64;   // Gram tile
-constexpr int kGK = 16;   // rows of A per LDS stage
+constexpr int kGT = 64;                 // Gram tile (64 x 64 entries of H)
+constexpr int kGS = 32;                 // rows of A per LDS stage
+constexpr int kGLd = kGT + 16;          // LDS row stride in doubles (see k_gram_partials)
+constexpr int kGramPartialTiles = 1024; // partial tiles the workspace holds (chunks x tiles of H)
+constexpr int kGramMaxChunks = 64;
+constexpr int64_t kGramMinRows = 1024;  // fewest rows of A a chunk is given
+constexpr int kXcds = 8;                // workgroups go to the accelerator dies round robin by index
 
-// Lower 64x64 tiles of H = A^T A + diag(d); the mirror of every off-diagonal
-// entry is written too, so H is stored whole.  256 threads x (4x4).
-__global__ __launch_bounds__(256) void k_gram(const double* __restrict__ A, int64_t m, int n, int64_t lda,
-                                              const double* __restrict__ damping, double* __restrict__ H) {
-  __shared__ double Ai[kGK][kGT + 1];
-  __shared__ double Aj[kGK][kGT + 1];
-  const int tt = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-  while (ti * (ti + 1) / 2 > tt) --ti;
-  while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
-  const int tj = tt - ti * (ti + 1) / 2;
+// H = A^T A is split over row chunks of A: workgroup (chunk, tile) forms the
+// 64 x 64 tile (ti, tj), tj <= ti, of the chunk's own product on the matrix
+// cores and stores it as a partial; k_gram_reduce adds the partials of each
+// entry in chunk order, so H does not depend on scheduling.
+struct GramShape {
+  int T = 0;          // tile rows of H
+  int tiles = 0;      // lower tiles, T (T + 1) / 2
+  int chunks = 0;     // row chunks of A
+  int64_t rows = 0;   // rows per chunk (a multiple of kGS)
+};
+
+int gram_chunk_cap(int n) {
+  const int T = (n + kGT - 1) / kGT, tiles = T * (T + 1) / 2;
+  return std::max(1, std::min(kGramMaxChunks, kGramPartialTiles / std::max(tiles, 1)));
+}
+
+// `slots`: workgroups of k_gram_partials resident on the device at once.
+// The chunk count is capped so that the grid fits one resident round when
+// the tiles allow it (a second, mostly empty round costs as much as a full
+// one).
+GramShape gram_shape(int64_t m, int n, int slots) {
+  GramShape g;
+  g.T = (n + kGT - 1) / kGT;
+  g.tiles = g.T * (g.T + 1) / 2;
+  int64_t c = std::min<int64_t>(gram_chunk_cap(n), std::max<int64_t>(1, (m + kGramMinRows - 1) / kGramMinRows));
+  if (c * g.tiles > slots) c = std::max<int64_t>(1, slots / g.tiles);
+  g.chunks = (int)c;
+  const int64_t per = (m + c - 1) / c;
+  g.rows = std::max<int64_t>(kGS, (per + kGS - 1) / kGS * kGS);
+  return g;
+}
+
+__device__ __forceinline__ void lower_tile(int t, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (ti * (ti + 1) / 2 > t) --ti;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// One partial tile: part[chunk][tile] = A_c[:, i-block]^T A_c[:, j-block]
+// over the chunk's rows, as v_mfma_f64_16x16x4f64 blocks.  Wave w owns the
+// 32 x 32 quadrant (w / 2, w % 2); in lane l, block b = 2 bi + bj holds in
+// item v the entry (32 (w / 2) + 16 bi + 4 v + l / 16, 32 (w % 2) + 16 bj +
+// l % 16).  The A operand of an instruction is the 16 x 4 slice
+// (lane l: H row l % 16, stage row l / 16), the B operand the 4 x 16 slice
+// (lane l: stage row l / 16, H column l % 16); both are rows of the staged
+// A.  The LDS row stride of 80 doubles puts the four 16-lane groups of one
+// operand read on alternating halves of the banks.
+// Work items (chunk, tile) are dealt to the dies (workgroup index mod kXcds)
+// in contiguous chunk-major slices, so the tiles of a chunk mostly share a die
+// and read its rows of A through that die's L2.
+// On a diagonal tile the upper-right quadrant is not needed: that wave forms
+// the chunk's part of A^T b for the tile's 64 columns instead.
+__global__ __launch_bounds__(256) void k_gram_partials(const double* __restrict__ A, int64_t m, int n, int64_t lda,
+                                                       const double* __restrict__ b, GramShape g,
+                                                       double* __restrict__ part, double* __restrict__ part_rhs) {
+  __shared__ double Ai[kGS][kGLd];
+  __shared__ double Aj[kGS][kGLd];
+  __shared__ double bs[kGS];
+  // die q % kXcds takes the contiguous slice [die * per, die * per + per) of
+  // the chunk-major work list; the grid is kXcds * per, so up to kXcds - 1
+  // workgroups find no work
+  const int q = blockIdx.x, work = g.chunks * g.tiles, per = (work + kXcds - 1) / kXcds;
+  const int item = (q % kXcds) * per + q / kXcds;
+  if (item >= work) return;
+  const int chunk = item / g.tiles, tile = item % g.tiles;
+  int ti, tj;
+  lower_tile(tile, ti, tj);
+  const bool diag = ti == tj;
   const int i0 = ti * kGT, j0 = tj * kGT;
-  const int tid = threadIdx.x;
-  const int tr = tid / 16, tc = tid % 16;
-  double acc[4][4];
+  const int64_t r_begin = (int64_t)chunk * g.rows;
+  const int64_t r_end = std::min<int64_t>(m, r_begin + g.rows);
+  const int tid = threadIdx.x, w = tid / 64, lane = tid % 64, li = lane & 15, lk = lane >> 4;
+  const bool rhs_wave = diag && w == 1;
+  const int sr = tid / 64, sc = tid % 64;  // staging: rows sr + 4 u, column sc
+  const bool ci_ok = i0 + sc < n, cj_ok = j0 + sc < n;
+  dx4 acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int k = 0; k < 4; ++k) acc[k] = dx4{0.0, 0.0, 0.0, 0.0};
+  double racc = 0.0;
+  double vi[kGS / 4], vj[kGS / 4], vb = 0.0;
+  auto fetch = [&](int64_t r0) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int64_t k0 = 0; k0 < m; k0 += kGK) {
-    for (int idx = tid; idx < kGK * kGT; idx += 256) {
-      const int r = idx / kGT, c = idx % kGT;
-      const int64_t k = k0 + r;
-      Ai[r][c] = (k < m && i0 + c < n) ? A[k * lda + i0 + c] : 0.0;
-      Aj[r][c] = (k < m && j0 + c < n) ? A[k * lda + j0 + c] : 0.0;
+    for (int u = 0; u < kGS / 4; ++u) {
+      const int64_t r = r0 + 4 * u + sr;
+      const bool ok = r < r_end;
+      vi[u] = (ok && ci_ok) ? A[r * lda + i0 + sc] : 0.0;
+      vj[u] = (!diag && ok && cj_ok) ? A[r * lda + j0 + sc] : 0.0;
     }
+    vb = (diag && tid < kGS && r0 + tid < r_end) ? b[r0 + tid] : 0.0;
+  };
+  if (r_begin < r_end) fetch(r_begin);
+  const int pr = 32 * (w >> 1) + li, qc = 32 * (w & 1) + li;
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += kGS) {
+    __syncthreads();  // the previous stage's readers are done
+#pragma unroll
+    for (int u = 0; u < kGS / 4; ++u) {
+      Ai[4 * u + sr][sc] = vi[u];
+      if (!diag) Aj[4 * u + sr][sc] = vj[u];
+    }
+    if (diag && tid < kGS) bs[tid] = vb;
     __syncthreads();
+    if (r0 + kGS < r_end) fetch(r0 + kGS);  // next stage's loads overlap this stage's products
+    const double(*Q)[kGLd] = diag ? Ai : Aj;
+    if (!rhs_wave) {
 #pragma unroll
-    for (int r = 0; r < kGK; ++r) {
-      double pa[4], pb[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        pa[a] = Ai[r][tr + 16 * a];
-        pb[a] = Aj[r][tc + 16 * a];
+      for (int k0 = 0; k0 < kGS; k0 += 4) {
+        const int k = k0 + lk;
+        const double a0 = Ai[k][pr], a1 = Ai[k][pr + 16];
+        const double b0 = Q[k][qc], b1 = Q[k][qc + 16];
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
       }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
+    } else {
+#pragma unroll 8
+      for (int k = 0; k < kGS; ++k) racc = fma(Ai[k][lane], bs[k], racc);
     }
-    __syncthreads();
   }
+  double* out = part + ((size_t)chunk * g.tiles + tile) * (kGT * kGT);
+  if (!rhs_wave) {
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = i0 + tr + 16 * a, j = j0 + tc + 16 * b;
-      if (i < n && j <= i) {
-        double v = acc[a][b];
-        if (i == j && damping) v += damping[i];
-        H[(size_t)i * n + j] = v;
-        if (i != j) H[(size_t)j * n + i] = v;
+      for (int v = 0; v < 4; ++v) {
+        const int row = 32 * (w >> 1) + 16 * (bb >> 1) + 4 * v + (lane >> 4);
+        const int col = 32 * (w & 1) + 16 * (bb & 1) + (lane & 15);
+        out[row * kGT + col] = acc[bb][v];
       }
-    }
+  } else {
+    part_rhs[(size_t)chunk * g.T * kGT + i0 + lane] = racc;
   }
 }
 
-// rhs = A^T b, one thread per column (rows read coalesced across columns)
-__global__ __launch_bounds__(256) void k_atb(const double* __restrict__ A, int64_t m, int n, int64_t lda,
-                                             const double* __restrict__ b, double* __restrict__ rhs) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int64_t k = 0; k < m; ++k) s += A[k * lda + i] * b[k];
-  rhs[i] = s;
+// H (both triangles) = sum of the partial tiles in chunk order + diag(d);
+// rhs = sum of the partial A^T b in chunk order.  Blocks [0, 16 tiles) take
+// 256 entries of a tile each, the rest 256 entries of rhs.
+__global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ part,
+                                                     const double* __restrict__ part_rhs, int n, GramShape g,
+                                                     const double* __restrict__ damping, double* __restrict__ H,
+                                                     double* __restrict__ rhs) {
+  constexpr int kPerTile = kGT * kGT / 256;
+  const int hb = g.tiles * kPerTile;
+  if ((int)blockIdx.x < hb) {
+    const int tile = blockIdx.x / kPerTile, e = (blockIdx.x % kPerTile) * 256 + threadIdx.x;
+    int ti, tj;
+    lower_tile(tile, ti, tj);
+    const int i = ti * kGT + e / kGT, j = tj * kGT + e % kGT;
+    if (i >= n || j > i) return;
+    double v = 0.0;
+    for (int c = 0; c < g.chunks; ++c) v += part[((size_t)c * g.tiles + tile) * (kGT * kGT) + e];
+    if (i == j && damping) v += damping[i];
+    H[(size_t)i * n + j] = v;
+    if (i != j) H[(size_t)j * n + i] = v;
+  } else {
+    const int col = (blockIdx.x - hb) * 256 + threadIdx.x;
+    if (col >= n) return;
+    double v = 0.0;
+    for (int c = 0; c < g.chunks; ++c) v += part_rhs[(size_t)c * g.T * kGT + col];
+    rhs[col] = v;
+  }
+}
+
+// Resident workgroups of k_gram_partials on the current device (cached per device).
+int gram_slots(int* slots) {
+  static std::mutex mu;
+  static int cached[64] = {};
+  int dev = 0;
+  CT_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0 || dev >= 64) {
+    set_error("device index out of range");
+    return CT_ERR_ARGUMENT;
+  }
+  if (cached[dev] == 0) {
+    int cus = 0, per_cu = 0;
+    CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gram_partials, 256, 0));
+    cached[dev] = std::max(1, cus) * std::max(1, per_cu);
+  }
+  *slots = cached[dev];
+  return CT_OK;
 }
 
 __global__ void k_clear_state(LMState* st) {
@@ -99,6 +225,8 @@ struct DenseLayout {
   double* H = nullptr;
   double* rhs = nullptr;
   double* tmp = nullptr;
+  double* part = nullptr;      // Gram partial tiles [chunks][tiles][kGT^2]
+  double* part_rhs = nullptr;  // partial A^T b [chunks][T kGT]
   LMState* st = nullptr;
   size_t bytes = 0;
 };
@@ -106,9 +234,12 @@ struct DenseLayout {
 DenseLayout dense_layout(int n, void* base) {
   Carver cv(base);
   DenseLayout L;
+  const size_t T = (size_t)(n + kGT - 1) / kGT, cap = (size_t)gram_chunk_cap(n);
   L.H = cv.take<double>((size_t)n * n);
   L.rhs = cv.take<double>((size_t)n);
   L.tmp = cv.take<double>(rcs_cholesky_tmp_doubles(n) + 1);
+  L.part = cv.take<double>(cap * (T * (T + 1) / 2) * kGT * kGT);
+  L.part_rhs = cv.take<double>(cap * T * kGT);
   L.st = cv.take<LMState>(1);
   L.bytes = cv.offset + 256;
   return L;
@@ -152,10 +283,14 @@ int launch_normal_equations(const double* A, int64_t m, int n, int64_t lda, cons
   hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, s, L.st);
   CT_LAUNCH_CHECK();
   CT_TRY(rcs_cholesky_prepare(L.tmp, n, s));
-  const int T = (n + kGT - 1) / kGT;
-  hipLaunchKernelGGL(k_gram, dim3(T * (T + 1) / 2), dim3(256), 0, s, A, m, n, lda, damping, L.H);
+  int slots = 0;
+  CT_TRY(gram_slots(&slots));
+  const GramShape g = gram_shape(m, n, slots);
+  const unsigned int ggrid = (unsigned)(((g.chunks * g.tiles + kXcds - 1) / kXcds) * kXcds);
+  hipLaunchKernelGGL(k_gram_partials, dim3(ggrid), dim3(256), 0, s, A, m, n, lda, b, g, L.part, L.part_rhs);
   CT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_atb, dim3((n + 255) / 256), dim3(256), 0, s, A, m, n, lda, b, L.rhs);
+  const unsigned int rgrid = (unsigned)(g.tiles * (kGT * kGT / 256) + (n + 255) / 256);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(rgrid), dim3(256), 0, s, L.part, L.part_rhs, n, g, damping, L.H, L.rhs);
   CT_LAUNCH_CHECK();
   return launch_rcs_cholesky_solve(L.H, L.rhs, x, n, L.st, L.tmp, s);
 }
@@ -168,7 +303,7 @@ int launch_normal_equations(const double* A, int64_t m, int n, int64_t lda, cons
 // stand-in for the symbol): the device address comes from
 // hipGetSymbolAddress, once per device, and only that address is used.
 constexpr int64_t kNoAllocMaxN = CT_NOALLOC_MAX_N;
-constexpr size_t kNoAllocScratchBytes = (size_t)4608 * 1024;  // >= the dense layout at n = 512 (4,502,624 bytes)
+constexpr size_t kNoAllocScratchBytes = (size_t)36800 * 1024;  // >= the largest dense layout for n <= 512 (n = 512: 37,647,456 bytes)
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {

@@ -655,7 +655,7 @@ int ct_normal_equations_solve_f64(const double* A, int64_t m, int64_t n, int64_t
  * in the reference.  It is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): the workspace is a static device array inside the library, one
- * copy per GPU, held until the process ends (4.5 MiB), so the call itself
+ * copy per GPU, held until the process ends (36 MiB), so the call itself
  * allocates nothing.  n <= CT_NOALLOC_MAX_N; larger n returns CT_ERR_ARGUMENT.
  * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
  * (other streams or threads) race and must be serialised by the caller.
