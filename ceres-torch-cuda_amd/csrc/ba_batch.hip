@@ -627,7 +627,7 @@ __device__ void solve_problem(const BatchArgs& a, int pi, BatchShared& sh) {
       continue;
     }
     reuse = false;
-    small_factor<9 * CT_BA_BATCH_MAX_CAMERAS>(w.S, w.rhs, pb.n, sh.chol);
+    small_factor<9 * CT_BA_BATCH_MAX_CAMERAS, false>(w.S, w.rhs, pb.n, sh.chol);
     double* y = sh.chol.Lp + packed_index(pb.n, 0);  // row n: y = L^-1 rhs
     small_backward(sh.chol.Lp, sh.chol.dinv, y, pb.n);
     int my_bad = 0;

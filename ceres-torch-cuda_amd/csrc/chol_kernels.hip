@@ -362,7 +362,9 @@ __device__ __forceinline__ bool inverse_tile(const double* __restrict__ A, int n
 //  * diagonal tile: wave 0 factors it in registers (lane l owns row l); the
 //    inverse of the factor is formed blockwise (16x16 diagonal blocks by
 //    substitution, then two doubling steps X21 = -V22 (L21 V11)) and kept for
-//    the off-diagonal tiles of the column and for the triangular solves;
+//    the off-diagonal tiles of the column and for the triangular solves; the
+//    tile is published once its inverse is stored (no other tile reads the
+//    diagonal tile of L) and its factor entries are stored after that;
 //  * off-diagonal tile: L_ij = A_ij V_jj^T with the published inverse.
 // Rows past n in the last tile row are zero; the diagonal tile pads them with
 // an identity block.  A pivot that is not positive and finite flags the step.
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
           lds_wave_sync();
           double pin = 0.0;
           int pbad = 0;
-          panel_steps_ahead(a, pin, pbad, l, kPW, std::make_integer_sequence<int, kPW>{});
+          panel_steps_ahead<true>(a, pin, pbad, l, kPW, std::make_integer_sequence<int, kPW>{});
           if (l < kPW) pinv[base + l] = pin;
           if (pbad && l == 0) s_bad = 1;
           if (row < kTB) {
@@ -486,8 +488,15 @@ __global__ __launch_bounds__(kFThreads) void k_chol_tiles(double* __restrict__ A
       for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
         const int r = idx / kTB, c = idx % kTB;
         vout[idx] = (r < cj && c < cj) ? V[r][c] : 0.0;
+      }
+      // other tiles read the inverse and the pivots, never this tile of L:
+      // publish first, then store the factor (complete when the launch ends)
+      publish_flag(&fw.ready[tile_index(j, j, nt)], epoch);
+      for (int idx = tid; idx < kTB * kTB; idx += kFThreads) {
+        const int r = idx / kTB, c = idx % kTB;
         if (r < cj && c <= r) A[(size_t)(c0 + r) * n + c0 + c] = D[r][c];
       }
+      continue;
     } else {
       if (!wait_flags(&fw.ready[tile_index(j, j, nt)], nullptr, epoch, &fw.ctl[3], &s_status)) break;
       double(*X)[kFLd] = lds[0];

@@ -74,19 +74,19 @@ __device__ __forceinline__ int packed_index(int i, int j) { return i * (i + 1) /
 // reciprocal 0 and change nothing.  On return lane JJ < 16 holds step JJ's
 // reciprocal in `pin`, and `bad` is set if a real pivot was not positive and
 // finite.  Steps are expanded at compile time, so a[] is only indexed by
-// constants.
-template <int JJ>
+// constants; kFull (every column real) drops the per-step column checks.
+template <int JJ, bool kFull>
 __device__ __forceinline__ void panel_step_ahead(double (&a)[kPW], double& piv, double& inv, double& pin, int& bad,
                                                  int l, int valid) {
   // piv, inv: this step's pivot and reciprocal square root, taken during the previous step
-  if (JJ < valid && (!(piv > 0.0) || !isfinite(piv))) bad = 1;
+  if ((kFull || JJ < valid) && (!(piv > 0.0) || !isfinite(piv))) bad = 1;
   if (l == JJ) pin = inv;
   a[JJ] = (l == JJ) ? piv * inv : a[JJ] * inv;
   const double f = l > JJ ? a[JJ] : 0.0;
   if constexpr (JJ + 1 < kPW) {
     a[JJ + 1] = fma(-f, lane_value(f, JJ + 1), a[JJ + 1]);
     const double next = lane_value(a[JJ + 1], JJ + 1);
-    const double next_inv = JJ + 1 < valid ? pivot_rsqrt(next) : 0.0;
+    const double next_inv = (kFull || JJ + 1 < valid) ? pivot_rsqrt(next) : 0.0;
 #pragma unroll
     for (int m = JJ + 2; m < kPW; ++m) a[m] = fma(-f, lane_value(f, m), a[m]);
     piv = next;
@@ -94,12 +94,12 @@ __device__ __forceinline__ void panel_step_ahead(double (&a)[kPW], double& piv, 
   }
 }
 
-template <int... J>
+template <bool kFull, int... J>
 __device__ __forceinline__ void panel_steps_ahead(double (&a)[kPW], double& pin, int& bad, int l, int valid,
                                                   std::integer_sequence<int, J...>) {
   double piv = lane_value(a[0], 0);
-  double inv = valid > 0 ? pivot_rsqrt(piv) : 0.0;
-  (panel_step_ahead<J>(a, piv, inv, pin, bad, l, valid), ...);
+  double inv = (kFull || valid > 0) ? pivot_rsqrt(piv) : 0.0;
+  (panel_step_ahead<J, kFull>(a, piv, inv, pin, bad, l, valid), ...);
 }
 
 // Lower tile t (row-major enumeration of the tiles with bj <= bi) -> (bi, bj).
@@ -230,8 +230,9 @@ __host__ __device__ constexpr int small_slots(int n) {
 // the packed factor with y = L^-1 rhs in row n, sh.dinv the pivots'
 // reciprocals and sh.bad the pivot check.  Every thread of the workgroup
 // (kSmallThreads) calls it; n <= kMaxN, whose tiles fill kSlots accumulator
-// slots per wave.
-template <int kMaxN = kSmallMax>
+// slots per wave.  kFullPanels adds a copy of the panel steps without the
+// per-column checks for panels of 16 real columns (faster, more registers).
+template <int kMaxN = kSmallMax, bool kFullPanels = true>
 __device__ __forceinline__ void small_factor(const double* __restrict__ A, const double* __restrict__ rhs, int n,
                                              SmallCholShared& sh) {
   constexpr int kSlots = small_slots(kMaxN);
@@ -303,7 +304,10 @@ __device__ __forceinline__ void small_factor(const double* __restrict__ A, const
     if (w < fwaves) {
       double pin = 0.0;
       int pbad = 0;
-      panel_steps_ahead(a, pin, pbad, lane, valid, std::make_integer_sequence<int, kPW>{});
+      if (kFullPanels && valid == kPW)
+        panel_steps_ahead<true>(a, pin, pbad, lane, valid, std::make_integer_sequence<int, kPW>{});
+      else
+        panel_steps_ahead<false>(a, pin, pbad, lane, valid, std::make_integer_sequence<int, kPW>{});
       if (w == 0) {
         if (lane < valid) sh.dinv[c0 + lane] = pin;
         if (pbad && lane == 0) sh.bad = 1;
