@@ -261,10 +261,7 @@ int enqueue_group(ct_ba_plan* pl, int group, hipStream_t s) {
                                        pl->options.solver_workgroups);
     case CT_KERNEL_REFINE:
       if (!refining(pl)) return CT_OK;
-      CT_TRY(launch_backsub<float>(v, L.bf, L.w, s));
-      for (int k = 0; k < pl->options.max_num_refinement_iterations; ++k)
-        CT_TRY(launch_refine_step<float>(v, L.bf, L.w, s));
-      return CT_OK;
+      return launch_refinement<float>(v, L.bf, L.w, pl->options.max_num_refinement_iterations, s);
     case CT_KERNEL_DOGLEG:
       if (!dogleg_options(&pl->options)) return CT_OK;
       return mixed ? launch_dogleg_step<float>(v, L.bf, L.w, s) : launch_dogleg_step<double>(v, L.bd, L.w, s);

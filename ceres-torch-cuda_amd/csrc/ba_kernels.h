@@ -39,8 +39,10 @@ int launch_dogleg_step(const BAView& v, const BABuffers<real>& b, const BAWork& 
 inline int dogleg_blocks(int num_points) { return num_points > 0 ? (num_points + kPointThreads - 1) / kPointThreads : 1; }
 // Test hook: NaN over the whole step when state->poison (ct_lm_options.debug_invalid_steps).
 int launch_poison_step(const BAView& v, const BAWork& w, hipStream_t s);
+// Mixed-precision refinement: `steps` refinement steps after the solve of
+// dx_c, the back-substitution of dx_p included (steps <= 0: that alone).
 template <typename real>
-int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s);
+int launch_refinement(const BAView& v, const BABuffers<real>& b, const BAWork& w, int steps, hipStream_t s);
 int launch_evaluate_only(const BAView& v, const double* cams, const double* pts, double* residuals, double* part,
                          hipStream_t s);
 int launch_evaluate_jacobian(const BAView& v, const double* cams, const double* pts, const int32_t* row_of_obs,

@@ -681,16 +681,53 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_backsub(BAView v, BABuffer
 //   delta_c = S^-1 (rho_c - sum_o Jc_o^T P_o L_p^-1 rho_p)
 //   delta_p = L_p^-T (L_p^-1 rho_p - sum_o P_o^T Jc_o delta_c)
 //   dx += delta
-// reusing the point factors and the FP64 Cholesky factor of S.
+// reusing the point factors and the FP64 Cholesky factor of S.  The passes
+// are fused across steps: the per-point pass of step k first completes the
+// point's step (step 1: back-substitution from the solved dx_c; later steps:
+// step k-1's update delta_p, with dx_c + delta_c standing for the updated
+// camera step), then forms the point's part of rho; the per-camera pass
+// applies the pending delta_c to its own camera before forming rho_c.  Only
+// the last step's update is a pass of its own.  The formulas and their order
+// are those of separate passes; the compiler's contraction into fused
+// multiply-adds may differ in the last bit.
 // ---------------------------------------------------------------------------
-template <typename real>
-__global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BABuffers<real> b, BAWork w) {
+template <typename real, bool kFirst>
+__global__ __launch_bounds__(kPointThreads, 2) void k_refine_pass(BAView v, BABuffers<real> b, BAWork w) {
   const LMState* st = w.state;
   if (st->done || st->skip_step) return;
   const int p = blockIdx.x * kPointThreads + threadIdx.x;
   if (p >= v.P) return;
   const int o0 = v.point_offsets[p], o1 = v.point_offsets[p + 1];
-  const double dp0 = w.dxp[3 * p], dp1 = w.dxp[3 * p + 1], dp2 = w.dxp[3 * p + 2];
+  const real* Li = b.Linv + 6 * (size_t)p;
+  double dp0, dp1, dp2;
+  if constexpr (kFirst) {
+    backsub_point(v, b, w.dxc, p, &dp0, &dp1, &dp2);
+  } else {
+    // the previous step's update of this point (its camera part waits for
+    // the per-camera pass)
+    double s0 = w.res_p[3 * p], s1 = w.res_p[3 * p + 1], s2 = w.res_p[3 * p + 2];
+    for (int o = o0; o < o1; ++o) {
+      const int ci = v.obs_camera[o];
+      const real* J = b.JP + kJPStride * (size_t)o;
+      double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double d = w.dxc_corr[9 * ci + j];
+        y0 += (double)J[j] * d;
+        y1 += (double)J[9 + j] * d;
+      }
+      const real* P = b.JP + kJPStride * (size_t)o + 18;
+      s0 -= (double)P[0] * y0 + (double)P[3] * y1;
+      s1 -= (double)P[1] * y0 + (double)P[4] * y1;
+      s2 -= (double)P[2] * y0 + (double)P[5] * y1;
+    }
+    dp0 = w.dxp[3 * p] + ((double)Li[0] * s0 + (double)Li[1] * s1 + (double)Li[3] * s2);
+    dp1 = w.dxp[3 * p + 1] + ((double)Li[2] * s1 + (double)Li[4] * s2);
+    dp2 = w.dxp[3 * p + 2] + (double)Li[5] * s2;
+  }
+  w.dxp[3 * p] = dp0;
+  w.dxp[3 * p + 1] = dp1;
+  w.dxp[3 * p + 2] = dp2;
   double rp0 = -b.gp[3 * p] - w.Dp[3 * p] * dp0;
   double rp1 = -b.gp[3 * p + 1] - w.Dp[3 * p + 1] * dp1;
   double rp2 = -b.gp[3 * p + 2] - w.Dp[3 * p + 2] * dp2;
@@ -702,7 +739,7 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
     double u1 = (double)Jq[3] * dp0 + (double)Jq[4] * dp1 + (double)Jq[5] * dp2;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
-      const double d = w.dxc[9 * ci + j];
+      const double d = kFirst ? w.dxc[9 * ci + j] : w.dxc[9 * ci + j] + w.dxc_corr[9 * ci + j];
       u0 += (double)J[j] * d;
       u1 += (double)J[9 + j] * d;
     }
@@ -712,7 +749,6 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
     rp1 -= (double)Jq[1] * u0 + (double)Jq[4] * u1;
     rp2 -= (double)Jq[2] * u0 + (double)Jq[5] * u1;
   }
-  const real* Li = b.Linv + 6 * (size_t)p;
   const double t0 = (double)Li[0] * rp0;
   const double t1 = (double)Li[1] * rp0 + (double)Li[2] * rp1;
   const double t2 = (double)Li[3] * rp0 + (double)Li[4] * rp1 + (double)Li[5] * rp2;
@@ -726,8 +762,9 @@ __global__ __launch_bounds__(kPointThreads, 2) void k_refine_points(BAView v, BA
   }
 }
 
-// One wavefront per camera: rho_c and the reduced right-hand side.
-template <typename real>
+// One wavefront per camera: rho_c and the reduced right-hand side (kUpdate:
+// first the previous step's correction of this camera, dx_c += delta_c).
+template <typename real, bool kUpdate>
 __global__ __launch_bounds__(kWave) void k_refine_cameras(BAView v, BABuffers<real> b, BAWork w) {
   __shared__ double part[kWave][9];
   const LMState* st = w.state;
@@ -752,7 +789,12 @@ __global__ __launch_bounds__(kWave) void k_refine_cameras(BAView v, BABuffers<re
     double ssum = 0.0;
     for (int l = 0; l < kWave; ++l) ssum += part[l][tid];
     const int k = 9 * ci + tid;
-    w.res_c[k] = -w.gc[k] - w.Dc[k] * w.dxc[k] - ssum;
+    double d = w.dxc[k];
+    if (kUpdate) {
+      d += w.dxc_corr[k];
+      w.dxc[k] = d;
+    }
+    w.res_c[k] = -w.gc[k] - w.Dc[k] * d - ssum;
   }
 }
 
@@ -1361,17 +1403,22 @@ int launch_poison_step(const BAView& v, const BAWork& w, hipStream_t s) {
 }
 
 template <typename real>
-int launch_refine_step(const BAView& v, const BABuffers<real>& b, const BAWork& w, hipStream_t s) {
+int launch_refinement(const BAView& v, const BABuffers<real>& b, const BAWork& w, int steps, hipStream_t s) {
+  if (steps <= 0) return launch_backsub<real>(v, b, w, s);
+  for (int k = 0; k < steps; ++k) {
+    if (v.P > 0) {
+      if (k == 0) hipLaunchKernelGGL((k_refine_pass<real, true>), dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
+      else hipLaunchKernelGGL((k_refine_pass<real, false>), dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
+      CT_LAUNCH_CHECK();
+    }
+    if (v.C > 0) {
+      if (k == 0) hipLaunchKernelGGL((k_refine_cameras<real, false>), dim3(v.C), dim3(kWave), 0, s, v, b, w);
+      else hipLaunchKernelGGL((k_refine_cameras<real, true>), dim3(v.C), dim3(kWave), 0, s, v, b, w);
+      CT_LAUNCH_CHECK();
+      CT_TRY(launch_rcs_tri_solve(w.S, w.res_c, w.dxc_corr, 9 * v.C, w.state, w.solve_tmp, s));
+    }
+  }
   const int work = v.P > 9 * v.C ? v.P : 9 * v.C;
-  if (v.P > 0) {
-    hipLaunchKernelGGL(k_refine_points<real>, dim3(blocks_for(v.P)), dim3(kPointThreads), 0, s, v, b, w);
-    CT_LAUNCH_CHECK();
-  }
-  if (v.C > 0) {
-    hipLaunchKernelGGL(k_refine_cameras<real>, dim3(v.C), dim3(kWave), 0, s, v, b, w);
-    CT_LAUNCH_CHECK();
-    CT_TRY(launch_rcs_tri_solve(w.S, w.res_c, w.dxc_corr, 9 * v.C, w.state, w.solve_tmp, s));
-  }
   if (work > 0) {
     hipLaunchKernelGGL(k_refine_update<real>, dim3(blocks_for(work)), dim3(kPointThreads), 0, s, v, b, w);
     CT_LAUNCH_CHECK();
@@ -1422,7 +1469,7 @@ template int launch_backsub<float>(const BAView&, const BABuffers<float>&, const
 template int launch_backsub<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
 template int launch_dogleg_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
 template int launch_dogleg_step<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
-template int launch_refine_step<float>(const BAView&, const BABuffers<float>&, const BAWork&, hipStream_t);
-template int launch_refine_step<double>(const BAView&, const BABuffers<double>&, const BAWork&, hipStream_t);
+template int launch_refinement<float>(const BAView&, const BABuffers<float>&, const BAWork&, int, hipStream_t);
+template int launch_refinement<double>(const BAView&, const BABuffers<double>&, const BAWork&, int, hipStream_t);
 
 }  // namespace ct

@@ -949,7 +949,12 @@ int launch_rcs_cholesky_solve(double* S, const double* rhs, double* x, int n, LM
   CT_TRY(device_cus(&cus));
   const size_t tiles = tri_blocks(n) * tri_blocks(n);  // factor and inverse tiles
   size_t G = std::min<size_t>(tiles, 2 * (size_t)cus);  // two workgroups fit a CU (LDS)
-  if (max_groups > 0) G = std::min<size_t>(G, (size_t)max_groups);  // tickets need no co-residency
+  if (max_groups > 0) {
+    // a capped launch shares the chip: two workgroups per tile column keep a
+    // single factorisation on its critical path, more only spin (tickets need
+    // no co-residency)
+    G = std::min<size_t>(G, std::min<size_t>((size_t)max_groups, std::max<size_t>(16, 2 * tri_blocks(n))));
+  }
   hipLaunchKernelGGL(k_chol_tiles, dim3((unsigned)G), dim3(kFThreads), 0, s, S, n, factor_work(tmp, n), st);
   CT_LAUNCH_CHECK();
   if (rhs == nullptr) return CT_OK;  // factorisation only
