@@ -95,32 +95,41 @@ def timed_iters(dev, data, *, precision, loop, linear, steps, warmup, refine=2):
         return lm_options(o, precision=precision, loop=loop, linear=linear)
 
     run_solve(dev, data, opts_for(max(1, warmup)))
-    seq, done = [], 0
-    while done < steps:
-        o = opts_for(steps - done)
-        rep, _, _ = run_solve(dev, data, o)
-        n = int(rep.num_records) - 1
-        if n <= 0:
-            raise RuntimeError("LM made no iteration")
-        seq.append(o)
-        done += n
     cams0 = torch.tensor(data.cameras, device=dev.device)
     pts0 = torch.tensor(data.points, device=dev.device)
-    lin, accepted, total = [], 0, 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for o in seq:
-        dev.set_parameters(cams0, pts0)
-        rep, rec = dev.run(o)
-        total += int(rep.num_records) - 1
-        lin.extend(rec[1:, 10].tolist())
-        accepted += int(rec[1:, 9].sum())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    if total != steps:
+    # The timed pass must repeat the learned solve lengths.  The same option
+    # set from the same start gives the same trajectory (repeated solves agree
+    # bit for bit); if a pass still differs, it is learned and timed again and
+    # the retry is reported.
+    for attempt in range(3):
+        seq, done = [], 0
+        while done < steps:
+            o = opts_for(steps - done)
+            rep, _, _ = run_solve(dev, data, o)
+            n = int(rep.num_records) - 1
+            if n <= 0:
+                raise RuntimeError("LM made no iteration")
+            seq.append(o)
+            done += n
+        lin, accepted, total = [], 0, 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for o in seq:
+            dev.set_parameters(cams0, pts0)
+            rep, rec = dev.run(o)
+            total += int(rep.num_records) - 1
+            lin.extend(rec[1:, 10].tolist())
+            accepted += int(rec[1:, 9].sum())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if total == steps:
+            break
+        log(f"[bench] timed pass ran {total} iterations where the learned solves ran {steps}; learning again")
+    else:
         raise RuntimeError(f"timed pass ran {total} iterations, expected {steps}")
     return dict(iters=total, wall_s=wall, iters_per_s=total / wall, solves=len(seq), accepted_steps=accepted,
-                mean_linear_iterations=float(sum(lin) / max(len(lin), 1)), final_cost=rep.final_cost), rep
+                mean_linear_iterations=float(sum(lin) / max(len(lin), 1)), final_cost=rep.final_cost,
+                timed_pass_retries=attempt), rep
 
 
 def time_to_tol(dev, data, *, precision, loop, linear, max_iters, target):
