@@ -645,7 +645,10 @@ int ct_ba_plan_covariance(ct_ba_plan* plan, const int32_t* requests, int64_t num
  * the reference's native normal_equations_solve / block_schur_solve
  * (ceres_torch_cuda.cpp:3-16, cuda_backends.py:267-307) behind the plugin
  * names sparse_normal_cholesky, sparse_cholesky, sparse_schur, block_schur.
- * FP64.  Synchronises the stream to report CT_ERR_NOT_SPD.                  */
+ * FP64.  A^T A and A^T b are formed from row chunks of A and the chunks are
+ * summed in a fixed order, so repeated calls give identical results; the
+ * workspace (ct_dense_workspace_bytes) depends on n only.  Synchronises the
+ * stream to report CT_ERR_NOT_SPD.                                          */
 /* ------------------------------------------------------------------------ */
 int ct_dense_workspace_bytes(int64_t n, size_t* bytes);
 int ct_normal_equations_solve_f64(const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
